@@ -1,0 +1,204 @@
+"""oxDNA on-disk formats: input files, topologies and configurations.
+
+Formats as the reference reads and writes them:
+  - input file: `key = value` lines, `#` comments, `{ ... }` blocks kept as
+    raw text (src/Utilities/parse_input/parse_input.cpp);
+  - topology, old style: header "N N_strands", then per particle
+    "strand base n3 n5"; new style: header "N N_strands 5->3", then one
+    strand per line listed 5'->3' (src/Utilities/TopologyParser.cpp:15-172,
+    src/Interactions/DNAInteraction.cpp:1506-1577);
+  - configuration: "t = ", "b = Lx Ly Lz", "E = ..." headers, then
+    "x y z a1 a3 [v L]" per particle; a1 and a3 are re-orthonormalised on
+    read and a2 = a3 x a1 (src/Backends/SimBackend.cpp:605-623,
+    src/Observables/Configurations/Configuration.cpp:85-140).
+"""
+from __future__ import annotations
+
+import re
+
+import numpy as np
+
+from .systems import System
+
+_BASES = {"A": 0, "G": 1, "C": 2, "T": 3, "a": 0, "g": 1, "c": 2, "t": 3}
+_NAMES = "AGCT"
+
+
+def base_name(btype: int) -> str:
+    if 0 <= btype < 4:
+        return _NAMES[btype]
+    return str(btype)
+
+
+def decode_base(tok: str) -> int:
+    if tok in _BASES:
+        return _BASES[tok]
+    return int(tok)
+
+
+def parse_input(text: str) -> dict:
+    """key = value pairs of an oxDNA input file.  Blocks in braces are kept
+    verbatim as the value string.  Later keys override earlier ones."""
+    out: dict = {}
+    lines = text.splitlines()
+    i = 0
+    while i < len(lines):
+        line = lines[i].split("#", 1)[0].strip()
+        i += 1
+        if not line or "=" not in line:
+            continue
+        key, val = [x.strip() for x in line.split("=", 1)]
+        if val.startswith("{") and val.count("{") > val.count("}"):
+            depth = val.count("{") - val.count("}")
+            block = [val]
+            while depth > 0 and i < len(lines):
+                block.append(lines[i])
+                depth += lines[i].count("{") - lines[i].count("}")
+                i += 1
+            val = "\n".join(block)
+        out[key] = val
+    return out
+
+
+def read_input(path: str, overrides=()) -> dict:
+    with open(path) as f:
+        d = parse_input(f.read())
+    for ov in overrides:
+        k, v = ov.split("=", 1)
+        d[k.strip()] = v.strip()
+    return d
+
+
+def temperature(raw: str) -> float:
+    """T in simulation units from "20C", "300K" or a plain number
+    (src/Utilities/Utils.cpp:316-352)."""
+    m = re.match(r"^\s*([-+0-9.eE]+)\s*([cCkK]?)\s*$", str(raw))
+    if not m:
+        raise ValueError(f"Unrecognizable temperature '{raw}'")
+    v, unit = float(m.group(1)), m.group(2).lower()
+    if unit == "c":
+        return (v + 273.15) * 0.1 / 300.0
+    if unit == "k":
+        return v * 0.1 / 300.0
+    return v
+
+
+def write_topology(path: str, s: System, new_style: bool = False) -> None:
+    N = s.N
+    ns = int(s.strand.max()) + 1 if N else 0
+    with open(path, "w") as f:
+        if not new_style:
+            f.write(f"{N} {ns}\n")
+            for i in range(N):
+                f.write(f"{s.strand[i] + 1} {base_name(int(s.btype[i]))} {s.n3[i]} {s.n5[i]}\n")
+            return
+        f.write(f"{N} {ns} 5->3\n")
+        for k in range(ns):
+            idx = np.nonzero(s.strand == k)[0]
+            # 5' end: the particle without an n5 inside the strand
+            head = [i for i in idx if s.n5[i] < 0]
+            circular = not head
+            i = int(head[0]) if head else int(idx[0])
+            seq = []
+            for _ in range(len(idx)):
+                b = int(s.btype[i])
+                seq.append(_NAMES[b] if 0 <= b < 4 else f"({b})")
+                i = int(s.n3[i])
+                if i < 0:
+                    break
+            f.write("".join(seq) + (" circular=true" if circular else "") + "\n")
+
+
+def read_topology(path: str):
+    """Returns (btype, n3, n5, strand) arrays for either topology style."""
+    with open(path) as f:
+        lines = [l.rstrip("\n") for l in f]
+    head = lines[0].split()
+    N = int(head[0])
+    btype = np.zeros(N, np.int32)
+    n3 = -np.ones(N, np.int32)
+    n5 = -np.ones(N, np.int32)
+    strand = np.zeros(N, np.int32)
+    if len(head) == 3 and head[2] == "5->3":
+        idx = 0
+        for k, line in enumerate(l for l in lines[1:] if l.strip()):
+            toks = line.split()
+            seq = toks[0]
+            circular = any(t.lower() in ("circular=true", "circular=1") for t in toks[1:])
+            bts = []
+            j = 0
+            while j < len(seq):
+                if seq[j] == "(":
+                    e = seq.index(")", j)
+                    bts.append(int(seq[j + 1:e]))
+                    j = e + 1
+                else:
+                    bts.append(decode_base(seq[j]))
+                    j += 1
+            n = len(bts)
+            for m in range(n):
+                i = idx + m
+                btype[i] = bts[m]
+                strand[i] = k
+                n5[i] = i - 1 if m > 0 else (idx + n - 1 if circular else -1)
+                n3[i] = i + 1 if m < n - 1 else (idx if circular else -1)
+            idx += n
+        if idx != N:
+            raise ValueError(f"topology header says {N} particles, found {idx}")
+        return btype, n3, n5, strand
+    i = 0
+    for line in lines[1:]:
+        if not line.strip() or line.startswith("#"):
+            continue
+        st, b, a, c = line.split()[:4]
+        strand[i] = int(st) - 1
+        btype[i] = decode_base(b)
+        n3[i] = int(a)
+        n5[i] = int(c)
+        i += 1
+    if i != N:
+        raise ValueError(f"topology header says {N} particles, found {i}")
+    return btype, n3, n5, strand
+
+
+def write_conf(path: str, s: System, step: int = 0, energies=(0.0, 0.0, 0.0)) -> None:
+    vel = s.vel if s.vel is not None else np.zeros((s.N, 3))
+    L = s.L if s.L is not None else np.zeros((s.N, 3))
+    with open(path, "w") as f:
+        f.write(f"t = {step}\n")
+        f.write("b = {:.17g} {:.17g} {:.17g}\n".format(*s.box))
+        f.write("E = {:.15g} {:.15g} {:.15g}\n".format(*energies))
+        for i in range(s.N):
+            row = list(s.pos[i]) + list(s.a1[i]) + list(s.a3[i]) + list(vel[i]) + list(L[i])
+            f.write(" ".join(f"{x:.17g}" for x in row) + "\n")
+
+
+def read_conf(path: str, N: int):
+    """Returns (step, box, pos, a1, a2, a3, vel, L) with the reference's
+    re-orthonormalisation of a1 and a3 (src/Backends/SimBackend.cpp:616-623)."""
+    with open(path) as f:
+        t = f.readline()
+        b = f.readline()
+        f.readline()
+        step = int(float(t.split("=")[1]))
+        box = np.array([float(x) for x in b.split("=")[1].split()])
+        data = np.loadtxt(f, ndmin=2)
+    if data.shape[0] != N:
+        raise ValueError(f"configuration has {data.shape[0]} particles, expected {N}")
+    pos = data[:, 0:3].copy()
+    a1 = data[:, 3:6] / np.linalg.norm(data[:, 3:6], axis=1, keepdims=True)
+    a3 = data[:, 6:9] / np.linalg.norm(data[:, 6:9], axis=1, keepdims=True)
+    a1 = a1 - a3 * np.sum(a1 * a3, axis=1, keepdims=True)
+    a1 /= np.linalg.norm(a1, axis=1, keepdims=True)
+    a2 = np.cross(a3, a1)
+    a2 /= np.linalg.norm(a2, axis=1, keepdims=True)
+    vel = data[:, 9:12].copy() if data.shape[1] >= 15 else np.zeros((N, 3))
+    L = data[:, 12:15].copy() if data.shape[1] >= 15 else np.zeros((N, 3))
+    return step, box, pos, a1, a2, a3, vel, L
+
+
+def load_system(topology: str, conf: str, T: float = 0.1, salt: float = 0.5, name: str = "") -> System:
+    btype, n3, n5, strand = read_topology(topology)
+    step, box, pos, a1, a2, a3, vel, L = read_conf(conf, len(btype))
+    return System(pos=pos, a1=a1, a3=a3, btype=btype, n3=n3, n5=n5, strand=strand, box=box, T=T, salt=salt,
+                  name=name, vel=vel, L=L)

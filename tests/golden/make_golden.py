@@ -1,0 +1,219 @@
+"""Writes the golden fixtures of tests/golden/ with the reference binary.
+
+Run in the build container (needs /root/reference and oracle/_ref/oxDNA,
+built by `make -f oracle/ref.mk`):
+
+    python tests/golden/make_golden.py
+
+For every case it writes a topology + configuration, runs the reference
+CPU backend (`backend = CPU`, `interaction_type = DNA2_nomesh`, `steps = 0`)
+and records what the reference computes at step 0:
+  - the 8-term energy split at 15 decimals (potential_energy observable,
+    split = true, precision = 15; src/Observables/PotentialEnergy.cpp:39-55);
+  - per-particle forces and body-frame torques at %.17g through the
+    project's ForceDump observable plugin (oracle/ForceDump.cpp), loaded by
+    the reference's own PluginManager.
+Each case becomes one small .npz (inputs and expected outputs only).
+"""
+from __future__ import annotations
+
+import math
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oxdna_amd import io, systems  # noqa: E402
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "oxDNA")
+PLUGIN_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+
+def run_reference(s: systems.System, T_str: str, salt: float, extra: dict | None = None, interaction="DNA2_nomesh"):
+    extra = extra or {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        io.write_topology(os.path.join(d, "t.top"), s)
+        io.write_conf(os.path.join(d, "c.dat"), s)
+        inp = {
+            "backend": "CPU", "sim_type": "MD", "interaction_type": interaction, "steps": "0", "seed": "1",
+            "T": T_str, "salt_concentration": repr(salt), "dt": "0.003", "verlet_skin": "0.05",
+            "thermostat": "no", "refresh_vel": "1", "topology": "t.top", "conf_file": "c.dat",
+            "trajectory_file": "traj.dat", "energy_file": "energy.dat", "print_energy_every": "1000",
+            "print_conf_interval": "100000", "no_stdout_energy": "1", "restart_step_counter": "1",
+            "box_type": "orthogonal", "time_scale": "linear", "plugin_search_path": PLUGIN_DIR, "log_file": "log.dat",
+        }
+        inp.update({k: str(v) for k, v in extra.items()})
+        with open(os.path.join(d, "input"), "w") as f:
+            for k, v in inp.items():
+                f.write(f"{k} = {v}\n")
+            f.write("data_output_1 = {\n name = split.dat\n print_every = 1\n col_1 = {\n type = potential_energy\n"
+                    " split = true\n precision = 15\n }\n}\n")
+            f.write("data_output_2 = {\n name = forces.dat\n print_every = 1\n col_1 = {\n type = ForceDump\n }\n}\n")
+        r = subprocess.run([REF_BIN, "input"], cwd=d, capture_output=True, text=True)
+        if r.returncode != 0:
+            log = open(os.path.join(d, "log.dat")).read() if os.path.exists(os.path.join(d, "log.dat")) else ""
+            raise RuntimeError(f"reference failed: {r.stdout[-2000:]} {r.stderr[-2000:]} {log[-2000:]}")
+        split = np.array([float(x) for x in open(os.path.join(d, "split.dat")).read().split()])
+        rows = np.loadtxt(os.path.join(d, "forces.dat"), ndmin=2)
+    idx = rows[:, 0].astype(int)
+    F = np.zeros((s.N, 3))
+    T = np.zeros((s.N, 3))
+    F[idx] = rows[:, 1:4]
+    T[idx] = rows[:, 4:7]
+    return split * s.N, F, T
+
+
+def tangle(seed: int, n_strands: int = 6, length: int = 8, box: float = 6.0) -> systems.System:
+    """Short strands grown as random walks of the backbone site with random
+    nucleotide orientations, packed in a small box, so that HB, cross and
+    coaxial stacking are sampled at non-ideal angles.  Candidates closer
+    than the excluded-volume ranges to any other nucleotide are rejected
+    (back-back 0.70, base-base 0.36, back-base 0.53; the bonded neighbour is
+    exempt from back-back only)."""
+    rng = np.random.default_rng(seed)
+    b = systems._Builder()
+    placed = []          # (back, base) of every placed nucleotide
+
+    def dist(p, q):
+        d = p - q
+        d -= box * np.rint(d / box)
+        return np.sqrt(d @ d)
+
+    def clear(back, base, skip_backback):
+        for k, (qb, qs) in enumerate(placed):
+            if k not in skip_backback and dist(back, qb) < 0.70:
+                return False
+            if dist(base, qs) < 0.36 or dist(back, qs) < 0.53 or dist(base, qb) < 0.53:
+                return False
+        return True
+
+    for _ in range(n_strands):
+        for attempt in range(2000):
+            pos, a1s, a3s, mine = [], [], [], []
+            back_prev = rng.random(3) * box
+            good = True
+            for k in range(length):
+                for tries in range(400):
+                    R = systems.random_rotation(rng)
+                    a1, a3 = R[:, 0], R[:, 2]
+                    a2 = np.cross(a3, a1)
+                    if k == 0:
+                        back = back_prev
+                    else:
+                        u = rng.normal(size=3)
+                        u /= np.linalg.norm(u)
+                        back = back_prev + u * rng.uniform(0.62, 0.9)
+                    c = back - (systems.BACK1 * a1 + systems.BACK2 * a2)
+                    base = c + 0.4 * a1
+                    skip = {len(placed) + len(mine) - 1} if k > 0 else set()
+                    saved = placed[:]
+                    placed.extend(mine)
+                    ok = clear(back, base, skip)
+                    placed[:] = saved
+                    if ok:
+                        break
+                else:
+                    good = False
+                    break
+                mine.append((back, base))
+                pos.append(c)
+                a1s.append(a1)
+                a3s.append(a3)
+                back_prev = back
+            if good:
+                placed.extend(mine)
+                b.add_strand(pos, a1s, a3s, rng.integers(0, 4, length))
+                break
+    return b.build([box] * 3, f"tangle{seed}")
+
+
+def equilibrate(s: systems.System, T_str: str, salt: float, steps: int, seed: int = 7) -> systems.System:
+    """Runs the reference binary for `steps` MD steps (Langevin, dt 0.003)
+    from s and returns its last configuration: thermal configurations
+    produced by the reference itself."""
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        io.write_topology(os.path.join(d, "t.top"), s)
+        io.write_conf(os.path.join(d, "c.dat"), s)
+        inp = {
+            "backend": "CPU", "sim_type": "MD", "interaction_type": "DNA2_nomesh", "steps": str(steps),
+            "seed": str(seed), "T": T_str, "salt_concentration": repr(salt), "dt": "0.003", "verlet_skin": "0.05",
+            "thermostat": "langevin", "diff_coeff": "2.5", "refresh_vel": "1", "topology": "t.top",
+            "conf_file": "c.dat", "trajectory_file": "traj.dat", "energy_file": "energy.dat",
+            "print_energy_every": "1000", "print_conf_interval": "1000000", "no_stdout_energy": "1",
+            "restart_step_counter": "1", "box_type": "orthogonal", "time_scale": "linear", "log_file": "log.dat",
+            "lastconf_file": "last.dat", "fix_diffusion": "0",
+        }
+        with open(os.path.join(d, "input"), "w") as f:
+            for k, v in inp.items():
+                f.write(f"{k} = {v}\n")
+        r = subprocess.run([REF_BIN, "input"], cwd=d, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("reference MD run failed: " + open(os.path.join(d, "log.dat")).read()[-2000:])
+        out = io.load_system(os.path.join(d, "t.top"), os.path.join(d, "last.dat"), name=s.name + "_md")
+    out.vel = out.L = None
+    return out
+
+
+def nicked_duplex(seed: int) -> systems.System:
+    """A 24-bp duplex whose first strand is cut in the middle: the two
+    nucleotides at the nick stack coaxially without being bonded."""
+    rng = np.random.default_rng(seed)
+    d = systems._Builder()
+    systems.add_duplex(d, rng.integers(0, 4, 24), (10.0, 10.0, 10.0))
+    s = d.build([20.0] * 3, "nicked")
+    s.n5[11] = -1
+    s.n3[12] = -1
+    s.strand[12:24] = 2
+    return s
+
+
+def bundle_small(seed: int) -> systems.System:
+    return systems.duplex_bundle(seed, nx=3, ny=2, bp=12, spacing=2.6, lz=12 * systems.BASE_BASE + 3.0)
+
+
+def cases():
+    ff = io.load_system("/root/reference/test/DNA/FORCE_FIELD/init.top", "/root/reference/test/DNA/FORCE_FIELD/init.dat")
+    ff.name = "ff16"
+    hp = systems.hairpin_set(3, n=4)
+    gas = systems.duplex_gas(5, n_duplex=8, bp=24, density=0.04)
+    yield "ff16_20C_1M", ff, "20C", 1.0, {}
+    yield "ff16_30C_0.2M", ff, "30C", 0.2, {}
+    yield "duplex20_20C_0.5M", systems.duplex_20bp(0), "20C", 0.5, {}
+    yield "hairpin_a", hp[0], "25C", 0.5, {}
+    yield "hairpin_b", hp[3], "75C", 0.5, {}
+    yield "bundle_30C_0.2M", bundle_small(1), "30C", 0.2, {}
+    yield "gas8_25C_0.5M", gas, "25C", 0.5, {}
+    yield "duplex20_md_20C", equilibrate(systems.duplex_20bp(2), "20C", 0.5, 4000), "20C", 0.5, {}
+    yield "nicked_md_30C", equilibrate(nicked_duplex(4), "30C", 0.5, 4000), "30C", 0.5, {}
+    yield "hairpin_md_50C", equilibrate(hp[1], "50C", 0.5, 4000), "50C", 0.5, {}
+    yield "bundle_md_30C", equilibrate(bundle_small(6), "30C", 0.2, 2000), "30C", 0.2, {}
+    yield "tangle1_20C_0.5M", tangle(1), "20C", 0.5, {}
+    yield "tangle2_45C_0.1M", tangle(2, n_strands=8, length=6, box=5.0), "45C", 0.1, {}
+    yield "tangle3_nohalf_dh", tangle(3), "37C", 0.3, {"dh_half_charged_ends": "false", "dh_lambda": "0.4",
+                                                        "dh_strength": "0.06"}
+
+
+def main():
+    if not os.path.exists(REF_BIN):
+        raise SystemExit("build the reference first: make -f oracle/ref.mk -j8")
+    for name, s, T_str, salt, extra in cases():
+        E, F, T = run_reference(s, T_str, salt, extra)
+        a2 = np.cross(s.a3, s.a1)
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), pos=s.pos, a1=s.a1, a2=a2, a3=s.a3, btype=s.btype,
+                            n3=s.n3, n5=s.n5, strand=s.strand, box=s.box, T=io.temperature(T_str), salt=salt,
+                            half_charged_ends=int(extra.get("dh_half_charged_ends", "true") != "false"),
+                            dh_lambda=float(extra.get("dh_lambda", -1.0)),
+                            dh_strength=float(extra.get("dh_strength", -1.0)),
+                            E8=E, F=F, Tbody=T)
+        print(f"{name}: N={s.N} U/N={E.sum() / s.N:.6f} max|F|={np.abs(F).max():.3g}")
+
+
+if __name__ == "__main__":
+    main()
