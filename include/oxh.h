@@ -1,0 +1,190 @@
+/*
+ * oxh: MI355X-native oxDNA2 molecular-dynamics hot path, C ABI.
+ *
+ * Plain C types only (pointers, sizes, scalars): no torch, no HIP types.
+ * Every call returns an oxh_status; 0 is success, and oxh_last_error()
+ * describes the last failure of the calling thread.  Nothing in the
+ * library calls exit(): a bad system is reported, not fatal (the
+ * reference exits on any CUDA error, src/CUDA/cuda_defs.h:12-22).
+ *
+ * A batch holds one or more independent systems resident in HBM.  The
+ * reference runs one system per process (src/Utilities/ConfigInfo.h:47,
+ * 94-98); the plain path here is a batch of one system, the batched and
+ * served forms are batches of many.
+ *
+ * Threading: thread-compatible, not thread-safe per batch handle.
+ */
+#ifndef OXH_H
+#define OXH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int oxh_status;
+
+#define OXH_OK 0
+#define OXH_ERR_ARG 1        /* invalid argument */
+#define OXH_ERR_HIP 2        /* HIP runtime failure */
+#define OXH_ERR_ALLOC 3      /* device allocation failed */
+#define OXH_ERR_SYSTEM 4     /* a system failed (broken bond, overflow); see oxh_batch_system_error */
+
+typedef struct oxh_batch oxh_batch;
+
+/* Thermostat codes, the reference's `thermostat` input key
+ * (src/Backends/Thermostats/ThermostatFactory.cpp): */
+#define OXH_THERMOSTAT_NO 0
+#define OXH_THERMOSTAT_LANGEVIN 1
+#define OXH_THERMOSTAT_BUSSI 2
+#define OXH_THERMOSTAT_BROWNIAN 3   /* "brownian" / "john" */
+
+/* One system: state, topology and the input-file keys that set up the
+ * interaction, the list and the integrator.  Arrays are host arrays in the
+ * system's own particle order; they are copied at upload.
+ * Keys and defaults follow the reference (docs/source/input.md:29-113):
+ *   T (simulation units; see src/Utilities/Utils.cpp:316-352 for "20C"),
+ *   salt_concentration, dh_lambda (0.3616455), dh_strength (0.0543),
+ *   debye_huckel_rhigh (3 lambda), dh_half_charged_ends (1),
+ *   hb_multiplier (1), max_backbone_force[_far] (off / 0.04), verlet_skin,
+ *   dt, thermostat, diff_coeff or gamma_trans, newtonian_steps, pt,
+ *   bussi_tau, seed, refresh_vel. */
+typedef struct oxh_system_desc {
+	int N;
+	const double *pos;       /* 3N */
+	const double *a1;        /* 3N, principal axis */
+	const double *a3;        /* 3N, stacking axis (a2 = a3 x a1) */
+	const double *vel;       /* 3N or NULL (zero, or refreshed) */
+	const double *L;         /* 3N or NULL, body frame angular momentum */
+	const int *btype;        /* N: A=0 G=1 C=2 T=3, or custom base types */
+	const int *n3;           /* N: 3' neighbour, -1 = none */
+	const int *n5;           /* N: 5' neighbour, -1 = none */
+	double box[3];
+
+	double T;
+	double salt;
+	int dh_half_charged_ends;
+	double dh_lambda;        /* < 0: default */
+	double dh_strength;      /* < 0: default */
+	double dh_rhigh;         /* <= 0: default */
+	double hb_multiplier;
+	double max_backbone_force;      /* < 0: off */
+	double max_backbone_force_far;  /* < 0: default 0.04 */
+	/* sequence-dependent well depths (use_average_seq = false): NULL keeps
+	 * the average-sequence oxDNA2 values.  stck[4][4] for A,G,C,T. */
+	const double *seq_stck;
+	double seq_stck_fact_eps;
+	double seq_hydr_AT, seq_hydr_GC;
+
+	double verlet_skin;
+	double dt;
+	int thermostat;
+	double diff_coeff;       /* Langevin / Brownian; <= 0: use gamma_trans / pt */
+	double gamma_trans;
+	double pt;               /* Brownian; <= 0: from diff_coeff */
+	int newtonian_steps;
+	int bussi_tau;
+	unsigned long long seed;
+	int refresh_vel;         /* 1: draw vel and L from N(0, T) at upload */
+} oxh_system_desc;
+
+/* Fills a descriptor with the reference's defaults (no arrays set). */
+void oxh_system_desc_defaults(oxh_system_desc *d);
+
+const char *oxh_last_error(void);
+int oxh_version(void);
+oxh_status oxh_device_count(int *n);
+
+/* Uploads n_sys systems to device `device` and builds their lists and
+ * forces.  Replaces the reference's backend set-up sequence
+ * MD_CUDABackend::init (src/CUDA/Backends/MD_CUDABackend.cu:674-755):
+ * CUDABaseBackend::init_cuda, apply_changes_to_simulation_data (:231-313),
+ * CUDASimpleVerletList::init (src/CUDA/Lists/CUDASimpleVerletList.cu:
+ * 165-202), thermostat init, first list build + compute_forces. */
+oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *systems, oxh_batch **out);
+oxh_status oxh_batch_destroy(oxh_batch *b);
+
+/* Rebuilds the Verlet lists: of every system when force != 0, else only of
+ * systems whose particles moved more than the skin.  Replaces
+ * CUDABaseList::update / CUDASimpleVerletList::update
+ * (src/CUDA/Lists/CUDABaseList.h:55, CUDASimpleVerletList.cu:231-289). */
+oxh_status oxh_batch_update_lists(oxh_batch *b, int force);
+
+/* Forces and torques of the current state into the device buffers (F lab
+ * frame, T body frame).  Replaces CUDABaseInteraction::compute_forces
+ * (src/CUDA/Interactions/CUDABaseInteraction.h:60) with
+ * CUDADNAInteraction's kernel (src/CUDA/Interactions/CUDA_DNA.cuh:832-904). */
+oxh_status oxh_batch_compute_forces(oxh_batch *b);
+
+/* Advances every system by n_steps MD steps: first half kick + drift +
+ * rotation, skin-triggered list rebuild, forces, second half kick,
+ * thermostat.  Replaces the loop of MD_CUDABackend::sim_step
+ * (src/CUDA/Backends/MD_CUDABackend.cu:567-619) driven by SimManager::run
+ * (src/Managers/SimManager.cpp:158-180); steps are captured in a HIP graph
+ * and the rebuild decision stays on the device. */
+oxh_status oxh_batch_run(oxh_batch *b, long long n_steps);
+
+/* Reorders particles of every system along a Hilbert curve (the
+ * reference's _sort_particles, src/CUDA/Backends/MD_CUDABackend.cu:535-550,
+ * with CUDA_sort_every).  Downloads keep the original particle order. */
+oxh_status oxh_batch_hilbert_sort(oxh_batch *b);
+
+/* Blocks until all queued work of the batch has finished. */
+oxh_status oxh_batch_sync(oxh_batch *b);
+
+/* State of system `sys` in its original particle order (any pointer may be
+ * NULL).  Replaces MD_CUDABackend::apply_simulation_data_changes
+ * (src/CUDA/Backends/MD_CUDABackend.cu:315-394). */
+oxh_status oxh_batch_get_state(oxh_batch *b, int sys, double *pos, double *a1, double *a3, double *vel, double *L);
+
+/* Forces (lab) and torques (body frame) of the last force evaluation, and
+ * each particle's share of the potential energy (Epart, N doubles). */
+oxh_status oxh_batch_get_forces(oxh_batch *b, int sys, double *F, double *T, double *Epart);
+
+/* Per-system FP64 sums: U (potential), U_hb, K_t, K_r (4 doubles per
+ * system, n_sys entries).  Replaces the reference's CUDA energy print
+ * (src/CUDA/Backends/MD_CUDABackend.cu:603-606, thrust transform_reduce). */
+oxh_status oxh_batch_energies(oxh_batch *b, double *out4);
+
+/* Potential energy of one system split by term, in the reference's order
+ * (FENE, bonded excl, stacking, non-bonded excl, HB, cross stacking,
+ * coaxial stacking, Debye-Hueckel), as BaseInteraction::
+ * get_system_energy_split (src/Interactions/BaseInteraction.cpp:61-86). */
+oxh_status oxh_batch_energy_split(oxh_batch *b, int sys, double *E8);
+
+/* Number of base pairs (pairs with E_HB < -0.1, the reference's HB_CUTOFF,
+ * src/defs.h:62, src/Observables/HBList.cpp:141-164) of every system. */
+oxh_status oxh_batch_hb_count(oxh_batch *b, int *out);
+
+/* The current Verlet pair set of system `sys` as unordered pairs
+ * (i < j, original indices), written to pairs (2 ints each) up to max
+ * pairs; *n receives the total.  Exposes what the reference keeps in
+ * CUDABaseList::d_matrix_neighs (src/CUDA/Lists/CUDABaseList.h:25-62). */
+oxh_status oxh_batch_get_pairs(oxh_batch *b, int sys, int *pairs, long long max, long long *n);
+
+/* List statistics of system `sys`: rebuild count and the last error bits
+ * (1 broken FENE bond, 2 neighbour overflow, 4 cell overflow). */
+oxh_status oxh_batch_system_info(oxh_batch *b, int sys, long long *n_rebuilds, int *error);
+
+/* Replaces the state (positions, orientations, velocities) of system `sys`
+ * in place; topology, N and box stay.  Used to reuse a resident slot. */
+oxh_status oxh_batch_set_state(oxh_batch *b, int sys, const double *pos, const double *a1, const double *a3, const double *vel,
+		const double *L);
+
+/* Neighbour-list and device-memory figures of the batch:
+ * out[0] N, out[1] max_neigh, out[2] mean neighbours per particle,
+ * out[3] device bytes held. */
+oxh_status oxh_batch_stats(oxh_batch *b, double *out4);
+
+/* Hilbert keys of n cells, each (x, y, z, bits) as 4 unsigned ints, on
+ * the device: the same function the sort uses, for checking. */
+oxh_status oxh_hilbert_keys(int device, int n, const unsigned int *xyzb, unsigned int *keys);
+
+/* Philox4x32-10 on the device for n (counter[4], key[2]) inputs: the
+ * thermostats' generator, for checking against published answers. */
+oxh_status oxh_philox(int device, int n, const unsigned int *ctr4, const unsigned int *key2, unsigned int *out4);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OXH_H */

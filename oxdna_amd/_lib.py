@@ -1,0 +1,102 @@
+"""ctypes binding of the C ABI in include/oxh.h (oxdna_amd/liboxh.so).
+
+There is no fallback: if the HIP library is not built, or no GPU is
+visible, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboxh.so")
+
+OXH_THERMOSTAT = {"no": 0, "langevin": 1, "bussi": 2, "brownian": 3, "john": 3}
+
+# every entry point declared in include/oxh.h
+EXPORTS = [
+    "oxh_system_desc_defaults", "oxh_last_error", "oxh_version", "oxh_device_count", "oxh_batch_create",
+    "oxh_batch_destroy", "oxh_batch_update_lists", "oxh_batch_compute_forces", "oxh_batch_run",
+    "oxh_batch_hilbert_sort", "oxh_batch_sync", "oxh_batch_get_state", "oxh_batch_get_forces", "oxh_batch_energies",
+    "oxh_batch_energy_split", "oxh_batch_hb_count", "oxh_batch_get_pairs", "oxh_batch_system_info",
+    "oxh_batch_set_state", "oxh_batch_stats", "oxh_hilbert_keys", "oxh_philox",
+]
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [
+        ("N", C.c_int),
+        ("pos", C.c_void_p), ("a1", C.c_void_p), ("a3", C.c_void_p), ("vel", C.c_void_p), ("L", C.c_void_p),
+        ("btype", C.c_void_p), ("n3", C.c_void_p), ("n5", C.c_void_p),
+        ("box", C.c_double * 3),
+        ("T", C.c_double), ("salt", C.c_double), ("dh_half_charged_ends", C.c_int),
+        ("dh_lambda", C.c_double), ("dh_strength", C.c_double), ("dh_rhigh", C.c_double),
+        ("hb_multiplier", C.c_double), ("max_backbone_force", C.c_double), ("max_backbone_force_far", C.c_double),
+        ("seq_stck", C.c_void_p), ("seq_stck_fact_eps", C.c_double), ("seq_hydr_AT", C.c_double),
+        ("seq_hydr_GC", C.c_double),
+        ("verlet_skin", C.c_double), ("dt", C.c_double), ("thermostat", C.c_int), ("diff_coeff", C.c_double),
+        ("gamma_trans", C.c_double), ("pt", C.c_double), ("newtonian_steps", C.c_int), ("bussi_tau", C.c_int),
+        ("seed", C.c_ulonglong), ("refresh_vel", C.c_int),
+    ]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m oxdna_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        V, I, D, P, LL = C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_longlong
+        L.oxh_last_error.restype = C.c_char_p
+        L.oxh_system_desc_defaults.argtypes = [P]
+        L.oxh_device_count.argtypes = [P]
+        L.oxh_batch_create.argtypes = [I, I, P, P]
+        L.oxh_batch_destroy.argtypes = [V]
+        L.oxh_batch_update_lists.argtypes = [V, I]
+        L.oxh_batch_compute_forces.argtypes = [V]
+        L.oxh_batch_run.argtypes = [V, LL]
+        L.oxh_batch_hilbert_sort.argtypes = [V]
+        L.oxh_batch_sync.argtypes = [V]
+        L.oxh_batch_get_state.argtypes = [V, I, P, P, P, P, P]
+        L.oxh_batch_get_forces.argtypes = [V, I, P, P, P]
+        L.oxh_batch_energies.argtypes = [V, P]
+        L.oxh_batch_energy_split.argtypes = [V, I, P]
+        L.oxh_batch_hb_count.argtypes = [V, P]
+        L.oxh_batch_get_pairs.argtypes = [V, I, P, LL, P]
+        L.oxh_batch_system_info.argtypes = [V, I, P, P]
+        L.oxh_batch_set_state.argtypes = [V, I, P, P, P, P, P]
+        L.oxh_batch_stats.argtypes = [V, P]
+        L.oxh_hilbert_keys.argtypes = [I, I, P, P]
+        L.oxh_philox.argtypes = [I, I, P, P, P]
+        for name in EXPORTS:
+            f = getattr(L, name)
+            if name not in ("oxh_last_error", "oxh_system_desc_defaults"):
+                f.restype = C.c_int
+        L.oxh_system_desc_defaults.restype = None
+        _LIB = L
+    return _LIB
+
+
+class OxhError(RuntimeError):
+    pass
+
+
+def check(status: int, what: str = ""):
+    if status != 0:
+        msg = lib().oxh_last_error().decode(errors="replace")
+        raise OxhError(f"{what}: oxh status {status}: {msg}")
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().oxh_device_count(C.byref(n)), "oxh_device_count")
+    return n.value
+
+
+def ptr(a):
+    return a.ctypes.data if a is not None else None

@@ -1,0 +1,205 @@
+"""Python host side of the HIP backend: systems resident on the GPU.
+
+`HIPBatch` owns one oxh_batch (include/oxh.h) holding one or more
+independent systems.  It mirrors what the reference's MD backend does with
+its device state (src/CUDA/Backends/MD_CUDABackend.cu): upload, list
+updates, force evaluation, MD steps and downloads, but for many systems at
+once.  The plain path is a batch of one system.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import SystemDesc, check, lib, ptr
+from .systems import System
+
+ENERGY_TERMS = ("FENE", "BONDED_EXCLUDED_VOLUME", "STACKING", "NONBONDED_EXCLUDED_VOLUME", "HYDROGEN_BONDING",
+                "CROSS_STACKING", "COAXIAL_STACKING", "DEBYE_HUCKEL")
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def make_desc(s: System, keep: list, **opts) -> SystemDesc:
+    """oxh_system_desc of System s; option names are the reference's
+    input keys (verlet_skin, dt, thermostat, diff_coeff, newtonian_steps,
+    bussi_tau, salt_concentration, dh_lambda, ...)."""
+    d = SystemDesc()
+    lib().oxh_system_desc_defaults(C.byref(d))
+    arrs = dict(pos=_c(s.pos, np.float64), a1=_c(s.a1, np.float64), a3=_c(s.a3, np.float64),
+                btype=_c(s.btype, np.int32), n3=_c(s.n3, np.int32), n5=_c(s.n5, np.int32))
+    if s.vel is not None and not opts.get("refresh_vel", False):
+        arrs["vel"] = _c(s.vel, np.float64)
+    if s.L is not None and not opts.get("refresh_vel", False):
+        arrs["L"] = _c(s.L, np.float64)
+    keep.append(arrs)
+    d.N = s.N
+    for k, a in arrs.items():
+        setattr(d, k, a.ctypes.data)
+    for k in range(3):
+        d.box[k] = float(s.box[k])
+    d.T = float(opts.get("T", s.T))
+    d.salt = float(opts.get("salt_concentration", s.salt))
+    d.dh_half_charged_ends = int(bool(opts.get("dh_half_charged_ends", True)))
+    d.dh_lambda = float(opts.get("dh_lambda", -1.0))
+    d.dh_strength = float(opts.get("dh_strength", -1.0))
+    d.dh_rhigh = float(opts.get("debye_huckel_rhigh", -1.0))
+    d.hb_multiplier = float(opts.get("hb_multiplier", 1.0))
+    d.max_backbone_force = float(opts.get("max_backbone_force", -1.0))
+    d.max_backbone_force_far = float(opts.get("max_backbone_force_far", -1.0))
+    d.verlet_skin = float(opts.get("verlet_skin", 0.05))
+    d.dt = float(opts.get("dt", 0.003))
+    th = opts.get("thermostat", "no")
+    d.thermostat = _lib.OXH_THERMOSTAT[th] if isinstance(th, str) else int(th)
+    d.diff_coeff = float(opts.get("diff_coeff", -1.0))
+    d.gamma_trans = float(opts.get("gamma_trans", -1.0))
+    d.pt = float(opts.get("pt", -1.0))
+    d.newtonian_steps = int(opts.get("newtonian_steps", 1))
+    d.bussi_tau = int(opts.get("bussi_tau", 1))
+    d.seed = int(opts.get("seed", 1)) & 0xFFFFFFFFFFFFFFFF
+    d.refresh_vel = int(bool(opts.get("refresh_vel", False)))
+    if "seq_stck" in opts:
+        st = _c(opts["seq_stck"], np.float64).ravel()
+        keep.append(st)
+        d.seq_stck = st.ctypes.data
+        d.seq_stck_fact_eps = float(opts["seq_stck_fact_eps"])
+        d.seq_hydr_AT = float(opts["seq_hydr_AT"])
+        d.seq_hydr_GC = float(opts["seq_hydr_GC"])
+    return d
+
+
+class HIPBatch:
+    """Independent systems resident on one GPU."""
+
+    def __init__(self, systems, device: int = 0, options=None, **common):
+        if isinstance(systems, System):
+            systems = [systems]
+        self.systems = list(systems)
+        if options is None:
+            options = [{} for _ in self.systems]
+        keep: list = []
+        descs = (SystemDesc * len(self.systems))()
+        for k, s in enumerate(self.systems):
+            o = dict(common)
+            o.update(options[k])
+            descs[k] = make_desc(s, keep, **o)
+        h = C.c_void_p()
+        check(lib().oxh_batch_create(device, len(self.systems), descs, C.byref(h)), "oxh_batch_create")
+        self.h = h
+        self.device = device
+
+    # -- lifecycle
+    def close(self):
+        if getattr(self, "h", None):
+            lib().oxh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def nsys(self):
+        return len(self.systems)
+
+    # -- operations
+    def run(self, steps: int):
+        check(lib().oxh_batch_run(self.h, int(steps)), "oxh_batch_run")
+
+    def compute_forces(self):
+        check(lib().oxh_batch_compute_forces(self.h), "oxh_batch_compute_forces")
+
+    def update_lists(self, force: bool = True):
+        check(lib().oxh_batch_update_lists(self.h, int(force)), "oxh_batch_update_lists")
+
+    def hilbert_sort(self):
+        check(lib().oxh_batch_hilbert_sort(self.h), "oxh_batch_hilbert_sort")
+
+    def sync(self):
+        check(lib().oxh_batch_sync(self.h), "oxh_batch_sync")
+
+    # -- downloads
+    def state(self, sys: int = 0) -> dict:
+        n = self.systems[sys].N
+        out = {k: np.zeros((n, 3)) for k in ("pos", "a1", "a3", "vel", "L")}
+        check(lib().oxh_batch_get_state(self.h, sys, *(ptr(out[k]) for k in ("pos", "a1", "a3", "vel", "L"))),
+              "oxh_batch_get_state")
+        return out
+
+    def forces(self, sys: int = 0):
+        n = self.systems[sys].N
+        F = np.zeros((n, 3))
+        T = np.zeros((n, 3))
+        E = np.zeros(n)
+        check(lib().oxh_batch_get_forces(self.h, sys, ptr(F), ptr(T), ptr(E)), "oxh_batch_get_forces")
+        return F, T, E
+
+    def energies(self) -> np.ndarray:
+        """(nsys, 4): U, U_hb, K_t, K_r, FP64 sums."""
+        out = np.zeros((self.nsys, 4))
+        check(lib().oxh_batch_energies(self.h, ptr(out)), "oxh_batch_energies")
+        return out
+
+    def energy_split(self, sys: int = 0) -> np.ndarray:
+        out = np.zeros(8)
+        check(lib().oxh_batch_energy_split(self.h, sys, ptr(out)), "oxh_batch_energy_split")
+        return out
+
+    def hb_count(self) -> np.ndarray:
+        out = np.zeros(self.nsys, np.int32)
+        check(lib().oxh_batch_hb_count(self.h, ptr(out)), "oxh_batch_hb_count")
+        return out
+
+    def pairs(self, sys: int = 0) -> np.ndarray:
+        n = C.c_longlong(0)
+        cap = max(1024, self.systems[sys].N * 64)
+        while True:
+            buf = np.zeros((cap, 2), np.int32)
+            check(lib().oxh_batch_get_pairs(self.h, sys, ptr(buf), cap, C.byref(n)), "oxh_batch_get_pairs")
+            if n.value <= cap:
+                return buf[: n.value]
+            cap = n.value
+
+    def info(self, sys: int = 0):
+        nr = C.c_longlong(0)
+        err = C.c_int(0)
+        check(lib().oxh_batch_system_info(self.h, sys, C.byref(nr), C.byref(err)), "oxh_batch_system_info")
+        return nr.value, err.value
+
+    def set_state(self, sys: int, pos=None, a1=None, a3=None, vel=None, L=None):
+        arrs = [None if a is None else _c(a, np.float64) for a in (pos, a1, a3, vel, L)]
+        check(lib().oxh_batch_set_state(self.h, sys, *(ptr(a) for a in arrs)), "oxh_batch_set_state")
+
+    def stats(self):
+        out = np.zeros(4)
+        check(lib().oxh_batch_stats(self.h, ptr(out)), "oxh_batch_stats")
+        return dict(N=int(out[0]), max_neigh=int(out[1]), mean_neigh=out[2], device_bytes=int(out[3]))
+
+
+def hilbert_keys(xyz: np.ndarray, bits: int, device: int = 0) -> np.ndarray:
+    xyzb = np.zeros((len(xyz), 4), np.uint32)
+    xyzb[:, :3] = xyz
+    xyzb[:, 3] = bits
+    out = np.zeros(len(xyz), np.uint32)
+    check(lib().oxh_hilbert_keys(device, len(xyz), ptr(xyzb), ptr(out)), "oxh_hilbert_keys")
+    return out
+
+
+def philox(ctr: np.ndarray, key: np.ndarray, device: int = 0) -> np.ndarray:
+    ctr = np.ascontiguousarray(ctr, np.uint32).reshape(-1, 4)
+    key = np.ascontiguousarray(key, np.uint32).reshape(-1, 2)
+    out = np.zeros_like(ctr)
+    check(lib().oxh_philox(device, len(ctr), ptr(ctr), ptr(key), ptr(out)), "oxh_philox")
+    return out

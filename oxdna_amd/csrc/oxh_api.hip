@@ -1,0 +1,976 @@
+// C ABI of the oxDNA2 hot path (include/oxh.h): batch set-up, stepping,
+// downloads.  Host code; kernels live in oxh_forces.hip, oxh_lists.hip and
+// oxh_integrate.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/oxh.h"
+#include "oxh_device.h"
+#include "oxh_internal.h"
+
+extern "C" int oxh_internal_hilbert_probe(int n, const void *d_xyzb, void *d_out, hipStream_t st);
+
+struct oxh_batch {
+	oxh::Batch b;
+	// per system host copies of what downloads need
+	std::vector<int> first, n;
+	// backup of the state at the start of a chunk (overflow recovery)
+	double4 *bk_pos = nullptr, *bk_vel = nullptr, *bk_Lm = nullptr, *bk_q64 = nullptr;
+	float4 *bk_F = nullptr, *bk_T = nullptr;
+	long long *bk_pstep = nullptr;
+	oxh_sys_state *bk_state = nullptr;
+	size_t device_bytes = 0;
+	bool bussi_aligned = true;
+	int bussi_ns = 0;
+	bool any_bussi = false;
+	int chunk = 0;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+oxh_status fail(oxh_status code, const char *fmt, ...) {
+	char buf[1024];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof(buf), fmt, ap);
+	va_end(ap);
+	g_err = buf;
+	return code;
+}
+
+#define HIPCHK(call)                                                                                      \
+	do {                                                                                                  \
+		hipError_t e_ = (call);                                                                           \
+		if(e_ != hipSuccess) return fail(OXH_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+	} while(0)
+
+template<typename T>
+oxh_status dalloc(oxh_batch *B, T **p, size_t count) {
+	size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+	hipError_t e = hipMalloc((void **) p, bytes);
+	if(e != hipSuccess) return fail(OXH_ERR_ALLOC, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+	B->device_bytes += bytes;
+	return OXH_OK;
+}
+
+#define TRY(x)                          \
+	do {                                \
+		oxh_status s_ = (x);            \
+		if(s_ != OXH_OK) return s_;     \
+	} while(0)
+
+inline double SQRd(double x) { return x * x; }
+
+// Model constants of one system, in FP64 then rounded to FP32 for the
+// kernels.  The formulas are the reference's: DNAInteraction::init
+// (src/Interactions/DNAInteraction.cpp:298-377), DNA2Interaction::
+// get_settings / init (src/Interactions/DNA2Interaction.cpp:62-155).
+// The reference's float-literal constants go through FP64 library calls.
+void model_constants(const oxh_system_desc &d, oxh_sys &S, double &rcut) {
+	const double T = d.T;
+	const float STCK_RC = 0.9f, STCK_R0 = 0.4f, STCK_A = 6.f, HYDR_RC = 0.75f, HYDR_R0 = 0.4f, HYDR_A = 8.f;
+	const double stck_shift = SQRd(1 - std::exp((double) (-(STCK_RC - STCK_R0) * STCK_A)));
+	const double hydr_shift = SQRd(1 - std::exp((double) (-(HYDR_RC - HYDR_R0) * HYDR_A)));
+	double eps[2][5][5];
+	for(int i = 0; i < 5; i++)
+		for(int j = 0; j < 5; j++) {
+			eps[1][i][j] = (double) 1.3523f + (double) 2.6717f * T;
+			eps[0][i][j] = (double) 1.0678f;
+		}
+	if(d.seq_stck) {
+		for(int i = 0; i < 4; i++)
+			for(int j = 0; j < 4; j++)
+				eps[1][i][j] = d.seq_stck[4 * i + j] * (1.0 - d.seq_stck_fact_eps + (T * 9.0 * d.seq_stck_fact_eps));
+		eps[0][0][3] = eps[0][3][0] = d.seq_hydr_AT;
+		eps[0][1][2] = eps[0][2][1] = d.seq_hydr_GC;
+	}
+	for(int t = 0; t < 2; t++)
+		for(int i = 0; i < 5; i++)
+			for(int j = 0; j < 5; j++) {
+				S.f1_eps[t][i][j] = (float) eps[t][i][j];
+				S.f1_shift[t][i][j] = (float) (eps[t][i][j] * (t == 1 ? stck_shift : hydr_shift));
+			}
+
+	const float salt_f = (float) d.salt;
+	const double lf = d.dh_lambda >= 0 ? d.dh_lambda : 0.3616455;
+	const double q = d.dh_strength >= 0 ? d.dh_strength : 0.0543;
+	const double lambda_set = lf * std::sqrt(T / 0.1f) / std::sqrt((double) salt_f);
+	const double rhigh = d.dh_rhigh > 0 ? d.dh_rhigh : 3.0 * lambda_set;
+	const double l = lf * std::sqrt(T / 0.1) / std::sqrt((double) salt_f);
+	const double x = rhigh;
+	const double B = -(std::exp(-x / l) * q * q * (x + l) * (x + l)) / (-4. * x * x * x * l * l * q);
+	const double RC = x * (q * x + 3. * q * l) / (q * (x + l));
+	S.dh_prefactor = (float) q;
+	S.dh_minus_kappa = (float) (-1.0 / l);
+	S.dh_rhigh = (float) rhigh;
+	S.dh_rc = (float) RC;
+	S.dh_b = (float) B;
+	S.dh_half = d.dh_half_charged_ends ? 1 : 0;
+	S.hb_multi = (float) d.hb_multiplier;
+	S.fene_r0 = 0.7564f;
+	S.use_mbf = 0;
+	if(d.max_backbone_force >= 0) {
+		const double FE = 2.0, FD2 = (double) 0.0625f;
+		double fmax = d.max_backbone_force;
+		S.use_mbf = 1;
+		S.mbf_fmax = (float) fmax;
+		S.mbf_xmax = (float) ((-FE + std::sqrt(FE * FE + 4.f * fmax * fmax * FD2)) / (2.f * fmax));
+		S.mbf_finf = (float) (d.max_backbone_force_far >= 0 ? d.max_backbone_force_far : 0.04f);
+	}
+	const double arm = std::sqrt(SQRd((double) -0.34f) + SQRd((double) 0.3408f));
+	rcut = std::max(2 * arm + (double) 0.711879214356f, 2 * (double) 0.4f + (double) 0.783775f);
+	rcut = std::max(rcut, 2.0 * arm + RC);
+}
+
+// orientation (a1, a3) -> unit quaternion (x, y, z, w); M has columns a1, a2, a3
+void frame_to_quat(const double *a1, const double *a3, double q[4]) {
+	double a2[3] = {a3[1] * a1[2] - a3[2] * a1[1], a3[2] * a1[0] - a3[0] * a1[2], a3[0] * a1[1] - a3[1] * a1[0]};
+	double M[3][3] = {{a1[0], a2[0], a3[0]}, {a1[1], a2[1], a3[1]}, {a1[2], a2[2], a3[2]}};
+	double tr = M[0][0] + M[1][1] + M[2][2];
+	double x, y, z, w;
+	if(tr > 0) {
+		double s = 0.5 / std::sqrt(tr + 1.0);
+		w = 0.25 / s;
+		x = (M[2][1] - M[1][2]) * s;
+		y = (M[0][2] - M[2][0]) * s;
+		z = (M[1][0] - M[0][1]) * s;
+	}
+	else if(M[0][0] > M[1][1] && M[0][0] > M[2][2]) {
+		double s = 0.5 / std::sqrt(1.0 + M[0][0] - M[1][1] - M[2][2]);
+		w = (M[2][1] - M[1][2]) * s;
+		x = 0.25 / s;
+		y = (M[0][1] + M[1][0]) * s;
+		z = (M[0][2] + M[2][0]) * s;
+	}
+	else if(M[1][1] > M[2][2]) {
+		double s = 0.5 / std::sqrt(1.0 + M[1][1] - M[0][0] - M[2][2]);
+		w = (M[0][2] - M[2][0]) * s;
+		x = (M[0][1] + M[1][0]) * s;
+		y = 0.25 / s;
+		z = (M[1][2] + M[2][1]) * s;
+	}
+	else {
+		double s = 0.5 / std::sqrt(1.0 + M[2][2] - M[0][0] - M[1][1]);
+		w = (M[1][0] - M[0][1]) * s;
+		x = (M[0][2] + M[2][0]) * s;
+		y = (M[1][2] + M[2][1]) * s;
+		z = 0.25 / s;
+	}
+	double n = std::sqrt(x * x + y * y + z * z + w * w);
+	q[0] = x / n;
+	q[1] = y / n;
+	q[2] = z / n;
+	q[3] = w / n;
+}
+
+void quat_to_axes(const double4 &q, double a1[3], double a3[3]) {
+	double sqx = q.x * q.x, sqy = q.y * q.y, sqz = q.z * q.z, sqw = q.w * q.w;
+	double xy = q.x * q.y, xz = q.x * q.z, xw = q.x * q.w, yz = q.y * q.z, yw = q.y * q.w, zw = q.z * q.w;
+	a1[0] = sqx - sqy - sqz + sqw;
+	a1[1] = 2. * (xy + zw);
+	a1[2] = 2. * (xz - yw);
+	a3[0] = 2. * (xz + yw);
+	a3[1] = 2. * (yz - xw);
+	a3[2] = -sqx - sqy + sqz + sqw;
+}
+
+__global__ void k_clear_need(int nsys, oxh_sys_state *st, int set_value) {
+	int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if(s < nsys) st[s].need_rebuild = set_value;
+}
+
+oxh_status sync_and_check(oxh_batch *B) {
+	HIPCHK(hipStreamSynchronize(B->b.stream));
+	HIPCHK(hipGetLastError());
+	return OXH_OK;
+}
+
+oxh_status read_states(oxh_batch *B, std::vector<oxh_sys_state> &st) {
+	st.resize(B->b.nsys);
+	HIPCHK(hipMemcpyAsync(st.data(), B->b.d_state, sizeof(oxh_sys_state) * B->b.nsys, hipMemcpyDeviceToHost, B->b.stream));
+	return sync_and_check(B);
+}
+
+// list build of the current state (systems flagged, or all)
+void enqueue_lists(oxh_batch *B, bool all) {
+	oxh::Batch &b = B->b;
+	if(all) k_clear_need<<<(b.nsys + 63) / 64, 64, 0, b.stream>>>(b.nsys, b.d_state, 1);
+	oxh::launch_cells(b.stream, b);
+	oxh::launch_build_list(b.stream, b, 0);
+}
+
+void enqueue_step(oxh_batch *B, long long step_index) {
+	oxh::Batch &b = B->b;
+	oxh::launch_first_step(b.stream, b);
+	oxh::launch_cells(b.stream, b);
+	oxh::launch_build_list(b.stream, b, 0);
+	oxh::launch_forces(b.stream, b, false);
+	oxh::launch_second_step(b.stream, b);
+	if(B->any_bussi) {
+		if(!B->bussi_aligned || step_index % B->bussi_ns == 0) oxh::launch_bussi(b.stream, b);
+	}
+}
+
+__global__ void k_clear_err(int nsys, oxh_sys_state *st, int mask) {
+	int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if(s < nsys) st[s].error &= ~mask;
+}
+
+// doubles the neighbour or cell capacity after an overflow and clears the
+// overflow bits (the caller rebuilds)
+oxh_status grow_lists(oxh_batch *B, const std::vector<oxh_sys_state> &st) {
+	oxh::Batch &b = B->b;
+	bool grow_neigh = false, grow_cells = false;
+	for(auto &s : st) {
+		if(s.error & 2) grow_neigh = true;
+		if(s.error & 4) grow_cells = true;
+	}
+	HIPCHK(hipStreamSynchronize(b.stream));
+	if(grow_neigh) {
+		HIPCHK(hipFree(b.nbr));
+		B->device_bytes -= (size_t) b.nstride * b.max_neigh * sizeof(int);
+		b.max_neigh = std::min(std::max(b.max_neigh * 2, 8), std::max(b.N, 1));
+		TRY(dalloc(B, &b.nbr, (size_t) b.nstride * b.max_neigh));
+	}
+	if(grow_cells && b.ncell_total) {
+		HIPCHK(hipFree(b.cell_items));
+		B->device_bytes -= (size_t) b.ncell_total * b.cell_cap * sizeof(int);
+		b.cell_cap = b.cell_cap * 2;
+		TRY(dalloc(B, &b.cell_items, (size_t) b.ncell_total * b.cell_cap));
+	}
+	k_clear_err<<<(b.nsys + 63) / 64, 64, 0, b.stream>>>(b.nsys, b.d_state, 6);
+	if(b.graph_exec) {
+		(void) hipGraphExecDestroy(b.graph_exec);
+		b.graph_exec = nullptr;
+	}
+	return OXH_OK;
+}
+
+// builds lists (all systems) and forces, growing buffers until nothing
+// overflows
+oxh_status build_all(oxh_batch *B) {
+	for(int attempt = 0; attempt < 12; attempt++) {
+		enqueue_lists(B, true);
+		oxh::launch_forces(B->b.stream, B->b, false);
+		k_clear_need<<<(B->b.nsys + 63) / 64, 64, 0, B->b.stream>>>(B->b.nsys, B->b.d_state, 0);
+		std::vector<oxh_sys_state> st;
+		TRY(read_states(B, st));
+		bool overflow = false;
+		for(auto &s : st) overflow |= (s.error & 6) != 0;
+		if(!overflow) return OXH_OK;
+		TRY(grow_lists(B, st));
+	}
+	return fail(OXH_ERR_SYSTEM, "neighbour list keeps overflowing");
+}
+
+oxh_status backup(oxh_batch *B) {
+	oxh::Batch &b = B->b;
+	size_t N = b.N;
+	HIPCHK(hipMemcpyAsync(B->bk_pos, b.pos, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(B->bk_vel, b.vel, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(B->bk_Lm, b.Lm, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(B->bk_q64, b.q64, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(B->bk_F, b.F, N * sizeof(float4), hipMemcpyDeviceToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(B->bk_T, b.T, N * sizeof(float4), hipMemcpyDeviceToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(B->bk_pstep, b.pstep, N * sizeof(long long), hipMemcpyDeviceToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(B->bk_state, b.d_state, b.nsys * sizeof(oxh_sys_state), hipMemcpyDeviceToDevice, b.stream));
+	return OXH_OK;
+}
+
+oxh_status restore(oxh_batch *B) {
+	oxh::Batch &b = B->b;
+	size_t N = b.N;
+	HIPCHK(hipMemcpyAsync(b.pos, B->bk_pos, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(b.vel, B->bk_vel, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(b.Lm, B->bk_Lm, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(b.q64, B->bk_q64, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(b.F, B->bk_F, N * sizeof(float4), hipMemcpyDeviceToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(b.T, B->bk_T, N * sizeof(float4), hipMemcpyDeviceToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(b.pstep, B->bk_pstep, N * sizeof(long long), hipMemcpyDeviceToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(b.d_state, B->bk_state, b.nsys * sizeof(oxh_sys_state), hipMemcpyDeviceToDevice, b.stream));
+	oxh::launch_prepare_mirror(b.stream, b);
+	return sync_and_check(B);
+}
+
+}  // namespace
+
+namespace {
+template<typename T>
+oxh_status download(oxh_batch *B, std::vector<T> &out, const T *src, int first, int n) {
+	out.resize(n);
+	HIPCHK(hipMemcpyAsync(out.data(), src + first, n * sizeof(T), hipMemcpyDeviceToHost, B->b.stream));
+	return OXH_OK;
+}
+oxh_status check_sys(oxh_batch *B, int sys) {
+	if(!B) return fail(OXH_ERR_ARG, "null batch");
+	if(sys < 0 || sys >= B->b.nsys) return fail(OXH_ERR_ARG, "system %d out of range", sys);
+	HIPCHK(hipSetDevice(B->b.device));
+	return OXH_OK;
+}
+}  // namespace
+
+extern "C" {
+
+void oxh_system_desc_defaults(oxh_system_desc *d) {
+	std::memset(d, 0, sizeof(*d));
+	d->T = 0.1;
+	d->salt = 0.5;
+	d->dh_half_charged_ends = 1;
+	d->dh_lambda = -1;
+	d->dh_strength = -1;
+	d->dh_rhigh = -1;
+	d->hb_multiplier = 1.0;
+	d->max_backbone_force = -1;
+	d->max_backbone_force_far = -1;
+	d->verlet_skin = 0.05;
+	d->dt = 0.003;
+	d->thermostat = OXH_THERMOSTAT_NO;
+	d->diff_coeff = -1;
+	d->gamma_trans = -1;
+	d->pt = -1;
+	d->newtonian_steps = 1;
+	d->bussi_tau = 1;
+	d->seed = 1;
+}
+
+const char *oxh_last_error(void) { return g_err.c_str(); }
+
+int oxh_version(void) { return 10000; }
+
+oxh_status oxh_device_count(int *n) {
+	HIPCHK(hipGetDeviceCount(n));
+	return OXH_OK;
+}
+
+oxh_status oxh_batch_destroy(oxh_batch *B) {
+	if(!B) return OXH_OK;
+	oxh::Batch &b = B->b;
+	(void) hipSetDevice(b.device);
+	if(b.stream) (void) hipStreamSynchronize(b.stream);
+	if(b.graph_exec) (void) hipGraphExecDestroy(b.graph_exec);
+	void *ptrs[] = {b.pos, b.vel, b.Lm, b.q64, b.rlist, b.q32, b.bsite, b.F, b.T, b.bonds, b.pinfo, b.orig, b.pstep, b.nbr, b.nnbr,
+			b.cell_count, b.cell_items, b.cell_owner, b.d_sys, b.d_state, b.red_blocks, b.red_sys_blocks, b.red_partial, b.red_out,
+			b.esplit, b.hbcount, b.keys, b.keys_alt, b.perm, b.perm_alt, b.inv_perm, b.radix_hist, b.scratch, B->bk_pos, B->bk_vel,
+			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state};
+	for(void *p : ptrs)
+		if(p) (void) hipFree(p);
+	if(b.stream) (void) hipStreamDestroy(b.stream);
+	delete B;
+	return OXH_OK;
+}
+
+oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs, oxh_batch **out) {
+	if(!out || n_sys <= 0 || !descs) return fail(OXH_ERR_ARG, "oxh_batch_create: bad arguments");
+	*out = nullptr;
+	int ndev = 0;
+	HIPCHK(hipGetDeviceCount(&ndev));
+	if(device < 0 || device >= ndev) return fail(OXH_ERR_ARG, "device %d not available (%d devices)", device, ndev);
+	HIPCHK(hipSetDevice(device));
+
+	oxh_batch *B = new oxh_batch();
+	oxh::Batch &b = B->b;
+	b.device = device;
+	b.nsys = n_sys;
+	auto bail = [&](oxh_status s) {
+		std::string msg = g_err;
+		oxh_batch_destroy(B);
+		g_err = msg;
+		return s;
+	};
+#define TRYB(x)                               \
+	do {                                      \
+		oxh_status s_ = (x);                  \
+		if(s_ != OXH_OK) return bail(s_);     \
+	} while(0)
+	if(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(OXH_ERR_HIP, "stream creation failed"));
+
+	// ---- per system parameters
+	b.h_sys.resize(n_sys);
+	B->first.resize(n_sys);
+	B->n.resize(n_sys);
+	int N = 0, ncell = 0, max_occ = 1;
+	int max_all_pairs_n = 1;
+	double max_neigh_est = 16;
+	b.min_newtonian = 1 << 30;
+	for(int s = 0; s < n_sys; s++) {
+		const oxh_system_desc &d = descs[s];
+		if(d.N <= 0 || !d.pos || !d.a1 || !d.a3 || !d.btype || !d.n3 || !d.n5)
+			return bail(fail(OXH_ERR_ARG, "system %d: missing arrays or N <= 0", s));
+		for(int k = 0; k < 3; k++)
+			if(!(d.box[k] > 0)) return bail(fail(OXH_ERR_ARG, "system %d: box side %d must be > 0", s, k));
+		if(!(d.T > 0)) return bail(fail(OXH_ERR_ARG, "system %d: T must be > 0", s));
+		if(!(d.salt > 0)) return bail(fail(OXH_ERR_ARG, "system %d: salt concentration must be > 0", s));
+		for(int i = 0; i < d.N; i++) {
+			if(d.n3[i] >= d.N || d.n5[i] >= d.N || d.n3[i] < -1 || d.n5[i] < -1)
+				return bail(fail(OXH_ERR_ARG, "system %d: wrong topology for particle %d", s, i));
+		}
+		oxh_sys &S = b.h_sys[s];
+		std::memset(&S, 0, sizeof(S));
+		double rcut = 0;
+		model_constants(d, S, rcut);
+		for(int k = 0; k < 3; k++) {
+			S.box[k] = d.box[k];
+			S.inv_box[k] = 1.0 / d.box[k];
+		}
+		S.first = N;
+		S.n = d.N;
+		B->first[s] = N;
+		B->n[s] = d.N;
+		double rv = rcut + 2.0 * d.verlet_skin;
+		S.rv2 = rv * rv;
+		S.skin2 = d.verlet_skin * d.verlet_skin;
+		// cells of side >= r_verlet, at most ~2 particles per cell on
+		// average (the reference's cells_auto_optimisation,
+		// src/CUDA/Lists/CUDASimpleVerletList.cu:96-110)
+		double V = d.box[0] * d.box[1] * d.box[2];
+		double maxf = std::pow(2.0 * d.N / V, 1.0 / 3.0);
+		int nc[3];
+		bool cells_ok = d.N > 1024;
+		for(int k = 0; k < 3; k++) {
+			nc[k] = (int) std::floor(d.box[k] / rv);
+			nc[k] = std::min(nc[k], (int) std::ceil(maxf * d.box[k]));
+			if(nc[k] < 3) cells_ok = false;
+		}
+		if(cells_ok) {
+			S.use_cells = 1;
+			for(int k = 0; k < 3; k++) S.nc[k] = nc[k];
+			S.cell_first = ncell;
+			S.ncell = nc[0] * nc[1] * nc[2];
+			ncell += S.ncell;
+			// occupancy of the initial configuration
+			std::vector<int> occ(S.ncell, 0);
+			for(int i = 0; i < d.N; i++) {
+				int c[3];
+				for(int k = 0; k < 3; k++) {
+					double sx = d.pos[3 * i + k] / d.box[k];
+					int ck = (int) ((sx - std::floor(sx)) * nc[k]);
+					c[k] = std::min(ck, nc[k] - 1);
+				}
+				int idx = (c[2] * nc[1] + c[1]) * nc[0] + c[0];
+				max_occ = std::max(max_occ, ++occ[idx]);
+			}
+			double cellvol = V / S.ncell;
+			double est = 4.0 / 3.0 * M_PI * rv * rv * rv * (3.0 * max_occ / cellvol);
+			max_neigh_est = std::max(max_neigh_est, std::min(est, (double) d.N - 1));
+		}
+		else {
+			S.use_cells = 0;
+			max_all_pairs_n = std::max(max_all_pairs_n, d.N);
+			max_neigh_est = std::max(max_neigh_est, (double) std::min(d.N - 1, 2048));
+		}
+		// integrator and thermostat (src/Backends/Thermostats/*.cpp)
+		S.dt = d.dt;
+		S.T = d.T;
+		S.thermostat = d.thermostat;
+		S.newtonian_steps = std::max(1, d.newtonian_steps);
+		S.seed_lo = (uint32_t) (d.seed & 0xFFFFFFFFull);
+		S.seed_hi = (uint32_t) (d.seed >> 32) ^ 0x6F78444Eu;
+		S.pad_ = d.refresh_vel;
+		if(d.thermostat == OXH_THERMOSTAT_LANGEVIN) {
+			double D = 0, gamma = 0;
+			if(d.diff_coeff > 0) {
+				D = (double) (float) d.diff_coeff;
+				gamma = d.T / D;
+			}
+			else if(d.gamma_trans > 0) {
+				gamma = (double) (float) d.gamma_trans;
+				D = d.T / gamma;
+			}
+			else return bail(fail(OXH_ERR_ARG, "system %d: Langevin needs diff_coeff or gamma_trans", s));
+			double Dr = 3. * D, gr = d.T / Dr;
+			S.gamma_t = gamma;
+			S.gamma_r = gr;
+			S.resc_t = std::sqrt(2. * gamma * d.T / d.dt);
+			S.resc_r = std::sqrt(2. * gr * d.T / d.dt);
+		}
+		else if(d.thermostat == OXH_THERMOSTAT_BROWNIAN) {
+			double ns = S.newtonian_steps, dt = (double) (float) d.dt, pt = 0, D = 0;
+			if(d.pt > 0) pt = (double) (float) d.pt;
+			else if(d.diff_coeff > 0) {
+				D = (double) (float) d.diff_coeff;
+				pt = (2 * d.T * ns * dt) / (d.T * ns * dt + 2 * D);
+			}
+			else return bail(fail(OXH_ERR_ARG, "system %d: brownian needs pt or diff_coeff", s));
+			if(pt > 1.) return bail(fail(OXH_ERR_ARG, "system %d: pt (%f) must be smaller than 1", s, pt));
+			D = d.T * ns * dt * (1. / pt - 1. / 2.);
+			S.pt = pt;
+			S.pr = (2 * d.T * ns * dt) / (d.T * ns * dt + 2 * 3 * D);
+			S.resc_brown = std::sqrt(d.T);
+		}
+		else if(d.thermostat == OXH_THERMOSTAT_BUSSI) {
+			if(d.bussi_tau <= 0) return bail(fail(OXH_ERR_ARG, "system %d: bussi_tau must be > 0", s));
+			S.bussi_c = std::exp(-S.newtonian_steps / (double) d.bussi_tau);
+			B->any_bussi = true;
+			if(B->bussi_ns == 0) B->bussi_ns = S.newtonian_steps;
+			else if(B->bussi_ns != S.newtonian_steps) B->bussi_aligned = false;
+		}
+		else if(d.thermostat != OXH_THERMOSTAT_NO) return bail(fail(OXH_ERR_ARG, "system %d: unknown thermostat %d", s, d.thermostat));
+		N += d.N;
+	}
+	b.N = N;
+	b.nstride = N;
+	b.ncell_total = ncell;
+	b.any_cells = ncell > 0;
+	b.cell_cap = std::max(8, 3 * max_occ);
+	b.max_neigh = std::max(8, (int) std::ceil(max_neigh_est));
+	if(b.max_neigh > N) b.max_neigh = std::max(1, N);
+
+	// ---- device buffers
+	TRYB(dalloc(B, &b.pos, N));
+	TRYB(dalloc(B, &b.vel, N));
+	TRYB(dalloc(B, &b.Lm, N));
+	TRYB(dalloc(B, &b.q64, N));
+	TRYB(dalloc(B, &b.rlist, N));
+	TRYB(dalloc(B, &b.q32, N));
+	TRYB(dalloc(B, &b.bsite, N));
+	TRYB(dalloc(B, &b.F, N));
+	TRYB(dalloc(B, &b.T, N));
+	TRYB(dalloc(B, &b.bonds, N));
+	TRYB(dalloc(B, &b.pinfo, N));
+	TRYB(dalloc(B, &b.orig, N));
+	TRYB(dalloc(B, &b.pstep, N));
+	TRYB(dalloc(B, &b.nnbr, N));
+	TRYB(dalloc(B, &b.nbr, (size_t) N * b.max_neigh));
+	TRYB(dalloc(B, &b.cell_count, std::max(ncell, 1)));
+	TRYB(dalloc(B, &b.cell_owner, std::max(ncell, 1)));
+	TRYB(dalloc(B, &b.cell_items, (size_t) std::max(ncell, 1) * b.cell_cap));
+	TRYB(dalloc(B, &b.d_sys, n_sys));
+	TRYB(dalloc(B, &b.d_state, n_sys));
+	TRYB(dalloc(B, &b.esplit, (size_t) 8 * N));
+	TRYB(dalloc(B, &b.hbcount, N));
+	TRYB(dalloc(B, &b.keys, N));
+	TRYB(dalloc(B, &b.keys_alt, N));
+	TRYB(dalloc(B, &b.perm, N));
+	TRYB(dalloc(B, &b.perm_alt, N));
+	TRYB(dalloc(B, &b.inv_perm, N));
+	TRYB(dalloc(B, &b.radix_hist, (size_t) 256 * ((N + 4095) / 4096)));
+	b.scratch_bytes = (size_t) std::max(N, 2 * n_sys) * sizeof(double4);
+	TRYB(dalloc(B, (char **) &b.scratch, b.scratch_bytes));
+	TRYB(dalloc(B, &B->bk_pos, N));
+	TRYB(dalloc(B, &B->bk_vel, N));
+	TRYB(dalloc(B, &B->bk_Lm, N));
+	TRYB(dalloc(B, &B->bk_q64, N));
+	TRYB(dalloc(B, &B->bk_F, N));
+	TRYB(dalloc(B, &B->bk_T, N));
+	TRYB(dalloc(B, &B->bk_pstep, N));
+	TRYB(dalloc(B, &B->bk_state, n_sys));
+
+	// reduction blocks: chunks of RED_CHUNK particles inside one system
+	std::vector<int3> rb;
+	std::vector<int2> rsb(n_sys);
+	for(int s = 0; s < n_sys; s++) {
+		rsb[s].x = (int) rb.size();
+		for(int a = B->first[s]; a < B->first[s] + B->n[s]; a += oxh::RED_CHUNK)
+			rb.push_back(make_int3(s, a, std::min(a + oxh::RED_CHUNK, B->first[s] + B->n[s])));
+		rsb[s].y = (int) rb.size() - rsb[s].x;
+	}
+	b.nred = (int) rb.size();
+	TRYB(dalloc(B, &b.red_blocks, b.nred));
+	TRYB(dalloc(B, &b.red_sys_blocks, n_sys));
+	TRYB(dalloc(B, &b.red_partial, (size_t) b.nred * oxh::RED_W));
+	TRYB(dalloc(B, &b.red_out, (size_t) n_sys * oxh::RED_W));
+
+	// ---- host staging
+	std::vector<double4> hpos(N), hvel(N), hL(N), hq(N);
+	std::vector<int2> hbonds(N), hinfo(N);
+	std::vector<int> horig(N), howner(std::max(ncell, 1));
+	std::vector<oxh_sys_state> hst(n_sys);
+	for(int s = 0; s < n_sys; s++) {
+		const oxh_system_desc &d = descs[s];
+		int f = B->first[s];
+		for(int i = 0; i < d.N; i++) {
+			hpos[f + i] = make_double4(d.pos[3 * i], d.pos[3 * i + 1], d.pos[3 * i + 2], 0.0);
+			hvel[f + i] = d.vel ? make_double4(d.vel[3 * i], d.vel[3 * i + 1], d.vel[3 * i + 2], 0.0) : make_double4(0, 0, 0, 0);
+			hL[f + i] = d.L ? make_double4(d.L[3 * i], d.L[3 * i + 1], d.L[3 * i + 2], 0.0) : make_double4(0, 0, 0, 0);
+			double q[4];
+			frame_to_quat(d.a1 + 3 * i, d.a3 + 3 * i, q);
+			hq[f + i] = make_double4(q[0], q[1], q[2], q[3]);
+			hbonds[f + i] = make_int2(d.n3[i] >= 0 ? f + d.n3[i] : -1, d.n5[i] >= 0 ? f + d.n5[i] : -1);
+			hinfo[f + i] = make_int2(d.btype[i], s);
+			horig[f + i] = i;
+		}
+		const oxh_sys &S = b.h_sys[s];
+		for(int c = 0; c < S.ncell; c++) howner[S.cell_first + c] = s;
+		std::memset(&hst[s], 0, sizeof(hst[s]));
+		hst[s].K_t = 0.5 * 3.0 * (d.N - 1) * d.T;
+		hst[s].K_r = 0.5 * 3.0 * d.N * d.T;
+	}
+	hipStream_t st = b.stream;
+	auto up = [&](void *dst, const void *src, size_t bytes) {
+		return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
+	};
+	bool ok = up(b.pos, hpos.data(), N * sizeof(double4)) && up(b.vel, hvel.data(), N * sizeof(double4)) &&
+			up(b.Lm, hL.data(), N * sizeof(double4)) && up(b.q64, hq.data(), N * sizeof(double4)) &&
+			up(b.bonds, hbonds.data(), N * sizeof(int2)) && up(b.pinfo, hinfo.data(), N * sizeof(int2)) &&
+			up(b.orig, horig.data(), N * sizeof(int)) && up(b.cell_owner, howner.data(), howner.size() * sizeof(int)) &&
+			up(b.d_sys, b.h_sys.data(), n_sys * sizeof(oxh_sys)) && up(b.d_state, hst.data(), n_sys * sizeof(oxh_sys_state)) &&
+			up(b.red_blocks, rb.data(), rb.size() * sizeof(int3)) && up(b.red_sys_blocks, rsb.data(), n_sys * sizeof(int2));
+	if(!ok) return bail(fail(OXH_ERR_HIP, "upload failed"));
+	if(hipMemsetAsync(b.pstep, 0, N * sizeof(long long), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
+	if(hipMemsetAsync(b.F, 0, N * sizeof(float4), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
+	if(hipMemsetAsync(b.T, 0, N * sizeof(float4), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
+
+	// refresh_vel systems: draw on the device, keep the others as uploaded
+	bool any_refresh = false;
+	for(int s = 0; s < n_sys; s++) any_refresh |= descs[s].refresh_vel != 0;
+	if(any_refresh) {
+		std::vector<double4> keep_v = hvel, keep_L = hL;
+		oxh::launch_refresh_vel(st, b);
+		std::vector<double4> nv(N), nL(N);
+		(void) hipMemcpyAsync(nv.data(), b.vel, N * sizeof(double4), hipMemcpyDeviceToHost, st);
+		(void) hipMemcpyAsync(nL.data(), b.Lm, N * sizeof(double4), hipMemcpyDeviceToHost, st);
+		if(hipStreamSynchronize(st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "refresh_vel failed"));
+		for(int s = 0; s < n_sys; s++) {
+			if(descs[s].refresh_vel) continue;
+			for(int i = B->first[s]; i < B->first[s] + B->n[s]; i++) {
+				nv[i] = keep_v[i];
+				nL[i] = keep_L[i];
+			}
+		}
+		(void) up(b.vel, nv.data(), N * sizeof(double4));
+		(void) up(b.Lm, nL.data(), N * sizeof(double4));
+	}
+	oxh::launch_prepare_mirror(st, b);
+	oxh_status s = build_all(B);
+	if(s != OXH_OK) return bail(s);
+	// graph chunk: enough steps to amortise the launch, at most ~2k kernels
+	B->chunk = 64;
+	if(B->any_bussi && B->bussi_aligned) B->chunk = std::max(B->bussi_ns, (64 / B->bussi_ns) * B->bussi_ns);
+	*out = B;
+	return OXH_OK;
+#undef TRYB
+}
+
+oxh_status oxh_batch_update_lists(oxh_batch *B, int force) {
+	if(!B) return fail(OXH_ERR_ARG, "null batch");
+	HIPCHK(hipSetDevice(B->b.device));
+	for(int attempt = 0; attempt < 12; attempt++) {
+		enqueue_lists(B, force != 0);
+		k_clear_need<<<(B->b.nsys + 63) / 64, 64, 0, B->b.stream>>>(B->b.nsys, B->b.d_state, 0);
+		std::vector<oxh_sys_state> st;
+		TRY(read_states(B, st));
+		bool overflow = false;
+		for(auto &x : st) overflow |= (x.error & 6) != 0;
+		if(!overflow) return OXH_OK;
+		TRY(grow_lists(B, st));
+		force = 1;
+	}
+	return fail(OXH_ERR_SYSTEM, "neighbour list keeps overflowing");
+}
+
+oxh_status oxh_batch_compute_forces(oxh_batch *B) {
+	if(!B) return fail(OXH_ERR_ARG, "null batch");
+	HIPCHK(hipSetDevice(B->b.device));
+	oxh::launch_forces(B->b.stream, B->b, false);
+	return sync_and_check(B);
+}
+
+oxh_status oxh_batch_run(oxh_batch *B, long long n_steps) {
+	if(!B) return fail(OXH_ERR_ARG, "null batch");
+	if(n_steps < 0) return fail(OXH_ERR_ARG, "negative step count");
+	oxh::Batch &b = B->b;
+	HIPCHK(hipSetDevice(b.device));
+	long long done = 0;
+	int retries = 0;
+	while(done < n_steps) {
+		long long todo = std::min<long long>(B->chunk, n_steps - done);
+		bool aligned = !B->any_bussi || !B->bussi_aligned || (b.step % B->bussi_ns == 0);
+		TRY(backup(B));
+		if(todo == B->chunk && aligned) {
+			if(!b.graph_exec) {
+				hipGraph_t g;
+				HIPCHK(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+				for(int k = 0; k < B->chunk; k++) enqueue_step(B, k);
+				HIPCHK(hipStreamEndCapture(b.stream, &g));
+				HIPCHK(hipGraphInstantiate(&b.graph_exec, g, nullptr, nullptr, 0));
+				HIPCHK(hipGraphDestroy(g));
+			}
+			HIPCHK(hipGraphLaunch(b.graph_exec, b.stream));
+		}
+		else {
+			for(long long k = 0; k < todo; k++) enqueue_step(B, b.step + k);
+		}
+		std::vector<oxh_sys_state> st;
+		TRY(read_states(B, st));
+		bool overflow = false;
+		int bad = -1;
+		for(int s = 0; s < b.nsys; s++) {
+			if(st[s].error & 6) overflow = true;
+			if((st[s].error & 1) && bad < 0) bad = s;
+		}
+		if(overflow) {
+			if(++retries > 8) return fail(OXH_ERR_SYSTEM, "neighbour list keeps overflowing");
+			TRY(restore(B));
+			TRY(grow_lists(B, st));
+			TRY(build_all(B));
+			continue;
+		}
+		b.step += todo;
+		done += todo;
+		if(bad >= 0) return fail(OXH_ERR_SYSTEM, "system %d: a bonded pair exceeded the FENE range", bad);
+	}
+	return OXH_OK;
+}
+
+oxh_status oxh_batch_hilbert_sort(oxh_batch *B) {
+	if(!B) return fail(OXH_ERR_ARG, "null batch");
+	HIPCHK(hipSetDevice(B->b.device));
+	if(oxh::launch_hilbert_sort(B->b.stream, B->b) != 0) return fail(OXH_ERR_ARG, "too many systems for the sort key");
+	TRY(sync_and_check(B));
+	return oxh_batch_update_lists(B, 1);
+}
+
+oxh_status oxh_batch_sync(oxh_batch *B) {
+	if(!B) return fail(OXH_ERR_ARG, "null batch");
+	HIPCHK(hipSetDevice(B->b.device));
+	return sync_and_check(B);
+}
+
+
+oxh_status oxh_batch_get_state(oxh_batch *B, int sys, double *pos, double *a1, double *a3, double *vel, double *L) {
+	TRY(check_sys(B, sys));
+	int f = B->first[sys], n = B->n[sys];
+	std::vector<double4> p, q, v, l;
+	std::vector<int> o;
+	TRY(download(B, p, B->b.pos, f, n));
+	TRY(download(B, q, B->b.q64, f, n));
+	TRY(download(B, v, B->b.vel, f, n));
+	TRY(download(B, l, B->b.Lm, f, n));
+	TRY(download(B, o, B->b.orig, f, n));
+	TRY(sync_and_check(B));
+	for(int k = 0; k < n; k++) {
+		int i = o[k];
+		if(pos) {
+			pos[3 * i] = p[k].x;
+			pos[3 * i + 1] = p[k].y;
+			pos[3 * i + 2] = p[k].z;
+		}
+		double x1[3], x3[3];
+		quat_to_axes(q[k], x1, x3);
+		for(int c = 0; c < 3; c++) {
+			if(a1) a1[3 * i + c] = x1[c];
+			if(a3) a3[3 * i + c] = x3[c];
+		}
+		if(vel) {
+			vel[3 * i] = v[k].x;
+			vel[3 * i + 1] = v[k].y;
+			vel[3 * i + 2] = v[k].z;
+		}
+		if(L) {
+			L[3 * i] = l[k].x;
+			L[3 * i + 1] = l[k].y;
+			L[3 * i + 2] = l[k].z;
+		}
+	}
+	return OXH_OK;
+}
+
+oxh_status oxh_batch_get_forces(oxh_batch *B, int sys, double *F, double *T, double *Epart) {
+	TRY(check_sys(B, sys));
+	int f = B->first[sys], n = B->n[sys];
+	std::vector<float4> hf, ht;
+	std::vector<int> o;
+	TRY(download(B, hf, B->b.F, f, n));
+	TRY(download(B, ht, B->b.T, f, n));
+	TRY(download(B, o, B->b.orig, f, n));
+	TRY(sync_and_check(B));
+	for(int k = 0; k < n; k++) {
+		int i = o[k];
+		if(F) {
+			F[3 * i] = hf[k].x;
+			F[3 * i + 1] = hf[k].y;
+			F[3 * i + 2] = hf[k].z;
+		}
+		if(T) {
+			T[3 * i] = ht[k].x;
+			T[3 * i + 1] = ht[k].y;
+			T[3 * i + 2] = ht[k].z;
+		}
+		if(Epart) Epart[i] = hf[k].w;
+	}
+	return OXH_OK;
+}
+
+oxh_status oxh_batch_energies(oxh_batch *B, double *out4) {
+	if(!B || !out4) return fail(OXH_ERR_ARG, "bad arguments");
+	HIPCHK(hipSetDevice(B->b.device));
+	oxh::launch_reduce(B->b.stream, B->b);
+	std::vector<double> r((size_t) B->b.nsys * oxh::RED_W);
+	HIPCHK(hipMemcpyAsync(r.data(), B->b.red_out, r.size() * sizeof(double), hipMemcpyDeviceToHost, B->b.stream));
+	TRY(sync_and_check(B));
+	for(int s = 0; s < B->b.nsys; s++)
+		for(int k = 0; k < 4; k++) out4[4 * s + k] = r[(size_t) s * oxh::RED_W + k];
+	return OXH_OK;
+}
+
+oxh_status oxh_batch_energy_split(oxh_batch *B, int sys, double *E8) {
+	TRY(check_sys(B, sys));
+	oxh::launch_forces(B->b.stream, B->b, true);
+	int f = B->first[sys], n = B->n[sys];
+	std::vector<float> e((size_t) 8 * n);
+	for(int t = 0; t < 8; t++)
+		HIPCHK(hipMemcpyAsync(e.data() + (size_t) t * n, B->b.esplit + (size_t) t * B->b.N + f, n * sizeof(float), hipMemcpyDeviceToHost,
+				B->b.stream));
+	TRY(sync_and_check(B));
+	for(int t = 0; t < 8; t++) {
+		double acc = 0;
+		for(int k = 0; k < n; k++) acc += e[(size_t) t * n + k];
+		E8[t] = acc;
+	}
+	return OXH_OK;
+}
+
+oxh_status oxh_batch_hb_count(oxh_batch *B, int *out) {
+	if(!B || !out) return fail(OXH_ERR_ARG, "bad arguments");
+	HIPCHK(hipSetDevice(B->b.device));
+	B->b.want_hbcount = true;
+	oxh::launch_forces(B->b.stream, B->b, false);
+	B->b.want_hbcount = false;
+	std::vector<int> h(B->b.N);
+	HIPCHK(hipMemcpyAsync(h.data(), B->b.hbcount, B->b.N * sizeof(int), hipMemcpyDeviceToHost, B->b.stream));
+	TRY(sync_and_check(B));
+	for(int s = 0; s < B->b.nsys; s++) {
+		long long acc = 0;
+		for(int i = B->first[s]; i < B->first[s] + B->n[s]; i++) acc += h[i];
+		out[s] = (int) (acc / 2);
+	}
+	return OXH_OK;
+}
+
+oxh_status oxh_batch_get_pairs(oxh_batch *B, int sys, int *pairs, long long max, long long *n_out) {
+	TRY(check_sys(B, sys));
+	int f = B->first[sys], n = B->n[sys];
+	std::vector<int> nn, o, all_orig;
+	TRY(download(B, nn, B->b.nnbr, f, n));
+	TRY(download(B, all_orig, B->b.orig, 0, B->b.N));
+	std::vector<int> rows((size_t) B->b.max_neigh * n);
+	for(int k = 0; k < B->b.max_neigh; k++)
+		HIPCHK(hipMemcpyAsync(rows.data() + (size_t) k * n, B->b.nbr + (size_t) k * B->b.nstride + f, n * sizeof(int), hipMemcpyDeviceToHost,
+				B->b.stream));
+	TRY(sync_and_check(B));
+	std::vector<std::pair<int, int>> pr;
+	for(int k = 0; k < n; k++) {
+		int i = all_orig[f + k];
+		for(int m = 0; m < nn[k]; m++) {
+			int j = all_orig[rows[(size_t) m * n + k]];
+			if(i < j) pr.push_back({i, j});
+			else pr.push_back({j, i});
+		}
+	}
+	std::sort(pr.begin(), pr.end());
+	pr.erase(std::unique(pr.begin(), pr.end()), pr.end());
+	*n_out = (long long) pr.size();
+	for(long long k = 0; k < (long long) pr.size() && k < max; k++) {
+		pairs[2 * k] = pr[k].first;
+		pairs[2 * k + 1] = pr[k].second;
+	}
+	return OXH_OK;
+}
+
+oxh_status oxh_batch_system_info(oxh_batch *B, int sys, long long *n_rebuilds, int *error) {
+	TRY(check_sys(B, sys));
+	std::vector<oxh_sys_state> st;
+	TRY(read_states(B, st));
+	if(n_rebuilds) *n_rebuilds = st[sys].n_rebuilds;
+	if(error) *error = st[sys].error;
+	return OXH_OK;
+}
+
+oxh_status oxh_batch_set_state(oxh_batch *B, int sys, const double *pos, const double *a1, const double *a3, const double *vel,
+		const double *L) {
+	TRY(check_sys(B, sys));
+	oxh::Batch &b = B->b;
+	int f = B->first[sys], n = B->n[sys];
+	std::vector<int> o;
+	std::vector<double4> hp, hq, hv, hl;
+	TRY(download(B, o, b.orig, f, n));
+	TRY(download(B, hp, b.pos, f, n));
+	TRY(download(B, hq, b.q64, f, n));
+	TRY(download(B, hv, b.vel, f, n));
+	TRY(download(B, hl, b.Lm, f, n));
+	TRY(sync_and_check(B));
+	for(int k = 0; k < n; k++) {
+		int i = o[k];
+		if(pos) hp[k] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.0);
+		if(a1 && a3) {
+			double q[4];
+			frame_to_quat(a1 + 3 * i, a3 + 3 * i, q);
+			hq[k] = make_double4(q[0], q[1], q[2], q[3]);
+		}
+		if(vel) hv[k] = make_double4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0);
+		if(L) hl[k] = make_double4(L[3 * i], L[3 * i + 1], L[3 * i + 2], 0.0);
+	}
+	HIPCHK(hipMemcpyAsync(b.pos + f, hp.data(), n * sizeof(double4), hipMemcpyHostToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(b.q64 + f, hq.data(), n * sizeof(double4), hipMemcpyHostToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(b.vel + f, hv.data(), n * sizeof(double4), hipMemcpyHostToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(b.Lm + f, hl.data(), n * sizeof(double4), hipMemcpyHostToDevice, b.stream));
+	oxh::launch_prepare_mirror(b.stream, b);
+	TRY(sync_and_check(B));
+	TRY(oxh_batch_update_lists(B, 1));
+	return oxh_batch_compute_forces(B);
+}
+
+oxh_status oxh_batch_stats(oxh_batch *B, double *out4) {
+	if(!B || !out4) return fail(OXH_ERR_ARG, "bad arguments");
+	HIPCHK(hipSetDevice(B->b.device));
+	std::vector<int> nn;
+	TRY(download(B, nn, B->b.nnbr, 0, B->b.N));
+	TRY(sync_and_check(B));
+	double acc = 0;
+	for(int x : nn) acc += x;
+	out4[0] = B->b.N;
+	out4[1] = B->b.max_neigh;
+	out4[2] = B->b.N ? acc / B->b.N : 0.0;
+	out4[3] = (double) B->device_bytes;
+	return OXH_OK;
+}
+
+oxh_status oxh_hilbert_keys(int device, int n, const unsigned int *xyzb, unsigned int *keys) {
+	if(n <= 0) return OXH_OK;
+	HIPCHK(hipSetDevice(device));
+	void *d_in = nullptr, *d_out = nullptr;
+	HIPCHK(hipMalloc(&d_in, (size_t) n * 16));
+	HIPCHK(hipMalloc(&d_out, (size_t) n * 4));
+	HIPCHK(hipMemcpy(d_in, xyzb, (size_t) n * 16, hipMemcpyHostToDevice));
+	oxh_internal_hilbert_probe(n, d_in, d_out, nullptr);
+	HIPCHK(hipDeviceSynchronize());
+	HIPCHK(hipMemcpy(keys, d_out, (size_t) n * 4, hipMemcpyDeviceToHost));
+	(void) hipFree(d_in);
+	(void) hipFree(d_out);
+	return OXH_OK;
+}
+
+}  // extern "C"
+
+__global__ void k_philox_probe(int n, const uint4 *ctr, const uint2 *key, uint4 *out) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i < n) out[i] = oxh::philox4x32_10(ctr[i], key[i]);
+}
+
+extern "C" oxh_status oxh_philox(int device, int n, const unsigned int *ctr4, const unsigned int *key2, unsigned int *out4) {
+	if(n <= 0) return OXH_OK;
+	HIPCHK(hipSetDevice(device));
+	uint4 *dc = nullptr, *dout = nullptr;
+	uint2 *dk = nullptr;
+	HIPCHK(hipMalloc(&dc, (size_t) n * 16));
+	HIPCHK(hipMalloc(&dk, (size_t) n * 8));
+	HIPCHK(hipMalloc(&dout, (size_t) n * 16));
+	HIPCHK(hipMemcpy(dc, ctr4, (size_t) n * 16, hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(dk, key2, (size_t) n * 8, hipMemcpyHostToDevice));
+	k_philox_probe<<<(n + 255) / 256, 256>>>(n, dc, dk, dout);
+	HIPCHK(hipDeviceSynchronize());
+	HIPCHK(hipMemcpy(out4, dout, (size_t) n * 16, hipMemcpyDeviceToHost));
+	(void) hipFree(dc);
+	(void) hipFree(dk);
+	(void) hipFree(dout);
+	return OXH_OK;
+}

@@ -1,0 +1,288 @@
+// Device-side building blocks of the oxDNA2 hot path for gfx950:
+// model constants, the FP32 radial/angular modulation functions,
+// quaternion algebra and the Philox4x32-10 counter generator.
+//
+// Model constants are the published oxDNA2 parameters as the reference
+// writes them (src/model.h:15-434, src/defs.h:14 for the float PI); the
+// modulation functions follow src/Interactions/DNAInteraction.cpp:1249-1454
+// and src/Interactions/DNA2Interaction.cpp:308-363 (analytic f4, the
+// reference's DNA2_nomesh form), evaluated in FP32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "oxh_types.h"
+
+namespace oxh {
+
+constexpr float PIf = 3.141592653589793238462643f;
+
+constexpr float POS_BACK = -0.4f;
+constexpr float POS_MM_BACK1 = -0.3400f;
+constexpr float POS_MM_BACK2 = 0.3408f;
+constexpr float POS_STACK = 0.34f;
+constexpr float POS_BASE = 0.4f;
+constexpr float GAMMA = POS_STACK - POS_BACK;
+
+constexpr float FENE_EPS = 2.0f;
+constexpr float FENE_DELTA = 0.25f;
+constexpr float FENE_DELTA2 = 0.0625f;
+
+constexpr float EXCL_EPS = 2.0f;
+constexpr float EXCL_S1 = 0.70f, EXCL_S2 = 0.33f, EXCL_S3 = 0.515f, EXCL_S4 = 0.515f;
+constexpr float EXCL_R1 = 0.675f, EXCL_R2 = 0.32f, EXCL_R3 = 0.50f, EXCL_R4 = 0.50f;
+constexpr float EXCL_B1 = 892.016223343f, EXCL_B2 = 4119.70450017f, EXCL_B3 = 1707.30627298f, EXCL_B4 = 1707.30627298f;
+constexpr float EXCL_RC1 = 0.711879214356f, EXCL_RC2 = 0.335388426126f, EXCL_RC3 = 0.52329943261f, EXCL_RC4 = 0.52329943261f;
+
+constexpr float HYDR_RCLOW = 0.276908f, HYDR_RCHIGH = 0.783775f;
+constexpr float CRST_RCLOW = 0.45f, CRST_RCHIGH = 0.7f;
+constexpr float CXST_RCLOW = 0.177778f, CXST_RCHIGH = 0.6222222f;
+
+// f1 radial: [0] HB, [1] stacking (src/model.h:92-102,145-157)
+constexpr float F1_A[2] = {8.f, 6.f};
+constexpr float F1_R0[2] = {0.4f, 0.4f};
+constexpr float F1_BLOW[2] = {-126.243f, -68.1857f};
+constexpr float F1_BHIGH[2] = {-7.87708f, -3.12992f};
+constexpr float F1_RLOW[2] = {0.34f, 0.32f};
+constexpr float F1_RHIGH[2] = {0.7f, 0.75f};
+constexpr float F1_RCLOW[2] = {0.276908f, 0.23239f};
+constexpr float F1_RCHIGH[2] = {0.783775f, 0.956f};
+
+// f2 radial: [0] cross stacking, [1] coaxial stacking, oxDNA2 K
+// (src/model.h:193-202,349-360; src/Interactions/DNA2Interaction.cpp:10)
+constexpr float F2_K[2] = {47.5f, 58.5f};
+constexpr float F2_RC[2] = {0.675f, 0.6f};
+constexpr float F2_R0[2] = {0.575f, 0.400f};
+constexpr float F2_BLOW[2] = {-0.888889f, -2.13158f};
+constexpr float F2_BHIGH[2] = {-0.888889f, -2.13158f};
+constexpr float F2_RLOW[2] = {0.495f, 0.22f};
+constexpr float F2_RHIGH[2] = {0.655f, 0.58f};
+constexpr float F2_RCLOW[2] = {0.45f, 0.177778f};
+constexpr float F2_RCHIGH[2] = {0.7f, 0.6222222f};
+
+// f4 slots, numbered as the reference numbers them (src/model.h:105-110,...)
+enum {
+	S_STCK_T4 = 0, S_STCK_T5 = 1, S_HYDR_T1 = 2, S_HYDR_T2 = 3, S_HYDR_T4 = 4, S_HYDR_T7 = 5,
+	S_CRST_T1 = 6, S_CRST_T2 = 7, S_CRST_T4 = 8, S_CRST_T7 = 9, S_CXST_T1 = 10, S_CXST_T4 = 11, S_CXST_T5 = 12
+};
+constexpr float F4_A[13] = {1.3f, 0.9f, 1.5f, 1.5f, 0.46f, 4.f, 2.25f, 1.70f, 1.50f, 1.70f, 2.f, 1.3f, 0.9f};
+constexpr float F4_B[13] = {6.4381f, 3.89361f, 4.16038f, 4.16038f, 0.133855f, 17.0526f, 7.00545f, 6.2469f, 2.59556f, 6.2469f,
+		10.9032f, 6.4381f, 3.89361f};
+constexpr float F4_T0[13] = {0.f, 0.f, 0.f, 0.f, PIf, PIf * 0.5f, PIf - 2.35f, 1.f, 0.f, 0.875f, PIf - 0.25f, 0.f, 0.f};
+constexpr float F4_TS[13] = {0.8f, 0.95f, 0.7f, 0.7f, 0.7f, 0.45f, 0.58f, 0.68f, 0.65f, 0.68f, 0.65f, 0.8f, 0.95f};
+constexpr float F4_TC[13] = {0.961538f, 1.16959f, 0.952381f, 0.952381f, 3.10559f, 0.555556f, 0.766284f, 0.865052f, 1.02564f,
+		0.865052f, 0.769231f, 0.961538f, 1.16959f};
+constexpr float CXST_THETA1_SA = 20.f;
+constexpr float CXST_THETA1_SB = (PIf - 0.1f * (PIf - (PIf - 0.25f)));
+
+// f5 (phi) modulation (src/model.h:180-189)
+constexpr float F5_A = 2.0f, F5_B = 10.9032f, F5_XC = -0.769231f, F5_XS = -0.65f;
+
+// centre distance beyond which only Debye-Hueckel can act: the largest
+// site-pair range (back-back excluded volume, 0.7119) plus twice the
+// backbone-site arm |(-0.34, 0.3408)| = 0.4814
+constexpr float R_CLOSE = 1.676f;
+
+// ------------------------------------------------------------ small vectors
+struct f3 {
+	float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 &operator+=(f3 &a, f3 b) { a.x += b.x; a.y += b.y; a.z += b.z; return a; }
+__device__ __forceinline__ f3 &operator-=(f3 &a, f3 b) { a.x -= b.x; a.y -= b.y; a.z -= b.z; return a; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ __forceinline__ float SQR(float x) { return x * x; }
+
+// body axes from a unit quaternion (x, y, z, w): a1, a2, a3 are the images
+// of the x, y, z axes (the columns of the rotation matrix)
+__device__ __forceinline__ void quat_axes(float4 q, f3 &a1, f3 &a2, f3 &a3) {
+	float sqx = q.x * q.x, sqy = q.y * q.y, sqz = q.z * q.z, sqw = q.w * q.w;
+	float xy = q.x * q.y, xz = q.x * q.z, xw = q.x * q.w, yz = q.y * q.z, yw = q.y * q.w, zw = q.z * q.w;
+	a1 = {sqx - sqy - sqz + sqw, 2.f * (xy + zw), 2.f * (xz - yw)};
+	a2 = {2.f * (xy - zw), -sqx + sqy - sqz + sqw, 2.f * (yz + xw)};
+	a3 = {2.f * (xz + yw), 2.f * (yz - xw), -sqx - sqy + sqz + sqw};
+}
+
+// --------------------------------------------------- modulation functions
+
+__device__ __forceinline__ float f1(const oxh_sys &S, float r, int t, int n3, int n5) {
+	float val = 0.f;
+	if(r < F1_RCHIGH[t]) {
+		float eps = S.f1_eps[t][n3][n5];
+		if(r > F1_RHIGH[t]) val = eps * F1_BHIGH[t] * SQR(r - F1_RCHIGH[t]);
+		else if(r > F1_RLOW[t]) {
+			float tmp = 1.f - expf(-(r - F1_R0[t]) * F1_A[t]);
+			val = eps * SQR(tmp) - S.f1_shift[t][n3][n5];
+		}
+		else if(r > F1_RCLOW[t]) val = eps * F1_BLOW[t] * SQR(r - F1_RCLOW[t]);
+	}
+	return val;
+}
+
+__device__ __forceinline__ float f1D(const oxh_sys &S, float r, int t, int n3, int n5) {
+	float val = 0.f;
+	if(r < F1_RCHIGH[t]) {
+		if(r > F1_RHIGH[t]) val = 2.f * F1_BHIGH[t] * (r - F1_RCHIGH[t]);
+		else if(r > F1_RLOW[t]) {
+			float tmp = expf(-(r - F1_R0[t]) * F1_A[t]);
+			val = 2.f * (1.f - tmp) * tmp * F1_A[t];
+		}
+		else if(r > F1_RCLOW[t]) val = 2.f * F1_BLOW[t] * (r - F1_RCLOW[t]);
+	}
+	return S.f1_eps[t][n3][n5] * val;
+}
+
+__device__ __forceinline__ float f2(float r, int t) {
+	float val = 0.f;
+	if(r < F2_RCHIGH[t]) {
+		if(r > F2_RHIGH[t]) val = F2_K[t] * F2_BHIGH[t] * SQR(r - F2_RCHIGH[t]);
+		else if(r > F2_RLOW[t]) val = (F2_K[t] * 0.5f) * (SQR(r - F2_R0[t]) - SQR(F2_RC[t] - F2_R0[t]));
+		else if(r > F2_RCLOW[t]) val = F2_K[t] * F2_BLOW[t] * SQR(r - F2_RCLOW[t]);
+	}
+	return val;
+}
+
+__device__ __forceinline__ float f2D(float r, int t) {
+	float val = 0.f;
+	if(r < F2_RCHIGH[t]) {
+		if(r > F2_RHIGH[t]) val = 2.f * F2_K[t] * F2_BHIGH[t] * (r - F2_RCHIGH[t]);
+		else if(r > F2_RLOW[t]) val = F2_K[t] * (r - F2_R0[t]);
+		else if(r > F2_RCLOW[t]) val = 2.f * F2_K[t] * F2_BLOW[t] * (r - F2_RCLOW[t]);
+	}
+	return val;
+}
+
+// angle from its cosine, clamped into [-1, 1]
+__device__ __forceinline__ float safe_acos(float c) { return acosf(fminf(1.f, fmaxf(-1.f, c))); }
+
+// f4(theta) and (df4/dtheta)/sin(theta) of slot s at the angle whose
+// cosine is c; the sine comes from the same c so theta/sin(theta) stays
+// well conditioned near 0 and pi.  Small-sine guard as the reference's
+// _f4Dsin (src/Interactions/DNAInteraction.cpp:1394-1420).
+struct f4pair {
+	float f, dsin;
+};
+
+__device__ __forceinline__ f4pair f4_both(float c, int s) {
+	c = fminf(1.f, fmaxf(-1.f, c));
+	float t = acosf(c);
+	float tt0 = t - F4_T0[s];
+	float m = 1.f;
+	if(tt0 < 0.f) {
+		tt0 = -tt0;
+		m = -1.f;
+	}
+	f4pair r = {0.f, 0.f};
+	if(tt0 < F4_TC[s]) {
+		float sint2 = (1.f - c) * (1.f + c);
+		float inv_sin = sint2 > 1e-8f ? rsqrtf(sint2) : 1.f;
+		if(tt0 > F4_TS[s]) {
+			r.f = F4_B[s] * SQR(F4_TC[s] - tt0);
+			r.dsin = m * 2.f * F4_B[s] * (tt0 - F4_TC[s]) * rsqrtf(fmaxf(sint2, 1e-30f));
+		}
+		else {
+			r.f = 1.f - F4_A[s] * SQR(tt0);
+			r.dsin = sint2 > 1e-8f ? -m * 2.f * F4_A[s] * tt0 * inv_sin : -m * 2.f * F4_A[s];
+		}
+	}
+	return r;
+}
+
+// custom_f4 / custom_f4D of the DNA2_nomesh interaction: f4(c) and
+// -(df4/dtheta)/sin(theta); coaxial theta1 adds the oxDNA2 harmonic term
+// (src/Interactions/DNA2Interaction.cpp:308-363)
+__device__ __forceinline__ void cf4(float c, int s, float &f, float &fD) {
+	f4pair p = f4_both(c, s);
+	f = p.f;
+	fD = -p.dsin;
+	if(s == S_CXST_T1) {
+		c = fminf(1.f, fmaxf(-1.f, c));
+		float t = acosf(c);
+		float tt0 = t - CXST_THETA1_SB;
+		if(tt0 >= 0.f) {
+			f += CXST_THETA1_SA * SQR(tt0);
+			float sint2 = (1.f - c) * (1.f + c);
+			fD -= sint2 > 1e-8f ? 2.f * CXST_THETA1_SA * tt0 * rsqrtf(sint2) : 2.f * CXST_THETA1_SA;
+		}
+	}
+}
+
+__device__ __forceinline__ float f5(float f) {
+	float val = 0.f;
+	if(f > F5_XC) {
+		if(f < F5_XS) val = F5_B * SQR(F5_XC - f);
+		else if(f < 0.f) val = 1.f - F5_A * SQR(f);
+		else val = 1.f;
+	}
+	return val;
+}
+
+__device__ __forceinline__ float f5D(float f) {
+	float val = 0.f;
+	if(f > F5_XC) {
+		if(f < F5_XS) val = 2.f * F5_B * (f - F5_XC);
+		else if(f < 0.f) val = -2.f * F5_A * f;
+	}
+	return val;
+}
+
+// truncated/smoothed LJ (src/Interactions/DNAInteraction.cpp:1182-1205);
+// returns the energy and sets force = force on the second site
+__device__ __forceinline__ float repulsive_lj(f3 r, f3 &force, float sigma, float rstar, float b, float rc) {
+	float rnorm = dot(r, r);
+	float energy = 0.f;
+	force = {0.f, 0.f, 0.f};
+	if(rnorm < rc * rc) {
+		if(rnorm > rstar * rstar) {
+			float rmod = sqrtf(rnorm);
+			float rrc = rmod - rc;
+			energy = EXCL_EPS * b * SQR(rrc);
+			force = r * (-(2.f * EXCL_EPS * b * rrc / rmod));
+		}
+		else {
+			float tmp = sigma * sigma / rnorm;
+			float lj = tmp * tmp * tmp;
+			energy = 4.f * EXCL_EPS * (SQR(lj) - lj);
+			force = r * (-(24.f * EXCL_EPS * (lj - 2.f * SQR(lj)) / rnorm));
+		}
+	}
+	return energy;
+}
+
+// ------------------------------------------------------------ Philox4x32-10
+// J. K. Salmon, M. A. Moraes, R. O. Dror, D. E. Shaw, "Parallel random
+// numbers: as easy as 1, 2, 3", SC'11.
+__host__ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k) {
+#pragma unroll
+	for(int r = 0; r < 10; r++) {
+		uint64_t p0 = (uint64_t) 0xD2511F53u * c.x;
+		uint64_t p1 = (uint64_t) 0xCD9E8D57u * c.z;
+		uint32_t hi0 = (uint32_t) (p0 >> 32), lo0 = (uint32_t) p0;
+		uint32_t hi1 = (uint32_t) (p1 >> 32), lo1 = (uint32_t) p1;
+		c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+		k.x += 0x9E3779B9u;
+		k.y += 0xBB67AE85u;
+	}
+	return c;
+}
+
+// uniform in (0, 1) from a 32-bit word
+__host__ __device__ __forceinline__ double u01(uint32_t x) { return ((double) x + 0.5) * 2.3283064365386963e-10; }
+
+// two standard normals (Box-Muller, FP64) from two 32-bit words
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, double &n0, double &n1) {
+	double r = sqrt(-2.0 * log(u01(a)));
+	double s, c;
+	sincospi(2.0 * u01(b), &s, &c);
+	n0 = r * c;
+	n1 = r * s;
+}
+
+}  // namespace oxh

@@ -1,0 +1,88 @@
+// Internal interface between the C ABI (oxh_api.hip) and the kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "oxh_types.h"
+
+namespace oxh {
+
+// A batch: one or more independent systems packed into one set of SoA
+// device arrays (system s owns particles [first, first + n)).  The plain
+// single-system path is a batch of one.
+struct Batch {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	int N = 0;          // particles in the batch
+	int nsys = 0;
+
+	// per-particle state (FP64 integration state, FP32 kernel mirror)
+	double4 *pos = nullptr, *vel = nullptr, *Lm = nullptr, *q64 = nullptr, *rlist = nullptr;
+	float4 *q32 = nullptr, *bsite = nullptr, *F = nullptr, *T = nullptr;
+	int2 *bonds = nullptr;   // (n3, n5) batch indices, -1 = none
+	int2 *pinfo = nullptr;   // (btype, system)
+	int *orig = nullptr;     // index of the particle inside its system as uploaded
+	long long *pstep = nullptr;  // per-particle step counter (RNG counter)
+
+	// neighbour list, column major: nbr[k * nstride + i]
+	int *nbr = nullptr, *nnbr = nullptr;
+	int nstride = 0, max_neigh = 0;
+
+	// cells (systems with use_cells), fixed capacity per cell
+	int ncell_total = 0, cell_cap = 0;
+	int *cell_count = nullptr, *cell_items = nullptr, *cell_owner = nullptr;
+
+	// per-system parameters and device-written state
+	oxh_sys *d_sys = nullptr;
+	oxh_sys_state *d_state = nullptr;
+	std::vector<oxh_sys> h_sys;
+
+	// reductions: blocks of at most RED_CHUNK particles, each inside one system
+	int nred = 0;
+	int3 *red_blocks = nullptr;        // (system, begin, end)
+	int2 *red_sys_blocks = nullptr;    // per system: (first block, n blocks)
+	double *red_partial = nullptr;     // nred * RED_W
+	double *red_out = nullptr;         // nsys * RED_W
+
+	// optional outputs
+	float *esplit = nullptr;   // 8 * N
+	int *hbcount = nullptr;    // N
+	bool want_hbcount = false;
+
+	// Hilbert sort scratch
+	unsigned int *keys = nullptr, *keys_alt = nullptr;
+	int *perm = nullptr, *perm_alt = nullptr, *inv_perm = nullptr;
+	unsigned int *radix_hist = nullptr;
+	void *scratch = nullptr;
+	size_t scratch_bytes = 0;
+
+	int force_block = 256;
+	int any_cells = 0;
+	int thermostat_mask = 0;     // bit t set if some system uses thermostat t
+	int min_newtonian = 1;
+
+	hipGraphExec_t graph_exec = nullptr;
+	int graph_steps = 0;
+	long long step = 0;          // steps run by this batch (host count)
+	std::string last_error;
+};
+
+constexpr int RED_CHUNK = 2048;
+constexpr int RED_W = 8;   // reduction width: U, Uhb, Kt, Kr, vx, vy, vz, count
+
+void launch_forces(hipStream_t st, const Batch &b, bool split);
+void launch_cells(hipStream_t st, const Batch &b);
+void launch_build_list(hipStream_t st, const Batch &b, int force_all);
+void launch_first_step(hipStream_t st, const Batch &b);
+void launch_second_step(hipStream_t st, const Batch &b);
+void launch_end_step(hipStream_t st, const Batch &b);
+void launch_bussi(hipStream_t st, const Batch &b);
+void launch_reduce(hipStream_t st, const Batch &b);
+void launch_refresh_vel(hipStream_t st, const Batch &b);
+void launch_prepare_mirror(hipStream_t st, const Batch &b);
+int launch_hilbert_sort(hipStream_t st, Batch &b);
+
+}  // namespace oxh

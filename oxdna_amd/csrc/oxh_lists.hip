@@ -1,0 +1,372 @@
+// Neighbour lists and Hilbert ordering on gfx950.
+//
+// Verlet list: every j != i of the same system, not bonded to i, with
+// |r_ij|^2 < r_verlet^2, r_verlet = r_cut + 2 skin; rebuilt when a particle
+// moved more than the skin since the last build (reference:
+// src/CUDA/Lists/CUDA_simple_verlet.cuh:14-95,
+// src/CUDA/Lists/CUDASimpleVerletList.cu:96-289).  The predicate is
+// evaluated in FP64 with a fixed operation sequence (explicit fma) so the
+// pair set is reproducible bit for bit on any IEEE FP64 machine.  Cells
+// hold a fixed number of slots and are sorted after filling, so list
+// contents and order are deterministic (the reference fills cells with
+// atomicInc slots in arbitrary order).
+//
+// Hilbert ordering (reference: src/CUDA/CUDA_sort.cu, with thrust's
+// sort_by_key): key = Hilbert index of the particle's cell after
+// J. Skilling, "Programming the Hilbert curve" (AIP Conf. Proc. 707, 2004),
+// sorted by a hand-written stable LSD radix sort with wave64 ballots.
+#include <type_traits>
+#include <utility>
+
+#include "oxh_device.h"
+#include "oxh_internal.h"
+
+namespace oxh {
+
+__device__ __forceinline__ int cell_coord(double x, double inv_box, int nc) {
+	double s = x * inv_box;
+	double f = s - floor(s);
+	int c = (int) (f * nc);
+	return c < nc ? c : nc - 1;
+}
+
+__global__ void k_cells_clear(int ncell, const int *__restrict__ owner, const oxh_sys_state *__restrict__ st, int *__restrict__ count) {
+	int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if(c >= ncell) return;
+	if(st[owner[c]].need_rebuild) count[c] = 0;
+}
+
+__global__ void k_cells_fill(int N, const double4 *__restrict__ pos, const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems,
+		oxh_sys_state *__restrict__ st, int *__restrict__ count, int *__restrict__ items, int cap) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= N) return;
+	int s = pinfo[i].y;
+	if(!st[s].need_rebuild) return;
+	const oxh_sys &S = systems[s];
+	if(!S.use_cells) return;
+	double4 p = pos[i];
+	int cx = cell_coord(p.x, S.inv_box[0], S.nc[0]);
+	int cy = cell_coord(p.y, S.inv_box[1], S.nc[1]);
+	int cz = cell_coord(p.z, S.inv_box[2], S.nc[2]);
+	int c = S.cell_first + (cz * S.nc[1] + cy) * S.nc[0] + cx;
+	int slot = atomicAdd(&count[c], 1);
+	if(slot < cap) items[(size_t) c * cap + slot] = i;
+	else atomicOr(&st[s].error, 4);
+}
+
+// sorts every rebuilt cell's slots by particle index (insertion sort;
+// cells hold tens of particles)
+__global__ void k_cells_sort(int ncell, const int *__restrict__ owner, const oxh_sys_state *__restrict__ st, const int *__restrict__ count,
+		int *__restrict__ items, int cap) {
+	int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if(c >= ncell) return;
+	if(!st[owner[c]].need_rebuild) return;
+	int n = min(count[c], cap);
+	int *it = items + (size_t) c * cap;
+	for(int a = 1; a < n; a++) {
+		int v = it[a];
+		int b = a - 1;
+		while(b >= 0 && it[b] > v) {
+			it[b + 1] = it[b];
+			b--;
+		}
+		it[b + 1] = v;
+	}
+}
+
+void launch_cells(hipStream_t st, const Batch &b) {
+	if(!b.any_cells) return;
+	int bs = 256;
+	k_cells_clear<<<(b.ncell_total + bs - 1) / bs, bs, 0, st>>>(b.ncell_total, b.cell_owner, b.d_state, b.cell_count);
+	k_cells_fill<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, b.pos, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_items, b.cell_cap);
+	k_cells_sort<<<(b.ncell_total + bs - 1) / bs, bs, 0, st>>>(b.ncell_total, b.cell_owner, b.d_state, b.cell_count, b.cell_items,
+			b.cell_cap);
+}
+
+// FP64 squared nearest-image distance, fixed operation sequence
+__device__ __forceinline__ double sqr_dist(double4 a, double4 b, const oxh_sys &S) {
+	double dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+	dx = fma(-S.box[0], rint(dx * S.inv_box[0]), dx);
+	dy = fma(-S.box[1], rint(dy * S.inv_box[1]), dy);
+	dz = fma(-S.box[2], rint(dz * S.inv_box[2]), dz);
+	return fma(dz, dz, fma(dy, dy, dx * dx));
+}
+
+__global__ void __launch_bounds__(256) k_build_list(int N, const double4 *__restrict__ pos, const int2 *__restrict__ bonds,
+		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st,
+		const int *__restrict__ count, const int *__restrict__ items, int cap, int *__restrict__ nbr, int *__restrict__ nnbr,
+		int nstride, int max_neigh, double4 *__restrict__ rlist, int force_all) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= N) return;
+	int s = pinfo[i].y;
+	if(!force_all && !st[s].need_rebuild) return;
+	const oxh_sys &S = systems[s];
+	const double4 p = pos[i];
+	const int2 bd = bonds[i];
+	int n = 0;
+	auto consider = [&](int j) {
+		if(j == i || j == bd.x || j == bd.y) return;
+		if(sqr_dist(p, pos[j], S) < S.rv2) {
+			if(n < max_neigh) nbr[(size_t) n * nstride + i] = j;
+			n++;
+		}
+	};
+	if(S.use_cells) {
+		int cx = cell_coord(p.x, S.inv_box[0], S.nc[0]);
+		int cy = cell_coord(p.y, S.inv_box[1], S.nc[1]);
+		int cz = cell_coord(p.z, S.inv_box[2], S.nc[2]);
+		for(int dz = -1; dz <= 1; dz++) {
+			int z = cz + dz;
+			z += (z < 0) ? S.nc[2] : (z >= S.nc[2] ? -S.nc[2] : 0);
+			for(int dy = -1; dy <= 1; dy++) {
+				int y = cy + dy;
+				y += (y < 0) ? S.nc[1] : (y >= S.nc[1] ? -S.nc[1] : 0);
+				for(int dx = -1; dx <= 1; dx++) {
+					int x = cx + dx;
+					x += (x < 0) ? S.nc[0] : (x >= S.nc[0] ? -S.nc[0] : 0);
+					int c = S.cell_first + (z * S.nc[1] + y) * S.nc[0] + x;
+					int m = min(count[c], cap);
+					const int *it = items + (size_t) c * cap;
+					for(int k = 0; k < m; k++) consider(it[k]);
+				}
+			}
+		}
+	}
+	else {
+		for(int j = S.first; j < S.first + S.n; j++) consider(j);
+	}
+	if(n > max_neigh) {
+		atomicOr(&st[s].error, 2);
+		n = max_neigh;
+	}
+	nnbr[i] = n;
+	rlist[i] = p;
+}
+
+void launch_build_list(hipStream_t st, const Batch &b, int force_all) {
+	int bs = 256;
+	k_build_list<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, b.pos, b.bonds, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_items,
+			b.cell_cap, b.nbr, b.nnbr, b.nstride, b.max_neigh, b.rlist, force_all);
+}
+
+// ------------------------------------------------------------ Hilbert sort
+
+__device__ __forceinline__ unsigned int hilbert_key(unsigned int x, unsigned int y, unsigned int z, int bits) {
+	unsigned int X[3] = {x, y, z};
+	unsigned int M = 1u << (bits - 1);
+	for(unsigned int Q = M; Q > 1; Q >>= 1) {
+		unsigned int P = Q - 1;
+#pragma unroll
+		for(int i = 0; i < 3; i++) {
+			if(X[i] & Q) X[0] ^= P;
+			else {
+				unsigned int t = (X[0] ^ X[i]) & P;
+				X[0] ^= t;
+				X[i] ^= t;
+			}
+		}
+	}
+	X[1] ^= X[0];
+	X[2] ^= X[1];
+	unsigned int t = 0;
+	for(unsigned int Q = M; Q > 1; Q >>= 1)
+		if(X[2] & Q) t ^= Q - 1;
+	X[0] ^= t;
+	X[1] ^= t;
+	X[2] ^= t;
+	unsigned int key = 0;
+	for(int bb = bits - 1; bb >= 0; bb--) {
+		key = (key << 3) | (((X[0] >> bb) & 1u) << 2) | (((X[1] >> bb) & 1u) << 1) | ((X[2] >> bb) & 1u);
+	}
+	return key;
+}
+
+__global__ void k_hilbert_keys(int N, const double4 *__restrict__ pos, const int2 *__restrict__ pinfo,
+		const oxh_sys *__restrict__ systems, int bits, unsigned int *__restrict__ keys, int *__restrict__ vals) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= N) return;
+	int s = pinfo[i].y;
+	const oxh_sys &S = systems[s];
+	double4 p = pos[i];
+	int n = 1 << bits;
+	unsigned int cx = cell_coord(p.x, S.inv_box[0], n);
+	unsigned int cy = cell_coord(p.y, S.inv_box[1], n);
+	unsigned int cz = cell_coord(p.z, S.inv_box[2], n);
+	keys[i] = ((unsigned int) s << (3 * bits)) | hilbert_key(cx, cy, cz, bits);
+	vals[i] = i;
+}
+
+// exported for the C ABI's key check (same function as the sort uses)
+__global__ void k_hilbert_probe(int n, const uint4 *__restrict__ xyzb, unsigned int *__restrict__ out) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= n) return;
+	uint4 v = xyzb[i];
+	out[i] = hilbert_key(v.x, v.y, v.z, (int) v.w);
+}
+
+constexpr int RADIX_TILE = 4096;   // keys per block and pass
+constexpr int RADIX_BS = 256;
+
+__global__ void __launch_bounds__(RADIX_BS) k_radix_hist(int N, const unsigned int *__restrict__ keys, int shift,
+		unsigned int *__restrict__ hist, int nblocks) {
+	__shared__ unsigned int h[256];
+	h[threadIdx.x] = 0;
+	__syncthreads();
+	int base = blockIdx.x * RADIX_TILE;
+	for(int k = threadIdx.x; k < RADIX_TILE; k += RADIX_BS) {
+		int i = base + k;
+		if(i < N) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+	}
+	__syncthreads();
+	hist[(size_t) threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan of n entries in one workgroup (n is small: 256 * blocks)
+__global__ void __launch_bounds__(1024) k_scan_single(unsigned int *__restrict__ data, int n) {
+	__shared__ unsigned int part[1024];
+	int t = threadIdx.x;
+	int per = (n + 1023) / 1024;
+	int b = t * per, e = min(n, b + per);
+	unsigned int s = 0;
+	for(int k = b; k < e; k++) s += data[k];
+	part[t] = s;
+	__syncthreads();
+	for(int off = 1; off < 1024; off <<= 1) {
+		unsigned int v = (t >= off) ? part[t - off] : 0u;
+		__syncthreads();
+		part[t] += v;
+		__syncthreads();
+	}
+	unsigned int run = (t > 0) ? part[t - 1] : 0u;
+	for(int k = b; k < e; k++) {
+		unsigned int v = data[k];
+		data[k] = run;
+		run += v;
+	}
+}
+
+__global__ void __launch_bounds__(RADIX_BS) k_radix_scatter(int N, const unsigned int *__restrict__ keys, const int *__restrict__ vals,
+		int shift, const unsigned int *__restrict__ offs, int nblocks, unsigned int *__restrict__ keys_out, int *__restrict__ vals_out) {
+	__shared__ unsigned int run[256];
+	__shared__ unsigned int wcount[RADIX_BS / 64][256];
+	const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+	run[t] = offs[(size_t) t * nblocks + blockIdx.x];
+	for(int k = 0; k < RADIX_BS / 64; k++) wcount[k][t] = 0;
+	__syncthreads();
+	const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+	int base = blockIdx.x * RADIX_TILE;
+	for(int c = 0; c < RADIX_TILE; c += RADIX_BS) {
+		int i = base + c + t;
+		bool valid = i < N;
+		unsigned int key = valid ? keys[i] : 0u;
+		unsigned int d = (key >> shift) & 255u;
+		uint64_t m = __ballot(valid);
+#pragma unroll
+		for(int bit = 0; bit < 8; bit++) {
+			uint64_t bal = __ballot(((d >> bit) & 1u) != 0u);
+			m &= ((d >> bit) & 1u) ? bal : ~bal;
+		}
+		unsigned int rank = __popcll(m & lt);
+		if(valid && rank == 0) wcount[w][d] = __popcll(m);
+		__syncthreads();
+		if(valid) {
+			unsigned int off = run[d] + rank;
+			for(int k = 0; k < w; k++) off += wcount[k][d];
+			keys_out[off] = key;
+			vals_out[off] = vals[i];
+		}
+		__syncthreads();
+		unsigned int tot = 0;
+		for(int k = 0; k < RADIX_BS / 64; k++) {
+			tot += wcount[k][t];
+			wcount[k][t] = 0;
+		}
+		run[t] += tot;
+		__syncthreads();
+	}
+}
+
+template<typename T>
+__global__ void k_gather(int N, const int *__restrict__ perm, const T *__restrict__ src, T *__restrict__ dst) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i < N) dst[i] = src[perm[i]];
+}
+
+__global__ void k_invert(int N, const int *__restrict__ perm, int *__restrict__ inv) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i < N) inv[perm[i]] = i;
+}
+
+__global__ void k_gather_bonds(int N, const int *__restrict__ perm, const int *__restrict__ inv, const int2 *__restrict__ src,
+		int2 *__restrict__ dst) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= N) return;
+	int2 b = src[perm[i]];
+	dst[i] = make_int2(b.x >= 0 ? inv[b.x] : -1, b.y >= 0 ? inv[b.y] : -1);
+}
+
+__global__ void k_mark_rebuild(int nsys, oxh_sys_state *st) {
+	int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if(s < nsys) st[s].need_rebuild = 1;
+}
+
+// Reorders every system's particles along a Hilbert curve of its box.
+// Returns 0 on success, -1 if the batch has too many systems to encode.
+int launch_hilbert_sort(hipStream_t st, Batch &b) {
+	int N = b.N;
+	if(N < 2) return 0;
+	int sys_bits = 0;
+	while((1 << sys_bits) < b.nsys) sys_bits++;
+	int bits = (32 - sys_bits) / 3;
+	if(bits > 10) bits = 10;
+	if(bits < 2) return -1;
+	int total_bits = 3 * bits + sys_bits;
+	int bs = 256, g = (N + bs - 1) / bs;
+	k_hilbert_keys<<<g, bs, 0, st>>>(N, b.pos, b.pinfo, b.d_sys, bits, b.keys, b.perm);
+	int nblocks = (N + RADIX_TILE - 1) / RADIX_TILE;
+	unsigned int *kin = b.keys, *kout = b.keys_alt;
+	int *vin = b.perm, *vout = b.perm_alt;
+	for(int shift = 0; shift < total_bits; shift += 8) {
+		k_radix_hist<<<nblocks, RADIX_BS, 0, st>>>(N, kin, shift, b.radix_hist, nblocks);
+		k_scan_single<<<1, 1024, 0, st>>>(b.radix_hist, 256 * nblocks);
+		k_radix_scatter<<<nblocks, RADIX_BS, 0, st>>>(N, kin, vin, shift, b.radix_hist, nblocks, kout, vout);
+		std::swap(kin, kout);
+		std::swap(vin, vout);
+	}
+	const int *perm = vin;
+	k_invert<<<g, bs, 0, st>>>(N, perm, b.inv_perm);
+	// gather every per-particle array through the scratch buffer
+	auto permute = [&](auto *arr) {
+		using T = std::remove_pointer_t<decltype(arr)>;
+		T *tmp = (T *) b.scratch;
+		k_gather<T><<<g, bs, 0, st>>>(N, perm, arr, tmp);
+		(void) hipMemcpyAsync(arr, tmp, sizeof(T) * N, hipMemcpyDeviceToDevice, st);
+	};
+	permute(b.pos);
+	permute(b.vel);
+	permute(b.Lm);
+	permute(b.q64);
+	permute(b.q32);
+	permute(b.bsite);
+	permute(b.F);
+	permute(b.T);
+	permute(b.pinfo);
+	permute(b.orig);
+	permute(b.pstep);
+	{
+		int2 *tmp = (int2 *) b.scratch;
+		k_gather_bonds<<<g, bs, 0, st>>>(N, perm, b.inv_perm, b.bonds, tmp);
+		(void) hipMemcpyAsync(b.bonds, tmp, sizeof(int2) * N, hipMemcpyDeviceToDevice, st);
+	}
+	k_mark_rebuild<<<(b.nsys + 63) / 64, 64, 0, st>>>(b.nsys, b.d_state);
+	return 0;
+}
+
+}  // namespace oxh
+
+// key check entry used by the C ABI (no allocation inside: caller buffers)
+extern "C" int oxh_internal_hilbert_probe(int n, const void *d_xyzb, void *d_out, hipStream_t st) {
+	if(n <= 0) return 0;
+	oxh::k_hilbert_probe<<<(n + 255) / 256, 256, 0, st>>>(n, (const uint4 *) d_xyzb, (unsigned int *) d_out);
+	return (int) hipGetLastError();
+}

@@ -1,0 +1,56 @@
+// Types shared by the host side and the gfx950 kernels.
+#pragma once
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// Per-system parameters, resident in HBM (one entry per system of a batch).
+// Everything the kernels need about a system: where its particles live in
+// the batch arrays, its box, its cells, its T-dependent model constants and
+// its integrator / thermostat settings.
+typedef struct oxh_sys {
+	// geometry, FP64 (minimum image and wrapping are done in FP64)
+	double box[3];
+	double inv_box[3];
+	// particle range [first, first + n) in the batch arrays
+	int first, n;
+	// cells: nc[3] per side, [cell_first, cell_first + ncell) in the batch's
+	// cell arrays; use_cells = 0 selects the all-pairs list build
+	int nc[3];
+	int cell_first, ncell, use_cells;
+	// Verlet list: r_verlet^2 and skin^2, FP64
+	double rv2, skin2;
+	// model constants (FP32) that depend on T / salt / options
+	float f1_eps[2][5][5];
+	float f1_shift[2][5][5];
+	float dh_prefactor, dh_minus_kappa, dh_rhigh, dh_rc, dh_b;
+	int dh_half;
+	float hb_multi;
+	int use_mbf;
+	float mbf_xmax, mbf_fmax, mbf_finf;
+	float fene_r0;
+	// integration, FP64
+	double dt, T;
+	int thermostat;       // 0 none, 1 Langevin, 2 Bussi, 3 Brownian
+	int newtonian_steps;
+	double gamma_t, resc_t, gamma_r, resc_r;   // Langevin
+	double pt, pr, resc_brown;                  // Brownian
+	double bussi_c;                             // exp(-newtonian_steps / tau)
+	uint32_t seed_lo, seed_hi;
+	int pad_;
+} oxh_sys;
+
+// per-system device-side state written by the kernels
+typedef struct oxh_sys_state {
+	double K_t, K_r;          // Bussi thermostat internal kinetic energies
+	int need_rebuild;         // skin trigger
+	int error;                // bit 0: FENE bond broken, bit 1: list overflow, bit 2: cell overflow
+	long long n_rebuilds;
+} oxh_sys_state;
+
+#ifdef __cplusplus
+}
+#endif
