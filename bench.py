@@ -1,0 +1,184 @@
+"""Benchmark of the oxDNA2 MD hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain1]
+
+Prints ONE JSON line.  A step is one full MD step of the workload: first
+half kick + drift + rotation, on-device skin check and (when due) Verlet
+rebuild, forces, second half kick, thermostat.
+
+Workloads:
+  plain1   512 random-sequence 24-bp duplexes (24,576 nt), cubic lattice,
+           random orientations, density 1e-3 nt/unit^3, T = 25 C, 0.5 M,
+           skin 0.05, dt 0.003, Bussi (newtonian_steps 10, bussi_tau 100)
+           -- the headline (default)
+  plain2   1,024 random-sequence 96-bp duplexes (196,608 nt), hexagonal
+           lattice of axis spacing 2.6, T = 30 C, 0.2 M, Langevin
+           (diff_coeff 2.5), skin 0.05, dt 0.003
+
+value = nucleotide-steps/s of the whole job, inputs resident in HBM.
+roofline: the force kernel (dominant), algorithmic bytes per launch /
+average launch time (HIP events on the batch's stream).
+cpu_baseline: the reference CPU binary (oracle/_ref/oxDNA_release, built
+from the reference's own sources with its release flags) on a bounded
+sample of the same system, single thread.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(name: str, seed: int = 1):
+    from oxdna_amd import systems
+    if name == "plain1":
+        s = systems.duplex_gas(seed, n_duplex=512, bp=24, density=1e-3, tc=25.0)
+        opts = dict(thermostat="bussi", newtonian_steps=10, bussi_tau=100, dt=0.003, verlet_skin=0.05,
+                    refresh_vel=True, seed=seed)
+        ref_keys = dict(T="25C", salt_concentration="0.5", thermostat="bussi", newtonian_steps="10",
+                        bussi_tau="100", box_type="cubic")
+        desc = ("512 x 24-bp duplexes (24,576 nt), density 1e-3, 25 C, 0.5 M, Bussi, dt 0.003, skin 0.05")
+        return s, opts, ref_keys, desc
+    if name == "plain2":
+        s = systems.duplex_bundle(seed, nx=32, ny=32, bp=96, spacing=2.6, lz=40.0, tc=30.0, salt=0.2)
+        opts = dict(thermostat="langevin", diff_coeff=2.5, dt=0.003, verlet_skin=0.05, refresh_vel=True, seed=seed)
+        ref_keys = dict(T="30C", salt_concentration="0.2", thermostat="langevin", diff_coeff="2.5",
+                        box_type="orthogonal")
+        desc = ("1,024 x 96-bp duplexes (196,608 nt), hexagonal lattice 2.6, 30 C, 0.2 M, Langevin, dt 0.003")
+        return s, opts, ref_keys, desc
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(s, ref_keys, steps_lo: int, steps_hi: int):
+    """Wall time of the reference CPU binary for steps_hi - steps_lo steps
+    (difference of two runs removes start-up), one thread."""
+    from oxdna_amd import io
+    binary = os.path.join(ROOT, "oracle", "_ref", "oxDNA_release")
+    if not os.path.exists(binary):
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        io.write_topology(os.path.join(d, "t.top"), s)
+        io.write_conf(os.path.join(d, "c.dat"), s)
+
+        def run(steps):
+            keys = dict(backend="CPU", sim_type="MD", interaction_type="DNA2", steps=str(steps), seed="1", dt="0.003",
+                        verlet_skin="0.05", refresh_vel="1", topology="t.top", conf_file="c.dat",
+                        trajectory_file="traj.dat", energy_file="energy.dat", print_energy_every="100000000",
+                        print_conf_interval="100000000", no_stdout_energy="1", restart_step_counter="1",
+                        time_scale="linear", log_file="log.dat", lastconf_file="last.dat")
+            keys.update(ref_keys)
+            with open(os.path.join(d, "input"), "w") as f:
+                for k, v in keys.items():
+                    f.write(f"{k} = {v}\n")
+            t0 = time.perf_counter()
+            r = subprocess.run(["taskset", "-c", "0", binary, "input"], cwd=d, capture_output=True, text=True)
+            if r.returncode != 0:
+                r = subprocess.run([binary, "input"], cwd=d, capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError("reference CPU run failed")
+            return time.perf_counter() - t0
+
+        t_lo = run(steps_lo)
+        t_hi = run(steps_hi)
+    dt = t_hi - t_lo
+    n = steps_hi - steps_lo
+    return dict(value=s.N * n / dt, unit="nucleotide-steps/s", cores=1, kind="reference",
+                sample=f"{n} MD steps of the same {s.N}-nt system (DNA2, release flags), "
+                       f"{dt:.1f} s, ms/step {1e3 * dt / n:.2f}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100000)
+    ap.add_argument("--warmup", type=int, default=2000)
+    ap.add_argument("--workload", default="plain1")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=300)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (systems are independent and a "
+              "single system is not decomposed across GPUs in this build)", file=sys.stderr)
+        sys.exit(2)
+
+    from oxdna_amd import _lib, backend
+    if _lib.device_count() < 1:
+        raise SystemExit("no GPU visible")
+    s, opts, ref_keys, desc = workload(args.workload, args.seed)
+    log(f"workload {args.workload}: N={s.N}")
+    t_up = time.perf_counter()
+    b = backend.HIPBatch(s, **opts)
+    log(f"upload + first list/forces {time.perf_counter() - t_up:.2f} s")
+    b.run(args.warmup)
+    b.sync()
+    reb0, _ = b.info()
+    t0 = time.perf_counter()
+    b.run(args.steps)
+    b.sync()
+    t1 = time.perf_counter()
+    reb1, err = b.info()
+    elapsed = t1 - t0
+    value = s.N * args.steps / elapsed
+    st = b.stats()
+    force_ms = b.time_forces(50)
+    E = b.energies()[0]
+    b.close()
+    nbar = st["mean_neigh"]
+    # force kernel, algorithmic bytes per nucleotide: own state read
+    # (pos 32, q32 16, bsite 16, bonds 8, pinfo 8, list length 4), F and T
+    # written (32), one 4-byte list entry per neighbour; neighbour gathers
+    # are counted as cache/LDS-resident
+    b_alg = 116.0 + 4.0 * nbar
+    achieved = s.N * b_alg / (force_ms * 1e-3) / 1e9
+    rebuild_every = args.steps / max(1, reb1 - reb0)
+    log(f"{elapsed:.3f} s for {args.steps} steps; {1e3 * elapsed / args.steps:.4f} ms/step; force kernel "
+        f"{force_ms * 1e3:.1f} us; mean neighbours {nbar:.1f}; rebuild every {rebuild_every:.1f} steps; "
+        f"U/N {E[0] / s.N:.4f} K/N {(E[2] + E[3]) / s.N:.4f}")
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(s, ref_keys, 20, 20 + args.cpu_steps)
+        except Exception as e:  # the GPU figure stands on its own
+            log(f"cpu baseline failed: {e}")
+
+    out = {
+        "metric": "nucleotide-steps/s",
+        "value": value,
+        "unit": "nucleotide-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 forces / f64 state",
+        "data": "synthetic",
+        "config": {"workload": desc, "n_nucleotides": s.N, "form": "plain (one system)", "parallelism": "none",
+                   "mean_neighbours": round(nbar, 2), "steps_per_rebuild": round(rebuild_every, 2)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_forces", "avg_launch_us": force_ms * 1e3, "bytes_per_nt": b_alg,
+                     "lanes_per_nt": st["lanes"], "block": st["block"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -170,10 +170,17 @@ oxh_status oxh_batch_system_info(oxh_batch *b, int sys, long long *n_rebuilds, i
 oxh_status oxh_batch_set_state(oxh_batch *b, int sys, const double *pos, const double *a1, const double *a3, const double *vel,
 		const double *L);
 
-/* Neighbour-list and device-memory figures of the batch:
+/* Neighbour-list, memory and launch-shape figures of the batch:
  * out[0] N, out[1] max_neigh, out[2] mean neighbours per particle,
- * out[3] device bytes held. */
-oxh_status oxh_batch_stats(oxh_batch *b, double *out4);
+ * out[3] device bytes held, out[4] threads per nucleotide and out[5] block
+ * size of the force kernel (picked by timing at upload; the environment
+ * variable OXH_FORCE_LANES fixes the former). */
+oxh_status oxh_batch_stats(oxh_batch *b, double *out6);
+
+/* Average duration in ms of n back-to-back launches of the force kernel on
+ * the batch's stream, timed with HIP events (the reference times its steps
+ * with CPU clocks around device syncs, src/Utilities/Timings.cpp:45-62). */
+oxh_status oxh_batch_time_forces(oxh_batch *b, int n_launches, double *avg_ms);
 
 /* Hilbert keys of n cells, each (x, y, z, bits) as 4 unsigned ints, on
  * the device: the same function the sort uses, for checking. */

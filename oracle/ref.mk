@@ -193,7 +193,14 @@ SRCS := \
 
 OBJS := $(patsubst %.cpp,$(OUT)/obj/%.o,$(SRCS))
 
-all: $(OUT)/oxDNA $(OUT)/ForceDump.so
+# the CPU baseline binary: the reference's Release flags (-O3 -ffast-math,
+# CMakeLists.txt:93-106) with a portable x86-64-v3 target in place of
+# -march=native, so the binary built here runs on the GPU box's host
+RELFLAGS := -std=c++14 -O3 -ffast-math -march=x86-64-v3 -DNOCUDA -DJSON_ENABLED -D_FORCE_INLINES \
+            -DRELEASE='"oracle-release"' -DGIT_COMMIT='"none"' -DBUILD_TIME='"none"' -I$(SRC)/extern -w
+RELOBJS := $(patsubst %.cpp,$(OUT)/relobj/%.o,$(SRCS))
+
+all: $(OUT)/oxDNA $(OUT)/ForceDump.so $(OUT)/oxDNA_release
 
 $(OUT)/obj/%.o: $(SRC)/%.cpp
 	@mkdir -p $(dir $@)
@@ -208,6 +215,17 @@ $(OUT)/obj/oxDNA_main.o: $(SRC)/oxDNA.cpp
 	$(CXX) $(CXXFLAGS) -c $< -o $@
 
 $(OUT)/oxDNA: $(OBJS) $(OUT)/obj/oxDNA_main.o $(OUT)/obj/tinyexpr.o
+	$(CXX) -rdynamic -o $@ $^ -ldl
+
+$(OUT)/relobj/%.o: $(SRC)/%.cpp
+	@mkdir -p $(dir $@)
+	$(CXX) $(RELFLAGS) -c $< -o $@
+
+$(OUT)/relobj/oxDNA_main.o: $(SRC)/oxDNA.cpp
+	@mkdir -p $(dir $@)
+	$(CXX) $(RELFLAGS) -c $< -o $@
+
+$(OUT)/oxDNA_release: $(RELOBJS) $(OUT)/relobj/oxDNA_main.o $(OUT)/obj/tinyexpr.o
 	$(CXX) -rdynamic -o $@ $^ -ldl
 
 # full-precision per-particle force/torque dump, loaded by the reference's

@@ -21,7 +21,7 @@ EXPORTS = [
     "oxh_batch_destroy", "oxh_batch_update_lists", "oxh_batch_compute_forces", "oxh_batch_run",
     "oxh_batch_hilbert_sort", "oxh_batch_sync", "oxh_batch_get_state", "oxh_batch_get_forces", "oxh_batch_energies",
     "oxh_batch_energy_split", "oxh_batch_hb_count", "oxh_batch_get_pairs", "oxh_batch_system_info",
-    "oxh_batch_set_state", "oxh_batch_stats", "oxh_hilbert_keys", "oxh_philox",
+    "oxh_batch_set_state", "oxh_batch_stats", "oxh_batch_time_forces", "oxh_hilbert_keys", "oxh_philox",
 ]
 
 
@@ -71,6 +71,7 @@ def lib():
         L.oxh_batch_system_info.argtypes = [V, I, P, P]
         L.oxh_batch_set_state.argtypes = [V, I, P, P, P, P, P]
         L.oxh_batch_stats.argtypes = [V, P]
+        L.oxh_batch_time_forces.argtypes = [V, I, P]
         L.oxh_hilbert_keys.argtypes = [I, I, P, P]
         L.oxh_philox.argtypes = [I, I, P, P, P]
         for name in EXPORTS:

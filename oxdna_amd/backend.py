@@ -182,10 +182,18 @@ class HIPBatch:
         arrs = [None if a is None else _c(a, np.float64) for a in (pos, a1, a3, vel, L)]
         check(lib().oxh_batch_set_state(self.h, sys, *(ptr(a) for a in arrs)), "oxh_batch_set_state")
 
+    def time_forces(self, n: int = 20) -> float:
+        """Average force-kernel launch duration (ms), HIP events on the
+        batch's stream."""
+        ms = C.c_double(0.0)
+        check(lib().oxh_batch_time_forces(self.h, int(n), C.byref(ms)), "oxh_batch_time_forces")
+        return ms.value
+
     def stats(self):
-        out = np.zeros(4)
+        out = np.zeros(6)
         check(lib().oxh_batch_stats(self.h, ptr(out)), "oxh_batch_stats")
-        return dict(N=int(out[0]), max_neigh=int(out[1]), mean_neigh=out[2], device_bytes=int(out[3]))
+        return dict(N=int(out[0]), max_neigh=int(out[1]), mean_neigh=out[2], device_bytes=int(out[3]),
+                    lanes=int(out[4]), block=int(out[5]))
 
 
 def hilbert_keys(xyz: np.ndarray, bits: int, device: int = 0) -> np.ndarray:

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -316,6 +317,48 @@ oxh_status check_sys(oxh_batch *B, int sys) {
 	HIPCHK(hipSetDevice(B->b.device));
 	return OXH_OK;
 }
+// Picks the force kernel's launch shape (threads per nucleotide, block
+// size) by timing every candidate on this batch's own lists: the best
+// shape depends on N (enough waves to fill 256 CUs) and on the neighbour
+// count (enough lanes to share a particle's list).
+oxh_status autotune_forces(oxh_batch *B) {
+	oxh::Batch &b = B->b;
+	const char *env = getenv("OXH_FORCE_LANES");
+	if(env) {
+		b.lanes = atoi(env);
+		return OXH_OK;
+	}
+	hipEvent_t e0, e1;
+	HIPCHK(hipEventCreate(&e0));
+	HIPCHK(hipEventCreate(&e1));
+	float best = 1e30f;
+	int best_lanes = b.lanes, best_block = b.force_block;
+	const int lanes[] = {1, 2, 4, 8, 16, 32};
+	const int blocks[] = {64, 128, 256};
+	for(int L : lanes)
+		for(int bs : blocks) {
+			b.lanes = L;
+			b.force_block = bs;
+			oxh::launch_forces(b.stream, b, false);
+			HIPCHK(hipEventRecord(e0, b.stream));
+			for(int k = 0; k < 3; k++) oxh::launch_forces(b.stream, b, false);
+			HIPCHK(hipEventRecord(e1, b.stream));
+			HIPCHK(hipEventSynchronize(e1));
+			float ms = 0.f;
+			HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+			if(ms < best) {
+				best = ms;
+				best_lanes = L;
+				best_block = bs;
+			}
+		}
+	(void) hipEventDestroy(e0);
+	(void) hipEventDestroy(e1);
+	b.lanes = best_lanes;
+	b.force_block = best_block;
+	return sync_and_check(B);
+}
+
 }  // namespace
 
 extern "C" {
@@ -357,7 +400,7 @@ oxh_status oxh_batch_destroy(oxh_batch *B) {
 	(void) hipSetDevice(b.device);
 	if(b.stream) (void) hipStreamSynchronize(b.stream);
 	if(b.graph_exec) (void) hipGraphExecDestroy(b.graph_exec);
-	void *ptrs[] = {b.pos, b.vel, b.Lm, b.q64, b.rlist, b.q32, b.bsite, b.F, b.T, b.bonds, b.pinfo, b.orig, b.pstep, b.nbr, b.nnbr,
+	void *ptrs[] = {b.pos, b.vel, b.Lm, b.q64, b.rlist, b.q32, b.bpos, b.F, b.T, b.bonds, b.pinfo, b.orig, b.pstep, b.nbr, b.nnbr,
 			b.cell_count, b.cell_items, b.cell_owner, b.d_sys, b.d_state, b.red_blocks, b.red_sys_blocks, b.red_partial, b.red_out,
 			b.esplit, b.hbcount, b.keys, b.keys_alt, b.perm, b.perm_alt, b.inv_perm, b.radix_hist, b.scratch, B->bk_pos, B->bk_vel,
 			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state};
@@ -428,6 +471,10 @@ oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs,
 		double rv = rcut + 2.0 * d.verlet_skin;
 		S.rv2 = rv * rv;
 		S.skin2 = d.verlet_skin * d.verlet_skin;
+		S.rclose2 = ((double) oxh::R_CLOSE + 2.0 * d.verlet_skin) * ((double) oxh::R_CLOSE + 2.0 * d.verlet_skin);
+		S.small_box = 0;
+		for(int k = 0; k < 3; k++)
+			if(d.box[k] < 2.0 * (rv + 2.0 * 0.4814)) S.small_box = 1;
 		// cells of side >= r_verlet, at most ~2 particles per cell on
 		// average (the reference's cells_auto_optimisation,
 		// src/CUDA/Lists/CUDASimpleVerletList.cu:96-110)
@@ -531,7 +578,7 @@ oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs,
 	TRYB(dalloc(B, &b.q64, N));
 	TRYB(dalloc(B, &b.rlist, N));
 	TRYB(dalloc(B, &b.q32, N));
-	TRYB(dalloc(B, &b.bsite, N));
+	TRYB(dalloc(B, &b.bpos, N));
 	TRYB(dalloc(B, &b.F, N));
 	TRYB(dalloc(B, &b.T, N));
 	TRYB(dalloc(B, &b.bonds, N));
@@ -641,6 +688,8 @@ oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs,
 	}
 	oxh::launch_prepare_mirror(st, b);
 	oxh_status s = build_all(B);
+	if(s != OXH_OK) return bail(s);
+	s = autotune_forces(B);
 	if(s != OXH_OK) return bail(s);
 	// graph chunk: enough steps to amortise the launch, at most ~2k kernels
 	B->chunk = 64;
@@ -849,22 +898,24 @@ oxh_status oxh_batch_hb_count(oxh_batch *B, int *out) {
 oxh_status oxh_batch_get_pairs(oxh_batch *B, int sys, int *pairs, long long max, long long *n_out) {
 	TRY(check_sys(B, sys));
 	int f = B->first[sys], n = B->n[sys];
-	std::vector<int> nn, o, all_orig;
+	const int M = B->b.max_neigh;
+	std::vector<int2> nn;
+	std::vector<int> all_orig;
 	TRY(download(B, nn, B->b.nnbr, f, n));
 	TRY(download(B, all_orig, B->b.orig, 0, B->b.N));
-	std::vector<int> rows((size_t) B->b.max_neigh * n);
-	for(int k = 0; k < B->b.max_neigh; k++)
-		HIPCHK(hipMemcpyAsync(rows.data() + (size_t) k * n, B->b.nbr + (size_t) k * B->b.nstride + f, n * sizeof(int), hipMemcpyDeviceToHost,
-				B->b.stream));
+	std::vector<int> rows((size_t) M * n);
+	HIPCHK(hipMemcpyAsync(rows.data(), B->b.nbr + (size_t) f * M, rows.size() * sizeof(int), hipMemcpyDeviceToHost, B->b.stream));
 	TRY(sync_and_check(B));
 	std::vector<std::pair<int, int>> pr;
 	for(int k = 0; k < n; k++) {
 		int i = all_orig[f + k];
-		for(int m = 0; m < nn[k]; m++) {
-			int j = all_orig[rows[(size_t) m * n + k]];
-			if(i < j) pr.push_back({i, j});
-			else pr.push_back({j, i});
-		}
+		const int *row = rows.data() + (size_t) k * M;
+		auto add = [&](int jb) {
+			int j = all_orig[jb];
+			pr.push_back(i < j ? std::make_pair(i, j) : std::make_pair(j, i));
+		};
+		for(int m = 0; m < nn[k].x; m++) add(row[m]);
+		for(int m = 0; m < nn[k].y; m++) add(row[M - 1 - m]);
 	}
 	std::sort(pr.begin(), pr.end());
 	pr.erase(std::unique(pr.begin(), pr.end()), pr.end());
@@ -919,19 +970,40 @@ oxh_status oxh_batch_set_state(oxh_batch *B, int sys, const double *pos, const d
 	return oxh_batch_compute_forces(B);
 }
 
-oxh_status oxh_batch_stats(oxh_batch *B, double *out4) {
-	if(!B || !out4) return fail(OXH_ERR_ARG, "bad arguments");
+oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {
+	if(!B || !out6) return fail(OXH_ERR_ARG, "bad arguments");
 	HIPCHK(hipSetDevice(B->b.device));
-	std::vector<int> nn;
+	std::vector<int2> nn;
 	TRY(download(B, nn, B->b.nnbr, 0, B->b.N));
 	TRY(sync_and_check(B));
 	double acc = 0;
-	for(int x : nn) acc += x;
-	out4[0] = B->b.N;
-	out4[1] = B->b.max_neigh;
-	out4[2] = B->b.N ? acc / B->b.N : 0.0;
-	out4[3] = (double) B->device_bytes;
+	for(int2 x : nn) acc += x.x + x.y;
+	out6[0] = B->b.N;
+	out6[1] = B->b.max_neigh;
+	out6[2] = B->b.N ? acc / B->b.N : 0.0;
+	out6[3] = (double) B->device_bytes;
+	out6[4] = B->b.lanes;
+	out6[5] = B->b.force_block;
 	return OXH_OK;
+}
+
+oxh_status oxh_batch_time_forces(oxh_batch *B, int n_launches, double *avg_ms) {
+	if(!B || n_launches <= 0 || !avg_ms) return fail(OXH_ERR_ARG, "bad arguments");
+	HIPCHK(hipSetDevice(B->b.device));
+	hipEvent_t e0, e1;
+	HIPCHK(hipEventCreate(&e0));
+	HIPCHK(hipEventCreate(&e1));
+	oxh::launch_forces(B->b.stream, B->b, false);   // warm
+	HIPCHK(hipEventRecord(e0, B->b.stream));
+	for(int k = 0; k < n_launches; k++) oxh::launch_forces(B->b.stream, B->b, false);
+	HIPCHK(hipEventRecord(e1, B->b.stream));
+	HIPCHK(hipEventSynchronize(e1));
+	float ms = 0.f;
+	HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+	(void) hipEventDestroy(e0);
+	(void) hipEventDestroy(e1);
+	*avg_ms = ms / n_launches;
+	return sync_and_check(B);
 }
 
 oxh_status oxh_hilbert_keys(int device, int n, const unsigned int *xyzb, unsigned int *keys) {

@@ -1,10 +1,12 @@
 // oxDNA2 forces and torques on gfx950.
 //
-// One thread per nucleotide, full (both-direction) neighbour list, so every
-// thread owns its force/torque accumulators and no atomics are needed: the
-// summation order is fixed and results are bitwise reproducible.  The
-// nearest-image separation is formed in FP64 from the FP64 positions and
-// rounded to FP32 once; all pair terms are FP32.
+// K threads per nucleotide (K picked by timing at upload) over a full
+// (both-direction) neighbour list split into a close sublist (every term)
+// and a far sublist (Debye-Hueckel only); each nucleotide's K partial sums
+// are combined by a fixed shuffle butterfly, so there are no atomics and
+// results are bitwise reproducible.  Nearest-image separations are formed
+// in FP64 from FP64 positions and rounded to FP32 once; all pair terms are
+// FP32.
 //
 // Terms follow the reference's CPU DNA2_nomesh formulation
 // (src/Interactions/DNAInteraction.cpp:415-1205,
@@ -370,20 +372,36 @@ __device__ __forceinline__ float debye_huckel(const oxh_sys &S, f3 rback, float 
 	return e * cf;
 }
 
-template<bool SPLIT>
+__device__ __forceinline__ f3 min_image_d(double4 a, double4 b, const oxh_sys &S) {
+	return min_image(a, b, S);
+}
+
+template<int K>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+	for(int off = K / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, K);
+	return v;
+}
+
+// K threads per nucleotide: lane l of nucleotide i takes list entries
+// l, l + K, ... of i's close sublist, then of its far sublist, and the n3 /
+// n5 bonded pairs go to lanes 0 / 1.  Partial sums are combined by a fixed
+// butterfly of wave shuffles, so results do not depend on timing.
+template<int K, bool SPLIT>
 __global__ void __launch_bounds__(256) k_forces(int N, const double4 *__restrict__ pos, const float4 *__restrict__ q32,
-		const float4 *__restrict__ bsite, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
-		const int *__restrict__ nbr, const int *__restrict__ nnbr, int nstride, const oxh_sys *__restrict__ systems,
+		const double4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
+		const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh, const oxh_sys *__restrict__ systems,
 		oxh_sys_state *__restrict__ sstate, float4 *__restrict__ Fout, float4 *__restrict__ Tout, float *__restrict__ esplit,
 		int *__restrict__ hbcount) {
-	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	const int i = t / K;
+	const int l = t % K;
 	if(i >= N) return;
 
 	const int2 pi = pinfo[i];
 	const oxh_sys &S = systems[pi.y];
 	const double4 P = pos[i];
 	const Frame fp = make_frame(q32[i], pi.x);
-	const float cf_i = bsite[i].w;
 	const int2 bd = bonds[i];
 
 	f3 F = {0, 0, 0}, T = {0, 0, 0};
@@ -393,7 +411,7 @@ __global__ void __launch_bounds__(256) k_forces(int N, const double4 *__restrict
 	int nhb = 0;
 
 	// bonded neighbours: i is the 5' partner of n3 and the 3' partner of n5
-	if(bd.x >= 0) {
+	if(l == 0 && bd.x >= 0) {
 		int j = bd.x;
 		Frame fq = make_frame(q32[j], pinfo[j].x);
 		f3 r = min_image(P, pos[j], S);
@@ -408,7 +426,7 @@ __global__ void __launch_bounds__(256) k_forces(int N, const double4 *__restrict
 			es[2] += 0.5f * e3[2];
 		}
 	}
-	if(bd.y >= 0) {
+	if(l == (K > 1 ? 1 : 0) && bd.y >= 0) {
 		int j = bd.y;
 		Frame fq = make_frame(q32[j], pinfo[j].x);
 		f3 r = min_image(pos[j], P, S);
@@ -424,23 +442,23 @@ __global__ void __launch_bounds__(256) k_forces(int N, const double4 *__restrict
 		}
 	}
 
-	const int nn = nnbr[i];
-	for(int k = 0; k < nn; k++) {
-		const int j = nbr[(size_t) k * nstride + i];
-		const double4 Pj = pos[j];
-		const f3 r = min_image(P, Pj, S);
-		const float4 bj = bsite[j];
-		f3 fq = {0, 0, 0};
-		// Debye-Hueckel on the backbone sites
-		f3 rback = r + mk(bj.x, bj.y, bj.z) - fp.back;
-		float edh = debye_huckel(S, rback, cf_i * bj.w, fq);
-		f3 tp = -cross(fp.back, fq);
+	const int2 nn = nnbr[i];
+	const int *row = nbr + (size_t) i * max_neigh;
+	const float cf_i = (float) bpos[i].w;
+
+	// close sublist: every term
+	for(int k = l; k < nn.x; k += K) {
+		const int j = row[k];
+		const f3 r = min_image(P, pos[j], S);
+		const Frame fq = make_frame(q32[j], pinfo[j].x);
+		f3 fq_ = {0, 0, 0};
+		float edh = debye_huckel(S, r + fq.back - fp.back, cf_i * (float) bpos[j].w, fq_);
+		f3 tp = -cross(fp.back, fq_);
 		float e = edh;
 		if(SPLIT) es[7] += 0.5f * edh;
 		if(dot(r, r) < R_CLOSE * R_CLOSE) {
-			Frame fqq = make_frame(q32[j], pinfo[j].x);
 			float e4[4];
-			close_pair(S, fp, fqq, r, fq, tp, e4);
+			close_pair(S, fp, fq, r, fq_, tp, e4);
 			e += e4[0] + e4[1] + e4[2] + e4[3];
 			Ehb += 0.5f * e4[1];
 			if(hbcount && e4[1] < -0.1f) nhb++;
@@ -451,33 +469,86 @@ __global__ void __launch_bounds__(256) k_forces(int N, const double4 *__restrict
 				es[6] += 0.5f * e4[3];
 			}
 		}
-		F -= fq;
+		F -= fq_;
 		T += tp;
 		E += 0.5f * e;
 	}
+
+	// far sublist: Debye-Hueckel between backbone sites only
+	const double4 Bi = bpos[i];
+	f3 Ff = {0, 0, 0};
+	float Ef = 0.f;
+	for(int k = l; k < nn.y; k += K) {
+		const int j = row[max_neigh - 1 - k];
+		const double4 Bj = bpos[j];
+		f3 rb;
+		if(!S.small_box) rb = min_image(Bi, Bj, S);
+		else {
+			// the reference's image choice: nearest image of the centres,
+			// then the site offsets (src/Interactions/DNA2Interaction.cpp:169)
+			const double4 Pj = pos[j];
+			rb = min_image(P, Pj, S) + mk((float) (Bj.x - Pj.x), (float) (Bj.y - Pj.y), (float) (Bj.z - Pj.z)) - fp.back;
+		}
+		f3 fq_ = {0, 0, 0};
+		float edh = debye_huckel(S, rb, cf_i * (float) Bj.w, fq_);
+		Ff += fq_;
+		Ef += edh;
+	}
+	F -= Ff;
+	T -= cross(fp.back, Ff);
+	E += 0.5f * Ef;
+	if(SPLIT) es[7] += 0.5f * Ef;
+
+	if(K > 1) {
+		F.x = group_sum<K>(F.x);
+		F.y = group_sum<K>(F.y);
+		F.z = group_sum<K>(F.z);
+		T.x = group_sum<K>(T.x);
+		T.y = group_sum<K>(T.y);
+		T.z = group_sum<K>(T.z);
+		E = group_sum<K>(E);
+		Ehb = group_sum<K>(Ehb);
+		if(SPLIT)
+			for(int q = 0; q < 8; q++) es[q] = group_sum<K>(es[q]);
+		if(hbcount) nhb = (int) group_sum<K>((float) nhb);
+		err = (int) group_sum<K>((float) err);
+	}
+	if(l != 0) return;
 
 	// torque into the body frame: orientationT * T_lab
 	f3 Tb = {dot(fp.a1, T), dot(fp.a2, T), dot(fp.a3, T)};
 	Fout[i] = make_float4(F.x, F.y, F.z, E);
 	Tout[i] = make_float4(Tb.x, Tb.y, Tb.z, Ehb);
 	if(SPLIT) {
-		for(int t = 0; t < 8; t++) esplit[(size_t) t * N + i] = es[t];
+		for(int q = 0; q < 8; q++) esplit[(size_t) q * N + i] = es[q];
 	}
 	if(hbcount) hbcount[i] = nhb;
-	if(err) atomicOr(&sstate[pi.y].error, err);
+	if(err) atomicOr(&sstate[pi.y].error, 1);
+}
+
+template<int K>
+static void launch_k(hipStream_t st, const Batch &b, bool split, int block) {
+	size_t threads = (size_t) b.N * K;
+	dim3 grid((unsigned) ((threads + block - 1) / block));
+	if(split)
+		k_forces<K, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh, b.d_sys,
+				b.d_state, b.F, b.T, b.esplit, b.hbcount);
+	else
+		k_forces<K, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh, b.d_sys,
+				b.d_state, b.F, b.T, nullptr, b.want_hbcount ? b.hbcount : nullptr);
 }
 
 void launch_forces(hipStream_t st, const Batch &b, bool split) {
-	int N = b.N;
-	if(N == 0) return;
-	dim3 block(b.force_block), grid((N + b.force_block - 1) / b.force_block);
-	if(split) {
-		k_forces<true><<<grid, block, 0, st>>>(N, b.pos, b.q32, b.bsite, b.bonds, b.pinfo, b.nbr, b.nnbr, b.nstride, b.d_sys,
-				b.d_state, b.F, b.T, b.esplit, b.hbcount);
-	}
-	else {
-		k_forces<false><<<grid, block, 0, st>>>(N, b.pos, b.q32, b.bsite, b.bonds, b.pinfo, b.nbr, b.nnbr, b.nstride, b.d_sys,
-				b.d_state, b.F, b.T, nullptr, b.want_hbcount ? b.hbcount : nullptr);
+	if(b.N == 0) return;
+	int block = b.force_block;
+	switch(b.lanes) {
+	case 1: launch_k<1>(st, b, split, block); break;
+	case 2: launch_k<2>(st, b, split, block); break;
+	case 4: launch_k<4>(st, b, split, block); break;
+	case 8: launch_k<8>(st, b, split, block); break;
+	case 16: launch_k<16>(st, b, split, block); break;
+	case 32: launch_k<32>(st, b, split, block); break;
+	default: launch_k<64>(st, b, split, block); break;
 	}
 }
 

@@ -37,33 +37,35 @@ __device__ __forceinline__ double4 qmul(double4 a, double4 b) {
 			a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z);
 }
 
-// FP32 mirror of the orientation and the backbone-site offset (+ the
-// Debye-Hueckel charge factor of the particle in .w)
-__device__ __forceinline__ void write_mirror(int i, double4 q, const oxh_sys &S, int2 bd, float4 *q32, float4 *bsite) {
+// FP32 mirror of the orientation, and the FP64 absolute backbone-site
+// position with the particle's Debye-Hueckel charge factor in .w (0.5 at
+// strand ends when dh_half_charged_ends, src/Interactions/DNA2Interaction.cpp:
+// 162-165)
+__device__ __forceinline__ void write_mirror(int i, double4 r, double4 q, const oxh_sys &S, int2 bd, float4 *q32, double4 *bpos) {
 	q32[i] = make_float4((float) q.x, (float) q.y, (float) q.z, (float) q.w);
 	double sqx = q.x * q.x, sqy = q.y * q.y, sqz = q.z * q.z, sqw = q.w * q.w;
 	double xy = q.x * q.y, xz = q.x * q.z, xw = q.x * q.w, yz = q.y * q.z, yw = q.y * q.w, zw = q.z * q.w;
 	double a1x = sqx - sqy - sqz + sqw, a1y = 2. * (xy + zw), a1z = 2. * (xz - yw);
 	double a2x = 2. * (xy - zw), a2y = -sqx + sqy - sqz + sqw, a2z = 2. * (yz + xw);
 	const double b1 = (double) POS_MM_BACK1, b2 = (double) POS_MM_BACK2;
-	float cf = (S.dh_half && (bd.x < 0 || bd.y < 0)) ? 0.5f : 1.f;
-	bsite[i] = make_float4((float) (b1 * a1x + b2 * a2x), (float) (b1 * a1y + b2 * a2y), (float) (b1 * a1z + b2 * a2z), cf);
+	double cf = (S.dh_half && (bd.x < 0 || bd.y < 0)) ? 0.5 : 1.0;
+	bpos[i] = make_double4(r.x + b1 * a1x + b2 * a2x, r.y + b1 * a1y + b2 * a2y, r.z + b1 * a1z + b2 * a2z, cf);
 }
 
-__global__ void k_prepare_mirror(int N, const double4 *__restrict__ q64, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
-		const oxh_sys *__restrict__ systems, float4 *__restrict__ q32, float4 *__restrict__ bsite) {
+__global__ void k_prepare_mirror(int N, const double4 *__restrict__ pos, const double4 *__restrict__ q64, const int2 *__restrict__ bonds,
+		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, float4 *__restrict__ q32, double4 *__restrict__ bpos) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= N) return;
-	write_mirror(i, q64[i], systems[pinfo[i].y], bonds[i], q32, bsite);
+	write_mirror(i, pos[i], q64[i], systems[pinfo[i].y], bonds[i], q32, bpos);
 }
 
 void launch_prepare_mirror(hipStream_t st, const Batch &b) {
 	if(b.N == 0) return;
-	k_prepare_mirror<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.q64, b.bonds, b.pinfo, b.d_sys, b.q32, b.bsite);
+	k_prepare_mirror<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.q64, b.bonds, b.pinfo, b.d_sys, b.q32, b.bpos);
 }
 
 __global__ void __launch_bounds__(256) k_first_step(int N, double4 *__restrict__ pos, double4 *__restrict__ vel, double4 *__restrict__ Lm,
-		double4 *__restrict__ q64, float4 *__restrict__ q32, float4 *__restrict__ bsite, const float4 *__restrict__ F,
+		double4 *__restrict__ q64, float4 *__restrict__ q32, double4 *__restrict__ bpos, const float4 *__restrict__ F,
 		const float4 *__restrict__ T, const double4 *__restrict__ rlist, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
 		long long *__restrict__ pstep, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -104,7 +106,7 @@ __global__ void __launch_bounds__(256) k_first_step(int N, double4 *__restrict__
 		q.w *= qn;
 		q64[i] = q;
 	}
-	write_mirror(i, q, S, bonds[i], q32, bsite);
+	write_mirror(i, r, q, S, bonds[i], q32, bpos);
 	pstep[i] += 1;
 
 	double4 rl = rlist[i];
@@ -113,7 +115,7 @@ __global__ void __launch_bounds__(256) k_first_step(int N, double4 *__restrict__
 }
 
 void launch_first_step(hipStream_t st, const Batch &b) {
-	k_first_step<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bsite, b.F, b.T, b.rlist, b.bonds, b.pinfo,
+	k_first_step<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.F, b.T, b.rlist, b.bonds, b.pinfo,
 			b.pstep, b.d_sys, b.d_state);
 }
 

@@ -21,15 +21,20 @@ struct Batch {
 
 	// per-particle state (FP64 integration state, FP32 kernel mirror)
 	double4 *pos = nullptr, *vel = nullptr, *Lm = nullptr, *q64 = nullptr, *rlist = nullptr;
-	float4 *q32 = nullptr, *bsite = nullptr, *F = nullptr, *T = nullptr;
+	float4 *q32 = nullptr, *F = nullptr, *T = nullptr;
+	double4 *bpos = nullptr;  // absolute backbone-site position, .w = Debye-Hueckel charge factor
 	int2 *bonds = nullptr;   // (n3, n5) batch indices, -1 = none
 	int2 *pinfo = nullptr;   // (btype, system)
 	int *orig = nullptr;     // index of the particle inside its system as uploaded
 	long long *pstep = nullptr;  // per-particle step counter (RNG counter)
 
-	// neighbour list, column major: nbr[k * nstride + i]
-	int *nbr = nullptr, *nnbr = nullptr;
+	// neighbour list, row major: particle i owns nbr[i * max_neigh + k];
+	// close pairs (may need every term) fill it from the front, far pairs
+	// (Debye-Hueckel only) from the back; nnbr[i] = (n_close, n_far)
+	int *nbr = nullptr;
+	int2 *nnbr = nullptr;
 	int nstride = 0, max_neigh = 0;
+	int lanes = 8;             // threads per nucleotide in the force kernel
 
 	// cells (systems with use_cells), fixed capacity per cell
 	int ncell_total = 0, cell_cap = 0;

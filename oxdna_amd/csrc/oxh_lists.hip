@@ -92,10 +92,14 @@ __device__ __forceinline__ double sqr_dist(double4 a, double4 b, const oxh_sys &
 	return fma(dz, dz, fma(dy, dy, dx * dx));
 }
 
+// One thread per particle writes its own row: pairs closer than
+// sqrt(rclose2) at build time from the front, the rest from the back.  A
+// pair cannot come closer than R_CLOSE before the next rebuild unless it was
+// within R_CLOSE + 2 skin now, so far pairs only ever need Debye-Hueckel.
 __global__ void __launch_bounds__(256) k_build_list(int N, const double4 *__restrict__ pos, const int2 *__restrict__ bonds,
 		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st,
-		const int *__restrict__ count, const int *__restrict__ items, int cap, int *__restrict__ nbr, int *__restrict__ nnbr,
-		int nstride, int max_neigh, double4 *__restrict__ rlist, int force_all) {
+		const int *__restrict__ count, const int *__restrict__ items, int cap, int *__restrict__ nbr, int2 *__restrict__ nnbr,
+		int max_neigh, double4 *__restrict__ rlist, int force_all) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= N) return;
 	int s = pinfo[i].y;
@@ -103,12 +107,18 @@ __global__ void __launch_bounds__(256) k_build_list(int N, const double4 *__rest
 	const oxh_sys &S = systems[s];
 	const double4 p = pos[i];
 	const int2 bd = bonds[i];
-	int n = 0;
+	int *row = nbr + (size_t) i * max_neigh;
+	int nc = 0, nf = 0;
 	auto consider = [&](int j) {
 		if(j == i || j == bd.x || j == bd.y) return;
-		if(sqr_dist(p, pos[j], S) < S.rv2) {
-			if(n < max_neigh) nbr[(size_t) n * nstride + i] = j;
-			n++;
+		double r2 = sqr_dist(p, pos[j], S);
+		if(r2 < S.rv2) {
+			if(nc + nf < max_neigh) {
+				if(r2 < S.rclose2) row[nc] = j;
+				else row[max_neigh - 1 - nf] = j;
+			}
+			if(r2 < S.rclose2) nc++;
+			else nf++;
 		}
 	};
 	if(S.use_cells) {
@@ -135,18 +145,18 @@ __global__ void __launch_bounds__(256) k_build_list(int N, const double4 *__rest
 	else {
 		for(int j = S.first; j < S.first + S.n; j++) consider(j);
 	}
-	if(n > max_neigh) {
+	if(nc + nf > max_neigh) {
 		atomicOr(&st[s].error, 2);
-		n = max_neigh;
+		nc = nf = 0;
 	}
-	nnbr[i] = n;
+	nnbr[i] = make_int2(nc, nf);
 	rlist[i] = p;
 }
 
 void launch_build_list(hipStream_t st, const Batch &b, int force_all) {
 	int bs = 256;
 	k_build_list<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, b.pos, b.bonds, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_items,
-			b.cell_cap, b.nbr, b.nnbr, b.nstride, b.max_neigh, b.rlist, force_all);
+			b.cell_cap, b.nbr, b.nnbr, b.max_neigh, b.rlist, force_all);
 }
 
 // ------------------------------------------------------------ Hilbert sort
@@ -346,8 +356,6 @@ int launch_hilbert_sort(hipStream_t st, Batch &b) {
 	permute(b.vel);
 	permute(b.Lm);
 	permute(b.q64);
-	permute(b.q32);
-	permute(b.bsite);
 	permute(b.F);
 	permute(b.T);
 	permute(b.pinfo);
@@ -358,6 +366,7 @@ int launch_hilbert_sort(hipStream_t st, Batch &b) {
 		k_gather_bonds<<<g, bs, 0, st>>>(N, perm, b.inv_perm, b.bonds, tmp);
 		(void) hipMemcpyAsync(b.bonds, tmp, sizeof(int2) * N, hipMemcpyDeviceToDevice, st);
 	}
+	launch_prepare_mirror(st, b);
 	k_mark_rebuild<<<(b.nsys + 63) / 64, 64, 0, st>>>(b.nsys, b.d_state);
 	return 0;
 }

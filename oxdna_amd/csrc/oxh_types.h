@@ -21,8 +21,13 @@ typedef struct oxh_sys {
 	// cell arrays; use_cells = 0 selects the all-pairs list build
 	int nc[3];
 	int cell_first, ncell, use_cells;
-	// Verlet list: r_verlet^2 and skin^2, FP64
-	double rv2, skin2;
+	// 1 when a box side is shorter than 2 (r_verlet + 2 backbone arms): the
+	// nearest image of two backbone sites may then differ from the image the
+	// centres select, so far pairs take the centres' image
+	int small_box;
+	// Verlet list: r_verlet^2, skin^2 and the build-time bound below which a
+	// pair goes to the close sublist, (R_CLOSE + 2 skin)^2, FP64
+	double rv2, skin2, rclose2;
 	// model constants (FP32) that depend on T / salt / options
 	float f1_eps[2][5][5];
 	float f1_shift[2][5][5];
