@@ -109,6 +109,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--sort-every", type=int, default=500, help="Hilbert re-sort interval in steps (0 = never)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (systems are independent and a "
@@ -122,6 +123,7 @@ def main():
     log(f"workload {args.workload}: N={s.N}")
     t_up = time.perf_counter()
     b = backend.HIPBatch(s, **opts)
+    b.set_sort_interval(args.sort_every)
     log(f"upload + first list/forces {time.perf_counter() - t_up:.2f} s")
     b.run(args.warmup)
     b.sync()
@@ -170,11 +172,13 @@ def main():
         "dtype": "f32 forces / f64 state",
         "data": "synthetic",
         "config": {"workload": desc, "n_nucleotides": s.N, "form": "plain (one system)", "parallelism": "none",
+                   "hilbert_sort_every_steps": args.sort_every,
                    "mean_neighbours": round(nbar, 2), "steps_per_rebuild": round(rebuild_every, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_forces", "avg_launch_us": force_ms * 1e3, "bytes_per_nt": b_alg,
-                     "lanes_per_nt": st["lanes"], "block": st["block"]},
+                     "lanes_per_nt": st["lanes"], "block": st["block"], "far_lanes_per_nt": st["far_lanes"],
+                     "far_block": st["far_block"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -128,6 +128,13 @@ oxh_status oxh_batch_run(oxh_batch *b, long long n_steps);
  * with CUDA_sort_every).  Downloads keep the original particle order. */
 oxh_status oxh_batch_hilbert_sort(oxh_batch *b);
 
+/* Re-sorts along the Hilbert curve every `steps` MD steps (0 = never),
+ * starting before the next step; the reference's CUDA_sort_every
+ * (src/CUDA/Backends/MD_CUDABackend.cu:575-577) counts list updates, this
+ * counts steps so that the decision stays on the host between graph
+ * chunks. */
+oxh_status oxh_batch_set_sort_interval(oxh_batch *b, long long steps);
+
 /* Blocks until all queued work of the batch has finished. */
 oxh_status oxh_batch_sync(oxh_batch *b);
 
@@ -170,12 +177,13 @@ oxh_status oxh_batch_system_info(oxh_batch *b, int sys, long long *n_rebuilds, i
 oxh_status oxh_batch_set_state(oxh_batch *b, int sys, const double *pos, const double *a1, const double *a3, const double *vel,
 		const double *L);
 
-/* Neighbour-list, memory and launch-shape figures of the batch:
+/* Neighbour-list, memory and launch-shape figures of the batch (8 doubles):
  * out[0] N, out[1] max_neigh, out[2] mean neighbours per particle,
  * out[3] device bytes held, out[4] threads per nucleotide and out[5] block
- * size of the force kernel (picked by timing at upload; the environment
- * variable OXH_FORCE_LANES fixes the former). */
-oxh_status oxh_batch_stats(oxh_batch *b, double *out6);
+ * size of the close-pair force kernel, out[6] threads per nucleotide (0 =
+ * inline) and out[7] block size of the far-pair kernel.  Shapes are picked
+ * by timing at upload; OXH_FORCE_LANES / OXH_FAR_LANES fix them. */
+oxh_status oxh_batch_stats(oxh_batch *b, double *out8);
 
 /* Average duration in ms of n back-to-back launches of the force kernel on
  * the batch's stream, timed with HIP events (the reference times its steps

@@ -19,7 +19,7 @@ OXH_THERMOSTAT = {"no": 0, "langevin": 1, "bussi": 2, "brownian": 3, "john": 3}
 EXPORTS = [
     "oxh_system_desc_defaults", "oxh_last_error", "oxh_version", "oxh_device_count", "oxh_batch_create",
     "oxh_batch_destroy", "oxh_batch_update_lists", "oxh_batch_compute_forces", "oxh_batch_run",
-    "oxh_batch_hilbert_sort", "oxh_batch_sync", "oxh_batch_get_state", "oxh_batch_get_forces", "oxh_batch_energies",
+    "oxh_batch_hilbert_sort", "oxh_batch_set_sort_interval", "oxh_batch_sync", "oxh_batch_get_state", "oxh_batch_get_forces", "oxh_batch_energies",
     "oxh_batch_energy_split", "oxh_batch_hb_count", "oxh_batch_get_pairs", "oxh_batch_system_info",
     "oxh_batch_set_state", "oxh_batch_stats", "oxh_batch_time_forces", "oxh_hilbert_keys", "oxh_philox",
 ]
@@ -62,6 +62,7 @@ def lib():
         L.oxh_batch_run.argtypes = [V, LL]
         L.oxh_batch_hilbert_sort.argtypes = [V]
         L.oxh_batch_sync.argtypes = [V]
+        L.oxh_batch_set_sort_interval.argtypes = [V, LL]
         L.oxh_batch_get_state.argtypes = [V, I, P, P, P, P, P]
         L.oxh_batch_get_forces.argtypes = [V, I, P, P, P]
         L.oxh_batch_energies.argtypes = [V, P]

@@ -127,6 +127,10 @@ class HIPBatch:
     def hilbert_sort(self):
         check(lib().oxh_batch_hilbert_sort(self.h), "oxh_batch_hilbert_sort")
 
+    def set_sort_interval(self, steps: int):
+        """Hilbert re-sort every `steps` MD steps (0 = never)."""
+        check(lib().oxh_batch_set_sort_interval(self.h, int(steps)), "oxh_batch_set_sort_interval")
+
     def sync(self):
         check(lib().oxh_batch_sync(self.h), "oxh_batch_sync")
 
@@ -190,10 +194,10 @@ class HIPBatch:
         return ms.value
 
     def stats(self):
-        out = np.zeros(6)
+        out = np.zeros(8)
         check(lib().oxh_batch_stats(self.h, ptr(out)), "oxh_batch_stats")
         return dict(N=int(out[0]), max_neigh=int(out[1]), mean_neigh=out[2], device_bytes=int(out[3]),
-                    lanes=int(out[4]), block=int(out[5]))
+                    lanes=int(out[4]), block=int(out[5]), far_lanes=int(out[6]), far_block=int(out[7]))
 
 
 def hilbert_keys(xyz: np.ndarray, bits: int, device: int = 0) -> np.ndarray:

@@ -32,6 +32,12 @@ struct oxh_batch {
 	int bussi_ns = 0;
 	bool any_bussi = false;
 	int chunk = 0;
+	// HIP graph replay of whole chunks (OXH_GRAPH=0 launches the same
+	// kernels one by one, e.g. under a profiler that does not follow graphs)
+	bool use_graph = true;
+	// Hilbert re-sort every sort_interval steps (0 = never), checked at
+	// chunk boundaries
+	long long sort_interval = 0, since_sort = 0;
 };
 
 namespace {
@@ -211,8 +217,8 @@ void enqueue_lists(oxh_batch *B, bool all) {
 
 void enqueue_step(oxh_batch *B, long long step_index) {
 	oxh::Batch &b = B->b;
-	oxh::launch_first_step(b.stream, b);
-	oxh::launch_cells(b.stream, b);
+	oxh::launch_first_step(b.stream, b);   // also zeroes the cell counters
+	oxh::launch_cells_fill_sort(b.stream, b);
 	oxh::launch_build_list(b.stream, b, 0);
 	oxh::launch_forces(b.stream, b, false);
 	oxh::launch_second_step(b.stream, b);
@@ -317,45 +323,69 @@ oxh_status check_sys(oxh_batch *B, int sys) {
 	HIPCHK(hipSetDevice(B->b.device));
 	return OXH_OK;
 }
-// Picks the force kernel's launch shape (threads per nucleotide, block
-// size) by timing every candidate on this batch's own lists: the best
-// shape depends on N (enough waves to fill 256 CUs) and on the neighbour
-// count (enough lanes to share a particle's list).
+// Picks the force kernels' launch shapes by timing every candidate on this
+// batch's own lists: threads per nucleotide and block size of the close
+// (+ bonded) kernel, and whether the far pairs run inline or in their own
+// low-register kernel (and its shape).  The best shapes depend on N (waves
+// to fill 256 CUs) and on the close / far neighbour counts.
 oxh_status autotune_forces(oxh_batch *B) {
 	oxh::Batch &b = B->b;
-	const char *env = getenv("OXH_FORCE_LANES");
-	if(env) {
+	if(const char *env = getenv("OXH_FORCE_LANES")) {
 		b.lanes = atoi(env);
+		if(const char *f = getenv("OXH_FAR_LANES")) b.far_lanes = atoi(f);
 		return OXH_OK;
 	}
 	hipEvent_t e0, e1;
 	HIPCHK(hipEventCreate(&e0));
 	HIPCHK(hipEventCreate(&e1));
-	float best = 1e30f;
-	int best_lanes = b.lanes, best_block = b.force_block;
+	auto timeit = [&](float &ms) -> oxh_status {
+		oxh::launch_forces(b.stream, b, false);
+		HIPCHK(hipEventRecord(e0, b.stream));
+		for(int k = 0; k < 3; k++) oxh::launch_forces(b.stream, b, false);
+		HIPCHK(hipEventRecord(e1, b.stream));
+		HIPCHK(hipEventSynchronize(e1));
+		HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+		return OXH_OK;
+	};
 	const int lanes[] = {1, 2, 4, 8, 16, 32};
 	const int blocks[] = {64, 128, 256};
+	float best = 1e30f;
+	int bl = b.lanes, bb = b.force_block, bfl = 0, bfb = 256;
+	// 1) close kernel shape with the far pairs inline
+	b.far_lanes = 0;
 	for(int L : lanes)
 		for(int bs : blocks) {
 			b.lanes = L;
 			b.force_block = bs;
-			oxh::launch_forces(b.stream, b, false);
-			HIPCHK(hipEventRecord(e0, b.stream));
-			for(int k = 0; k < 3; k++) oxh::launch_forces(b.stream, b, false);
-			HIPCHK(hipEventRecord(e1, b.stream));
-			HIPCHK(hipEventSynchronize(e1));
-			float ms = 0.f;
-			HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+			float ms;
+			TRY(timeit(ms));
 			if(ms < best) {
 				best = ms;
-				best_lanes = L;
-				best_block = bs;
+				bl = L;
+				bb = bs;
+			}
+		}
+	// 2) far pairs in their own kernel, close kernel at its best shape
+	b.lanes = bl;
+	b.force_block = bb;
+	for(int L : lanes)
+		for(int bs : blocks) {
+			b.far_lanes = L;
+			b.far_block = bs;
+			float ms;
+			TRY(timeit(ms));
+			if(ms < best) {
+				best = ms;
+				bfl = L;
+				bfb = bs;
 			}
 		}
 	(void) hipEventDestroy(e0);
 	(void) hipEventDestroy(e1);
-	b.lanes = best_lanes;
-	b.force_block = best_block;
+	b.far_lanes = bfl;
+	b.far_block = bfb;
+	// forces of the current state with the chosen shapes
+	oxh::launch_forces(b.stream, b, false);
 	return sync_and_check(B);
 }
 
@@ -401,7 +431,7 @@ oxh_status oxh_batch_destroy(oxh_batch *B) {
 	if(b.stream) (void) hipStreamSynchronize(b.stream);
 	if(b.graph_exec) (void) hipGraphExecDestroy(b.graph_exec);
 	void *ptrs[] = {b.pos, b.vel, b.Lm, b.q64, b.rlist, b.q32, b.bpos, b.F, b.T, b.bonds, b.pinfo, b.orig, b.pstep, b.nbr, b.nnbr,
-			b.cell_count, b.cell_items, b.cell_owner, b.d_sys, b.d_state, b.red_blocks, b.red_sys_blocks, b.red_partial, b.red_out,
+			b.cell_count, b.cell_items, b.cell_owner, b.pos32, b.d_sys, b.d_state, b.red_blocks, b.red_sys_blocks, b.red_partial, b.red_out, b.red_done,
 			b.esplit, b.hbcount, b.keys, b.keys_alt, b.perm, b.perm_alt, b.inv_perm, b.radix_hist, b.scratch, B->bk_pos, B->bk_vel,
 			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state};
 	for(void *p : ptrs)
@@ -463,6 +493,8 @@ oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs,
 		for(int k = 0; k < 3; k++) {
 			S.box[k] = d.box[k];
 			S.inv_box[k] = 1.0 / d.box[k];
+			S.boxf[k] = (float) d.box[k];
+			S.inv_boxf[k] = (float) (1.0 / d.box[k]);
 		}
 		S.first = N;
 		S.n = d.N;
@@ -472,6 +504,15 @@ oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs,
 		S.rv2 = rv * rv;
 		S.skin2 = d.verlet_skin * d.verlet_skin;
 		S.rclose2 = ((double) oxh::R_CLOSE + 2.0 * d.verlet_skin) * ((double) oxh::R_CLOSE + 2.0 * d.verlet_skin);
+		{
+			double lmax = std::max(d.box[0], std::max(d.box[1], d.box[2]));
+			// |d r| <= 4 L 2^-24 per component after wrapping, rounding and the
+			// nearest image; |d r^2| <= 2 |r| |d r| sqrt(3) + rounding of r^2
+			double dr = 4.0 * lmax * std::ldexp(1.0, -24);
+			S.eps2 = (float) (4.0 * (2.0 * rv * dr * std::sqrt(3.0) + 3.0 * dr * dr + 8.0 * rv * rv * std::ldexp(1.0, -24)));
+			S.rv2f = (float) S.rv2;
+			S.rclose2f = (float) S.rclose2;
+		}
 		S.small_box = 0;
 		for(int k = 0; k < 3; k++)
 			if(d.box[k] < 2.0 * (rv + 2.0 * 0.4814)) S.small_box = 1;
@@ -589,6 +630,7 @@ oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs,
 	TRYB(dalloc(B, &b.nbr, (size_t) N * b.max_neigh));
 	TRYB(dalloc(B, &b.cell_count, std::max(ncell, 1)));
 	TRYB(dalloc(B, &b.cell_owner, std::max(ncell, 1)));
+	TRYB(dalloc(B, &b.pos32, N));
 	TRYB(dalloc(B, &b.cell_items, (size_t) std::max(ncell, 1) * b.cell_cap));
 	TRYB(dalloc(B, &b.d_sys, n_sys));
 	TRYB(dalloc(B, &b.d_state, n_sys));
@@ -625,6 +667,7 @@ oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs,
 	TRYB(dalloc(B, &b.red_sys_blocks, n_sys));
 	TRYB(dalloc(B, &b.red_partial, (size_t) b.nred * oxh::RED_W));
 	TRYB(dalloc(B, &b.red_out, (size_t) n_sys * oxh::RED_W));
+	TRYB(dalloc(B, &b.red_done, n_sys));
 
 	// ---- host staging
 	std::vector<double4> hpos(N), hvel(N), hL(N), hq(N);
@@ -663,6 +706,8 @@ oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs,
 			up(b.red_blocks, rb.data(), rb.size() * sizeof(int3)) && up(b.red_sys_blocks, rsb.data(), n_sys * sizeof(int2));
 	if(!ok) return bail(fail(OXH_ERR_HIP, "upload failed"));
 	if(hipMemsetAsync(b.pstep, 0, N * sizeof(long long), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
+	if(hipMemsetAsync(b.red_done, 0, n_sys * sizeof(int), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
+	if(hipMemsetAsync(b.cell_count, 0, std::max(ncell, 1) * sizeof(int), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
 	if(hipMemsetAsync(b.F, 0, N * sizeof(float4), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
 	if(hipMemsetAsync(b.T, 0, N * sizeof(float4), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
 
@@ -693,6 +738,7 @@ oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs,
 	if(s != OXH_OK) return bail(s);
 	// graph chunk: enough steps to amortise the launch, at most ~2k kernels
 	B->chunk = 64;
+	if(const char *g = getenv("OXH_GRAPH")) B->use_graph = atoi(g) != 0;
 	if(B->any_bussi && B->bussi_aligned) B->chunk = std::max(B->bussi_ns, (64 / B->bussi_ns) * B->bussi_ns);
 	*out = B;
 	return OXH_OK;
@@ -731,10 +777,15 @@ oxh_status oxh_batch_run(oxh_batch *B, long long n_steps) {
 	long long done = 0;
 	int retries = 0;
 	while(done < n_steps) {
+		if(B->sort_interval > 0 && B->since_sort >= B->sort_interval) {
+			if(oxh::launch_hilbert_sort(b.stream, b) != 0) return fail(OXH_ERR_ARG, "too many systems for the sort key");
+			TRY(build_all(B));
+			B->since_sort = 0;
+		}
 		long long todo = std::min<long long>(B->chunk, n_steps - done);
 		bool aligned = !B->any_bussi || !B->bussi_aligned || (b.step % B->bussi_ns == 0);
 		TRY(backup(B));
-		if(todo == B->chunk && aligned) {
+		if(todo == B->chunk && aligned && B->use_graph) {
 			if(!b.graph_exec) {
 				hipGraph_t g;
 				HIPCHK(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
@@ -765,6 +816,7 @@ oxh_status oxh_batch_run(oxh_batch *B, long long n_steps) {
 		}
 		b.step += todo;
 		done += todo;
+		B->since_sort += todo;
 		if(bad >= 0) return fail(OXH_ERR_SYSTEM, "system %d: a bonded pair exceeded the FENE range", bad);
 	}
 	return OXH_OK;
@@ -776,6 +828,13 @@ oxh_status oxh_batch_hilbert_sort(oxh_batch *B) {
 	if(oxh::launch_hilbert_sort(B->b.stream, B->b) != 0) return fail(OXH_ERR_ARG, "too many systems for the sort key");
 	TRY(sync_and_check(B));
 	return oxh_batch_update_lists(B, 1);
+}
+
+oxh_status oxh_batch_set_sort_interval(oxh_batch *B, long long steps) {
+	if(!B || steps < 0) return fail(OXH_ERR_ARG, "bad arguments");
+	B->sort_interval = steps;
+	B->since_sort = steps;   // sort before the next chunk
+	return OXH_OK;
 }
 
 oxh_status oxh_batch_sync(oxh_batch *B) {
@@ -970,7 +1029,7 @@ oxh_status oxh_batch_set_state(oxh_batch *B, int sys, const double *pos, const d
 	return oxh_batch_compute_forces(B);
 }
 
-oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {
+oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {  // 8 doubles
 	if(!B || !out6) return fail(OXH_ERR_ARG, "bad arguments");
 	HIPCHK(hipSetDevice(B->b.device));
 	std::vector<int2> nn;
@@ -984,6 +1043,8 @@ oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {
 	out6[3] = (double) B->device_bytes;
 	out6[4] = B->b.lanes;
 	out6[5] = B->b.force_block;
+	out6[6] = B->b.far_lanes;
+	out6[7] = B->b.far_block;
 	return OXH_OK;
 }
 

@@ -372,8 +372,13 @@ __device__ __forceinline__ float debye_huckel(const oxh_sys &S, f3 rback, float 
 	return e * cf;
 }
 
-__device__ __forceinline__ f3 min_image_d(double4 a, double4 b, const oxh_sys &S) {
-	return min_image(a, b, S);
+// FP32 nearest image of two wrapped backbone sites (far pairs only)
+__device__ __forceinline__ f3 min_image_f(float4 a, float4 b, const oxh_sys &S) {
+	float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+	dx -= S.boxf[0] * rintf(dx * S.inv_boxf[0]);
+	dy -= S.boxf[1] * rintf(dy * S.inv_boxf[1]);
+	dz -= S.boxf[2] * rintf(dz * S.inv_boxf[2]);
+	return mk(dx, dy, dz);
 }
 
 template<int K>
@@ -387,9 +392,9 @@ __device__ __forceinline__ float group_sum(float v) {
 // l, l + K, ... of i's close sublist, then of its far sublist, and the n3 /
 // n5 bonded pairs go to lanes 0 / 1.  Partial sums are combined by a fixed
 // butterfly of wave shuffles, so results do not depend on timing.
-template<int K, bool SPLIT>
+template<int K, bool SPLIT, bool FAR>
 __global__ void __launch_bounds__(256) k_forces(int N, const double4 *__restrict__ pos, const float4 *__restrict__ q32,
-		const double4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
+		const float4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
 		const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh, const oxh_sys *__restrict__ systems,
 		oxh_sys_state *__restrict__ sstate, float4 *__restrict__ Fout, float4 *__restrict__ Tout, float *__restrict__ esplit,
 		int *__restrict__ hbcount) {
@@ -444,7 +449,7 @@ __global__ void __launch_bounds__(256) k_forces(int N, const double4 *__restrict
 
 	const int2 nn = nnbr[i];
 	const int *row = nbr + (size_t) i * max_neigh;
-	const float cf_i = (float) bpos[i].w;
+	const float cf_i = bpos[i].w;
 
 	// close sublist: every term
 	for(int k = l; k < nn.x; k += K) {
@@ -452,7 +457,7 @@ __global__ void __launch_bounds__(256) k_forces(int N, const double4 *__restrict
 		const f3 r = min_image(P, pos[j], S);
 		const Frame fq = make_frame(q32[j], pinfo[j].x);
 		f3 fq_ = {0, 0, 0};
-		float edh = debye_huckel(S, r + fq.back - fp.back, cf_i * (float) bpos[j].w, fq_);
+		float edh = debye_huckel(S, r + fq.back - fp.back, cf_i * bpos[j].w, fq_);
 		f3 tp = -cross(fp.back, fq_);
 		float e = edh;
 		if(SPLIT) es[7] += 0.5f * edh;
@@ -474,23 +479,24 @@ __global__ void __launch_bounds__(256) k_forces(int N, const double4 *__restrict
 		E += 0.5f * e;
 	}
 
-	// far sublist: Debye-Hueckel between backbone sites only
-	const double4 Bi = bpos[i];
+	// far sublist: Debye-Hueckel between backbone sites only (or left to
+	// k_forces_far when FAR is false)
+	const float4 Bi = bpos[i];
 	f3 Ff = {0, 0, 0};
 	float Ef = 0.f;
-	for(int k = l; k < nn.y; k += K) {
+	for(int k = l; FAR && k < nn.y; k += K) {
 		const int j = row[max_neigh - 1 - k];
-		const double4 Bj = bpos[j];
+		const float4 Bj = bpos[j];
 		f3 rb;
-		if(!S.small_box) rb = min_image(Bi, Bj, S);
+		if(!S.small_box) rb = min_image_f(Bi, Bj, S);
 		else {
 			// the reference's image choice: nearest image of the centres,
 			// then the site offsets (src/Interactions/DNA2Interaction.cpp:169)
-			const double4 Pj = pos[j];
-			rb = min_image(P, Pj, S) + mk((float) (Bj.x - Pj.x), (float) (Bj.y - Pj.y), (float) (Bj.z - Pj.z)) - fp.back;
+			const Frame fq = make_frame(q32[j], 0);
+			rb = min_image(P, pos[j], S) + fq.back - fp.back;
 		}
 		f3 fq_ = {0, 0, 0};
-		float edh = debye_huckel(S, rb, cf_i * (float) Bj.w, fq_);
+		float edh = debye_huckel(S, rb, cf_i * Bj.w, fq_);
 		Ff += fq_;
 		Ef += edh;
 	}
@@ -526,29 +532,126 @@ __global__ void __launch_bounds__(256) k_forces(int N, const double4 *__restrict
 	if(err) atomicOr(&sstate[pi.y].error, 1);
 }
 
+// Far pairs on their own: Debye-Hueckel between backbone sites, few
+// registers, so many more waves stay resident to hide the gather latency.
+// Adds onto the F / T written by k_forces (same thread owns the
+// particle's outputs in both kernels, fixed order, no atomics).  Particle
+// i receives -Ff at its backbone site, i.e. the torque back x (-Ff); in the
+// body frame back = (b1, b2, 0), so with Ff_body = (f1, f2, f3) that torque
+// is (-b2 f3, b1 f3, b2 f1 - b1 f2).
+template<int K, bool SPLIT>
+__global__ void __launch_bounds__(256) k_forces_far(int N, const double4 *__restrict__ pos, const float4 *__restrict__ q32,
+		const float4 *__restrict__ bpos, const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh,
+		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, float4 *__restrict__ Fio, float4 *__restrict__ Tio,
+		float *__restrict__ esplit) {
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	const int i = t / K;
+	const int l = t % K;
+	if(i >= N) return;
+	const oxh_sys &S = systems[pinfo[i].y];
+	const int nf = nnbr[i].y;
+	const int *row = nbr + (size_t) i * max_neigh + max_neigh - 1;
+	const float4 Bi = bpos[i];
+	f3 Ff = {0, 0, 0};
+	float Ef = 0.f;
+	Frame fp;
+	if(S.small_box) fp = make_frame(q32[i], 0);
+	for(int k = l; k < nf; k += K) {
+		const int j = row[-k];
+		const float4 Bj = bpos[j];
+		f3 rb;
+		if(!S.small_box) rb = min_image_f(Bi, Bj, S);
+		else {
+			const Frame fq = make_frame(q32[j], 0);
+			rb = min_image(pos[i], pos[j], S) + fq.back - fp.back;
+		}
+		f3 fq_ = {0, 0, 0};
+		Ef += debye_huckel(S, rb, Bi.w * Bj.w, fq_);
+		Ff += fq_;
+	}
+	if(K > 1) {
+		Ff.x = group_sum<K>(Ff.x);
+		Ff.y = group_sum<K>(Ff.y);
+		Ff.z = group_sum<K>(Ff.z);
+		Ef = group_sum<K>(Ef);
+	}
+	if(l != 0) return;
+	f3 a1, a2, a3;
+	quat_axes(q32[i], a1, a2, a3);
+	const float f1b = dot(a1, Ff), f2b = dot(a2, Ff), f3b = dot(a3, Ff);
+	float4 F = Fio[i], T = Tio[i];
+	F.x -= Ff.x;
+	F.y -= Ff.y;
+	F.z -= Ff.z;
+	F.w += 0.5f * Ef;
+	T.x += -POS_MM_BACK2 * f3b;
+	T.y += POS_MM_BACK1 * f3b;
+	T.z += POS_MM_BACK2 * f1b - POS_MM_BACK1 * f2b;
+	Fio[i] = F;
+	Tio[i] = T;
+	if(SPLIT) esplit[(size_t) 7 * N + i] += 0.5f * Ef;
+}
+
 template<int K>
-static void launch_k(hipStream_t st, const Batch &b, bool split, int block) {
+static void launch_k(hipStream_t st, const Batch &b, bool split, int block, bool far_inline) {
+	size_t threads = (size_t) b.N * K;
+	dim3 grid((unsigned) ((threads + block - 1) / block));
+	int *hbc = b.want_hbcount ? b.hbcount : nullptr;
+	if(split) {
+		if(far_inline)
+			k_forces<K, true, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
+					b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount);
+		else
+			k_forces<K, true, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
+					b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount);
+	}
+	else {
+		if(far_inline)
+			k_forces<K, false, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
+					b.d_sys, b.d_state, b.F, b.T, nullptr, hbc);
+		else
+			k_forces<K, false, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
+					b.d_sys, b.d_state, b.F, b.T, nullptr, hbc);
+	}
+}
+
+template<int K>
+static void launch_far(hipStream_t st, const Batch &b, bool split, int block) {
 	size_t threads = (size_t) b.N * K;
 	dim3 grid((unsigned) ((threads + block - 1) / block));
 	if(split)
-		k_forces<K, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh, b.d_sys,
-				b.d_state, b.F, b.T, b.esplit, b.hbcount);
+		k_forces_far<K, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F, b.T,
+				b.esplit);
 	else
-		k_forces<K, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh, b.d_sys,
-				b.d_state, b.F, b.T, nullptr, b.want_hbcount ? b.hbcount : nullptr);
+		k_forces_far<K, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F, b.T,
+				nullptr);
 }
 
+// close (+ bonded) kernel; the far pairs either inline (far_lanes == 0) or
+// in k_forces_far right after it
 void launch_forces(hipStream_t st, const Batch &b, bool split) {
 	if(b.N == 0) return;
+	const bool inl = b.far_lanes == 0;
 	int block = b.force_block;
 	switch(b.lanes) {
-	case 1: launch_k<1>(st, b, split, block); break;
-	case 2: launch_k<2>(st, b, split, block); break;
-	case 4: launch_k<4>(st, b, split, block); break;
-	case 8: launch_k<8>(st, b, split, block); break;
-	case 16: launch_k<16>(st, b, split, block); break;
-	case 32: launch_k<32>(st, b, split, block); break;
-	default: launch_k<64>(st, b, split, block); break;
+	case 1: launch_k<1>(st, b, split, block, inl); break;
+	case 2: launch_k<2>(st, b, split, block, inl); break;
+	case 4: launch_k<4>(st, b, split, block, inl); break;
+	case 8: launch_k<8>(st, b, split, block, inl); break;
+	case 16: launch_k<16>(st, b, split, block, inl); break;
+	case 32: launch_k<32>(st, b, split, block, inl); break;
+	default: launch_k<64>(st, b, split, block, inl); break;
+	}
+	if(inl) return;
+	block = b.far_block;
+	switch(b.far_lanes) {
+	case 1: launch_far<1>(st, b, split, block); break;
+	case 2: launch_far<2>(st, b, split, block); break;
+	case 4: launch_far<4>(st, b, split, block); break;
+	case 8: launch_far<8>(st, b, split, block); break;
+	case 16: launch_far<16>(st, b, split, block); break;
+	case 32: launch_far<32>(st, b, split, block); break;
+	default: launch_far<64>(st, b, split, block); break;
 	}
 }
 

@@ -13,6 +13,8 @@
 // system's seed and counted by (particle, step, stream), so a system's
 // trajectory does not depend on which other systems share the batch or on
 // the particle order.  Bussi's kinetic-energy reductions are FP64.
+#include <algorithm>
+
 #include "oxh_device.h"
 #include "oxh_internal.h"
 
@@ -37,23 +39,27 @@ __device__ __forceinline__ double4 qmul(double4 a, double4 b) {
 			a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z);
 }
 
-// FP32 mirror of the orientation, and the FP64 absolute backbone-site
-// position with the particle's Debye-Hueckel charge factor in .w (0.5 at
-// strand ends when dh_half_charged_ends, src/Interactions/DNA2Interaction.cpp:
-// 162-165)
-__device__ __forceinline__ void write_mirror(int i, double4 r, double4 q, const oxh_sys &S, int2 bd, float4 *q32, double4 *bpos) {
+// FP32 mirror of the orientation, and the backbone-site position wrapped
+// into the box in FP64 and rounded to FP32, with the particle's
+// Debye-Hueckel charge factor in .w (0.5 at strand ends when
+// dh_half_charged_ends, src/Interactions/DNA2Interaction.cpp:162-165)
+__device__ __forceinline__ void write_mirror(int i, double4 r, double4 q, const oxh_sys &S, int2 bd, float4 *q32, float4 *bpos) {
 	q32[i] = make_float4((float) q.x, (float) q.y, (float) q.z, (float) q.w);
 	double sqx = q.x * q.x, sqy = q.y * q.y, sqz = q.z * q.z, sqw = q.w * q.w;
 	double xy = q.x * q.y, xz = q.x * q.z, xw = q.x * q.w, yz = q.y * q.z, yw = q.y * q.w, zw = q.z * q.w;
 	double a1x = sqx - sqy - sqz + sqw, a1y = 2. * (xy + zw), a1z = 2. * (xz - yw);
 	double a2x = 2. * (xy - zw), a2y = -sqx + sqy - sqz + sqw, a2z = 2. * (yz + xw);
 	const double b1 = (double) POS_MM_BACK1, b2 = (double) POS_MM_BACK2;
-	double cf = (S.dh_half && (bd.x < 0 || bd.y < 0)) ? 0.5 : 1.0;
-	bpos[i] = make_double4(r.x + b1 * a1x + b2 * a2x, r.y + b1 * a1y + b2 * a2y, r.z + b1 * a1z + b2 * a2z, cf);
+	float cf = (S.dh_half && (bd.x < 0 || bd.y < 0)) ? 0.5f : 1.f;
+	double bx = r.x + b1 * a1x + b2 * a2x, by = r.y + b1 * a1y + b2 * a2y, bz = r.z + b1 * a1z + b2 * a2z;
+	bx -= S.box[0] * floor(bx * S.inv_box[0]);
+	by -= S.box[1] * floor(by * S.inv_box[1]);
+	bz -= S.box[2] * floor(bz * S.inv_box[2]);
+	bpos[i] = make_float4((float) bx, (float) by, (float) bz, cf);
 }
 
 __global__ void k_prepare_mirror(int N, const double4 *__restrict__ pos, const double4 *__restrict__ q64, const int2 *__restrict__ bonds,
-		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, float4 *__restrict__ q32, double4 *__restrict__ bpos) {
+		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, float4 *__restrict__ q32, float4 *__restrict__ bpos) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= N) return;
 	write_mirror(i, pos[i], q64[i], systems[pinfo[i].y], bonds[i], q32, bpos);
@@ -65,10 +71,14 @@ void launch_prepare_mirror(hipStream_t st, const Batch &b) {
 }
 
 __global__ void __launch_bounds__(256) k_first_step(int N, double4 *__restrict__ pos, double4 *__restrict__ vel, double4 *__restrict__ Lm,
-		double4 *__restrict__ q64, float4 *__restrict__ q32, double4 *__restrict__ bpos, const float4 *__restrict__ F,
+		double4 *__restrict__ q64, float4 *__restrict__ q32, float4 *__restrict__ bpos, const float4 *__restrict__ F,
 		const float4 *__restrict__ T, const double4 *__restrict__ rlist, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
-		long long *__restrict__ pstep, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st) {
+		long long *__restrict__ pstep, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st, int ncell,
+		int *__restrict__ cell_count) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	// cell counters start every step at zero; they are only filled (after
+	// this kernel) in steps that rebuild the lists
+	if(i < ncell) cell_count[i] = 0;
 	if(i >= N) return;
 	const int s = pinfo[i].y;
 	const oxh_sys &S = systems[s];
@@ -115,8 +125,9 @@ __global__ void __launch_bounds__(256) k_first_step(int N, double4 *__restrict__
 }
 
 void launch_first_step(hipStream_t st, const Batch &b) {
-	k_first_step<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.F, b.T, b.rlist, b.bonds, b.pinfo,
-			b.pstep, b.d_sys, b.d_state);
+	int n = std::max(b.N, b.ncell_total);
+	k_first_step<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.F, b.T, b.rlist, b.bonds, b.pinfo,
+			b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count);
 }
 
 // second half kick + Langevin / Brownian thermostats; the first particle
@@ -278,35 +289,75 @@ __device__ double bussi_K(const oxh_sys &S, long long step, uint32_t stream, dou
 	return K + dK;
 }
 
-__global__ void k_bussi_update(int nsys, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st,
-		const double *__restrict__ red, const long long *__restrict__ pstep, double4 *__restrict__ factors) {
-	int s = blockIdx.x * blockDim.x + threadIdx.x;
-	if(s >= nsys) return;
+// One pass over velocities and angular momenta: every block reduces its
+// chunk (fixed tree) into partial[]; the last block of a system to finish
+// (device-scope counter) sums the system's partials in block order and
+// advances the thermostat's kinetic energies
+// (src/Backends/Thermostats/BussiThermostat.cpp:54-90).
+__global__ void __launch_bounds__(256) k_bussi_reduce(const int3 *__restrict__ blocks, const int2 *__restrict__ sys_blocks,
+		const double4 *__restrict__ vel, const double4 *__restrict__ Lm, const oxh_sys *__restrict__ systems,
+		oxh_sys_state *__restrict__ st, const long long *__restrict__ pstep, double *__restrict__ partial, int *__restrict__ done,
+		double4 *__restrict__ factors, int nsys) {
+	__shared__ double sh[5][256];
+	__shared__ int last;
+	const int3 bl = blocks[blockIdx.x];
+	const int s = bl.x;
 	const oxh_sys &S = systems[s];
-	factors[s] = make_double4(1.0, 1.0, 0.0, 0.0);
-	if(S.thermostat != 2 || S.n == 0) return;
-	long long step = pstep[S.first] - 1;
-	if(step % S.newtonian_steps != 0) return;
-	const double *r = red + (size_t) s * RED_W;
-	double n = r[7];
-	double vcx = r[4] / n, vcy = r[5] / n, vcz = r[6] / n;
-	double K_now_t = r[2] - 0.5 * n * (vcx * vcx + vcy * vcy + vcz * vcz);
-	double K_now_r = r[3];
-	int dof_t = 3 * (S.n - 1), dof_r = 3 * S.n;
+	if(S.thermostat != 2) return;
+	const long long step = pstep[S.first] - 1;
+	if(step % S.newtonian_steps != 0) {
+		if(threadIdx.x == 0 && blockIdx.x == sys_blocks[s].x) factors[s] = make_double4(1.0, 1.0, 0.0, 0.0);
+		return;
+	}
+	double acc[5] = {0, 0, 0, 0, 0};
+	for(int i = bl.y + threadIdx.x; i < bl.z; i += blockDim.x) {
+		double4 v = vel[i], L = Lm[i];
+		acc[0] += 0.5 * (v.x * v.x + v.y * v.y + v.z * v.z);
+		acc[1] += 0.5 * (L.x * L.x + L.y * L.y + L.z * L.z);
+		acc[2] += v.x;
+		acc[3] += v.y;
+		acc[4] += v.z;
+	}
+	for(int k = 0; k < 5; k++) sh[k][threadIdx.x] = acc[k];
+	__syncthreads();
+	for(int off = 128; off > 0; off >>= 1) {
+		if(threadIdx.x < off)
+			for(int k = 0; k < 5; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
+		__syncthreads();
+	}
+	if(threadIdx.x == 0) {
+		for(int k = 0; k < 5; k++) partial[(size_t) blockIdx.x * RED_W + k] = sh[k][0];
+		__threadfence();
+		const int2 sb = sys_blocks[s];
+		last = (atomicAdd(&done[s], 1) == sb.y - 1);
+	}
+	__syncthreads();
+	if(!last || threadIdx.x != 0) return;
+	__threadfence();
+	const int2 sb = sys_blocks[s];
+	double r[5] = {0, 0, 0, 0, 0};
+	for(int b = sb.x; b < sb.x + sb.y; b++)
+		for(int k = 0; k < 5; k++) r[k] += __builtin_nontemporal_load(&partial[(size_t) b * RED_W + k]);
+	done[s] = 0;
+	const double n = S.n;
+	const double vcx = r[2] / n, vcy = r[3] / n, vcz = r[4] / n;
+	const double K_now_t = r[0] - 0.5 * n * (vcx * vcx + vcy * vcy + vcz * vcz);
+	const double K_now_r = r[1];
+	const int dof_t = 3 * (S.n - 1), dof_r = 3 * S.n;
 	st[s].K_t = bussi_K(S, step, RNG_BUSSI, st[s].K_t, dof_t);
 	st[s].K_r = bussi_K(S, step, RNG_BUSSI + 4, st[s].K_r, dof_r);
-	double ft = K_now_t > 0. ? sqrt(st[s].K_t / K_now_t) : 1.0;
-	double fr = K_now_r > 0. ? sqrt(st[s].K_r / K_now_r) : 1.0;
+	const double ft = K_now_t > 0. ? sqrt(st[s].K_t / K_now_t) : 1.0;
+	const double fr = K_now_r > 0. ? sqrt(st[s].K_r / K_now_r) : 1.0;
 	factors[s] = make_double4(ft, fr, 1.0, 0.0);
-	// stash the centre-of-mass velocity next to the factors
 	factors[nsys + s] = make_double4(vcx, vcy, vcz, 0.0);
 }
 
 __global__ void k_bussi_rescale(int N, double4 *__restrict__ vel, double4 *__restrict__ Lm, const int2 *__restrict__ pinfo,
-		const double4 *__restrict__ factors, int nsys) {
+		const double4 *__restrict__ factors, int nsys, const oxh_sys *__restrict__ systems) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= N) return;
 	int s = pinfo[i].y;
+	if(systems[s].thermostat != 2) return;
 	double4 f = factors[s];
 	if(f.z == 0.0) return;
 	double4 vc = factors[nsys + s];
@@ -323,10 +374,10 @@ __global__ void k_bussi_rescale(int N, double4 *__restrict__ vel, double4 *__res
 }
 
 void launch_bussi(hipStream_t st, const Batch &b) {
-	launch_reduce(st, b);
 	double4 *factors = (double4 *) b.scratch;
-	k_bussi_update<<<(b.nsys + 63) / 64, 64, 0, st>>>(b.nsys, b.d_sys, b.d_state, b.red_out, b.pstep, factors);
-	k_bussi_rescale<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.vel, b.Lm, b.pinfo, factors, b.nsys);
+	k_bussi_reduce<<<b.nred, 256, 0, st>>>(b.red_blocks, b.red_sys_blocks, b.vel, b.Lm, b.d_sys, b.d_state, b.pstep, b.red_partial,
+			b.red_done, factors, b.nsys);
+	k_bussi_rescale<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.vel, b.Lm, b.pinfo, factors, b.nsys, b.d_sys);
 }
 
 // initial velocities ~ N(0, T) per component (src/Backends/MDBackend.cpp:140-165)

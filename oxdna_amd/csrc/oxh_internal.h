@@ -22,7 +22,7 @@ struct Batch {
 	// per-particle state (FP64 integration state, FP32 kernel mirror)
 	double4 *pos = nullptr, *vel = nullptr, *Lm = nullptr, *q64 = nullptr, *rlist = nullptr;
 	float4 *q32 = nullptr, *F = nullptr, *T = nullptr;
-	double4 *bpos = nullptr;  // absolute backbone-site position, .w = Debye-Hueckel charge factor
+	float4 *bpos = nullptr;   // backbone-site position wrapped into the box (FP32), .w = Debye-Hueckel charge factor
 	int2 *bonds = nullptr;   // (n3, n5) batch indices, -1 = none
 	int2 *pinfo = nullptr;   // (btype, system)
 	int *orig = nullptr;     // index of the particle inside its system as uploaded
@@ -35,10 +35,13 @@ struct Batch {
 	int2 *nnbr = nullptr;
 	int nstride = 0, max_neigh = 0;
 	int lanes = 8;             // threads per nucleotide in the force kernel
+	int far_lanes = 0;         // threads per nucleotide of the separate far-pair kernel (0 = far pairs inline)
+	int far_block = 256;
 
 	// cells (systems with use_cells), fixed capacity per cell
 	int ncell_total = 0, cell_cap = 0;
 	int *cell_count = nullptr, *cell_items = nullptr, *cell_owner = nullptr;
+	float4 *pos32 = nullptr;   // wrapped FP32 centres, written by the cell fill of a rebuild
 
 	// per-system parameters and device-written state
 	oxh_sys *d_sys = nullptr;
@@ -51,6 +54,7 @@ struct Batch {
 	int2 *red_sys_blocks = nullptr;    // per system: (first block, n blocks)
 	double *red_partial = nullptr;     // nred * RED_W
 	double *red_out = nullptr;         // nsys * RED_W
+	int *red_done = nullptr;           // per system: blocks finished (Bussi's last-block reduction)
 
 	// optional outputs
 	float *esplit = nullptr;   // 8 * N
@@ -80,6 +84,7 @@ constexpr int RED_W = 8;   // reduction width: U, Uhb, Kt, Kr, vx, vy, vz, count
 
 void launch_forces(hipStream_t st, const Batch &b, bool split);
 void launch_cells(hipStream_t st, const Batch &b);
+void launch_cells_fill_sort(hipStream_t st, const Batch &b);
 void launch_build_list(hipStream_t st, const Batch &b, int force_all);
 void launch_first_step(hipStream_t st, const Batch &b);
 void launch_second_step(hipStream_t st, const Batch &b);

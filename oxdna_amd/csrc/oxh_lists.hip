@@ -15,6 +15,7 @@
 // sort_by_key): key = Hilbert index of the particle's cell after
 // J. Skilling, "Programming the Hilbert curve" (AIP Conf. Proc. 707, 2004),
 // sorted by a hand-written stable LSD radix sort with wave64 ballots.
+#include <algorithm>
 #include <type_traits>
 #include <utility>
 
@@ -37,7 +38,7 @@ __global__ void k_cells_clear(int ncell, const int *__restrict__ owner, const ox
 }
 
 __global__ void k_cells_fill(int N, const double4 *__restrict__ pos, const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems,
-		oxh_sys_state *__restrict__ st, int *__restrict__ count, int *__restrict__ items, int cap) {
+		oxh_sys_state *__restrict__ st, int *__restrict__ count, int *__restrict__ items, int cap, float4 *__restrict__ pos32) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= N) return;
 	int s = pinfo[i].y;
@@ -49,38 +50,55 @@ __global__ void k_cells_fill(int N, const double4 *__restrict__ pos, const int2 
 	int cy = cell_coord(p.y, S.inv_box[1], S.nc[1]);
 	int cz = cell_coord(p.z, S.inv_box[2], S.nc[2]);
 	int c = S.cell_first + (cz * S.nc[1] + cy) * S.nc[0] + cx;
+	pos32[i] = make_float4((float) (p.x - S.box[0] * floor(p.x * S.inv_box[0])), (float) (p.y - S.box[1] * floor(p.y * S.inv_box[1])),
+			(float) (p.z - S.box[2] * floor(p.z * S.inv_box[2])), 0.f);
 	int slot = atomicAdd(&count[c], 1);
 	if(slot < cap) items[(size_t) c * cap + slot] = i;
 	else atomicOr(&st[s].error, 4);
 }
 
-// sorts every rebuilt cell's slots by particle index (insertion sort;
-// cells hold tens of particles)
-__global__ void k_cells_sort(int ncell, const int *__restrict__ owner, const oxh_sys_state *__restrict__ st, const int *__restrict__ count,
-		int *__restrict__ items, int cap) {
-	int c = blockIdx.x * blockDim.x + threadIdx.x;
-	if(c >= ncell) return;
-	if(!st[owner[c]].need_rebuild) return;
-	int n = min(count[c], cap);
-	int *it = items + (size_t) c * cap;
-	for(int a = 1; a < n; a++) {
-		int v = it[a];
-		int b = a - 1;
-		while(b >= 0 && it[b] > v) {
-			it[b + 1] = it[b];
-			b--;
+// Sorts every rebuilt cell's slots by particle index: one wavefront per
+// cell (grid-stride over cells), each lane ranks its slots against the
+// whole cell staged in LDS (indices are distinct, so rank = number of
+// smaller indices).
+__global__ void __launch_bounds__(64) k_cells_sort(int ncell, const int *__restrict__ owner, const oxh_sys_state *__restrict__ st,
+		const int *__restrict__ count, int *__restrict__ items, int cap) {
+	extern __shared__ int cell_buf[];
+	for(int c = blockIdx.x; c < ncell; c += gridDim.x) {
+		if(!st[owner[c]].need_rebuild) continue;
+		const int n = min(count[c], cap);
+		if(n <= 1) continue;
+		int *it = items + (size_t) c * cap;
+		for(int k = threadIdx.x; k < n; k += 64) cell_buf[k] = it[k];
+		__syncthreads();
+		for(int k = threadIdx.x; k < n; k += 64) {
+			const int v = cell_buf[k];
+			int rank = 0;
+			for(int m = 0; m < n; m++) rank += cell_buf[m] < v;
+			it[rank] = v;
 		}
-		it[b + 1] = v;
+		__syncthreads();
 	}
+}
+
+// fill + sort of flagged systems' cells; the counters must be zero
+void launch_cells_fill_sort(hipStream_t st, const Batch &b) {
+	if(!b.any_cells) return;
+	int bs = 256;
+	k_cells_fill<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, b.pos, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_items, b.cell_cap,
+			b.pos32);
+	k_cells_sort<<<std::min(b.ncell_total, 4096), 64, (size_t) b.cell_cap * sizeof(int), st>>>(b.ncell_total, b.cell_owner, b.d_state,
+			b.cell_count, b.cell_items, b.cell_cap);
 }
 
 void launch_cells(hipStream_t st, const Batch &b) {
 	if(!b.any_cells) return;
 	int bs = 256;
 	k_cells_clear<<<(b.ncell_total + bs - 1) / bs, bs, 0, st>>>(b.ncell_total, b.cell_owner, b.d_state, b.cell_count);
-	k_cells_fill<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, b.pos, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_items, b.cell_cap);
-	k_cells_sort<<<(b.ncell_total + bs - 1) / bs, bs, 0, st>>>(b.ncell_total, b.cell_owner, b.d_state, b.cell_count, b.cell_items,
-			b.cell_cap);
+	k_cells_fill<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, b.pos, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_items, b.cell_cap,
+			b.pos32);
+	k_cells_sort<<<std::min(b.ncell_total, 4096), 64, (size_t) b.cell_cap * sizeof(int), st>>>(b.ncell_total, b.cell_owner, b.d_state,
+			b.cell_count, b.cell_items, b.cell_cap);
 }
 
 // FP64 squared nearest-image distance, fixed operation sequence
@@ -92,15 +110,23 @@ __device__ __forceinline__ double sqr_dist(double4 a, double4 b, const oxh_sys &
 	return fma(dz, dz, fma(dy, dy, dx * dx));
 }
 
-// One thread per particle writes its own row: pairs closer than
-// sqrt(rclose2) at build time from the front, the rest from the back.  A
-// pair cannot come closer than R_CLOSE before the next rebuild unless it was
-// within R_CLOSE + 2 skin now, so far pairs only ever need Debye-Hueckel.
+constexpr int BUILD_LANES = 16;
+
+// BUILD_LANES threads per particle share the candidate scan: in every chunk
+// each lane tests one candidate, a wave ballot gives every passing lane its
+// slot, so the row is written in candidate order whatever the timing.
+// Pairs closer than sqrt(rclose2) go to the front of the row, the rest to
+// the back.  A pair cannot come closer than R_CLOSE before the next rebuild
+// unless it is within R_CLOSE + 2 skin now, so far pairs only ever need
+// Debye-Hueckel.
 __global__ void __launch_bounds__(256) k_build_list(int N, const double4 *__restrict__ pos, const int2 *__restrict__ bonds,
 		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st,
 		const int *__restrict__ count, const int *__restrict__ items, int cap, int *__restrict__ nbr, int2 *__restrict__ nnbr,
-		int max_neigh, double4 *__restrict__ rlist, int force_all) {
-	int i = blockIdx.x * blockDim.x + threadIdx.x;
+		int max_neigh, double4 *__restrict__ rlist, int force_all, const float4 *__restrict__ pos32) {
+	constexpr int K = BUILD_LANES;
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	const int i = t / K;
+	const int l = t % K;
 	if(i >= N) return;
 	int s = pinfo[i].y;
 	if(!force_all && !st[s].need_rebuild) return;
@@ -108,17 +134,60 @@ __global__ void __launch_bounds__(256) k_build_list(int N, const double4 *__rest
 	const double4 p = pos[i];
 	const int2 bd = bonds[i];
 	int *row = nbr + (size_t) i * max_neigh;
+	const float4 p32 = S.use_cells ? pos32[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+	const int g0 = (threadIdx.x & 63) & ~(K - 1);          // first lane of this group in the wave
+	const uint64_t gmask = (K == 64) ? ~0ull : (((1ull << K) - 1ull) << g0);
+	const uint64_t below = ((1ull << (threadIdx.x & 63)) - 1ull) & gmask;
 	int nc = 0, nf = 0;
-	auto consider = [&](int j) {
-		if(j == i || j == bd.x || j == bd.y) return;
-		double r2 = sqr_dist(p, pos[j], S);
-		if(r2 < S.rv2) {
-			if(nc + nf < max_neigh) {
-				if(r2 < S.rclose2) row[nc] = j;
-				else row[max_neigh - 1 - nf] = j;
+	// tests candidates base[0 .. n) in chunks of K
+	auto scan = [&](const int *cand, int first, int n) {
+		for(int m0 = 0; m0 < n; m0 += K) {
+			const int m = m0 + l;
+			bool close = false, far = false;
+			int j = -1;
+			if(m < n) {
+				j = cand ? cand[m] : first + m;
+				if(j != i && j != bd.x && j != bd.y) {
+					if(cand) {
+						// FP32 decides unless within its error bound of a threshold
+						const float4 q = pos32[j];
+						float dx = q.x - p32.x, dy = q.y - p32.y, dz = q.z - p32.z;
+						dx -= S.boxf[0] * rintf(dx * S.inv_boxf[0]);
+						dy -= S.boxf[1] * rintf(dy * S.inv_boxf[1]);
+						dz -= S.boxf[2] * rintf(dz * S.inv_boxf[2]);
+						const float r2f = dx * dx + dy * dy + dz * dz;
+						if(fabsf(r2f - S.rv2f) < S.eps2 || fabsf(r2f - S.rclose2f) < S.eps2) {
+							double r2 = sqr_dist(p, pos[j], S);
+							if(r2 < S.rv2) {
+								close = r2 < S.rclose2;
+								far = !close;
+							}
+						}
+						else if(r2f < S.rv2f) {
+							close = r2f < S.rclose2f;
+							far = !close;
+						}
+					}
+					else {
+						double r2 = sqr_dist(p, pos[j], S);
+						if(r2 < S.rv2) {
+							close = r2 < S.rclose2;
+							far = !close;
+						}
+					}
+				}
 			}
-			if(r2 < S.rclose2) nc++;
-			else nf++;
+			const uint64_t bc = __ballot(close) & gmask, bf = __ballot(far) & gmask;
+			if(close) {
+				int k = nc + __popcll(bc & below);
+				if(k + nf < max_neigh) row[k] = j;
+			}
+			if(far) {
+				int k = nf + __popcll(bf & below);
+				if(nc + k < max_neigh) row[max_neigh - 1 - k] = j;
+			}
+			nc += __popcll(bc);
+			nf += __popcll(bf);
 		}
 	};
 	if(S.use_cells) {
@@ -135,16 +204,15 @@ __global__ void __launch_bounds__(256) k_build_list(int N, const double4 *__rest
 					int x = cx + dx;
 					x += (x < 0) ? S.nc[0] : (x >= S.nc[0] ? -S.nc[0] : 0);
 					int c = S.cell_first + (z * S.nc[1] + y) * S.nc[0] + x;
-					int m = min(count[c], cap);
-					const int *it = items + (size_t) c * cap;
-					for(int k = 0; k < m; k++) consider(it[k]);
+					scan(items + (size_t) c * cap, 0, min(count[c], cap));
 				}
 			}
 		}
 	}
 	else {
-		for(int j = S.first; j < S.first + S.n; j++) consider(j);
+		scan(nullptr, S.first, S.n);
 	}
+	if(l != 0) return;
 	if(nc + nf > max_neigh) {
 		atomicOr(&st[s].error, 2);
 		nc = nf = 0;
@@ -155,8 +223,9 @@ __global__ void __launch_bounds__(256) k_build_list(int N, const double4 *__rest
 
 void launch_build_list(hipStream_t st, const Batch &b, int force_all) {
 	int bs = 256;
-	k_build_list<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, b.pos, b.bonds, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_items,
-			b.cell_cap, b.nbr, b.nnbr, b.max_neigh, b.rlist, force_all);
+	size_t threads = (size_t) b.N * BUILD_LANES;
+	k_build_list<<<(unsigned) ((threads + bs - 1) / bs), bs, 0, st>>>(b.N, b.pos, b.bonds, b.pinfo, b.d_sys, b.d_state, b.cell_count,
+			b.cell_items, b.cell_cap, b.nbr, b.nnbr, b.max_neigh, b.rlist, force_all, b.pos32);
 }
 
 // ------------------------------------------------------------ Hilbert sort

@@ -15,6 +15,7 @@ typedef struct oxh_sys {
 	// geometry, FP64 (minimum image and wrapping are done in FP64)
 	double box[3];
 	double inv_box[3];
+	float boxf[3], inv_boxf[3];   // FP32 copies for the far-pair nearest image
 	// particle range [first, first + n) in the batch arrays
 	int first, n;
 	// cells: nc[3] per side, [cell_first, cell_first + ncell) in the batch's
@@ -28,6 +29,9 @@ typedef struct oxh_sys {
 	// Verlet list: r_verlet^2, skin^2 and the build-time bound below which a
 	// pair goes to the close sublist, (R_CLOSE + 2 skin)^2, FP64
 	double rv2, skin2, rclose2;
+	// FP32 squared distances within eps2 of rv2 or rclose2 are re-decided in
+	// FP64 (bound on the FP32 rounding error of wrapped coordinates)
+	float rv2f, rclose2f, eps2;
 	// model constants (FP32) that depend on T / salt / options
 	float f1_eps[2][5][5];
 	float f1_shift[2][5][5];

@@ -67,3 +67,23 @@ def test_deterministic(hip):
         outs.append((st["pos"], st["vel"], F, T))
     for a, c in zip(*outs):
         assert np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("lanes,far_lanes", [(1, 0), (8, 0), (4, 4), (8, 16), (32, 1)])
+@pytest.mark.parametrize("name", ["bundle_md_30C", "tangle3_nohalf_dh", "gas8_25C_0.5M"])
+def test_launch_shapes(hip, monkeypatch, name, lanes, far_lanes):
+    """Every launch shape the autotuner may pick gives the same physics:
+    far pairs inline or in their own kernel, any lanes per nucleotide."""
+    monkeypatch.setenv("OXH_FORCE_LANES", str(lanes))
+    monkeypatch.setenv("OXH_FAR_LANES", str(far_lanes))
+    s, opts, ref = load_golden(name)
+    with hip.HIPBatch(s, **opts) as b:
+        b.compute_forces()
+        F, T, _ = b.forces()
+        e8 = b.energy_split()
+        st = b.stats()
+    assert st["lanes"] == lanes and st["far_lanes"] == far_lanes
+    assert vec_rel_err(F, ref["F"])[0] <= F_TOL
+    assert vec_rel_err(T, ref["T"])[0] <= F_TOL
+    scale = np.abs(ref["E8"]).sum() + 1e-6 * s.N
+    assert np.all(np.abs(e8 - ref["E8"]) <= 1e-5 * scale)
