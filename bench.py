@@ -59,6 +59,13 @@ def workload(name: str, seed: int = 1):
                         box_type="orthogonal")
         desc = ("1,024 x 96-bp duplexes (196,608 nt), hexagonal lattice 2.6, 30 C, 0.2 M, Langevin, dt 0.003")
         return s, opts, ref_keys, desc
+    if name == "batched":
+        hs = systems.hairpin_set(seed, n=256, box=30.0, t_lo=25.0, t_hi=75.0)
+        opts = dict(thermostat="langevin", diff_coeff=2.5, dt=0.003, verlet_skin=0.05, refresh_vel=True)
+        per = [dict(seed=seed * 1000 + k) for k in range(len(hs))]
+        desc = ("256 hairpins, 6-12 bp stems, 4-36 nt loops, 16-60 nt each, own 30-unit boxes, 25-75 C, 0.5 M, "
+                "Langevin, dt 0.003")
+        return hs, (opts, per), None, desc
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -120,9 +127,17 @@ def main():
     if _lib.device_count() < 1:
         raise SystemExit("no GPU visible")
     s, opts, ref_keys, desc = workload(args.workload, args.seed)
-    log(f"workload {args.workload}: N={s.N}")
+    if isinstance(s, list):
+        common, per = opts
+        systems_ = s
+        b_args = dict(options=per, **common)
+    else:
+        systems_ = [s]
+        b_args = dict(opts)
+    n_total = sum(x.N for x in systems_)
+    log(f"workload {args.workload}: {len(systems_)} system(s), N={n_total}")
     t_up = time.perf_counter()
-    b = backend.HIPBatch(s, **opts)
+    b = backend.HIPBatch(systems_, **b_args)
     b.set_sort_interval(args.sort_every)
     log(f"upload + first list/forces {time.perf_counter() - t_up:.2f} s")
     b.run(args.warmup)
@@ -134,10 +149,10 @@ def main():
     t1 = time.perf_counter()
     reb1, err = b.info()
     elapsed = t1 - t0
-    value = s.N * args.steps / elapsed
+    value = n_total * args.steps / elapsed
     st = b.stats()
     force_ms = b.time_forces(50)
-    E = b.energies()[0]
+    E = b.energies().sum(axis=0)
     b.close()
     nbar = st["mean_neigh"]
     # force kernel, algorithmic bytes per nucleotide: own state read
@@ -145,14 +160,14 @@ def main():
     # written (32), one 4-byte list entry per neighbour; neighbour gathers
     # are counted as cache/LDS-resident
     b_alg = 116.0 + 4.0 * nbar
-    achieved = s.N * b_alg / (force_ms * 1e-3) / 1e9
+    achieved = n_total * b_alg / (force_ms * 1e-3) / 1e9
     rebuild_every = args.steps / max(1, reb1 - reb0)
     log(f"{elapsed:.3f} s for {args.steps} steps; {1e3 * elapsed / args.steps:.4f} ms/step; force kernel "
         f"{force_ms * 1e3:.1f} us; mean neighbours {nbar:.1f}; rebuild every {rebuild_every:.1f} steps; "
-        f"U/N {E[0] / s.N:.4f} K/N {(E[2] + E[3]) / s.N:.4f}")
+        f"U/N {E[0] / n_total:.4f} K/N {(E[2] + E[3]) / n_total:.4f}")
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and ref_keys is not None:
         try:
             cpu = cpu_baseline(s, ref_keys, 20, 20 + args.cpu_steps)
         except Exception as e:  # the GPU figure stands on its own
@@ -171,7 +186,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 forces / f64 state",
         "data": "synthetic",
-        "config": {"workload": desc, "n_nucleotides": s.N, "form": "plain (one system)", "parallelism": "none",
+        "config": {"workload": desc, "n_nucleotides": n_total, "n_systems": len(systems_),
+                   "form": "plain (one system)" if len(systems_) == 1 else "batched (one launch for all systems)",
+                   "parallelism": "none",
                    "hilbert_sort_every_steps": args.sort_every,
                    "mean_neighbours": round(nbar, 2), "steps_per_rebuild": round(rebuild_every, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
