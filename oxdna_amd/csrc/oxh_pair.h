@@ -1,0 +1,380 @@
+// Pair physics of the oxDNA2 interaction on gfx950, shared by the grid
+// force kernels (oxh_forces.hip) and the workgroup-resident engine
+// (oxh_persist.hip).
+//
+// Terms follow the reference's CPU DNA2_nomesh formulation
+// (src/Interactions/DNAInteraction.cpp:415-1205,
+//  src/Interactions/DNA2Interaction.cpp:157-306), in FP32.
+#pragma once
+
+#include "oxh_device.h"
+
+namespace oxh {
+
+struct Frame {
+	f3 a1, a2, a3, back, stack, base;
+	int btype, type;
+};
+
+__device__ __forceinline__ int type_of(int btype) { return (btype < 0) ? 3 - ((3 - btype) % 4) : btype % 4; }
+
+__device__ __forceinline__ Frame make_frame(float4 q, int btype) {
+	Frame fr;
+	quat_axes(q, fr.a1, fr.a2, fr.a3);
+	fr.back = fr.a1 * POS_MM_BACK1 + fr.a2 * POS_MM_BACK2;
+	fr.stack = fr.a1 * POS_STACK;
+	fr.base = fr.a1 * POS_BASE;
+	fr.btype = btype;
+	fr.type = type_of(btype);
+	return fr;
+}
+
+__device__ __forceinline__ f3 min_image(double4 a, double4 b, const oxh_sys &S) {
+	double dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+	dx = fma(-S.box[0], rint(dx * S.inv_box[0]), dx);
+	dy = fma(-S.box[1], rint(dy * S.inv_box[1]), dy);
+	dz = fma(-S.box[2], rint(dz * S.inv_box[2]), dz);
+	return mk((float) dx, (float) dy, (float) dz);
+}
+
+// Result of one pair term evaluated for the ordered pair (p, q), r = q - p:
+// force on q (p receives -force), torques on p and q (lab frame), energy.
+struct PairOut {
+	f3 force, tp, tq;
+	float e;
+};
+
+// ------------------------------------------------------------ bonded terms
+// FENE, bonded excluded volume and stacking of the bonded pair p -> n3 = q
+// (src/Interactions/DNAInteraction.cpp:415-705); returns the three energies
+// in e3.
+__device__ PairOut bonded_pair(const oxh_sys &S, const Frame &p, const Frame &q, f3 r, float e3[3], int &err) {
+	PairOut o = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, 0.f};
+
+	// FENE backbone
+	f3 rback = r + q.back - p.back;
+	float rbackmod = sqrtf(dot(rback, rback));
+	float rbackr0 = rbackmod - S.fene_r0;
+	f3 force;
+	float e_fene;
+	if(S.use_mbf && fabsf(rbackr0) > S.mbf_xmax) {
+		float fene_xmax = -(FENE_EPS * 0.5f) * logf(1.f - SQR(S.mbf_xmax) / FENE_DELTA2);
+		float long_xmax = (S.mbf_fmax - S.mbf_finf) * S.mbf_xmax * logf(S.mbf_xmax) + S.mbf_finf * S.mbf_xmax;
+		e_fene = (S.mbf_fmax - S.mbf_finf) * S.mbf_xmax * logf(fabsf(rbackr0)) + S.mbf_finf * fabsf(rbackr0) - long_xmax + fene_xmax;
+		force = rback * (-copysignf(1.f, rbackr0) * ((S.mbf_fmax - S.mbf_finf) * S.mbf_xmax / fabsf(rbackr0) + S.mbf_finf) / rbackmod);
+	}
+	else {
+		if(fabsf(rbackr0) >= FENE_DELTA) {
+			err |= 1;
+			rbackr0 = copysignf(FENE_DELTA * 0.999f, rbackr0);
+		}
+		e_fene = -(FENE_EPS * 0.5f) * logf(1.f - SQR(rbackr0) / FENE_DELTA2);
+		force = rback * (-(FENE_EPS * rbackr0 / (FENE_DELTA2 - SQR(rbackr0))) / rbackmod);
+	}
+	o.force = force;
+	o.tp = -cross(p.back, force);
+	o.tq = cross(q.back, force);
+	e3[0] = e_fene;
+
+	// bonded excluded volume: base-base, p.base-q.back, p.back-q.base
+	f3 fx;
+	float ex = repulsive_lj(r + q.base - p.base, fx, EXCL_S2, EXCL_R2, EXCL_B2, EXCL_RC2);
+	o.force += fx;
+	o.tp -= cross(p.base, fx);
+	o.tq += cross(q.base, fx);
+	ex += repulsive_lj(r + q.back - p.base, fx, EXCL_S3, EXCL_R3, EXCL_B3, EXCL_RC3);
+	o.force += fx;
+	o.tp -= cross(p.base, fx);
+	o.tq += cross(q.back, fx);
+	ex += repulsive_lj(r + q.base - p.back, fx, EXCL_S4, EXCL_R4, EXCL_B4, EXCL_RC4);
+	o.force += fx;
+	o.tp -= cross(p.back, fx);
+	o.tq += cross(q.base, fx);
+	e3[1] = ex;
+
+	// stacking
+	const f3 &a1 = p.a1, &a2 = p.a2, &a3 = p.a3, &b1 = q.a1, &b2 = q.a2, &b3 = q.a3;
+	f3 rbackref = r + b1 * POS_BACK - a1 * POS_BACK;
+	float rbackrefmod = sqrtf(dot(rbackref, rbackref));
+	f3 rstack = r + q.stack - p.stack;
+	float rstackmod = sqrtf(dot(rstack, rstack));
+	f3 rstackdir = rstack * (1.f / rstackmod);
+
+	float cost4 = dot(a3, b3);
+	float cost5 = dot(a3, rstackdir);
+	float cost6 = -dot(b3, rstackdir);
+	float inv_rbr = 1.f / rbackrefmod;
+	float cosphi1 = dot(a2, rbackref) * inv_rbr;
+	float cosphi2 = dot(b2, rbackref) * inv_rbr;
+
+	float vf1 = f1(S, rstackmod, 1, q.type, p.type);
+	float f4t4, f4t4D, f4t5, f4t5D, f4t6, f4t6D;
+	cf4(cost4, S_STCK_T4, f4t4, f4t4D);
+	cf4(-cost5, S_STCK_T5, f4t5, f4t5D);
+	cf4(cost6, S_STCK_T5, f4t6, f4t6D);
+	float f5phi1 = f5(cosphi1);
+	float f5phi2 = f5(cosphi2);
+	float es = vf1 * f4t4 * f4t5 * f4t6 * f5phi1 * f5phi2;
+	e3[2] = es;
+
+	if(es != 0.f) {
+		float vf1D = f1D(S, rstackmod, 1, q.type, p.type);
+		float f4t4Dsin = -f4t4D;
+		float f4t5Dsin = -f4t5D;
+		float f4t6Dsin = -f4t6D;
+		float f5phi1D = f5D(cosphi1);
+		float f5phi2D = f5D(cosphi2);
+		float inv_rs = 1.f / rstackmod;
+
+		f3 sf = -rstackdir * (vf1D * f4t4 * f4t5 * f4t6 * f5phi1 * f5phi2);
+		sf += -(a3 - rstackdir * cost5) * (vf1 * f4t4 * f4t5Dsin * f4t6 * f5phi1 * f5phi2 * inv_rs);
+		sf += -(b3 + rstackdir * cost6) * (vf1 * f4t4 * f4t5 * f4t6Dsin * f5phi1 * f5phi2 * inv_rs);
+
+		float rbrc = rbackrefmod * rbackrefmod * rbackrefmod;
+		float inv_rbrc = 1.f / rbrc;
+		float ra2 = dot(rstackdir, a2);
+		float ra1 = dot(rstackdir, a1);
+		float rb1 = dot(rstackdir, b1);
+		float a2b1 = dot(a2, b1);
+		float par = rstackmod * ra2 - a2b1 * GAMMA;
+		float dcosphi1dr = (SQR(rstackmod) * ra2 - ra2 * SQR(rbackrefmod) - rstackmod * (a2b1 + ra2 * (-ra1 + rb1)) * GAMMA +
+				a2b1 * (-ra1 + rb1) * SQR(GAMMA)) * inv_rbrc;
+		float dcosphi1dra1 = rstackmod * GAMMA * par * inv_rbrc;
+		float dcosphi1dra2 = -rstackmod * inv_rbr;
+		float dcosphi1drb1 = -rstackmod * GAMMA * par * inv_rbrc;
+		float dcosphi1da1b1 = -SQR(GAMMA) * par * inv_rbrc;
+		float dcosphi1da2b1 = GAMMA * inv_rbr;
+		float fpp1 = -vf1 * f4t4 * f4t5 * f4t6 * f5phi1D * f5phi2;
+		sf += -(rstackdir * dcosphi1dr + ((a2 - rstackdir * ra2) * dcosphi1dra2 + (a1 - rstackdir * ra1) * dcosphi1dra1 +
+				(b1 - rstackdir * rb1) * dcosphi1drb1) * inv_rs) * fpp1;
+
+		float qa2 = dot(rstackdir, b2);
+		float qa1 = dot(rstackdir, b1);
+		float qb1 = dot(rstackdir, a1);
+		float qa2b1 = dot(b2, a1);
+		float par2 = rstackmod * qa2 + qa2b1 * GAMMA;
+		float dcosphi2dr = (par2 * (rstackmod + (qb1 - qa1) * GAMMA) - qa2 * SQR(rbackrefmod)) * inv_rbrc;
+		float dcosphi2dra1 = -rstackmod * GAMMA * par2 * inv_rbrc;
+		float dcosphi2dra2 = -rstackmod * inv_rbr;
+		float dcosphi2drb1 = rstackmod * GAMMA * par2 * inv_rbrc;
+		float dcosphi2da1b1 = -SQR(GAMMA) * par2 * inv_rbrc;
+		float dcosphi2da2b1 = -GAMMA * inv_rbr;
+		float fpp2 = -vf1 * f4t4 * f4t5 * f4t6 * f5phi1 * f5phi2D;
+		sf += -(rstackdir * dcosphi2dr + ((b2 - rstackdir * qa2) * dcosphi2dra2 + (b1 - rstackdir * qa1) * dcosphi2dra1 +
+				(a1 - rstackdir * qb1) * dcosphi2drb1) * inv_rs) * fpp2;
+
+		o.force += sf;
+		f3 tp = -cross(p.stack, sf);
+		f3 tq = cross(q.stack, sf);
+
+		f3 t4dir = cross(b3, a3);
+		float tm = vf1 * f4t4Dsin * f4t5 * f4t6 * f5phi1 * f5phi2;
+		tp -= t4dir * tm;
+		tq += t4dir * tm;
+		tm = -vf1 * f4t4 * f4t5Dsin * f4t6 * f5phi1 * f5phi2;
+		tp -= cross(rstackdir, a3) * tm;
+		tm = vf1 * f4t4 * f4t5 * f4t6Dsin * f5phi1 * f5phi2;
+		tq += cross(rstackdir, b3) * tm;
+
+		tp += cross(rstackdir, a2) * (fpp1 * dcosphi1dra2) + cross(rstackdir, a1) * (fpp1 * dcosphi1dra1);
+		tq += cross(rstackdir, b1) * (fpp1 * dcosphi1drb1);
+		f3 pure = cross(a2, b1) * (fpp1 * dcosphi1da2b1) + cross(a1, b1) * (fpp1 * dcosphi1da1b1);
+		tp -= pure;
+		tq += pure;
+
+		tp += cross(rstackdir, a1) * (fpp2 * dcosphi2drb1);
+		tq += cross(rstackdir, b2) * (fpp2 * dcosphi2dra2) + cross(rstackdir, b1) * (fpp2 * dcosphi2dra1);
+		pure = cross(a1, b2) * (fpp2 * dcosphi2da2b1) + cross(a1, b1) * (fpp2 * dcosphi2da1b1);
+		tp -= pure;
+		tq += pure;
+
+		o.tp += tp;
+		o.tq += tq;
+	}
+	o.e = e3[0] + e3[1] + e3[2];
+	return o;
+}
+
+// ----------------------------------------------------- non-bonded, close
+// excluded volume, hydrogen bonding, cross stacking and oxDNA2 coaxial
+// stacking of a non-bonded pair whose centres are closer than R_CLOSE
+// (src/Interactions/DNAInteraction.cpp:707-1021,
+//  src/Interactions/DNA2Interaction.cpp:214-306).  Adds the force on q to
+// fq and the lab torque on p to tp; energies into e4 (excl, HB, CRST, CXST).
+__device__ void close_pair(const oxh_sys &S, const Frame &p, const Frame &q, f3 r, f3 &fq, f3 &tp, float e4[4]) {
+	// excluded volume: base-base, p.back-q.base, p.base-q.back, back-back
+	f3 fx;
+	float ex = repulsive_lj(r + q.base - p.base, fx, EXCL_S2, EXCL_R2, EXCL_B2, EXCL_RC2);
+	fq += fx;
+	tp -= cross(p.base, fx);
+	ex += repulsive_lj(r + q.base - p.back, fx, EXCL_S4, EXCL_R4, EXCL_B4, EXCL_RC4);
+	fq += fx;
+	tp -= cross(p.back, fx);
+	ex += repulsive_lj(r + q.back - p.base, fx, EXCL_S3, EXCL_R3, EXCL_B3, EXCL_RC3);
+	fq += fx;
+	tp -= cross(p.base, fx);
+	ex += repulsive_lj(r + q.back - p.back, fx, EXCL_S1, EXCL_R1, EXCL_B1, EXCL_RC1);
+	fq += fx;
+	tp -= cross(p.back, fx);
+	e4[0] = ex;
+	e4[1] = e4[2] = e4[3] = 0.f;
+
+	const f3 &a1 = p.a1, &a3 = p.a3, &b1 = q.a1, &b3 = q.a3;
+	f3 rhydro = r + q.base - p.base;
+	float rh2 = dot(rhydro, rhydro);
+
+	// hydrogen bonding
+	if(p.btype + q.btype == 3 && SQR(HYDR_RCLOW) < rh2 && rh2 < SQR(HYDR_RCHIGH)) {
+		float hb_multi = (abs(q.btype) >= 300 && abs(p.btype) >= 300) ? S.hb_multi : 1.f;
+		float rmod = sqrtf(rh2);
+		float inv_r = 1.f / rmod;
+		f3 dir = rhydro * inv_r;
+		float cost1 = -dot(a1, b1), cost2 = -dot(b1, dir), cost3 = dot(a1, dir);
+		float cost4 = dot(a3, b3), cost7 = -dot(b3, dir), cost8 = dot(a3, dir);
+		float vf1 = hb_multi * f1(S, rmod, 0, q.type, p.type);
+		float f4t1, f4t2, f4t3, f4t4, f4t7, f4t8, d1, d2, d3, d4, d7, d8;
+		cf4(cost1, S_HYDR_T1, f4t1, d1);
+		cf4(cost2, S_HYDR_T2, f4t2, d2);
+		cf4(cost3, S_HYDR_T2, f4t3, d3);
+		cf4(cost4, S_HYDR_T4, f4t4, d4);
+		cf4(cost7, S_HYDR_T7, f4t7, d7);
+		cf4(cost8, S_HYDR_T7, f4t8, d8);
+		float e = vf1 * f4t1 * f4t2 * f4t3 * f4t4 * f4t7 * f4t8;
+		e4[1] = e;
+		if(e != 0.f) {
+			float vf1D = hb_multi * f1D(S, rmod, 0, q.type, p.type);
+			float f4t1Dsin = d1, f4t2Dsin = d2, f4t3Dsin = -d3, f4t4Dsin = -d4, f4t7Dsin = d7, f4t8Dsin = -d8;
+			f3 force = -dir * (vf1D * f4t1 * f4t2 * f4t3 * f4t4 * f4t7 * f4t8);
+			f3 t = {0, 0, 0};
+			t -= cross(a3, b3) * (-vf1 * f4t1 * f4t2 * f4t3 * f4t4Dsin * f4t7 * f4t8);
+			t -= cross(a1, b1) * (-vf1 * f4t1Dsin * f4t2 * f4t3 * f4t4 * f4t7 * f4t8);
+			force += (b1 + dir * cost2) * (vf1 * f4t1 * f4t2Dsin * f4t3 * f4t4 * f4t7 * f4t8 * inv_r);
+			force += (a1 - dir * cost3) * (vf1 * f4t1 * f4t2 * f4t3Dsin * f4t4 * f4t7 * f4t8 * inv_r);
+			t += cross(dir, a1) * (-vf1 * f4t1 * f4t2 * f4t3Dsin * f4t4 * f4t7 * f4t8);
+			force += (b3 + dir * cost7) * (vf1 * f4t1 * f4t2 * f4t3 * f4t4 * f4t7Dsin * f4t8 * inv_r);
+			force += (a3 - dir * cost8) * (vf1 * f4t1 * f4t2 * f4t3 * f4t4 * f4t7 * f4t8Dsin * inv_r);
+			t += cross(dir, a3) * (-vf1 * f4t1 * f4t2 * f4t3 * f4t4 * f4t7 * f4t8Dsin);
+			t -= cross(p.base, force);
+			fq += force;
+			tp += t;
+		}
+	}
+
+	// cross stacking
+	if(SQR(CRST_RCLOW) < rh2 && rh2 < SQR(CRST_RCHIGH)) {
+		float rmod = sqrtf(rh2);
+		float inv_r = 1.f / rmod;
+		f3 dir = rhydro * inv_r;
+		float cost1 = -dot(a1, b1), cost2 = -dot(b1, dir), cost3 = dot(a1, dir);
+		float cost4 = dot(a3, b3), cost7 = -dot(b3, dir), cost8 = dot(a3, dir);
+		float vf2 = f2(rmod, 0);
+		float f4t1, f4t2, f4t3, d1, d2, d3, ga, gaD, gb, gbD;
+		cf4(cost1, S_CRST_T1, f4t1, d1);
+		cf4(cost2, S_CRST_T2, f4t2, d2);
+		cf4(cost3, S_CRST_T2, f4t3, d3);
+		cf4(cost4, S_CRST_T4, ga, gaD);
+		cf4(-cost4, S_CRST_T4, gb, gbD);
+		float f4t4 = ga + gb, f4t4Dsin = -gaD + gbD;
+		cf4(cost7, S_CRST_T7, ga, gaD);
+		cf4(-cost7, S_CRST_T7, gb, gbD);
+		float f4t7 = ga + gb, f4t7Dsin = gaD - gbD;
+		cf4(cost8, S_CRST_T7, ga, gaD);
+		cf4(-cost8, S_CRST_T7, gb, gbD);
+		float f4t8 = ga + gb, f4t8Dsin = -gaD + gbD;
+		float e = vf2 * f4t1 * f4t2 * f4t3 * f4t4 * f4t7 * f4t8;
+		e4[2] = e;
+		if(e != 0.f) {
+			float vf2D = f2D(rmod, 0);
+			float f4t1Dsin = d1, f4t2Dsin = d2, f4t3Dsin = -d3;
+			f3 force = -dir * (vf2D * f4t1 * f4t2 * f4t3 * f4t4 * f4t7 * f4t8);
+			f3 t = {0, 0, 0};
+			t -= cross(a1, b1) * (-vf2 * f4t1Dsin * f4t2 * f4t3 * f4t4 * f4t7 * f4t8);
+			force += (b1 + dir * cost2) * (vf2 * f4t1 * f4t2Dsin * f4t3 * f4t4 * f4t7 * f4t8 * inv_r);
+			force += (a1 - dir * cost3) * (vf2 * f4t1 * f4t2 * f4t3Dsin * f4t4 * f4t7 * f4t8 * inv_r);
+			t += cross(dir, a1) * (-vf2 * f4t1 * f4t2 * f4t3Dsin * f4t4 * f4t7 * f4t8);
+			t -= cross(a3, b3) * (-vf2 * f4t1 * f4t2 * f4t3 * f4t4Dsin * f4t7 * f4t8);
+			force += (b3 + dir * cost7) * (vf2 * f4t1 * f4t2 * f4t3 * f4t4 * f4t7Dsin * f4t8 * inv_r);
+			force += (a3 - dir * cost8) * (vf2 * f4t1 * f4t2 * f4t3 * f4t4 * f4t7 * f4t8Dsin * inv_r);
+			t += cross(dir, a3) * (-vf2 * f4t1 * f4t2 * f4t3 * f4t4 * f4t7 * f4t8Dsin);
+			t -= cross(p.base, force);
+			fq += force;
+			tp += t;
+		}
+	}
+
+	// coaxial stacking (oxDNA2 form)
+	f3 rstack = r + q.stack - p.stack;
+	float rs2 = dot(rstack, rstack);
+	if(SQR(CXST_RCLOW) < rs2 && rs2 < SQR(CXST_RCHIGH)) {
+		float rmod = sqrtf(rs2);
+		float inv_r = 1.f / rmod;
+		f3 dir = rstack * inv_r;
+		float cost1 = -dot(a1, b1), cost4 = dot(a3, b3), cost5 = dot(a3, dir), cost6 = -dot(b3, dir);
+		float vf2 = f2(rmod, 1);
+		float f4t1, d1, f4t4, d4, ga, gaD, gb, gbD;
+		cf4(cost1, S_CXST_T1, f4t1, d1);
+		cf4(cost4, S_CXST_T4, f4t4, d4);
+		cf4(cost5, S_CXST_T5, ga, gaD);
+		cf4(-cost5, S_CXST_T5, gb, gbD);
+		float f4t5 = ga + gb, f4t5Dsin = -gaD + gbD;
+		cf4(cost6, S_CXST_T5, ga, gaD);
+		cf4(-cost6, S_CXST_T5, gb, gbD);
+		float f4t6 = ga + gb, f4t6Dsin = gaD - gbD;
+		float e = vf2 * f4t1 * f4t4 * f4t5 * f4t6;
+		e4[3] = e;
+		if(e != 0.f) {
+			float vf2D = f2D(rmod, 1);
+			float f4t1Dsin = d1, f4t4Dsin = -d4;
+			f3 force = -dir * (vf2D * f4t1 * f4t4 * f4t5 * f4t6);
+			f3 t = {0, 0, 0};
+			t -= cross(a1, b1) * (-vf2 * f4t1Dsin * f4t4 * f4t5 * f4t6);
+			t -= cross(a3, b3) * (-vf2 * f4t1 * f4t4Dsin * f4t5 * f4t6);
+			float fact = vf2 * f4t1 * f4t4 * f4t5Dsin * f4t6;
+			force += (a3 - dir * cost5) * (fact * inv_r);
+			t -= cross(dir, a3) * fact;
+			fact = vf2 * f4t1 * f4t4 * f4t5 * f4t6Dsin;
+			force += (b3 + dir * cost6) * (fact * inv_r);
+			t -= cross(p.stack, force);
+			fq += force;
+			tp += t;
+		}
+	}
+}
+
+// Debye-Hueckel between backbone sites (src/Interactions/DNA2Interaction.cpp:
+// 157-212); cf = product of the two charge factors
+__device__ __forceinline__ float debye_huckel(const oxh_sys &S, f3 rback, float cf, f3 &fq) {
+	float rb2 = dot(rback, rback);
+	if(rb2 >= S.dh_rc * S.dh_rc) return 0.f;
+	float rbm = sqrtf(rb2);
+	float inv = 1.f / rbm;
+	float e;
+	if(rbm < S.dh_rhigh) {
+		float ex = expf(S.dh_minus_kappa * rbm);
+		e = ex * (S.dh_prefactor * inv);
+		fq = rback * (inv * (-S.dh_prefactor * ex) * (S.dh_minus_kappa * inv - inv * inv) * cf);
+	}
+	else {
+		e = S.dh_b * SQR(rbm - S.dh_rc);
+		fq = rback * (inv * (-2.f * S.dh_b * (rbm - S.dh_rc)) * cf);
+	}
+	return e * cf;
+}
+
+// FP32 nearest image of two wrapped backbone sites (far pairs only)
+__device__ __forceinline__ f3 min_image_f(float4 a, float4 b, const oxh_sys &S) {
+	float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+	dx -= S.boxf[0] * rintf(dx * S.inv_boxf[0]);
+	dy -= S.boxf[1] * rintf(dy * S.inv_boxf[1]);
+	dz -= S.boxf[2] * rintf(dz * S.inv_boxf[2]);
+	return mk(dx, dy, dz);
+}
+
+template<int K>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+	for(int off = K / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, K);
+	return v;
+}
+
+}  // namespace oxh
