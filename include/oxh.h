@@ -177,13 +177,17 @@ oxh_status oxh_batch_system_info(oxh_batch *b, int sys, long long *n_rebuilds, i
 oxh_status oxh_batch_set_state(oxh_batch *b, int sys, const double *pos, const double *a1, const double *a3, const double *vel,
 		const double *L);
 
-/* Neighbour-list, memory and launch-shape figures of the batch (8 doubles):
- * out[0] N, out[1] max_neigh, out[2] mean neighbours per particle,
- * out[3] device bytes held, out[4] threads per nucleotide and out[5] block
- * size of the close-pair force kernel, out[6] threads per nucleotide (0 =
- * inline) and out[7] block size of the far-pair kernel.  Shapes are picked
- * by timing at upload; OXH_FORCE_LANES / OXH_FAR_LANES fix them. */
-oxh_status oxh_batch_stats(oxh_batch *b, double *out8);
+/* Neighbour-list, memory and launch-shape figures of the batch (10
+ * doubles): out[0] N, out[1] max_neigh, out[2] mean neighbours per
+ * particle, out[3] device bytes held, out[4] threads per nucleotide and
+ * out[5] block size of the close-pair force kernel, out[6] threads per
+ * nucleotide (0 = inline) and out[7] block size of the far-pair kernel,
+ * out[8] engine of oxh_batch_run (0: one kernel per phase, 1: one
+ * workgroup per system for many steps per launch, used when every system
+ * has at most 512 nucleotides and no cells), out[9] number of systems.
+ * Shapes are picked by timing at upload; OXH_FORCE_LANES / OXH_FAR_LANES
+ * fix them, OXH_PERSIST=0 disables the workgroup engine. */
+oxh_status oxh_batch_stats(oxh_batch *b, double *out10);
 
 /* Average duration in ms of n back-to-back launches of the force kernel on
  * the batch's stream, timed with HIP events (the reference times its steps

@@ -194,10 +194,11 @@ class HIPBatch:
         return ms.value
 
     def stats(self):
-        out = np.zeros(8)
+        out = np.zeros(10)
         check(lib().oxh_batch_stats(self.h, ptr(out)), "oxh_batch_stats")
         return dict(N=int(out[0]), max_neigh=int(out[1]), mean_neigh=out[2], device_bytes=int(out[3]),
-                    lanes=int(out[4]), block=int(out[5]), far_lanes=int(out[6]), far_block=int(out[7]))
+                    lanes=int(out[4]), block=int(out[5]), far_lanes=int(out[6]), far_block=int(out[7]),
+                    engine="workgroup" if out[8] else "grid", n_systems=int(out[9]))
 
 
 def hilbert_keys(xyz: np.ndarray, bits: int, device: int = 0) -> np.ndarray:

@@ -38,6 +38,11 @@ struct oxh_batch {
 	// Hilbert re-sort every sort_interval steps (0 = never), checked at
 	// chunk boundaries
 	long long sort_interval = 0, since_sort = 0;
+	// workgroup-resident engine (every system small and without cells;
+	// OXH_PERSIST=0 selects the one-kernel-per-phase path instead), steps
+	// per launch
+	bool persist = false;
+	int persist_chunk = 256;
 };
 
 namespace {
@@ -227,6 +232,12 @@ void enqueue_step(oxh_batch *B, long long step_index) {
 	}
 }
 
+// forces of the current state with the engine oxh_batch_run uses
+void enqueue_forces(oxh_batch *B, bool split = false) {
+	if(B->persist) oxh::launch_persist_forces(B->b.stream, B->b, split);
+	else oxh::launch_forces(B->b.stream, B->b, split);
+}
+
 __global__ void k_clear_err(int nsys, oxh_sys_state *st, int mask) {
 	int s = blockIdx.x * blockDim.x + threadIdx.x;
 	if(s < nsys) st[s].error &= ~mask;
@@ -267,7 +278,7 @@ oxh_status grow_lists(oxh_batch *B, const std::vector<oxh_sys_state> &st) {
 oxh_status build_all(oxh_batch *B) {
 	for(int attempt = 0; attempt < 12; attempt++) {
 		enqueue_lists(B, true);
-		oxh::launch_forces(B->b.stream, B->b, false);
+		enqueue_forces(B);
 		k_clear_need<<<(B->b.nsys + 63) / 64, 64, 0, B->b.stream>>>(B->b.nsys, B->b.d_state, 0);
 		std::vector<oxh_sys_state> st;
 		TRY(read_states(B, st));
@@ -335,6 +346,8 @@ oxh_status autotune_forces(oxh_batch *B) {
 		if(const char *f = getenv("OXH_FAR_LANES")) b.far_lanes = atoi(f);
 		return OXH_OK;
 	}
+	// the workgroup engine fixes its own lanes per system
+	if(B->persist) return OXH_OK;
 	hipEvent_t e0, e1;
 	HIPCHK(hipEventCreate(&e0));
 	HIPCHK(hipEventCreate(&e1));
@@ -385,7 +398,7 @@ oxh_status autotune_forces(oxh_batch *B) {
 	b.far_lanes = bfl;
 	b.far_block = bfb;
 	// forces of the current state with the chosen shapes
-	oxh::launch_forces(b.stream, b, false);
+	enqueue_forces(B);
 	return sync_and_check(B);
 }
 
@@ -732,6 +745,9 @@ oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs,
 		(void) up(b.Lm, nL.data(), N * sizeof(double4));
 	}
 	oxh::launch_prepare_mirror(st, b);
+	B->persist = true;
+	for(const oxh_sys &S : b.h_sys) B->persist &= !S.use_cells && S.n <= oxh::PERSIST_MAX_N;
+	if(const char *p = getenv("OXH_PERSIST")) B->persist &= atoi(p) != 0;
 	oxh_status s = build_all(B);
 	if(s != OXH_OK) return bail(s);
 	s = autotune_forces(B);
@@ -765,7 +781,7 @@ oxh_status oxh_batch_update_lists(oxh_batch *B, int force) {
 oxh_status oxh_batch_compute_forces(oxh_batch *B) {
 	if(!B) return fail(OXH_ERR_ARG, "null batch");
 	HIPCHK(hipSetDevice(B->b.device));
-	oxh::launch_forces(B->b.stream, B->b, false);
+	enqueue_forces(B);
 	return sync_and_check(B);
 }
 
@@ -777,15 +793,18 @@ oxh_status oxh_batch_run(oxh_batch *B, long long n_steps) {
 	long long done = 0;
 	int retries = 0;
 	while(done < n_steps) {
-		if(B->sort_interval > 0 && B->since_sort >= B->sort_interval) {
+		if(B->sort_interval > 0 && B->since_sort >= B->sort_interval && !B->persist) {
 			if(oxh::launch_hilbert_sort(b.stream, b) != 0) return fail(OXH_ERR_ARG, "too many systems for the sort key");
 			TRY(build_all(B));
 			B->since_sort = 0;
 		}
-		long long todo = std::min<long long>(B->chunk, n_steps - done);
+		long long todo = std::min<long long>(B->persist ? B->persist_chunk : B->chunk, n_steps - done);
 		bool aligned = !B->any_bussi || !B->bussi_aligned || (b.step % B->bussi_ns == 0);
 		TRY(backup(B));
-		if(todo == B->chunk && aligned && B->use_graph) {
+		if(B->persist) {
+			oxh::launch_persist(b.stream, b, (int) todo);
+		}
+		else if(todo == B->chunk && aligned && B->use_graph) {
 			if(!b.graph_exec) {
 				hipGraph_t g;
 				HIPCHK(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
@@ -922,7 +941,7 @@ oxh_status oxh_batch_energies(oxh_batch *B, double *out4) {
 
 oxh_status oxh_batch_energy_split(oxh_batch *B, int sys, double *E8) {
 	TRY(check_sys(B, sys));
-	oxh::launch_forces(B->b.stream, B->b, true);
+	enqueue_forces(B, true);
 	int f = B->first[sys], n = B->n[sys];
 	std::vector<float> e((size_t) 8 * n);
 	for(int t = 0; t < 8; t++)
@@ -941,7 +960,7 @@ oxh_status oxh_batch_hb_count(oxh_batch *B, int *out) {
 	if(!B || !out) return fail(OXH_ERR_ARG, "bad arguments");
 	HIPCHK(hipSetDevice(B->b.device));
 	B->b.want_hbcount = true;
-	oxh::launch_forces(B->b.stream, B->b, false);
+	enqueue_forces(B);
 	B->b.want_hbcount = false;
 	std::vector<int> h(B->b.N);
 	HIPCHK(hipMemcpyAsync(h.data(), B->b.hbcount, B->b.N * sizeof(int), hipMemcpyDeviceToHost, B->b.stream));
@@ -1029,7 +1048,7 @@ oxh_status oxh_batch_set_state(oxh_batch *B, int sys, const double *pos, const d
 	return oxh_batch_compute_forces(B);
 }
 
-oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {  // 8 doubles
+oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {  // 10 doubles
 	if(!B || !out6) return fail(OXH_ERR_ARG, "bad arguments");
 	HIPCHK(hipSetDevice(B->b.device));
 	std::vector<int2> nn;
@@ -1045,6 +1064,8 @@ oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {  // 8 doubles
 	out6[5] = B->b.force_block;
 	out6[6] = B->b.far_lanes;
 	out6[7] = B->b.far_block;
+	out6[8] = B->persist ? 1.0 : 0.0;
+	out6[9] = B->b.nsys;
 	return OXH_OK;
 }
 
@@ -1058,6 +1079,8 @@ oxh_status oxh_batch_time_forces(oxh_batch *B, int n_launches, double *avg_ms) {
 	HIPCHK(hipEventRecord(e0, B->b.stream));
 	for(int k = 0; k < n_launches; k++) oxh::launch_forces(B->b.stream, B->b, false);
 	HIPCHK(hipEventRecord(e1, B->b.stream));
+	// leave the forces of the engine that runs the steps
+	if(B->persist) enqueue_forces(B);
 	HIPCHK(hipEventSynchronize(e1));
 	float ms = 0.f;
 	HIPCHK(hipEventElapsedTime(&ms, e0, e1));

@@ -38,133 +38,7 @@ __global__ void __launch_bounds__(256) k_forces(int N, const double4 *__restrict
 	const int l = t % K;
 	if(i >= N) return;
 
-	const int2 pi = pinfo[i];
-	const oxh_sys &S = systems[pi.y];
-	const double4 P = pos[i];
-	const Frame fp = make_frame(q32[i], pi.x);
-	const int2 bd = bonds[i];
-
-	f3 F = {0, 0, 0}, T = {0, 0, 0};
-	float E = 0.f, Ehb = 0.f;
-	float es[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-	int err = 0;
-	int nhb = 0;
-
-	// bonded neighbours: i is the 5' partner of n3 and the 3' partner of n5
-	if(l == 0 && bd.x >= 0) {
-		int j = bd.x;
-		Frame fq = make_frame(q32[j], pinfo[j].x);
-		f3 r = min_image(P, pos[j], S);
-		float e3[3];
-		PairOut o = bonded_pair(S, fp, fq, r, e3, err);
-		F -= o.force;
-		T += o.tp;
-		E += 0.5f * o.e;
-		if(SPLIT) {
-			es[0] += 0.5f * e3[0];
-			es[1] += 0.5f * e3[1];
-			es[2] += 0.5f * e3[2];
-		}
-	}
-	if(l == (K > 1 ? 1 : 0) && bd.y >= 0) {
-		int j = bd.y;
-		Frame fq = make_frame(q32[j], pinfo[j].x);
-		f3 r = min_image(pos[j], P, S);
-		float e3[3];
-		PairOut o = bonded_pair(S, fq, fp, r, e3, err);
-		F += o.force;
-		T += o.tq;
-		E += 0.5f * o.e;
-		if(SPLIT) {
-			es[0] += 0.5f * e3[0];
-			es[1] += 0.5f * e3[1];
-			es[2] += 0.5f * e3[2];
-		}
-	}
-
-	const int2 nn = nnbr[i];
-	const int *row = nbr + (size_t) i * max_neigh;
-	const float cf_i = bpos[i].w;
-
-	// close sublist: every term
-	for(int k = l; k < nn.x; k += K) {
-		const int j = row[k];
-		const f3 r = min_image(P, pos[j], S);
-		const Frame fq = make_frame(q32[j], pinfo[j].x);
-		f3 fq_ = {0, 0, 0};
-		float edh = debye_huckel(S, r + fq.back - fp.back, cf_i * bpos[j].w, fq_);
-		f3 tp = -cross(fp.back, fq_);
-		float e = edh;
-		if(SPLIT) es[7] += 0.5f * edh;
-		if(dot(r, r) < R_CLOSE * R_CLOSE) {
-			float e4[4];
-			close_pair(S, fp, fq, r, fq_, tp, e4);
-			e += e4[0] + e4[1] + e4[2] + e4[3];
-			Ehb += 0.5f * e4[1];
-			if(hbcount && e4[1] < -0.1f) nhb++;
-			if(SPLIT) {
-				es[3] += 0.5f * e4[0];
-				es[4] += 0.5f * e4[1];
-				es[5] += 0.5f * e4[2];
-				es[6] += 0.5f * e4[3];
-			}
-		}
-		F -= fq_;
-		T += tp;
-		E += 0.5f * e;
-	}
-
-	// far sublist: Debye-Hueckel between backbone sites only (or left to
-	// k_forces_far when FAR is false)
-	const float4 Bi = bpos[i];
-	f3 Ff = {0, 0, 0};
-	float Ef = 0.f;
-	for(int k = l; FAR && k < nn.y; k += K) {
-		const int j = row[max_neigh - 1 - k];
-		const float4 Bj = bpos[j];
-		f3 rb;
-		if(!S.small_box) rb = min_image_f(Bi, Bj, S);
-		else {
-			// the reference's image choice: nearest image of the centres,
-			// then the site offsets (src/Interactions/DNA2Interaction.cpp:169)
-			const Frame fq = make_frame(q32[j], 0);
-			rb = min_image(P, pos[j], S) + fq.back - fp.back;
-		}
-		f3 fq_ = {0, 0, 0};
-		float edh = debye_huckel(S, rb, cf_i * Bj.w, fq_);
-		Ff += fq_;
-		Ef += edh;
-	}
-	F -= Ff;
-	T -= cross(fp.back, Ff);
-	E += 0.5f * Ef;
-	if(SPLIT) es[7] += 0.5f * Ef;
-
-	if(K > 1) {
-		F.x = group_sum<K>(F.x);
-		F.y = group_sum<K>(F.y);
-		F.z = group_sum<K>(F.z);
-		T.x = group_sum<K>(T.x);
-		T.y = group_sum<K>(T.y);
-		T.z = group_sum<K>(T.z);
-		E = group_sum<K>(E);
-		Ehb = group_sum<K>(Ehb);
-		if(SPLIT)
-			for(int q = 0; q < 8; q++) es[q] = group_sum<K>(es[q]);
-		if(hbcount) nhb = (int) group_sum<K>((float) nhb);
-		err = (int) group_sum<K>((float) err);
-	}
-	if(l != 0) return;
-
-	// torque into the body frame: orientationT * T_lab
-	f3 Tb = {dot(fp.a1, T), dot(fp.a2, T), dot(fp.a3, T)};
-	Fout[i] = make_float4(F.x, F.y, F.z, E);
-	Tout[i] = make_float4(Tb.x, Tb.y, Tb.z, Ehb);
-	if(SPLIT) {
-		for(int q = 0; q < 8; q++) esplit[(size_t) q * N + i] = es[q];
-	}
-	if(hbcount) hbcount[i] = nhb;
-	if(err) atomicOr(&sstate[pi.y].error, 1);
+	particle_forces<SPLIT, FAR>(K, i, l, N, pos, q32, bpos, bonds, pinfo, nbr, nnbr, max_neigh, systems, sstate, Fout, Tout, esplit, hbcount);
 }
 
 // Far pairs on their own: Debye-Hueckel between backbone sites, few
@@ -205,10 +79,10 @@ __global__ void __launch_bounds__(256) k_forces_far(int N, const double4 *__rest
 		Ff += fq_;
 	}
 	if(K > 1) {
-		Ff.x = group_sum<K>(Ff.x);
-		Ff.y = group_sum<K>(Ff.y);
-		Ff.z = group_sum<K>(Ff.z);
-		Ef = group_sum<K>(Ef);
+		Ff.x = group_sum(Ff.x, K);
+		Ff.y = group_sum(Ff.y, K);
+		Ff.z = group_sum(Ff.z, K);
+		Ef = group_sum(Ef, K);
 	}
 	if(l != 0) return;
 	f3 a1, a2, a3;

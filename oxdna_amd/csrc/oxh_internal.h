@@ -82,6 +82,23 @@ struct Batch {
 constexpr int RED_CHUNK = 2048;
 constexpr int RED_W = 8;   // reduction width: U, Uhb, Kt, Kr, vx, vy, vz, count
 
+// workgroup-resident engine: one workgroup of PERSIST_BLOCK threads per
+// system, for batches whose systems all have at most PERSIST_MAX_N
+// particles and no cells
+constexpr int PERSIST_BLOCK = 512;
+constexpr int PERSIST_MAX_N = 512;
+
+// lanes per nucleotide of the workgroup-resident force phase: the largest
+// power of two <= 16 with n * K <= PERSIST_BLOCK
+__host__ __device__ inline int persist_lanes(int n) {
+	int k = 16;
+	while(k > 1 && k * n > PERSIST_BLOCK) k >>= 1;
+	return k;
+}
+
+void launch_persist(hipStream_t st, const Batch &b, int nsteps);
+void launch_persist_forces(hipStream_t st, const Batch &b, bool split);
+
 void launch_forces(hipStream_t st, const Batch &b, bool split);
 void launch_cells(hipStream_t st, const Batch &b);
 void launch_cells_fill_sort(hipStream_t st, const Batch &b);

@@ -370,11 +370,155 @@ __device__ __forceinline__ f3 min_image_f(float4 a, float4 b, const oxh_sys &S) 
 	return mk(dx, dy, dz);
 }
 
-template<int K>
-__device__ __forceinline__ float group_sum(float v) {
+// sum over the K lanes of a group (K a power of two, groups aligned to K
+// inside the wave), fixed butterfly order
+__device__ __forceinline__ float group_sum(float v, int K) {
 #pragma unroll
 	for(int off = K / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, K);
 	return v;
+}
+
+// Forces, torques and energy of particle i from the K threads l = 0..K-1
+// of its group (consecutive lanes of one wave, K | 64): lane 0 takes the
+// 3' bond, lane 1 the 5' bond, neighbour k goes to lane k mod K; the lanes'
+// partial sums are combined by a fixed butterfly, so the result does not
+// depend on timing.  Lane 0 writes F (xyz, .w = energy) and the body-frame
+// torque (.w = HB energy).  Far pairs are added here when FAR, else by
+// k_forces_far.  Every lane of the group must call it.
+// K may be a run-time value (the workgroup-resident engine) or a constant
+// (the grid kernels, where it folds into the loop strides and shuffles).
+template<bool SPLIT, bool FAR>
+__device__ __forceinline__ void particle_forces(int K, int i, int l, int N, const double4 *__restrict__ pos, const float4 *__restrict__ q32,
+		const float4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo, const int *__restrict__ nbr,
+		const int2 *__restrict__ nnbr, int max_neigh, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ sstate,
+		float4 *__restrict__ Fout, float4 *__restrict__ Tout, float *__restrict__ esplit, int *__restrict__ hbcount) {
+	const int2 pi = pinfo[i];
+	const oxh_sys &S = systems[pi.y];
+	const double4 P = pos[i];
+	const Frame fp = make_frame(q32[i], pi.x);
+	const int2 bd = bonds[i];
+
+	f3 F = {0, 0, 0}, T = {0, 0, 0};
+	float E = 0.f, Ehb = 0.f;
+	float es[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	int err = 0;
+	int nhb = 0;
+
+	// bonded neighbours: i is the 5' partner of n3 and the 3' partner of n5
+	if(l == 0 && bd.x >= 0) {
+		int j = bd.x;
+		Frame fq = make_frame(q32[j], pinfo[j].x);
+		f3 r = min_image(P, pos[j], S);
+		float e3[3];
+		PairOut o = bonded_pair(S, fp, fq, r, e3, err);
+		F -= o.force;
+		T += o.tp;
+		E += 0.5f * o.e;
+		if(SPLIT) {
+			es[0] += 0.5f * e3[0];
+			es[1] += 0.5f * e3[1];
+			es[2] += 0.5f * e3[2];
+		}
+	}
+	if(l == (K > 1 ? 1 : 0) && bd.y >= 0) {
+		int j = bd.y;
+		Frame fq = make_frame(q32[j], pinfo[j].x);
+		f3 r = min_image(pos[j], P, S);
+		float e3[3];
+		PairOut o = bonded_pair(S, fq, fp, r, e3, err);
+		F += o.force;
+		T += o.tq;
+		E += 0.5f * o.e;
+		if(SPLIT) {
+			es[0] += 0.5f * e3[0];
+			es[1] += 0.5f * e3[1];
+			es[2] += 0.5f * e3[2];
+		}
+	}
+
+	const int2 nn = nnbr[i];
+	const int *row = nbr + (size_t) i * max_neigh;
+	const float cf_i = bpos[i].w;
+
+	// close sublist: every term
+	for(int k = l; k < nn.x; k += K) {
+		const int j = row[k];
+		const f3 r = min_image(P, pos[j], S);
+		const Frame fq = make_frame(q32[j], pinfo[j].x);
+		f3 fq_ = {0, 0, 0};
+		float edh = debye_huckel(S, r + fq.back - fp.back, cf_i * bpos[j].w, fq_);
+		f3 tp = -cross(fp.back, fq_);
+		float e = edh;
+		if(SPLIT) es[7] += 0.5f * edh;
+		if(dot(r, r) < R_CLOSE * R_CLOSE) {
+			float e4[4];
+			close_pair(S, fp, fq, r, fq_, tp, e4);
+			e += e4[0] + e4[1] + e4[2] + e4[3];
+			Ehb += 0.5f * e4[1];
+			if(hbcount && e4[1] < -0.1f) nhb++;
+			if(SPLIT) {
+				es[3] += 0.5f * e4[0];
+				es[4] += 0.5f * e4[1];
+				es[5] += 0.5f * e4[2];
+				es[6] += 0.5f * e4[3];
+			}
+		}
+		F -= fq_;
+		T += tp;
+		E += 0.5f * e;
+	}
+
+	// far sublist: Debye-Hueckel between backbone sites only (or left to
+	// k_forces_far when FAR is false)
+	const float4 Bi = bpos[i];
+	f3 Ff = {0, 0, 0};
+	float Ef = 0.f;
+	for(int k = l; FAR && k < nn.y; k += K) {
+		const int j = row[max_neigh - 1 - k];
+		const float4 Bj = bpos[j];
+		f3 rb;
+		if(!S.small_box) rb = min_image_f(Bi, Bj, S);
+		else {
+			// the reference's image choice: nearest image of the centres,
+			// then the site offsets (src/Interactions/DNA2Interaction.cpp:169)
+			const Frame fq = make_frame(q32[j], 0);
+			rb = min_image(P, pos[j], S) + fq.back - fp.back;
+		}
+		f3 fq_ = {0, 0, 0};
+		float edh = debye_huckel(S, rb, cf_i * Bj.w, fq_);
+		Ff += fq_;
+		Ef += edh;
+	}
+	F -= Ff;
+	T -= cross(fp.back, Ff);
+	E += 0.5f * Ef;
+	if(SPLIT) es[7] += 0.5f * Ef;
+
+	if(K > 1) {
+		F.x = group_sum(F.x, K);
+		F.y = group_sum(F.y, K);
+		F.z = group_sum(F.z, K);
+		T.x = group_sum(T.x, K);
+		T.y = group_sum(T.y, K);
+		T.z = group_sum(T.z, K);
+		E = group_sum(E, K);
+		Ehb = group_sum(Ehb, K);
+		if(SPLIT)
+			for(int q = 0; q < 8; q++) es[q] = group_sum(es[q], K);
+		if(hbcount) nhb = (int) group_sum((float) nhb, K);
+		err = (int) group_sum((float) err, K);
+	}
+	if(l != 0) return;
+
+	// torque into the body frame: orientationT * T_lab
+	f3 Tb = {dot(fp.a1, T), dot(fp.a2, T), dot(fp.a3, T)};
+	Fout[i] = make_float4(F.x, F.y, F.z, E);
+	Tout[i] = make_float4(Tb.x, Tb.y, Tb.z, Ehb);
+	if(SPLIT) {
+		for(int q = 0; q < 8; q++) esplit[(size_t) q * N + i] = es[q];
+	}
+	if(hbcount) hbcount[i] = nhb;
+	if(err) atomicOr(&sstate[pi.y].error, 1);
 }
 
 }  // namespace oxh

@@ -1,0 +1,166 @@
+// Workgroup-resident MD engine for small systems on gfx950.
+//
+// One workgroup integrates one system for a whole chunk of steps in a
+// single launch: the phases of a step (first half kick + drift + rotation,
+// skin check, Verlet rebuild when due, forces, second half kick +
+// thermostat, Bussi rescaling) are separated by workgroup barriers instead
+// of kernel launches.  A system of a few dozen nucleotides costs microseconds
+// per step on one CU, so a batch of independent small systems (hairpins,
+// short duplexes, short served jobs) runs one system per CU with no launch
+// or grid-synchronisation cost per step.
+//
+// Every phase calls the same per-particle device functions as the
+// one-kernel-per-phase path (oxh_step.h, oxh_list.h, oxh_pair.h) on the
+// same HBM arrays, so with the same lanes per nucleotide both paths give the
+// same bits.  The lanes per nucleotide depend only on the system's size,
+// so a system's trajectory does not depend on the other systems of the
+// batch.
+//
+// Reference semantics: the per-step sequence of MD_CUDABackend::sim_step
+// (src/CUDA/Backends/MD_CUDABackend.cu:567-619) with the mixed-precision
+// kernels (src/CUDA/Backends/CUDA_mixed.cuh:23-93), the simple Verlet list
+// (src/CUDA/Lists/CUDA_simple_verlet.cuh:40-82) and the thermostats
+// (src/CUDA/Thermostats/CUDA{Langevin,Bussi,Brownian}Thermostat.cu).
+#include "oxh_device.h"
+#include "oxh_internal.h"
+#include "oxh_list.h"
+#include "oxh_pair.h"
+#include "oxh_step.h"
+
+namespace oxh {
+
+namespace {
+
+struct PersistArgs {
+	int N, max_neigh, nsteps, cell_cap;
+	double4 *pos, *vel, *Lm, *q64, *rlist;
+	float4 *q32, *F, *T, *bpos;
+	const int2 *bonds, *pinfo;
+	const int *orig;
+	long long *pstep;
+	int *nbr;
+	int2 *nnbr;
+	const oxh_sys *systems;
+	oxh_sys_state *state;
+	const int *cell_count, *cell_items;
+	const float4 *pos32;
+	int *hbcount;
+};
+
+__global__ void __launch_bounds__(PERSIST_BLOCK) k_persist(PersistArgs a) {
+	__shared__ double sh[5][256];
+	__shared__ double fac[5];
+	__shared__ int need;
+	const int s = blockIdx.x;
+	const oxh_sys &S = a.systems[s];
+	const int first = S.first, n = S.n;
+	const int tid = threadIdx.x;
+	const int K = persist_lanes(n);
+	const long long step0 = a.pstep[first];
+	if(tid == 0) need = 0;
+	__syncthreads();
+	for(int it = 0; it < a.nsteps; it++) {
+		// 1. first half kick, drift, rotation, FP32 mirror, skin check
+		if(tid < n && first_step_particle(first + tid, S, a.pos, a.vel, a.Lm, a.q64, a.q32, a.bpos, a.F, a.T, a.rlist, a.bonds, a.pstep))
+			need = 1;
+		__syncthreads();
+		// 2. Verlet rebuild (all pairs of the system; such systems use no cells)
+		const bool rebuild = need != 0;
+		if(rebuild) {
+			for(int u = tid; u < n * BUILD_LANES; u += PERSIST_BLOCK)
+				build_row<BUILD_LANES>(first + u / BUILD_LANES, u % BUILD_LANES, s, a.pos, a.bonds, a.systems, a.state, a.cell_count,
+						a.cell_items, a.cell_cap, a.nbr, a.nnbr, a.max_neigh, a.rlist, a.pos32);
+			__syncthreads();
+		}
+		// 3. forces and torques, K lanes per nucleotide, far pairs inline
+		if(tid < n * K)
+			particle_forces<false, true>(K, first + tid / K, tid % K, a.N, a.pos, a.q32, a.bpos, a.bonds, a.pinfo, a.nbr, a.nnbr,
+					a.max_neigh, a.systems, a.state, a.F, a.T, nullptr, a.hbcount);
+		__syncthreads();
+		// 4. second half kick + Langevin / Brownian
+		if(tid < n) second_step_particle(first + tid, S, a.vel, a.Lm, a.F, a.T, a.orig, a.pstep);
+		if(tid == 0) {
+			if(rebuild) a.state[s].n_rebuilds += 1;
+			need = 0;
+		}
+		// 5. Bussi: kinetic energies about the centre of mass, new targets,
+		// rescaling
+		const long long step = step0 + it;
+		if(S.thermostat == 2 && step % S.newtonian_steps == 0) {
+			__syncthreads();
+			bussi_chunk_sums(first, first + n, a.vel, a.Lm, sh);
+			if(tid == 0) {
+				double r[5];
+				for(int k = 0; k < 5; k++) r[k] = 0.0 + sh[k][0];
+				double ft, fr, vc[3];
+				bussi_update(S, a.state[s], step, r, ft, fr, vc);
+				fac[0] = ft;
+				fac[1] = fr;
+				fac[2] = vc[0];
+				fac[3] = vc[1];
+				fac[4] = vc[2];
+			}
+			__syncthreads();
+			if(tid < n) bussi_rescale_particle(first + tid, a.vel, a.Lm, fac[0], fac[1], fac[2], fac[3], fac[4]);
+		}
+		__syncthreads();
+	}
+}
+
+// Forces of the current state with the workgroup engine's lanes per
+// nucleotide (after uploads, list rebuilds and for observables), so that
+// runs continue from exactly the forces the engine itself computes.
+template<bool SPLIT>
+__global__ void __launch_bounds__(PERSIST_BLOCK) k_persist_forces(PersistArgs a, float *esplit) {
+	const oxh_sys &S = a.systems[blockIdx.x];
+	const int K = persist_lanes(S.n);
+	const int tid = threadIdx.x;
+	if(tid < S.n * K)
+		particle_forces<SPLIT, true>(K, S.first + tid / K, tid % K, a.N, a.pos, a.q32, a.bpos, a.bonds, a.pinfo, a.nbr, a.nnbr,
+				a.max_neigh, a.systems, a.state, a.F, a.T, esplit, a.hbcount);
+}
+
+PersistArgs make_args(const Batch &b, int nsteps) {
+	PersistArgs a;
+	a.N = b.N;
+	a.max_neigh = b.max_neigh;
+	a.nsteps = nsteps;
+	a.cell_cap = b.cell_cap;
+	a.pos = b.pos;
+	a.vel = b.vel;
+	a.Lm = b.Lm;
+	a.q64 = b.q64;
+	a.rlist = b.rlist;
+	a.q32 = b.q32;
+	a.F = b.F;
+	a.T = b.T;
+	a.bpos = b.bpos;
+	a.bonds = b.bonds;
+	a.pinfo = b.pinfo;
+	a.orig = b.orig;
+	a.pstep = b.pstep;
+	a.nbr = b.nbr;
+	a.nnbr = b.nnbr;
+	a.systems = b.d_sys;
+	a.state = b.d_state;
+	a.cell_count = b.cell_count;
+	a.cell_items = b.cell_items;
+	a.pos32 = b.pos32;
+	a.hbcount = b.want_hbcount ? b.hbcount : nullptr;
+	return a;
+}
+
+}  // namespace
+
+void launch_persist(hipStream_t st, const Batch &b, int nsteps) {
+	if(b.N == 0 || nsteps <= 0) return;
+	k_persist<<<b.nsys, PERSIST_BLOCK, 0, st>>>(make_args(b, nsteps));
+}
+
+void launch_persist_forces(hipStream_t st, const Batch &b, bool split) {
+	if(b.N == 0) return;
+	if(split) k_persist_forces<true><<<b.nsys, PERSIST_BLOCK, 0, st>>>(make_args(b, 0), b.esplit);
+	else k_persist_forces<false><<<b.nsys, PERSIST_BLOCK, 0, st>>>(make_args(b, 0), nullptr);
+}
+
+}  // namespace oxh
