@@ -14,6 +14,10 @@ Workloads:
   plain2   1,024 random-sequence 96-bp duplexes (196,608 nt), hexagonal
            lattice of axis spacing 2.6, T = 30 C, 0.2 M, Langevin
            (diff_coeff 2.5), skin 0.05, dt 0.003
+  batched  256 hairpins of 16-60 nt in their own 30-unit boxes, 25-75 C,
+           one launch for all systems
+  served   a stream of jobs of 40-20,000 nt through resident slot pools
+           (continuous batching), against the plain path one job at a time
 
 value = nucleotide-steps/s of the whole job, inputs resident in HBM.
 roofline: the force kernel (dominant), algorithmic bytes per launch /
@@ -31,6 +35,8 @@ import subprocess
 import sys
 import tempfile
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -107,6 +113,95 @@ def cpu_baseline(s, ref_keys, steps_lo: int, steps_hi: int):
                        f"{dt:.1f} s, ms/step {1e3 * dt / n:.2f}")
 
 
+def served_main(args):
+    """Served stream: jobs of log-uniform size 40-20,000 nt (single
+    strands, duplexes, hairpins, bundles), T 20-60 C, Langevin, with step
+    counts log-uniform over [1e4, 1e6] x step_scale.  First the plain path
+    runs the jobs one at a time (grid engine, one batch per job, as the
+    reference runs one system per process); then the server takes the same
+    jobs as a Poisson stream at rate_mult x the plain path's job rate, and
+    once more with every job arriving at t = 0 (capacity)."""
+    from oxdna_amd import backend, serve, systems
+    rng_seed = args.seed
+    jobs = []
+    for k in range(args.jobs):
+        s, steps = systems.served_job(k, seed=rng_seed)
+        steps = max(100, int(round(steps * args.step_scale)))
+        opts = dict(thermostat="langevin", diff_coeff=2.5, dt=0.003, verlet_skin=0.05, refresh_vel=True,
+                    seed=(rng_seed * 1000003 + k) & 0xFFFFFFFF)
+        jobs.append(serve.Job(k, s, steps, opts=opts))
+    work = sum(j.work for j in jobs)
+    sizes = np.array([j.system.N for j in jobs])
+    log(f"served: {len(jobs)} jobs, N {sizes.min()}-{sizes.max()} (median {int(np.median(sizes))}), "
+        f"{work:.3e} nt-steps")
+
+    # plain path, one job at a time
+    os.environ["OXH_PERSIST"] = "0"
+    t_setup = 0.0
+    t_run = 0.0
+    t0 = time.perf_counter()
+    for j in jobs:
+        ta = time.perf_counter()
+        b = backend.HIPBatch(j.system, **j.opts)
+        tb = time.perf_counter()
+        b.run(j.steps)
+        b.sync()
+        tc = time.perf_counter()
+        b.close()
+        t_setup += tb - ta
+        t_run += tc - tb
+    t_plain = time.perf_counter() - t0
+    del os.environ["OXH_PERSIST"]
+    plain = work / t_plain
+    log(f"plain one at a time: {t_plain:.2f} s ({t_setup:.2f} s upload/setup), {plain:.3e} nt-steps/s; "
+        f"stepping only {work / t_run:.3e}")
+
+    srv = serve.Server(budget_bytes=args.budget_gb << 30)
+    hbm = srv.hbm_bytes()
+    rate = args.rate_mult * len(jobs) / t_plain
+    arr = serve.poisson_arrivals(len(jobs), rate, seed=rng_seed)
+    for j, a in zip(jobs, arr):
+        j.arrival = float(a)
+    res = srv.run(jobs)
+    log(f"served at {rate:.2f} jobs/s ({args.rate_mult}x plain): makespan {res['makespan']:.2f} s, "
+        f"{res['nt_steps_per_s']:.3e} nt-steps/s, p50 {res['p50']:.3f} s p99 {res['p99']:.3f} s")
+    for j in jobs:
+        j.arrival = 0.0
+        j.admitted = j.finished = None
+    cap = srv.run(jobs)
+    log(f"served, all at t=0: makespan {cap['makespan']:.2f} s, {cap['nt_steps_per_s']:.3e} nt-steps/s")
+    srv.close()
+    out = {
+        "metric": "nucleotide-steps/s",
+        "value": res["nt_steps_per_s"],
+        "unit": "nucleotide-steps/s",
+        "n_gpus": 1,
+        "steps": int(sum(j.steps for j in jobs)),
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 forces / f64 state",
+        "data": "synthetic",
+        "config": {"workload": f"served stream: {len(jobs)} jobs, 40-20,000 nt log-uniform (single strands, duplexes, "
+                               f"hairpins, bundles), 20-60 C, 0.5 M, Langevin, dt 0.003, steps log-uniform "
+                               f"[1e4, 1e6] x {args.step_scale}, Poisson arrivals at {args.rate_mult}x the plain "
+                               f"path's job rate",
+                   "form": "served (slot pools, continuous batching)", "n_jobs": len(jobs), "work_nt_steps": work,
+                   "arrival_rate_jobs_per_s": rate, "hbm_bytes_resident": hbm, "parallelism": "none"},
+        "served": {"p50_latency_s": res["p50"], "p99_latency_s": res["p99"], "makespan_s": res["makespan"],
+                   "capacity_nt_steps_per_s": cap["nt_steps_per_s"], "capacity_makespan_s": cap["makespan"]},
+        "plain_one_at_a_time": {"value": plain, "seconds": t_plain, "setup_seconds": t_setup,
+                                "stepping_only_value": work / t_run},
+        "served_over_plain": res["nt_steps_per_s"] / plain,
+        "capacity_over_plain": cap["nt_steps_per_s"] / plain,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +212,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--sort-every", type=int, default=500, help="Hilbert re-sort interval in steps (0 = never)")
+    ap.add_argument("--jobs", type=int, default=64, help="served: number of jobs")
+    ap.add_argument("--step-scale", type=float, default=0.01, help="served: factor on the jobs' step counts")
+    ap.add_argument("--rate-mult", type=float, default=4.0, help="served: arrival rate / plain path's job rate")
+    ap.add_argument("--budget-gb", type=int, default=8, help="served: HBM budget of the resident pools")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (systems are independent and a "
@@ -126,6 +225,8 @@ def main():
     from oxdna_amd import _lib, backend
     if _lib.device_count() < 1:
         raise SystemExit("no GPU visible")
+    if args.workload == "served":
+        return served_main(args)
     s, opts, ref_keys, desc = workload(args.workload, args.seed)
     if isinstance(s, list):
         common, per = opts

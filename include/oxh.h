@@ -194,6 +194,35 @@ oxh_status oxh_batch_stats(oxh_batch *b, double *out10);
  * with CPU clocks around device syncs, src/Utilities/Timings.cpp:45-62). */
 oxh_status oxh_batch_time_forces(oxh_batch *b, int n_launches, double *avg_ms);
 
+/* ---- slot pools (served form).  The reference runs one system per
+ * process (src/Utilities/ConfigInfo.h:47) and has no counterpart: a job
+ * server keeps pools resident and admits jobs into free slots while the
+ * other slots keep running (continuous batching).
+ *
+ * A pool is an oxh_batch of n_slots empty slots; slot s reserves
+ * capacity[s] nucleotides and cell_budget[s] cells (NULL: capacity / 2
+ * cells for slots above 1024 nucleotides, none below).  Pools whose slots
+ * all hold at most 512 nucleotides run the workgroup engine.  Slots start
+ * empty; oxh_batch_run advances every occupied slot by at most its
+ * remaining steps, so jobs stop exactly at their targets.  A job's
+ * trajectory depends only on the job: lanes per nucleotide are fixed per
+ * pool, lists are rebuilt on each system's own skin trigger, and capacity
+ * changes rebuild lists from the last build's positions.  All other
+ * oxh_batch_* calls take a slot index as the system index. */
+oxh_status oxh_pool_create(int device, int n_slots, const int *capacity, const int *cell_budget, oxh_batch **out);
+
+/* Uploads job desc into empty slot `slot` (desc->N <= capacity), builds
+ * its list and forces; it will run n_steps steps. */
+oxh_status oxh_pool_admit(oxh_batch *pool, int slot, const oxh_system_desc *desc, long long n_steps);
+
+/* Empties a slot (download results first). */
+oxh_status oxh_pool_release(oxh_batch *pool, int slot);
+
+/* Per slot: steps done, steps targeted, nucleotides (0 = empty) and error
+ * bits (bit 0: a bonded pair exceeded the FENE range; that slot then stops
+ * while the others continue).  Any pointer may be NULL. */
+oxh_status oxh_pool_status(oxh_batch *pool, long long *steps_done, long long *steps_target, int *n_nucleotides, int *error);
+
 /* Hilbert keys of n cells, each (x, y, z, bits) as 4 unsigned ints, on
  * the device: the same function the sort uses, for checking. */
 oxh_status oxh_hilbert_keys(int device, int n, const unsigned int *xyzb, unsigned int *keys);

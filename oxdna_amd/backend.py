@@ -201,6 +201,46 @@ class HIPBatch:
                     engine="workgroup" if out[8] else "grid", n_systems=int(out[9]))
 
 
+class HIPPool(HIPBatch):
+    """A slot pool (oxh_pool_*): slots of fixed capacity that jobs are
+    admitted into and released from while the other slots keep running.
+    Pools of slots of at most 512 nucleotides run the workgroup engine."""
+
+    def __init__(self, capacity, device: int = 0, cell_budget=None):
+        cap = np.ascontiguousarray(capacity, np.int32)
+        cb = None if cell_budget is None else np.ascontiguousarray(cell_budget, np.int32)
+        h = C.c_void_p()
+        check(lib().oxh_pool_create(device, len(cap), ptr(cap), None if cb is None else ptr(cb), C.byref(h)),
+              "oxh_pool_create")
+        self.h = h
+        self.device = device
+        self.capacity = cap
+        self.systems = [None] * len(cap)
+
+    @property
+    def nsys(self):
+        return len(self.capacity)
+
+    def admit(self, slot: int, system: System, steps: int, **opts):
+        keep: list = []
+        d = make_desc(system, keep, **opts)
+        check(lib().oxh_pool_admit(self.h, int(slot), C.byref(d), int(steps)), "oxh_pool_admit")
+        self.systems[slot] = system
+
+    def release(self, slot: int):
+        check(lib().oxh_pool_release(self.h, int(slot)), "oxh_pool_release")
+        self.systems[slot] = None
+
+    def status(self):
+        n = self.nsys
+        done = np.zeros(n, np.int64)
+        target = np.zeros(n, np.int64)
+        nn = np.zeros(n, np.int32)
+        err = np.zeros(n, np.int32)
+        check(lib().oxh_pool_status(self.h, ptr(done), ptr(target), ptr(nn), ptr(err)), "oxh_pool_status")
+        return dict(done=done, target=target, n=nn, error=err)
+
+
 def hilbert_keys(xyz: np.ndarray, bits: int, device: int = 0) -> np.ndarray:
     xyzb = np.zeros((len(xyz), 4), np.uint32)
     xyzb[:, :3] = xyz

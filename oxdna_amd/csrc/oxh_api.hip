@@ -43,6 +43,13 @@ struct oxh_batch {
 	// per launch
 	bool persist = false;
 	int persist_chunk = 256;
+	int persist_block = 512;   // threads per workgroup (OXH_PERSIST_BLOCK = 256 or 512)
+	// slot pool (oxh_pool_create): capacity and reserved cells per slot,
+	// step target and steps done per slot, per-chunk step limits
+	bool pool = false;
+	std::vector<int> cap, cell_budget, h_limit;
+	std::vector<long long> target, done_steps;
+	double4 *bk_rlist = nullptr;
 };
 
 namespace {
@@ -216,25 +223,26 @@ oxh_status read_states(oxh_batch *B, std::vector<oxh_sys_state> &st) {
 void enqueue_lists(oxh_batch *B, bool all) {
 	oxh::Batch &b = B->b;
 	if(all) k_clear_need<<<(b.nsys + 63) / 64, 64, 0, b.stream>>>(b.nsys, b.d_state, 1);
-	oxh::launch_cells(b.stream, b);
-	oxh::launch_build_list(b.stream, b, 0);
+	oxh::launch_cells(b.stream, b, b.pos);
+	oxh::launch_build_list(b.stream, b, 0, b.pos);
 }
 
-void enqueue_step(oxh_batch *B, long long step_index) {
+// step k of a chunk; step_index = the batch's step count (Bussi alignment)
+void enqueue_step(oxh_batch *B, int k, long long step_index) {
 	oxh::Batch &b = B->b;
-	oxh::launch_first_step(b.stream, b);   // also zeroes the cell counters
+	oxh::launch_first_step(b.stream, b, k);   // also zeroes the cell counters
 	oxh::launch_cells_fill_sort(b.stream, b);
-	oxh::launch_build_list(b.stream, b, 0);
-	oxh::launch_forces(b.stream, b, false);
-	oxh::launch_second_step(b.stream, b);
+	oxh::launch_build_list(b.stream, b, 0, b.pos);
+	oxh::launch_forces(b.stream, b, false, k);
+	oxh::launch_second_step(b.stream, b, k);
 	if(B->any_bussi) {
-		if(!B->bussi_aligned || step_index % B->bussi_ns == 0) oxh::launch_bussi(b.stream, b);
+		if(!B->bussi_aligned || step_index % B->bussi_ns == 0) oxh::launch_bussi(b.stream, b, k);
 	}
 }
 
 // forces of the current state with the engine oxh_batch_run uses
 void enqueue_forces(oxh_batch *B, bool split = false) {
-	if(B->persist) oxh::launch_persist_forces(B->b.stream, B->b, split);
+	if(B->persist) oxh::launch_persist_forces(B->b.stream, B->b, split, B->persist_block);
 	else oxh::launch_forces(B->b.stream, B->b, split);
 }
 
@@ -301,6 +309,7 @@ oxh_status backup(oxh_batch *B) {
 	HIPCHK(hipMemcpyAsync(B->bk_T, b.T, N * sizeof(float4), hipMemcpyDeviceToDevice, b.stream));
 	HIPCHK(hipMemcpyAsync(B->bk_pstep, b.pstep, N * sizeof(long long), hipMemcpyDeviceToDevice, b.stream));
 	HIPCHK(hipMemcpyAsync(B->bk_state, b.d_state, b.nsys * sizeof(oxh_sys_state), hipMemcpyDeviceToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(B->bk_rlist, b.rlist, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
 	return OXH_OK;
 }
 
@@ -315,10 +324,216 @@ oxh_status restore(oxh_batch *B) {
 	HIPCHK(hipMemcpyAsync(b.T, B->bk_T, N * sizeof(float4), hipMemcpyDeviceToDevice, b.stream));
 	HIPCHK(hipMemcpyAsync(b.pstep, B->bk_pstep, N * sizeof(long long), hipMemcpyDeviceToDevice, b.stream));
 	HIPCHK(hipMemcpyAsync(b.d_state, B->bk_state, b.nsys * sizeof(oxh_sys_state), hipMemcpyDeviceToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(b.rlist, B->bk_rlist, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
 	oxh::launch_prepare_mirror(b.stream, b);
 	return sync_and_check(B);
 }
 
+__global__ void k_set_need_one(oxh_sys_state *st, int s) { st[s].need_rebuild = 1; }
+
+// Rebuilds every system's list (or only system `only`, if >= 0) from the
+// positions of its last build (rlist): the same pairs in the same order as
+// that build produced, so a capacity change or a replay after an overflow
+// does not change any trajectory.  Grows buffers (and then rebuilds every
+// system) until nothing overflows.
+oxh_status rebuild_from_rlist(oxh_batch *B, int only = -1) {
+	oxh::Batch &b = B->b;
+	for(int attempt = 0; attempt < 12; attempt++) {
+		if(only >= 0 && attempt == 0) k_set_need_one<<<1, 1, 0, b.stream>>>(b.d_state, only);
+		else k_clear_need<<<(b.nsys + 63) / 64, 64, 0, b.stream>>>(b.nsys, b.d_state, 1);
+		oxh::launch_cells(b.stream, b, b.rlist);
+		oxh::launch_build_list(b.stream, b, 0, b.rlist);
+		k_clear_need<<<(b.nsys + 63) / 64, 64, 0, b.stream>>>(b.nsys, b.d_state, 0);
+		std::vector<oxh_sys_state> st;
+		TRY(read_states(B, st));
+		bool overflow = false;
+		for(auto &x : st) overflow |= (x.error & 6) != 0;
+		if(!overflow) return OXH_OK;
+		TRY(grow_lists(B, st));
+	}
+	return fail(OXH_ERR_SYSTEM, "neighbour list keeps overflowing");
+}
+
+
+// Per-system parameters of desc d (validation, T-dependent model constants,
+// box, Verlet radii, integrator and thermostat); first / n / cells are the
+// caller's.  Returns r_verlet in rv.
+oxh_status sys_setup(oxh_batch *B, const oxh_system_desc &d, int s, oxh_sys &S, double &rv) {
+	if(d.N <= 0 || !d.pos || !d.a1 || !d.a3 || !d.btype || !d.n3 || !d.n5)
+		return fail(OXH_ERR_ARG, "system %d: missing arrays or N <= 0", s);
+	for(int k = 0; k < 3; k++)
+		if(!(d.box[k] > 0)) return fail(OXH_ERR_ARG, "system %d: box side %d must be > 0", s, k);
+	if(!(d.T > 0)) return fail(OXH_ERR_ARG, "system %d: T must be > 0", s);
+	if(!(d.salt > 0)) return fail(OXH_ERR_ARG, "system %d: salt concentration must be > 0", s);
+	for(int i = 0; i < d.N; i++) {
+		if(d.n3[i] >= d.N || d.n5[i] >= d.N || d.n3[i] < -1 || d.n5[i] < -1)
+			return fail(OXH_ERR_ARG, "system %d: wrong topology for particle %d", s, i);
+	}
+	std::memset(&S, 0, sizeof(S));
+	double rcut = 0;
+	model_constants(d, S, rcut);
+	for(int k = 0; k < 3; k++) {
+		S.box[k] = d.box[k];
+		S.inv_box[k] = 1.0 / d.box[k];
+		S.boxf[k] = (float) d.box[k];
+		S.inv_boxf[k] = (float) (1.0 / d.box[k]);
+	}
+	rv = rcut + 2.0 * d.verlet_skin;
+	S.rv2 = rv * rv;
+	S.skin2 = d.verlet_skin * d.verlet_skin;
+	S.rclose2 = ((double) oxh::R_CLOSE + 2.0 * d.verlet_skin) * ((double) oxh::R_CLOSE + 2.0 * d.verlet_skin);
+	{
+		double lmax = std::max(d.box[0], std::max(d.box[1], d.box[2]));
+		// |d r| <= 4 L 2^-24 per component after wrapping, rounding and the
+		// nearest image; |d r^2| <= 2 |r| |d r| sqrt(3) + rounding of r^2
+		double dr = 4.0 * lmax * std::ldexp(1.0, -24);
+		S.eps2 = (float) (4.0 * (2.0 * rv * dr * std::sqrt(3.0) + 3.0 * dr * dr + 8.0 * rv * rv * std::ldexp(1.0, -24)));
+		S.rv2f = (float) S.rv2;
+		S.rclose2f = (float) S.rclose2;
+	}
+	S.small_box = 0;
+	for(int k = 0; k < 3; k++)
+		if(d.box[k] < 2.0 * (rv + 2.0 * 0.4814)) S.small_box = 1;
+	// integrator and thermostat (src/Backends/Thermostats/*.cpp)
+	S.dt = d.dt;
+	S.T = d.T;
+	S.thermostat = d.thermostat;
+	S.newtonian_steps = std::max(1, d.newtonian_steps);
+	S.seed_lo = (uint32_t) (d.seed & 0xFFFFFFFFull);
+	S.seed_hi = (uint32_t) (d.seed >> 32) ^ 0x6F78444Eu;
+	S.pad_ = d.refresh_vel;
+	if(d.thermostat == OXH_THERMOSTAT_LANGEVIN) {
+		double D = 0, gamma = 0;
+		if(d.diff_coeff > 0) {
+			D = (double) (float) d.diff_coeff;
+			gamma = d.T / D;
+		}
+		else if(d.gamma_trans > 0) {
+			gamma = (double) (float) d.gamma_trans;
+			D = d.T / gamma;
+		}
+		else return fail(OXH_ERR_ARG, "system %d: Langevin needs diff_coeff or gamma_trans", s);
+		double Dr = 3. * D, gr = d.T / Dr;
+		S.gamma_t = gamma;
+		S.gamma_r = gr;
+		S.resc_t = std::sqrt(2. * gamma * d.T / d.dt);
+		S.resc_r = std::sqrt(2. * gr * d.T / d.dt);
+	}
+	else if(d.thermostat == OXH_THERMOSTAT_BROWNIAN) {
+		double ns = S.newtonian_steps, dt = (double) (float) d.dt, pt = 0, D = 0;
+		if(d.pt > 0) pt = (double) (float) d.pt;
+		else if(d.diff_coeff > 0) {
+			D = (double) (float) d.diff_coeff;
+			pt = (2 * d.T * ns * dt) / (d.T * ns * dt + 2 * D);
+		}
+		else return fail(OXH_ERR_ARG, "system %d: brownian needs pt or diff_coeff", s);
+		if(pt > 1.) return fail(OXH_ERR_ARG, "system %d: pt (%f) must be smaller than 1", s, pt);
+		D = d.T * ns * dt * (1. / pt - 1. / 2.);
+		S.pt = pt;
+		S.pr = (2 * d.T * ns * dt) / (d.T * ns * dt + 2 * 3 * D);
+		S.resc_brown = std::sqrt(d.T);
+	}
+	else if(d.thermostat == OXH_THERMOSTAT_BUSSI) {
+		if(d.bussi_tau <= 0) return fail(OXH_ERR_ARG, "system %d: bussi_tau must be > 0", s);
+		S.bussi_c = std::exp(-S.newtonian_steps / (double) d.bussi_tau);
+		B->any_bussi = true;
+		if(B->bussi_ns == 0) B->bussi_ns = S.newtonian_steps;
+		else if(B->bussi_ns != S.newtonian_steps) B->bussi_aligned = false;
+	}
+	else if(d.thermostat != OXH_THERMOSTAT_NO) return fail(OXH_ERR_ARG, "system %d: unknown thermostat %d", s, d.thermostat);
+	return OXH_OK;
+}
+
+// Cells of side >= r_verlet, at most ~2 particles per cell on average (the
+// reference's cells_auto_optimisation, src/CUDA/Lists/
+// CUDASimpleVerletList.cu:96-110); systems of at most 1024 particles (or
+// fewer than 3 cells a side) use the all-pairs build.  max_cells > 0 caps
+// the cell count (slots of a pool have a fixed cell budget: cells then
+// get larger, which only adds candidates).  Updates the estimates of the
+// largest cell occupancy and of the neighbours per particle.
+void sys_cells(const oxh_system_desc &d, oxh_sys &S, double rv, int cell_first, int max_cells, int &max_occ,
+		double &max_neigh_est) {
+	double V = d.box[0] * d.box[1] * d.box[2];
+	double maxf = std::pow(2.0 * d.N / V, 1.0 / 3.0);
+	int nc[3];
+	bool cells_ok = d.N > 1024;
+	for(int k = 0; k < 3; k++) {
+		nc[k] = (int) std::floor(d.box[k] / rv);
+		nc[k] = std::min(nc[k], (int) std::ceil(maxf * d.box[k]));
+	}
+	if(max_cells > 0) {
+		while((long long) nc[0] * nc[1] * nc[2] > max_cells) {
+			int kmax = (nc[0] >= nc[1] && nc[0] >= nc[2]) ? 0 : (nc[1] >= nc[2] ? 1 : 2);
+			nc[kmax]--;
+		}
+	}
+	for(int k = 0; k < 3; k++)
+		if(nc[k] < 3) cells_ok = false;
+	S.nc[0] = S.nc[1] = S.nc[2] = 0;
+	S.cell_first = cell_first;
+	S.ncell = 0;
+	if(cells_ok) {
+		S.use_cells = 1;
+		for(int k = 0; k < 3; k++) S.nc[k] = nc[k];
+		S.ncell = nc[0] * nc[1] * nc[2];
+		// occupancy of the initial configuration
+		std::vector<int> occ(S.ncell, 0);
+		for(int i = 0; i < d.N; i++) {
+			int c[3];
+			for(int k = 0; k < 3; k++) {
+				double sx = d.pos[3 * i + k] / d.box[k];
+				int ck = (int) ((sx - std::floor(sx)) * nc[k]);
+				c[k] = std::min(ck, nc[k] - 1);
+			}
+			int idx = (c[2] * nc[1] + c[1]) * nc[0] + c[0];
+			max_occ = std::max(max_occ, ++occ[idx]);
+		}
+		double cellvol = V / S.ncell;
+		double est = 4.0 / 3.0 * M_PI * rv * rv * rv * (3.0 * max_occ / cellvol);
+		max_neigh_est = std::max(max_neigh_est, std::min(est, (double) d.N - 1));
+	}
+	else {
+		S.use_cells = 0;
+		max_neigh_est = std::max(max_neigh_est, (double) std::min(d.N - 1, 2048));
+	}
+}
+
+// host staging of system s (desc d) that starts at batch index f; the
+// arrays point at the system's first particle
+void stage_system(const oxh_system_desc &d, int s, int f, double4 *hpos, double4 *hvel, double4 *hL, double4 *hq, int2 *hbonds,
+		int2 *hinfo, int *horig) {
+	for(int i = 0; i < d.N; i++) {
+		hpos[i] = make_double4(d.pos[3 * i], d.pos[3 * i + 1], d.pos[3 * i + 2], 0.0);
+		hvel[i] = d.vel ? make_double4(d.vel[3 * i], d.vel[3 * i + 1], d.vel[3 * i + 2], 0.0) : make_double4(0, 0, 0, 0);
+		hL[i] = d.L ? make_double4(d.L[3 * i], d.L[3 * i + 1], d.L[3 * i + 2], 0.0) : make_double4(0, 0, 0, 0);
+		double q[4];
+		frame_to_quat(d.a1 + 3 * i, d.a3 + 3 * i, q);
+		hq[i] = make_double4(q[0], q[1], q[2], q[3]);
+		hbonds[i] = make_int2(d.n3[i] >= 0 ? f + d.n3[i] : -1, d.n5[i] >= 0 ? f + d.n5[i] : -1);
+		hinfo[i] = make_int2(d.btype[i], s);
+		horig[i] = i;
+	}
+}
+
+void initial_state(const oxh_system_desc &d, oxh_sys_state &st) {
+	std::memset(&st, 0, sizeof(st));
+	st.K_t = 0.5 * 3.0 * (d.N - 1) * d.T;
+	st.K_r = 0.5 * 3.0 * d.N * d.T;
+}
+
+// reduction blocks: chunks of RED_CHUNK particles inside each system's
+// current range; entries past the used ones are (-1, 0, 0)
+void red_layout(const oxh_batch *B, std::vector<int3> &rb, std::vector<int2> &rsb, int nred_total) {
+	rb.clear();
+	rsb.assign(B->b.nsys, make_int2(0, 0));
+	for(int s = 0; s < B->b.nsys; s++) {
+		rsb[s].x = (int) rb.size();
+		for(int a = B->first[s]; a < B->first[s] + B->n[s]; a += oxh::RED_CHUNK)
+			rb.push_back(make_int3(s, a, std::min(a + oxh::RED_CHUNK, B->first[s] + B->n[s])));
+		rsb[s].y = (int) rb.size() - rsb[s].x;
+	}
+	while((int) rb.size() < nred_total) rb.push_back(make_int3(-1, 0, 0));
+}
 }  // namespace
 
 namespace {
@@ -446,7 +661,7 @@ oxh_status oxh_batch_destroy(oxh_batch *B) {
 	void *ptrs[] = {b.pos, b.vel, b.Lm, b.q64, b.rlist, b.q32, b.bpos, b.F, b.T, b.bonds, b.pinfo, b.orig, b.pstep, b.nbr, b.nnbr,
 			b.cell_count, b.cell_items, b.cell_owner, b.pos32, b.d_sys, b.d_state, b.red_blocks, b.red_sys_blocks, b.red_partial, b.red_out, b.red_done,
 			b.esplit, b.hbcount, b.keys, b.keys_alt, b.perm, b.perm_alt, b.inv_perm, b.radix_hist, b.scratch, B->bk_pos, B->bk_vel,
-			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state};
+			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state, B->bk_rlist, b.limit};
 	for(void *p : ptrs)
 		if(p) (void) hipFree(p);
 	if(b.stream) (void) hipStreamDestroy(b.stream);
@@ -454,8 +669,15 @@ oxh_status oxh_batch_destroy(oxh_batch *B) {
 	return OXH_OK;
 }
 
-oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs, oxh_batch **out) {
-	if(!out || n_sys <= 0 || !descs) return fail(OXH_ERR_ARG, "oxh_batch_create: bad arguments");
+}  // extern "C"
+
+namespace {
+
+// A batch of n_sys systems (descs), or a slot pool (capacity != nullptr):
+// slot s reserves capacity[s] particles and cell_budget[s] cells and holds
+// descs[s] if given and non-empty, else nothing.
+oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, const int *capacity, const int *cell_budget,
+		oxh_batch **out) {
 	*out = nullptr;
 	int ndev = 0;
 	HIPCHK(hipGetDeviceCount(&ndev));
@@ -466,6 +688,7 @@ oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs,
 	oxh::Batch &b = B->b;
 	b.device = device;
 	b.nsys = n_sys;
+	B->pool = capacity != nullptr;
 	auto bail = [&](oxh_status s) {
 		std::string msg = g_err;
 		oxh_batch_destroy(B);
@@ -484,139 +707,35 @@ oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs,
 	B->first.resize(n_sys);
 	B->n.resize(n_sys);
 	int N = 0, ncell = 0, max_occ = 1;
-	int max_all_pairs_n = 1;
 	double max_neigh_est = 16;
 	b.min_newtonian = 1 << 30;
+	if(B->pool) {
+		B->cap.assign(capacity, capacity + n_sys);
+		B->cell_budget.assign(cell_budget, cell_budget + n_sys);
+		B->target.assign(n_sys, 0);
+		B->done_steps.assign(n_sys, 0);
+		B->h_limit.assign(n_sys, 0);
+	}
 	for(int s = 0; s < n_sys; s++) {
-		const oxh_system_desc &d = descs[s];
-		if(d.N <= 0 || !d.pos || !d.a1 || !d.a3 || !d.btype || !d.n3 || !d.n5)
-			return bail(fail(OXH_ERR_ARG, "system %d: missing arrays or N <= 0", s));
-		for(int k = 0; k < 3; k++)
-			if(!(d.box[k] > 0)) return bail(fail(OXH_ERR_ARG, "system %d: box side %d must be > 0", s, k));
-		if(!(d.T > 0)) return bail(fail(OXH_ERR_ARG, "system %d: T must be > 0", s));
-		if(!(d.salt > 0)) return bail(fail(OXH_ERR_ARG, "system %d: salt concentration must be > 0", s));
-		for(int i = 0; i < d.N; i++) {
-			if(d.n3[i] >= d.N || d.n5[i] >= d.N || d.n3[i] < -1 || d.n5[i] < -1)
-				return bail(fail(OXH_ERR_ARG, "system %d: wrong topology for particle %d", s, i));
-		}
 		oxh_sys &S = b.h_sys[s];
-		std::memset(&S, 0, sizeof(S));
-		double rcut = 0;
-		model_constants(d, S, rcut);
-		for(int k = 0; k < 3; k++) {
-			S.box[k] = d.box[k];
-			S.inv_box[k] = 1.0 / d.box[k];
-			S.boxf[k] = (float) d.box[k];
-			S.inv_boxf[k] = (float) (1.0 / d.box[k]);
-		}
-		S.first = N;
-		S.n = d.N;
-		B->first[s] = N;
-		B->n[s] = d.N;
-		double rv = rcut + 2.0 * d.verlet_skin;
-		S.rv2 = rv * rv;
-		S.skin2 = d.verlet_skin * d.verlet_skin;
-		S.rclose2 = ((double) oxh::R_CLOSE + 2.0 * d.verlet_skin) * ((double) oxh::R_CLOSE + 2.0 * d.verlet_skin);
-		{
-			double lmax = std::max(d.box[0], std::max(d.box[1], d.box[2]));
-			// |d r| <= 4 L 2^-24 per component after wrapping, rounding and the
-			// nearest image; |d r^2| <= 2 |r| |d r| sqrt(3) + rounding of r^2
-			double dr = 4.0 * lmax * std::ldexp(1.0, -24);
-			S.eps2 = (float) (4.0 * (2.0 * rv * dr * std::sqrt(3.0) + 3.0 * dr * dr + 8.0 * rv * rv * std::ldexp(1.0, -24)));
-			S.rv2f = (float) S.rv2;
-			S.rclose2f = (float) S.rclose2;
-		}
-		S.small_box = 0;
-		for(int k = 0; k < 3; k++)
-			if(d.box[k] < 2.0 * (rv + 2.0 * 0.4814)) S.small_box = 1;
-		// cells of side >= r_verlet, at most ~2 particles per cell on
-		// average (the reference's cells_auto_optimisation,
-		// src/CUDA/Lists/CUDASimpleVerletList.cu:96-110)
-		double V = d.box[0] * d.box[1] * d.box[2];
-		double maxf = std::pow(2.0 * d.N / V, 1.0 / 3.0);
-		int nc[3];
-		bool cells_ok = d.N > 1024;
-		for(int k = 0; k < 3; k++) {
-			nc[k] = (int) std::floor(d.box[k] / rv);
-			nc[k] = std::min(nc[k], (int) std::ceil(maxf * d.box[k]));
-			if(nc[k] < 3) cells_ok = false;
-		}
-		if(cells_ok) {
-			S.use_cells = 1;
-			for(int k = 0; k < 3; k++) S.nc[k] = nc[k];
-			S.cell_first = ncell;
-			S.ncell = nc[0] * nc[1] * nc[2];
-			ncell += S.ncell;
-			// occupancy of the initial configuration
-			std::vector<int> occ(S.ncell, 0);
-			for(int i = 0; i < d.N; i++) {
-				int c[3];
-				for(int k = 0; k < 3; k++) {
-					double sx = d.pos[3 * i + k] / d.box[k];
-					int ck = (int) ((sx - std::floor(sx)) * nc[k]);
-					c[k] = std::min(ck, nc[k] - 1);
-				}
-				int idx = (c[2] * nc[1] + c[1]) * nc[0] + c[0];
-				max_occ = std::max(max_occ, ++occ[idx]);
-			}
-			double cellvol = V / S.ncell;
-			double est = 4.0 / 3.0 * M_PI * rv * rv * rv * (3.0 * max_occ / cellvol);
-			max_neigh_est = std::max(max_neigh_est, std::min(est, (double) d.N - 1));
+		const bool has = descs && descs[s].N > 0;
+		if(has) {
+			double rv = 0;
+			TRYB(sys_setup(B, descs[s], s, S, rv));
+			sys_cells(descs[s], S, rv, ncell, B->pool ? cell_budget[s] : 0, max_occ, max_neigh_est);
 		}
 		else {
-			S.use_cells = 0;
-			max_all_pairs_n = std::max(max_all_pairs_n, d.N);
-			max_neigh_est = std::max(max_neigh_est, (double) std::min(d.N - 1, 2048));
+			std::memset(&S, 0, sizeof(S));
+			S.cell_first = ncell;
 		}
-		// integrator and thermostat (src/Backends/Thermostats/*.cpp)
-		S.dt = d.dt;
-		S.T = d.T;
-		S.thermostat = d.thermostat;
-		S.newtonian_steps = std::max(1, d.newtonian_steps);
-		S.seed_lo = (uint32_t) (d.seed & 0xFFFFFFFFull);
-		S.seed_hi = (uint32_t) (d.seed >> 32) ^ 0x6F78444Eu;
-		S.pad_ = d.refresh_vel;
-		if(d.thermostat == OXH_THERMOSTAT_LANGEVIN) {
-			double D = 0, gamma = 0;
-			if(d.diff_coeff > 0) {
-				D = (double) (float) d.diff_coeff;
-				gamma = d.T / D;
-			}
-			else if(d.gamma_trans > 0) {
-				gamma = (double) (float) d.gamma_trans;
-				D = d.T / gamma;
-			}
-			else return bail(fail(OXH_ERR_ARG, "system %d: Langevin needs diff_coeff or gamma_trans", s));
-			double Dr = 3. * D, gr = d.T / Dr;
-			S.gamma_t = gamma;
-			S.gamma_r = gr;
-			S.resc_t = std::sqrt(2. * gamma * d.T / d.dt);
-			S.resc_r = std::sqrt(2. * gr * d.T / d.dt);
-		}
-		else if(d.thermostat == OXH_THERMOSTAT_BROWNIAN) {
-			double ns = S.newtonian_steps, dt = (double) (float) d.dt, pt = 0, D = 0;
-			if(d.pt > 0) pt = (double) (float) d.pt;
-			else if(d.diff_coeff > 0) {
-				D = (double) (float) d.diff_coeff;
-				pt = (2 * d.T * ns * dt) / (d.T * ns * dt + 2 * D);
-			}
-			else return bail(fail(OXH_ERR_ARG, "system %d: brownian needs pt or diff_coeff", s));
-			if(pt > 1.) return bail(fail(OXH_ERR_ARG, "system %d: pt (%f) must be smaller than 1", s, pt));
-			D = d.T * ns * dt * (1. / pt - 1. / 2.);
-			S.pt = pt;
-			S.pr = (2 * d.T * ns * dt) / (d.T * ns * dt + 2 * 3 * D);
-			S.resc_brown = std::sqrt(d.T);
-		}
-		else if(d.thermostat == OXH_THERMOSTAT_BUSSI) {
-			if(d.bussi_tau <= 0) return bail(fail(OXH_ERR_ARG, "system %d: bussi_tau must be > 0", s));
-			S.bussi_c = std::exp(-S.newtonian_steps / (double) d.bussi_tau);
-			B->any_bussi = true;
-			if(B->bussi_ns == 0) B->bussi_ns = S.newtonian_steps;
-			else if(B->bussi_ns != S.newtonian_steps) B->bussi_aligned = false;
-		}
-		else if(d.thermostat != OXH_THERMOSTAT_NO) return bail(fail(OXH_ERR_ARG, "system %d: unknown thermostat %d", s, d.thermostat));
-		N += d.N;
+		S.first = N;
+		S.n = has ? descs[s].N : 0;
+		B->first[s] = N;
+		B->n[s] = S.n;
+		ncell += B->pool ? cell_budget[s] : S.ncell;
+		N += B->pool ? capacity[s] : S.n;
 	}
+	if(B->pool) B->bussi_aligned = false;   // slots start at different times
 	b.N = N;
 	b.nstride = N;
 	b.ncell_total = ncell;
@@ -668,14 +787,13 @@ oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs,
 
 	// reduction blocks: chunks of RED_CHUNK particles inside one system
 	std::vector<int3> rb;
-	std::vector<int2> rsb(n_sys);
-	for(int s = 0; s < n_sys; s++) {
-		rsb[s].x = (int) rb.size();
-		for(int a = B->first[s]; a < B->first[s] + B->n[s]; a += oxh::RED_CHUNK)
-			rb.push_back(make_int3(s, a, std::min(a + oxh::RED_CHUNK, B->first[s] + B->n[s])));
-		rsb[s].y = (int) rb.size() - rsb[s].x;
-	}
-	b.nred = (int) rb.size();
+	std::vector<int2> rsb;
+	int nred_total = 0;   // pools: enough blocks for every slot at capacity
+	if(B->pool)
+		for(int s = 0; s < n_sys; s++) nred_total += (capacity[s] + oxh::RED_CHUNK - 1) / oxh::RED_CHUNK;
+	red_layout(B, rb, rsb, nred_total);
+	b.nred = std::max(1, (int) rb.size());
+	if(rb.empty()) rb.push_back(make_int3(-1, 0, 0));
 	TRYB(dalloc(B, &b.red_blocks, b.nred));
 	TRYB(dalloc(B, &b.red_sys_blocks, n_sys));
 	TRYB(dalloc(B, &b.red_partial, (size_t) b.nred * oxh::RED_W));
@@ -684,28 +802,22 @@ oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs,
 
 	// ---- host staging
 	std::vector<double4> hpos(N), hvel(N), hL(N), hq(N);
-	std::vector<int2> hbonds(N), hinfo(N);
+	std::vector<int2> hbonds(N, make_int2(-1, -1)), hinfo(N);
 	std::vector<int> horig(N), howner(std::max(ncell, 1));
 	std::vector<oxh_sys_state> hst(n_sys);
 	for(int s = 0; s < n_sys; s++) {
-		const oxh_system_desc &d = descs[s];
-		int f = B->first[s];
-		for(int i = 0; i < d.N; i++) {
-			hpos[f + i] = make_double4(d.pos[3 * i], d.pos[3 * i + 1], d.pos[3 * i + 2], 0.0);
-			hvel[f + i] = d.vel ? make_double4(d.vel[3 * i], d.vel[3 * i + 1], d.vel[3 * i + 2], 0.0) : make_double4(0, 0, 0, 0);
-			hL[f + i] = d.L ? make_double4(d.L[3 * i], d.L[3 * i + 1], d.L[3 * i + 2], 0.0) : make_double4(0, 0, 0, 0);
-			double q[4];
-			frame_to_quat(d.a1 + 3 * i, d.a3 + 3 * i, q);
-			hq[f + i] = make_double4(q[0], q[1], q[2], q[3]);
-			hbonds[f + i] = make_int2(d.n3[i] >= 0 ? f + d.n3[i] : -1, d.n5[i] >= 0 ? f + d.n5[i] : -1);
-			hinfo[f + i] = make_int2(d.btype[i], s);
-			horig[f + i] = i;
-		}
+		static const oxh_system_desc empty = {};
+		const oxh_system_desc &d = descs ? descs[s] : empty;
 		const oxh_sys &S = b.h_sys[s];
-		for(int c = 0; c < S.ncell; c++) howner[S.cell_first + c] = s;
-		std::memset(&hst[s], 0, sizeof(hst[s]));
-		hst[s].K_t = 0.5 * 3.0 * (d.N - 1) * d.T;
-		hst[s].K_r = 0.5 * 3.0 * d.N * d.T;
+		const int f = B->first[s];
+		int nres = S.ncell;   // cells owned by this system
+		if(B->pool) nres = B->cell_budget[s];
+		for(int c = 0; c < nres; c++) howner[S.cell_first + c] = s;
+		for(int i = f; i < f + (B->pool ? B->cap[s] : B->n[s]); i++) hinfo[i] = make_int2(0, s);   // padding belongs to its slot
+		if(!descs || d.N <= 0) continue;
+		stage_system(d, s, f, hpos.data() + f, hvel.data() + f, hL.data() + f, hq.data() + f, hbonds.data() + f, hinfo.data() + f,
+				horig.data() + f);
+		initial_state(d, hst[s]);
 	}
 	hipStream_t st = b.stream;
 	auto up = [&](void *dst, const void *src, size_t bytes) {
@@ -726,10 +838,10 @@ oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs,
 
 	// refresh_vel systems: draw on the device, keep the others as uploaded
 	bool any_refresh = false;
-	for(int s = 0; s < n_sys; s++) any_refresh |= descs[s].refresh_vel != 0;
+	for(int s = 0; s < n_sys && descs; s++) any_refresh |= descs[s].N > 0 && descs[s].refresh_vel != 0;
 	if(any_refresh) {
 		std::vector<double4> keep_v = hvel, keep_L = hL;
-		oxh::launch_refresh_vel(st, b);
+		oxh::launch_refresh_vel(st, b, 0, N);
 		std::vector<double4> nv(N), nL(N);
 		(void) hipMemcpyAsync(nv.data(), b.vel, N * sizeof(double4), hipMemcpyDeviceToHost, st);
 		(void) hipMemcpyAsync(nL.data(), b.Lm, N * sizeof(double4), hipMemcpyDeviceToHost, st);
@@ -745,13 +857,28 @@ oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs,
 		(void) up(b.Lm, nL.data(), N * sizeof(double4));
 	}
 	oxh::launch_prepare_mirror(st, b);
+	// the workgroup engine takes systems of at most one particle per thread
+	if(const char *p = getenv("OXH_PERSIST_BLOCK")) B->persist_block = atoi(p) == 256 ? 256 : 512;
 	B->persist = true;
-	for(const oxh_sys &S : b.h_sys) B->persist &= !S.use_cells && S.n <= oxh::PERSIST_MAX_N;
+	for(int s = 0; s < n_sys; s++) {
+		const oxh_sys &S = b.h_sys[s];
+		const int reach = B->pool ? B->cap[s] : S.n;
+		B->persist &= !S.use_cells && reach <= std::min(oxh::PERSIST_MAX_N, B->persist_block);
+	}
 	if(const char *p = getenv("OXH_PERSIST")) B->persist &= atoi(p) != 0;
+	TRYB(dalloc(B, &B->bk_rlist, N));
+	if(B->pool) TRYB(dalloc(B, &b.limit, n_sys));
 	oxh_status s = build_all(B);
 	if(s != OXH_OK) return bail(s);
-	s = autotune_forces(B);
-	if(s != OXH_OK) return bail(s);
+	if(!B->pool) {
+		s = autotune_forces(B);
+		if(s != OXH_OK) return bail(s);
+	}
+	else if(const char *env = getenv("OXH_FORCE_LANES")) {
+		b.lanes = atoi(env);
+		b.far_lanes = 0;
+		if(const char *f = getenv("OXH_FAR_LANES")) b.far_lanes = atoi(f);
+	}
 	// graph chunk: enough steps to amortise the launch, at most ~2k kernels
 	B->chunk = 64;
 	if(const char *g = getenv("OXH_GRAPH")) B->use_graph = atoi(g) != 0;
@@ -759,6 +886,17 @@ oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs,
 	*out = B;
 	return OXH_OK;
 #undef TRYB
+}
+
+}  // namespace
+
+extern "C" {
+
+oxh_status oxh_batch_create(int device, int n_sys, const oxh_system_desc *descs, oxh_batch **out) {
+	if(!out || n_sys <= 0 || !descs) return fail(OXH_ERR_ARG, "oxh_batch_create: bad arguments");
+	for(int s = 0; s < n_sys; s++)
+		if(descs[s].N <= 0) return fail(OXH_ERR_ARG, "system %d: N must be > 0", s);
+	return create_impl(device, n_sys, descs, nullptr, nullptr, out);
 }
 
 oxh_status oxh_batch_update_lists(oxh_batch *B, int force) {
@@ -793,22 +931,34 @@ oxh_status oxh_batch_run(oxh_batch *B, long long n_steps) {
 	long long done = 0;
 	int retries = 0;
 	while(done < n_steps) {
-		if(B->sort_interval > 0 && B->since_sort >= B->sort_interval && !B->persist) {
+		if(B->sort_interval > 0 && B->since_sort >= B->sort_interval && !B->persist && !B->pool) {
 			if(oxh::launch_hilbert_sort(b.stream, b) != 0) return fail(OXH_ERR_ARG, "too many systems for the sort key");
 			TRY(build_all(B));
 			B->since_sort = 0;
 		}
 		long long todo = std::min<long long>(B->persist ? B->persist_chunk : B->chunk, n_steps - done);
 		bool aligned = !B->any_bussi || !B->bussi_aligned || (b.step % B->bussi_ns == 0);
+		if(B->pool) {
+			// every slot advances by at most its remaining steps; slots
+			// with nothing left stay frozen
+			bool any = false;
+			for(int s = 0; s < b.nsys; s++) {
+				long long left = B->n[s] > 0 ? B->target[s] - B->done_steps[s] : 0;
+				B->h_limit[s] = (int) std::max(0LL, std::min(left, todo));
+				any |= B->h_limit[s] > 0;
+			}
+			if(!any) break;
+			HIPCHK(hipMemcpyAsync(b.limit, B->h_limit.data(), b.nsys * sizeof(int), hipMemcpyHostToDevice, b.stream));
+		}
 		TRY(backup(B));
 		if(B->persist) {
-			oxh::launch_persist(b.stream, b, (int) todo);
+			oxh::launch_persist(b.stream, b, (int) todo, B->persist_block);
 		}
 		else if(todo == B->chunk && aligned && B->use_graph) {
 			if(!b.graph_exec) {
 				hipGraph_t g;
 				HIPCHK(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
-				for(int k = 0; k < B->chunk; k++) enqueue_step(B, k);
+				for(int k = 0; k < B->chunk; k++) enqueue_step(B, k, k);
 				HIPCHK(hipStreamEndCapture(b.stream, &g));
 				HIPCHK(hipGraphInstantiate(&b.graph_exec, g, nullptr, nullptr, 0));
 				HIPCHK(hipGraphDestroy(g));
@@ -816,7 +966,7 @@ oxh_status oxh_batch_run(oxh_batch *B, long long n_steps) {
 			HIPCHK(hipGraphLaunch(b.graph_exec, b.stream));
 		}
 		else {
-			for(long long k = 0; k < todo; k++) enqueue_step(B, b.step + k);
+			for(int k = 0; k < (int) todo; k++) enqueue_step(B, k, b.step + k);
 		}
 		std::vector<oxh_sys_state> st;
 		TRY(read_states(B, st));
@@ -827,16 +977,158 @@ oxh_status oxh_batch_run(oxh_batch *B, long long n_steps) {
 			if((st[s].error & 1) && bad < 0) bad = s;
 		}
 		if(overflow) {
+			// replay the chunk from its start with larger buffers and the
+			// same lists
 			if(++retries > 8) return fail(OXH_ERR_SYSTEM, "neighbour list keeps overflowing");
 			TRY(restore(B));
 			TRY(grow_lists(B, st));
-			TRY(build_all(B));
+			TRY(rebuild_from_rlist(B));
 			continue;
 		}
 		b.step += todo;
 		done += todo;
 		B->since_sort += todo;
+		if(B->pool) {
+			for(int s = 0; s < b.nsys; s++) {
+				B->done_steps[s] += B->h_limit[s];
+				// a broken bond ends that job only: its slot stops here
+				if((st[s].error & 1) && B->n[s] > 0) B->target[s] = B->done_steps[s];
+			}
+			continue;
+		}
 		if(bad >= 0) return fail(OXH_ERR_SYSTEM, "system %d: a bonded pair exceeded the FENE range", bad);
+	}
+	return OXH_OK;
+}
+
+oxh_status oxh_pool_create(int device, int n_slots, const int *capacity, const int *cell_budget, oxh_batch **out) {
+	if(!out || n_slots <= 0 || !capacity) return fail(OXH_ERR_ARG, "oxh_pool_create: bad arguments");
+	std::vector<int> cells(n_slots, 0);
+	for(int s = 0; s < n_slots; s++) {
+		if(capacity[s] <= 0) return fail(OXH_ERR_ARG, "slot %d: capacity must be > 0", s);
+		// default budget: one cell per ~2 particles, enough for a system
+		// of that size at any density
+		cells[s] = cell_budget ? std::max(0, cell_budget[s]) : (capacity[s] > 1024 ? std::max(27, capacity[s] / 2) : 0);
+	}
+	return create_impl(device, n_slots, nullptr, capacity, cells.data(), out);
+}
+
+oxh_status oxh_pool_admit(oxh_batch *B, int slot, const oxh_system_desc *desc, long long n_steps) {
+	TRY(check_sys(B, slot));
+	if(!B->pool) return fail(OXH_ERR_ARG, "not a slot pool");
+	if(!desc || n_steps < 0) return fail(OXH_ERR_ARG, "oxh_pool_admit: bad arguments");
+	if(B->n[slot] > 0) return fail(OXH_ERR_ARG, "slot %d is in use", slot);
+	const oxh_system_desc &d = *desc;
+	if(d.N <= 0 || d.N > B->cap[slot]) return fail(OXH_ERR_ARG, "slot %d holds at most %d nucleotides (job has %d)", slot, B->cap[slot], d.N);
+	oxh::Batch &b = B->b;
+	oxh_sys S;
+	double rv = 0;
+	TRY(sys_setup(B, d, slot, S, rv));
+	B->bussi_aligned = false;
+	const int f = B->first[slot];
+	int max_occ = 1;
+	double est = 16;
+	sys_cells(d, S, rv, b.h_sys[slot].cell_first, B->cell_budget[slot], max_occ, est);
+	if(S.use_cells && B->persist) return fail(OXH_ERR_ARG, "slot %d: the workgroup engine takes no cell lists", slot);
+	S.first = f;
+	S.n = d.N;
+	// host staging and upload of the slot's particles
+	std::vector<double4> hpos(d.N), hvel(d.N), hL(d.N), hq(d.N);
+	std::vector<int2> hbonds(d.N), hinfo(d.N);
+	std::vector<int> horig(d.N);
+	stage_system(d, slot, f, hpos.data(), hvel.data(), hL.data(), hq.data(), hbonds.data(), hinfo.data(), horig.data());
+	oxh_sys_state hst;
+	initial_state(d, hst);
+	hipStream_t st = b.stream;
+	HIPCHK(hipStreamSynchronize(st));   // staging of an earlier admission may still be in flight
+	HIPCHK(hipMemcpyAsync(b.pos + f, hpos.data(), d.N * sizeof(double4), hipMemcpyHostToDevice, st));
+	HIPCHK(hipMemcpyAsync(b.rlist + f, hpos.data(), d.N * sizeof(double4), hipMemcpyHostToDevice, st));
+	HIPCHK(hipMemcpyAsync(b.vel + f, hvel.data(), d.N * sizeof(double4), hipMemcpyHostToDevice, st));
+	HIPCHK(hipMemcpyAsync(b.Lm + f, hL.data(), d.N * sizeof(double4), hipMemcpyHostToDevice, st));
+	HIPCHK(hipMemcpyAsync(b.q64 + f, hq.data(), d.N * sizeof(double4), hipMemcpyHostToDevice, st));
+	HIPCHK(hipMemcpyAsync(b.bonds + f, hbonds.data(), d.N * sizeof(int2), hipMemcpyHostToDevice, st));
+	HIPCHK(hipMemcpyAsync(b.pinfo + f, hinfo.data(), d.N * sizeof(int2), hipMemcpyHostToDevice, st));
+	HIPCHK(hipMemcpyAsync(b.orig + f, horig.data(), d.N * sizeof(int), hipMemcpyHostToDevice, st));
+	HIPCHK(hipMemsetAsync(b.pstep + f, 0, d.N * sizeof(long long), st));
+	HIPCHK(hipMemsetAsync(b.F + f, 0, d.N * sizeof(float4), st));
+	HIPCHK(hipMemsetAsync(b.T + f, 0, d.N * sizeof(float4), st));
+	b.h_sys[slot] = S;
+	HIPCHK(hipMemcpyAsync(b.d_sys + slot, &b.h_sys[slot], sizeof(oxh_sys), hipMemcpyHostToDevice, st));
+	HIPCHK(hipMemcpyAsync(b.d_state + slot, &hst, sizeof(hst), hipMemcpyHostToDevice, st));
+	B->n[slot] = d.N;
+	B->target[slot] = n_steps;
+	B->done_steps[slot] = 0;
+	if(d.refresh_vel) oxh::launch_refresh_vel(st, b, f, d.N);
+	oxh::launch_prepare_mirror(st, b);
+	// reduction blocks follow the slots' sizes
+	std::vector<int3> rb;
+	std::vector<int2> rsb;
+	red_layout(B, rb, rsb, b.nred);
+	HIPCHK(hipMemcpyAsync(b.red_blocks, rb.data(), b.nred * sizeof(int3), hipMemcpyHostToDevice, st));
+	HIPCHK(hipMemcpyAsync(b.red_sys_blocks, rsb.data(), b.nsys * sizeof(int2), hipMemcpyHostToDevice, st));
+	// capacity for this system's cells and neighbours; growing rebuilds the
+	// other slots' lists exactly (from their last build positions)
+	bool regrow = false;
+	if(S.use_cells && 3 * max_occ > b.cell_cap) {
+		HIPCHK(hipStreamSynchronize(st));
+		HIPCHK(hipFree(b.cell_items));
+		B->device_bytes -= (size_t) b.ncell_total * b.cell_cap * sizeof(int);
+		b.cell_cap = 3 * max_occ;
+		TRY(dalloc(B, &b.cell_items, (size_t) b.ncell_total * b.cell_cap));
+		regrow = true;
+	}
+	if((int) std::ceil(est) > b.max_neigh) {
+		HIPCHK(hipStreamSynchronize(st));
+		HIPCHK(hipFree(b.nbr));
+		B->device_bytes -= (size_t) b.nstride * b.max_neigh * sizeof(int);
+		b.max_neigh = std::min((int) std::ceil(est), std::max(b.N, 1));
+		TRY(dalloc(B, &b.nbr, (size_t) b.nstride * b.max_neigh));
+		regrow = true;
+	}
+	if(regrow && b.graph_exec) {
+		(void) hipGraphExecDestroy(b.graph_exec);
+		b.graph_exec = nullptr;
+	}
+	// the new system's list (its rlist is its position); with regrown
+	// buffers every slot's list is rebuilt from its rlist, which gives the
+	// same lists as before
+	TRY(rebuild_from_rlist(B, regrow ? -1 : slot));
+	enqueue_forces(B);
+	return sync_and_check(B);
+}
+
+oxh_status oxh_pool_release(oxh_batch *B, int slot) {
+	TRY(check_sys(B, slot));
+	if(!B->pool) return fail(OXH_ERR_ARG, "not a slot pool");
+	oxh::Batch &b = B->b;
+	HIPCHK(hipStreamSynchronize(b.stream));
+	oxh_sys &S = b.h_sys[slot];
+	S.n = 0;
+	S.use_cells = 0;
+	S.ncell = 0;
+	B->n[slot] = 0;
+	B->target[slot] = B->done_steps[slot] = 0;
+	oxh_sys_state hst;
+	std::memset(&hst, 0, sizeof(hst));
+	HIPCHK(hipMemcpyAsync(b.d_sys + slot, &S, sizeof(oxh_sys), hipMemcpyHostToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(b.d_state + slot, &hst, sizeof(hst), hipMemcpyHostToDevice, b.stream));
+	std::vector<int3> rb;
+	std::vector<int2> rsb;
+	red_layout(B, rb, rsb, b.nred);
+	HIPCHK(hipMemcpyAsync(b.red_blocks, rb.data(), b.nred * sizeof(int3), hipMemcpyHostToDevice, b.stream));
+	HIPCHK(hipMemcpyAsync(b.red_sys_blocks, rsb.data(), b.nsys * sizeof(int2), hipMemcpyHostToDevice, b.stream));
+	return sync_and_check(B);
+}
+
+oxh_status oxh_pool_status(oxh_batch *B, long long *steps_done, long long *steps_target, int *n_nucleotides, int *error) {
+	if(!B || !B->pool) return fail(OXH_ERR_ARG, "not a slot pool");
+	std::vector<oxh_sys_state> st;
+	TRY(read_states(B, st));
+	for(int s = 0; s < B->b.nsys; s++) {
+		if(steps_done) steps_done[s] = B->done_steps[s];
+		if(steps_target) steps_target[s] = B->target[s];
+		if(n_nucleotides) n_nucleotides[s] = B->n[s];
+		if(error) error[s] = B->n[s] > 0 ? st[s].error : 0;
 	}
 	return OXH_OK;
 }

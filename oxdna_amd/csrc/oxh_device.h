@@ -285,4 +285,13 @@ __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, double &n0, d
 	n1 = r * s;
 }
 
+// particle i is padding of a slot: beyond its system's current size
+__device__ __forceinline__ bool padding(int i, const oxh_sys &S) { return i - S.first >= S.n; }
+
+// particle i takes no part in step k of the chunk: padding, or its system
+// has used up its steps (limit[s], slot pools)
+__device__ __forceinline__ bool idle(int i, const oxh_sys &S, int s, int k, const int *__restrict__ limit) {
+	return padding(i, S) || (limit && k >= limit[s]);
+}
+
 }  // namespace oxh

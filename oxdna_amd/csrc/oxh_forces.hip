@@ -28,17 +28,20 @@ namespace oxh {
 // n5 bonded pairs go to lanes 0 / 1.  Partial sums are combined by a fixed
 // butterfly of wave shuffles, so results do not depend on timing.
 template<int K, bool SPLIT, bool FAR>
-__global__ void __launch_bounds__(256) k_forces(int N, const double4 *__restrict__ pos, const float4 *__restrict__ q32,
+__global__ void __launch_bounds__(256) k_forces(int N, const double4 *__restrict__ pos, const double4 *__restrict__ q64,
+		const float4 *__restrict__ q32,
 		const float4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
 		const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh, const oxh_sys *__restrict__ systems,
 		oxh_sys_state *__restrict__ sstate, float4 *__restrict__ Fout, float4 *__restrict__ Tout, float *__restrict__ esplit,
-		int *__restrict__ hbcount) {
+		int *__restrict__ hbcount, int k, const int *__restrict__ limit) {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
 	const int i = t / K;
 	const int l = t % K;
 	if(i >= N) return;
+	const int s = pinfo[i].y;
+	if(idle(i, systems[s], s, k, limit)) return;
 
-	particle_forces<SPLIT, FAR>(K, i, l, N, pos, q32, bpos, bonds, pinfo, nbr, nnbr, max_neigh, systems, sstate, Fout, Tout, esplit, hbcount);
+	particle_forces<SPLIT, FAR>(K, i, l, N, pos, q64, q32, bpos, bonds, pinfo, nbr, nnbr, max_neigh, systems, sstate, Fout, Tout, esplit, hbcount);
 }
 
 // Far pairs on their own: Debye-Hueckel between backbone sites, few
@@ -52,12 +55,14 @@ template<int K, bool SPLIT>
 __global__ void __launch_bounds__(256) k_forces_far(int N, const double4 *__restrict__ pos, const float4 *__restrict__ q32,
 		const float4 *__restrict__ bpos, const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh,
 		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, float4 *__restrict__ Fio, float4 *__restrict__ Tio,
-		float *__restrict__ esplit) {
+		float *__restrict__ esplit, int k_step, const int *__restrict__ limit) {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
 	const int i = t / K;
 	const int l = t % K;
 	if(i >= N) return;
-	const oxh_sys &S = systems[pinfo[i].y];
+	const int s = pinfo[i].y;
+	const oxh_sys &S = systems[s];
+	if(idle(i, S, s, k_step, limit)) return;
 	const int nf = nnbr[i].y;
 	const int *row = nbr + (size_t) i * max_neigh + max_neigh - 1;
 	const float4 Bi = bpos[i];
@@ -102,65 +107,65 @@ __global__ void __launch_bounds__(256) k_forces_far(int N, const double4 *__rest
 }
 
 template<int K>
-static void launch_k(hipStream_t st, const Batch &b, bool split, int block, bool far_inline) {
+static void launch_k(hipStream_t st, const Batch &b, bool split, int block, bool far_inline, int k) {
 	size_t threads = (size_t) b.N * K;
 	dim3 grid((unsigned) ((threads + block - 1) / block));
 	int *hbc = b.want_hbcount ? b.hbcount : nullptr;
 	if(split) {
 		if(far_inline)
-			k_forces<K, true, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-					b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount);
+			k_forces<K, true, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
+					b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr);
 		else
-			k_forces<K, true, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-					b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount);
+			k_forces<K, true, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
+					b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr);
 	}
 	else {
 		if(far_inline)
-			k_forces<K, false, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-					b.d_sys, b.d_state, b.F, b.T, nullptr, hbc);
+			k_forces<K, false, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
+					b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr);
 		else
-			k_forces<K, false, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-					b.d_sys, b.d_state, b.F, b.T, nullptr, hbc);
+			k_forces<K, false, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
+					b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr);
 	}
 }
 
 template<int K>
-static void launch_far(hipStream_t st, const Batch &b, bool split, int block) {
+static void launch_far(hipStream_t st, const Batch &b, bool split, int block, int k) {
 	size_t threads = (size_t) b.N * K;
 	dim3 grid((unsigned) ((threads + block - 1) / block));
 	if(split)
 		k_forces_far<K, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F, b.T,
-				b.esplit);
+				b.esplit, k, k >= 0 ? b.limit : nullptr);
 	else
 		k_forces_far<K, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F, b.T,
-				nullptr);
+				nullptr, k, k >= 0 ? b.limit : nullptr);
 }
 
 // close (+ bonded) kernel; the far pairs either inline (far_lanes == 0) or
 // in k_forces_far right after it
-void launch_forces(hipStream_t st, const Batch &b, bool split) {
+void launch_forces(hipStream_t st, const Batch &b, bool split, int k) {
 	if(b.N == 0) return;
 	const bool inl = b.far_lanes == 0;
 	int block = b.force_block;
 	switch(b.lanes) {
-	case 1: launch_k<1>(st, b, split, block, inl); break;
-	case 2: launch_k<2>(st, b, split, block, inl); break;
-	case 4: launch_k<4>(st, b, split, block, inl); break;
-	case 8: launch_k<8>(st, b, split, block, inl); break;
-	case 16: launch_k<16>(st, b, split, block, inl); break;
-	case 32: launch_k<32>(st, b, split, block, inl); break;
-	default: launch_k<64>(st, b, split, block, inl); break;
+	case 1: launch_k<1>(st, b, split, block, inl, k); break;
+	case 2: launch_k<2>(st, b, split, block, inl, k); break;
+	case 4: launch_k<4>(st, b, split, block, inl, k); break;
+	case 8: launch_k<8>(st, b, split, block, inl, k); break;
+	case 16: launch_k<16>(st, b, split, block, inl, k); break;
+	case 32: launch_k<32>(st, b, split, block, inl, k); break;
+	default: launch_k<64>(st, b, split, block, inl, k); break;
 	}
 	if(inl) return;
 	block = b.far_block;
 	switch(b.far_lanes) {
-	case 1: launch_far<1>(st, b, split, block); break;
-	case 2: launch_far<2>(st, b, split, block); break;
-	case 4: launch_far<4>(st, b, split, block); break;
-	case 8: launch_far<8>(st, b, split, block); break;
-	case 16: launch_far<16>(st, b, split, block); break;
-	case 32: launch_far<32>(st, b, split, block); break;
-	default: launch_far<64>(st, b, split, block); break;
+	case 1: launch_far<1>(st, b, split, block, k); break;
+	case 2: launch_far<2>(st, b, split, block, k); break;
+	case 4: launch_far<4>(st, b, split, block, k); break;
+	case 8: launch_far<8>(st, b, split, block, k); break;
+	case 16: launch_far<16>(st, b, split, block, k); break;
+	case 32: launch_far<32>(st, b, split, block, k); break;
+	default: launch_far<64>(st, b, split, block, k); break;
 	}
 }
 

@@ -25,7 +25,9 @@ __global__ void k_prepare_mirror(int N, const double4 *__restrict__ pos, const d
 		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, float4 *__restrict__ q32, float4 *__restrict__ bpos) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= N) return;
-	write_mirror(i, pos[i], q64[i], systems[pinfo[i].y], bonds[i], q32, bpos);
+	const oxh_sys &S = systems[pinfo[i].y];
+	if(padding(i, S)) return;
+	write_mirror(i, pos[i], q64[i], S, bonds[i], q32, bpos);
 }
 
 void launch_prepare_mirror(hipStream_t st, const Batch &b) {
@@ -37,31 +39,35 @@ __global__ void __launch_bounds__(256) k_first_step(int N, double4 *__restrict__
 		double4 *__restrict__ q64, float4 *__restrict__ q32, float4 *__restrict__ bpos, const float4 *__restrict__ F,
 		const float4 *__restrict__ T, const double4 *__restrict__ rlist, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
 		long long *__restrict__ pstep, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st, int ncell,
-		int *__restrict__ cell_count) {
+		int *__restrict__ cell_count, int k, const int *__restrict__ limit) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	// cell counters start every step at zero; they are only filled (after
 	// this kernel) in steps that rebuild the lists
 	if(i < ncell) cell_count[i] = 0;
 	if(i >= N) return;
 	const int s = pinfo[i].y;
-	if(first_step_particle(i, systems[s], pos, vel, Lm, q64, q32, bpos, F, T, rlist, bonds, pstep)) st[s].need_rebuild = 1;
+	const oxh_sys &S = systems[s];
+	if(idle(i, S, s, k, limit)) return;
+	if(first_step_particle(i, S, pos, vel, Lm, q64, q32, bpos, F, T, rlist, bonds, pstep)) st[s].need_rebuild = 1;
 }
 
-void launch_first_step(hipStream_t st, const Batch &b) {
+void launch_first_step(hipStream_t st, const Batch &b, int k) {
 	int n = std::max(b.N, b.ncell_total);
 	k_first_step<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.F, b.T, b.rlist, b.bonds, b.pinfo,
-			b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count);
+			b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit);
 }
 
 // second half kick + Langevin / Brownian thermostats; the first particle
 // of each system closes the system's list-rebuild flag for this step
 __global__ void __launch_bounds__(256) k_second_step(int N, double4 *__restrict__ vel, double4 *__restrict__ Lm,
 		const float4 *__restrict__ F, const float4 *__restrict__ T, const int2 *__restrict__ pinfo, const int *__restrict__ orig,
-		const long long *__restrict__ pstep, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st) {
+		const long long *__restrict__ pstep, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st, int k,
+		const int *__restrict__ limit) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= N) return;
 	const int s = pinfo[i].y;
 	const oxh_sys &S = systems[s];
+	if(idle(i, S, s, k, limit)) return;
 	second_step_particle(i, S, vel, Lm, F, T, orig, pstep);
 	if(i == S.first && st[s].need_rebuild) {
 		st[s].need_rebuild = 0;
@@ -69,11 +75,10 @@ __global__ void __launch_bounds__(256) k_second_step(int N, double4 *__restrict_
 	}
 }
 
-void launch_second_step(hipStream_t st, const Batch &b) {
-	k_second_step<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.vel, b.Lm, b.F, b.T, b.pinfo, b.orig, b.pstep, b.d_sys, b.d_state);
+void launch_second_step(hipStream_t st, const Batch &b, int k) {
+	k_second_step<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.vel, b.Lm, b.F, b.T, b.pinfo, b.orig, b.pstep, b.d_sys, b.d_state, k,
+			b.limit);
 }
-
-void launch_end_step(hipStream_t, const Batch &) {}
 
 // -------------------------------------------------------- FP64 reductions
 // per system: U = sum F.w, U_hb = sum T.w, K_t = 1/2 sum v^2, K_r = 1/2 sum L^2,
@@ -83,6 +88,7 @@ __global__ void __launch_bounds__(256) k_reduce_partial(const int3 *__restrict__
 		const float4 *__restrict__ T, const double4 *__restrict__ vel, const double4 *__restrict__ Lm, double *__restrict__ partial) {
 	__shared__ double sh[RED_W][256];
 	int3 bl = blocks[blockIdx.x];
+	if(bl.x < 0) bl.y = bl.z = 0;   // unused block of a slot pool
 	double acc[RED_W] = {0, 0, 0, 0, 0, 0, 0, 0};
 	for(int i = bl.y + threadIdx.x; i < bl.z; i += blockDim.x) {
 		float4 f = F[i], t = T[i];
@@ -131,15 +137,16 @@ void launch_reduce(hipStream_t st, const Batch &b) {
 __global__ void __launch_bounds__(256) k_bussi_reduce(const int3 *__restrict__ blocks, const int2 *__restrict__ sys_blocks,
 		const double4 *__restrict__ vel, const double4 *__restrict__ Lm, const oxh_sys *__restrict__ systems,
 		oxh_sys_state *__restrict__ st, const long long *__restrict__ pstep, double *__restrict__ partial, int *__restrict__ done,
-		double4 *__restrict__ factors, int nsys) {
+		double4 *__restrict__ factors, int nsys, int k, const int *__restrict__ limit) {
 	__shared__ double sh[5][256];
 	__shared__ int last;
 	const int3 bl = blocks[blockIdx.x];
 	const int s = bl.x;
+	if(s < 0) return;   // unused block of a slot pool
 	const oxh_sys &S = systems[s];
 	if(S.thermostat != 2) return;
 	const long long step = pstep[S.first] - 1;
-	if(step % S.newtonian_steps != 0) {
+	if(step % S.newtonian_steps != 0 || (limit && k >= limit[s])) {
 		if(threadIdx.x == 0 && blockIdx.x == sys_blocks[s].x) factors[s] = make_double4(1.0, 1.0, 0.0, 0.0);
 		return;
 	}
@@ -169,26 +176,28 @@ __global__ void k_bussi_rescale(int N, double4 *__restrict__ vel, double4 *__res
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= N) return;
 	int s = pinfo[i].y;
-	if(systems[s].thermostat != 2) return;
+	if(systems[s].thermostat != 2 || padding(i, systems[s])) return;
 	double4 f = factors[s];
 	if(f.z == 0.0) return;
 	double4 vc = factors[nsys + s];
 	bussi_rescale_particle(i, vel, Lm, f.x, f.y, vc.x, vc.y, vc.z);
 }
 
-void launch_bussi(hipStream_t st, const Batch &b) {
+void launch_bussi(hipStream_t st, const Batch &b, int k) {
 	double4 *factors = (double4 *) b.scratch;
 	k_bussi_reduce<<<b.nred, 256, 0, st>>>(b.red_blocks, b.red_sys_blocks, b.vel, b.Lm, b.d_sys, b.d_state, b.pstep, b.red_partial,
-			b.red_done, factors, b.nsys);
+			b.red_done, factors, b.nsys, k, b.limit);
 	k_bussi_rescale<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.vel, b.Lm, b.pinfo, factors, b.nsys, b.d_sys);
 }
 
 // initial velocities ~ N(0, T) per component (src/Backends/MDBackend.cpp:140-165)
-__global__ void k_refresh_vel(int N, double4 *__restrict__ vel, double4 *__restrict__ Lm, const int2 *__restrict__ pinfo,
+__global__ void k_refresh_vel(int i0, int n_, double4 *__restrict__ vel, double4 *__restrict__ Lm, const int2 *__restrict__ pinfo,
 		const int *__restrict__ orig, const oxh_sys *__restrict__ systems) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if(i >= N) return;
+	if(i >= n_) return;
+	i += i0;
 	const oxh_sys &S = systems[pinfo[i].y];
+	if(padding(i, S)) return;
 	double n[8];
 	normals4(S, (uint32_t) orig[i], -1, RNG_REFRESH, n);
 	normals4(S, (uint32_t) orig[i], -1, RNG_REFRESH + 1, n + 4);
@@ -197,9 +206,9 @@ __global__ void k_refresh_vel(int N, double4 *__restrict__ vel, double4 *__restr
 	Lm[i] = make_double4(n[3] * sT, n[4] * sT, n[5] * sT, 0.0);
 }
 
-void launch_refresh_vel(hipStream_t st, const Batch &b) {
-	if(b.N == 0) return;
-	k_refresh_vel<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.vel, b.Lm, b.pinfo, b.orig, b.d_sys);
+void launch_refresh_vel(hipStream_t st, const Batch &b, int i0, int n) {
+	if(n <= 0) return;
+	k_refresh_vel<<<(n + 255) / 256, 256, 0, st>>>(i0, n, b.vel, b.Lm, b.pinfo, b.orig, b.d_sys);
 }
 
 }  // namespace oxh

@@ -47,6 +47,11 @@ struct Batch {
 	oxh_sys *d_sys = nullptr;
 	oxh_sys_state *d_state = nullptr;
 	std::vector<oxh_sys> h_sys;
+	// slot pools: steps each system may take in the current chunk (step k of
+	// the chunk runs for system s iff k < limit[s]); nullptr = no limit.
+	// Particles [first + n, first + capacity) of a slot are padding, which
+	// every kernel skips.
+	int *limit = nullptr;
 
 	// reductions: blocks of at most RED_CHUNK particles, each inside one system
 	int nred = 0;
@@ -82,33 +87,36 @@ struct Batch {
 constexpr int RED_CHUNK = 2048;
 constexpr int RED_W = 8;   // reduction width: U, Uhb, Kt, Kr, vx, vy, vz, count
 
-// workgroup-resident engine: one workgroup of PERSIST_BLOCK threads per
-// system, for batches whose systems all have at most PERSIST_MAX_N
-// particles and no cells
-constexpr int PERSIST_BLOCK = 512;
+// workgroup-resident engine: one workgroup of `block` threads (256 or
+// 512) per system, for batches whose systems all have at most
+// PERSIST_MAX_N particles and no cells
 constexpr int PERSIST_MAX_N = 512;
 
 // lanes per nucleotide of the workgroup-resident force phase: the largest
-// power of two <= 16 with n * K <= PERSIST_BLOCK
-__host__ __device__ inline int persist_lanes(int n) {
+// power of two <= 16 with n * K <= block
+__host__ __device__ inline int persist_lanes(int n, int block) {
 	int k = 16;
-	while(k > 1 && k * n > PERSIST_BLOCK) k >>= 1;
+	while(k > 1 && k * n > block) k >>= 1;
 	return k;
 }
 
-void launch_persist(hipStream_t st, const Batch &b, int nsteps);
-void launch_persist_forces(hipStream_t st, const Batch &b, bool split);
+void launch_persist(hipStream_t st, const Batch &b, int nsteps, int block);
+void launch_persist_forces(hipStream_t st, const Batch &b, bool split, int block);
 
-void launch_forces(hipStream_t st, const Batch &b, bool split);
-void launch_cells(hipStream_t st, const Batch &b);
+// k = index of the step inside the current chunk (checked against limit);
+// k < 0: forces of every system's current state, whatever its limit
+void launch_forces(hipStream_t st, const Batch &b, bool split, int k = -1);
+// list builds read positions from src (the current positions, or rlist to
+// rebuild exactly the lists of the last build)
+void launch_cells(hipStream_t st, const Batch &b, const double4 *src);
 void launch_cells_fill_sort(hipStream_t st, const Batch &b);
-void launch_build_list(hipStream_t st, const Batch &b, int force_all);
-void launch_first_step(hipStream_t st, const Batch &b);
-void launch_second_step(hipStream_t st, const Batch &b);
-void launch_end_step(hipStream_t st, const Batch &b);
-void launch_bussi(hipStream_t st, const Batch &b);
+void launch_build_list(hipStream_t st, const Batch &b, int force_all, const double4 *src);
+void launch_first_step(hipStream_t st, const Batch &b, int k);
+void launch_second_step(hipStream_t st, const Batch &b, int k);
+void launch_bussi(hipStream_t st, const Batch &b, int k);
 void launch_reduce(hipStream_t st, const Batch &b);
-void launch_refresh_vel(hipStream_t st, const Batch &b);
+// velocities ~ N(0, T) for particles [i0, i0 + n)
+void launch_refresh_vel(hipStream_t st, const Batch &b, int i0, int n);
 void launch_prepare_mirror(hipStream_t st, const Batch &b);
 int launch_hilbert_sort(hipStream_t st, Batch &b);
 

@@ -127,7 +127,7 @@ __device__ __forceinline__ void build_row(int i, int l, int s, const double4 *__
 		nc = nf = 0;
 	}
 	nnbr[i] = make_int2(nc, nf);
-	rlist[i] = p;
+	if(rlist) rlist[i] = p;
 }
 
 }  // namespace oxh

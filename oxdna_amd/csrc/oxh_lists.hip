@@ -38,7 +38,7 @@ __global__ void k_cells_fill(int N, const double4 *__restrict__ pos, const int2 
 	int s = pinfo[i].y;
 	if(!st[s].need_rebuild) return;
 	const oxh_sys &S = systems[s];
-	if(!S.use_cells) return;
+	if(!S.use_cells || padding(i, S)) return;
 	double4 p = pos[i];
 	int cx = cell_coord(p.x, S.inv_box[0], S.nc[0]);
 	int cy = cell_coord(p.y, S.inv_box[1], S.nc[1]);
@@ -85,11 +85,11 @@ void launch_cells_fill_sort(hipStream_t st, const Batch &b) {
 			b.cell_count, b.cell_items, b.cell_cap);
 }
 
-void launch_cells(hipStream_t st, const Batch &b) {
+void launch_cells(hipStream_t st, const Batch &b, const double4 *src) {
 	if(!b.any_cells) return;
 	int bs = 256;
 	k_cells_clear<<<(b.ncell_total + bs - 1) / bs, bs, 0, st>>>(b.ncell_total, b.cell_owner, b.d_state, b.cell_count);
-	k_cells_fill<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, b.pos, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_items, b.cell_cap,
+	k_cells_fill<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, src, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_items, b.cell_cap,
 			b.pos32);
 	k_cells_sort<<<std::min(b.ncell_total, 4096), 64, (size_t) b.cell_cap * sizeof(int), st>>>(b.ncell_total, b.cell_owner, b.d_state,
 			b.cell_count, b.cell_items, b.cell_cap);
@@ -106,14 +106,17 @@ __global__ void __launch_bounds__(256) k_build_list(int N, const double4 *__rest
 	if(i >= N) return;
 	int s = pinfo[i].y;
 	if(!force_all && !st[s].need_rebuild) return;
+	if(padding(i, systems[s])) return;
 	build_row<K>(i, l, s, pos, bonds, systems, st, count, items, cap, nbr, nnbr, max_neigh, rlist, pos32);
 }
 
-void launch_build_list(hipStream_t st, const Batch &b, int force_all) {
+// src = b.pos (the usual build) or b.rlist (the lists of the last build,
+// rebuilt exactly, e.g. after a capacity change); rlist is then left as is
+void launch_build_list(hipStream_t st, const Batch &b, int force_all, const double4 *src) {
 	int bs = 256;
 	size_t threads = (size_t) b.N * BUILD_LANES;
-	k_build_list<<<(unsigned) ((threads + bs - 1) / bs), bs, 0, st>>>(b.N, b.pos, b.bonds, b.pinfo, b.d_sys, b.d_state, b.cell_count,
-			b.cell_items, b.cell_cap, b.nbr, b.nnbr, b.max_neigh, b.rlist, force_all, b.pos32);
+	k_build_list<<<(unsigned) ((threads + bs - 1) / bs), bs, 0, st>>>(b.N, src, b.bonds, b.pinfo, b.d_sys, b.d_state, b.cell_count,
+			b.cell_items, b.cell_cap, b.nbr, b.nnbr, b.max_neigh, src == b.rlist ? nullptr : b.rlist, force_all, b.pos32);
 }
 
 // ------------------------------------------------------------ Hilbert sort

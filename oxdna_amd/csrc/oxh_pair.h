@@ -44,17 +44,107 @@ struct PairOut {
 	float e;
 };
 
+// ------------------------------------------------ FP64 site geometry
+// The pair terms are FP32, but the site-to-site separations they start
+// from are formed in FP64 and rounded once: the radial factors (FENE,
+// Morse, harmonic, the steep excluded-volume walls) depend on
+// |r_site| - r0, which is small next to |r_site|, so FP32 sites would lose
+// ~1e-6 of the force to cancellation.  Axes come from the FP64
+// quaternion (as the FP32 mirror forms them) and are rounded once.
+struct d3 {
+	double x, y, z;
+};
+
+// a nucleotide's FP64 a1 axis and backbone offset (the stacking and base
+// offsets are POS_STACK a1 and POS_BASE a1)
+struct Sites64 {
+	d3 a1, back;
+};
+
+__device__ __forceinline__ Frame make_frame_d(double4 q, int btype, Sites64 &st) {
+	const double sqx = q.x * q.x, sqy = q.y * q.y, sqz = q.z * q.z, sqw = q.w * q.w;
+	const double xy = q.x * q.y, xz = q.x * q.z, xw = q.x * q.w, yz = q.y * q.z, yw = q.y * q.w, zw = q.z * q.w;
+	const d3 a1 = {sqx - sqy - sqz + sqw, 2. * (xy + zw), 2. * (xz - yw)};
+	const d3 a2 = {2. * (xy - zw), -sqx + sqy - sqz + sqw, 2. * (yz + xw)};
+	const d3 a3 = {2. * (xz + yw), 2. * (yz - xw), -sqx - sqy + sqz + sqw};
+	const double b1 = (double) POS_MM_BACK1, b2 = (double) POS_MM_BACK2, ps = (double) POS_STACK, pb = (double) POS_BASE;
+	st.a1 = a1;
+	st.back = {b1 * a1.x + b2 * a2.x, b1 * a1.y + b2 * a2.y, b1 * a1.z + b2 * a2.z};
+	Frame fr;
+	fr.a1 = mk((float) a1.x, (float) a1.y, (float) a1.z);
+	fr.a2 = mk((float) a2.x, (float) a2.y, (float) a2.z);
+	fr.a3 = mk((float) a3.x, (float) a3.y, (float) a3.z);
+	fr.back = mk((float) st.back.x, (float) st.back.y, (float) st.back.z);
+	fr.stack = mk((float) (ps * a1.x), (float) (ps * a1.y), (float) (ps * a1.z));
+	fr.base = mk((float) (pb * a1.x), (float) (pb * a1.y), (float) (pb * a1.z));
+	fr.btype = btype;
+	fr.type = type_of(btype);
+	return fr;
+}
+
+__device__ __forceinline__ d3 min_image_d(double4 a, double4 b, const oxh_sys &S) {
+	double dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+	dx = fma(-S.box[0], rint(dx * S.inv_box[0]), dx);
+	dy = fma(-S.box[1], rint(dy * S.inv_box[1]), dy);
+	dz = fma(-S.box[2], rint(dz * S.inv_box[2]), dz);
+	return {dx, dy, dz};
+}
+
+// (r + qs) - ps in FP64, rounded to FP32
+__device__ __forceinline__ f3 site_sep(const d3 &r, const d3 &qs, const d3 &ps) {
+	return mk((float) ((r.x + qs.x) - ps.x), (float) ((r.y + qs.y) - ps.y), (float) ((r.z + qs.z) - ps.z));
+}
+
+// separations of the site pairs the terms use, for the ordered pair
+// (p, q), r = q - p
+struct PairGeom {
+	f3 r;          // centres
+	f3 back;       // back(q) - back(p)
+	f3 base;       // base(q) - base(p)
+	f3 base_back;  // base(q) - back(p)
+	f3 back_base;  // back(q) - base(p)
+	f3 stack;      // stack(q) - stack(p)
+};
+
+__device__ __forceinline__ d3 scaled(const d3 &v, double c) { return {c * v.x, c * v.y, c * v.z}; }
+
+__device__ __forceinline__ PairGeom pair_geom(const d3 &r, const Sites64 &p, const Sites64 &q) {
+	const double pb = (double) POS_BASE, ps = (double) POS_STACK;
+	const d3 pbase = scaled(p.a1, pb), qbase = scaled(q.a1, pb);
+	PairGeom g;
+	g.r = mk((float) r.x, (float) r.y, (float) r.z);
+	g.back = site_sep(r, q.back, p.back);
+	g.base = site_sep(r, qbase, pbase);
+	g.base_back = site_sep(r, qbase, p.back);
+	g.back_base = site_sep(r, q.back, pbase);
+	g.stack = site_sep(r, scaled(q.a1, ps), scaled(p.a1, ps));
+	return g;
+}
+
+// bonded pairs also need the FENE stretch |back(q) - back(p)| - r0 (FP64)
+// and the separation of the POS_BACK reference sites of the stacking's
+// phi angles
+__device__ __forceinline__ void bonded_geom(const d3 &r, const Sites64 &p, const Sites64 &q, const oxh_sys &S, float &stretch,
+		f3 &backref) {
+	const double dx = (r.x + q.back.x) - p.back.x, dy = (r.y + q.back.y) - p.back.y, dz = (r.z + q.back.z) - p.back.z;
+	stretch = (float) (sqrt(dx * dx + dy * dy + dz * dz) - (double) S.fene_r0);
+	const double pbk = (double) POS_BACK;
+	backref = mk((float) (r.x + pbk * (q.a1.x - p.a1.x)), (float) (r.y + pbk * (q.a1.y - p.a1.y)),
+			(float) (r.z + pbk * (q.a1.z - p.a1.z)));
+}
+
 // ------------------------------------------------------------ bonded terms
 // FENE, bonded excluded volume and stacking of the bonded pair p -> n3 = q
-// (src/Interactions/DNAInteraction.cpp:415-705); returns the three energies
-// in e3.
-__device__ PairOut bonded_pair(const oxh_sys &S, const Frame &p, const Frame &q, f3 r, float e3[3], int &err) {
+// (src/Interactions/DNAInteraction.cpp:415-705); stretch = fene_stretch of
+// the pair; returns the three energies in e3.
+__device__ PairOut bonded_pair(const oxh_sys &S, const Frame &p, const Frame &q, const PairGeom &g, float stretch, f3 rbackref,
+		float e3[3], int &err) {
 	PairOut o = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, 0.f};
 
 	// FENE backbone
-	f3 rback = r + q.back - p.back;
-	float rbackmod = sqrtf(dot(rback, rback));
-	float rbackr0 = rbackmod - S.fene_r0;
+	f3 rback = g.back;
+	float rbackr0 = stretch;
+	float rbackmod = S.fene_r0 + stretch;
 	f3 force;
 	float e_fene;
 	if(S.use_mbf && fabsf(rbackr0) > S.mbf_xmax) {
@@ -78,15 +168,15 @@ __device__ PairOut bonded_pair(const oxh_sys &S, const Frame &p, const Frame &q,
 
 	// bonded excluded volume: base-base, p.base-q.back, p.back-q.base
 	f3 fx;
-	float ex = repulsive_lj(r + q.base - p.base, fx, EXCL_S2, EXCL_R2, EXCL_B2, EXCL_RC2);
+	float ex = repulsive_lj(g.base, fx, EXCL_S2, EXCL_R2, EXCL_B2, EXCL_RC2);
 	o.force += fx;
 	o.tp -= cross(p.base, fx);
 	o.tq += cross(q.base, fx);
-	ex += repulsive_lj(r + q.back - p.base, fx, EXCL_S3, EXCL_R3, EXCL_B3, EXCL_RC3);
+	ex += repulsive_lj(g.back_base, fx, EXCL_S3, EXCL_R3, EXCL_B3, EXCL_RC3);
 	o.force += fx;
 	o.tp -= cross(p.base, fx);
 	o.tq += cross(q.back, fx);
-	ex += repulsive_lj(r + q.base - p.back, fx, EXCL_S4, EXCL_R4, EXCL_B4, EXCL_RC4);
+	ex += repulsive_lj(g.base_back, fx, EXCL_S4, EXCL_R4, EXCL_B4, EXCL_RC4);
 	o.force += fx;
 	o.tp -= cross(p.back, fx);
 	o.tq += cross(q.base, fx);
@@ -94,9 +184,8 @@ __device__ PairOut bonded_pair(const oxh_sys &S, const Frame &p, const Frame &q,
 
 	// stacking
 	const f3 &a1 = p.a1, &a2 = p.a2, &a3 = p.a3, &b1 = q.a1, &b2 = q.a2, &b3 = q.a3;
-	f3 rbackref = r + b1 * POS_BACK - a1 * POS_BACK;
 	float rbackrefmod = sqrtf(dot(rbackref, rbackref));
-	f3 rstack = r + q.stack - p.stack;
+	f3 rstack = g.stack;
 	float rstackmod = sqrtf(dot(rstack, rstack));
 	f3 rstackdir = rstack * (1.f / rstackmod);
 
@@ -201,26 +290,26 @@ __device__ PairOut bonded_pair(const oxh_sys &S, const Frame &p, const Frame &q,
 // (src/Interactions/DNAInteraction.cpp:707-1021,
 //  src/Interactions/DNA2Interaction.cpp:214-306).  Adds the force on q to
 // fq and the lab torque on p to tp; energies into e4 (excl, HB, CRST, CXST).
-__device__ void close_pair(const oxh_sys &S, const Frame &p, const Frame &q, f3 r, f3 &fq, f3 &tp, float e4[4]) {
+__device__ void close_pair(const oxh_sys &S, const Frame &p, const Frame &q, const PairGeom &g, f3 &fq, f3 &tp, float e4[4]) {
 	// excluded volume: base-base, p.back-q.base, p.base-q.back, back-back
 	f3 fx;
-	float ex = repulsive_lj(r + q.base - p.base, fx, EXCL_S2, EXCL_R2, EXCL_B2, EXCL_RC2);
+	float ex = repulsive_lj(g.base, fx, EXCL_S2, EXCL_R2, EXCL_B2, EXCL_RC2);
 	fq += fx;
 	tp -= cross(p.base, fx);
-	ex += repulsive_lj(r + q.base - p.back, fx, EXCL_S4, EXCL_R4, EXCL_B4, EXCL_RC4);
+	ex += repulsive_lj(g.base_back, fx, EXCL_S4, EXCL_R4, EXCL_B4, EXCL_RC4);
 	fq += fx;
 	tp -= cross(p.back, fx);
-	ex += repulsive_lj(r + q.back - p.base, fx, EXCL_S3, EXCL_R3, EXCL_B3, EXCL_RC3);
+	ex += repulsive_lj(g.back_base, fx, EXCL_S3, EXCL_R3, EXCL_B3, EXCL_RC3);
 	fq += fx;
 	tp -= cross(p.base, fx);
-	ex += repulsive_lj(r + q.back - p.back, fx, EXCL_S1, EXCL_R1, EXCL_B1, EXCL_RC1);
+	ex += repulsive_lj(g.back, fx, EXCL_S1, EXCL_R1, EXCL_B1, EXCL_RC1);
 	fq += fx;
 	tp -= cross(p.back, fx);
 	e4[0] = ex;
 	e4[1] = e4[2] = e4[3] = 0.f;
 
 	const f3 &a1 = p.a1, &a3 = p.a3, &b1 = q.a1, &b3 = q.a3;
-	f3 rhydro = r + q.base - p.base;
+	f3 rhydro = g.base;
 	float rh2 = dot(rhydro, rhydro);
 
 	// hydrogen bonding
@@ -303,7 +392,7 @@ __device__ void close_pair(const oxh_sys &S, const Frame &p, const Frame &q, f3 
 	}
 
 	// coaxial stacking (oxDNA2 form)
-	f3 rstack = r + q.stack - p.stack;
+	f3 rstack = g.stack;
 	float rs2 = dot(rstack, rstack);
 	if(SQR(CXST_RCLOW) < rs2 && rs2 < SQR(CXST_RCHIGH)) {
 		float rmod = sqrtf(rs2);
@@ -388,14 +477,16 @@ __device__ __forceinline__ float group_sum(float v, int K) {
 // K may be a run-time value (the workgroup-resident engine) or a constant
 // (the grid kernels, where it folds into the loop strides and shuffles).
 template<bool SPLIT, bool FAR>
-__device__ __forceinline__ void particle_forces(int K, int i, int l, int N, const double4 *__restrict__ pos, const float4 *__restrict__ q32,
+__device__ __forceinline__ void particle_forces(int K, int i, int l, int N, const double4 *__restrict__ pos,
+		const double4 *__restrict__ q64, const float4 *__restrict__ q32,
 		const float4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo, const int *__restrict__ nbr,
 		const int2 *__restrict__ nnbr, int max_neigh, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ sstate,
 		float4 *__restrict__ Fout, float4 *__restrict__ Tout, float *__restrict__ esplit, int *__restrict__ hbcount) {
 	const int2 pi = pinfo[i];
 	const oxh_sys &S = systems[pi.y];
 	const double4 P = pos[i];
-	const Frame fp = make_frame(q32[i], pi.x);
+	Sites64 sp;
+	const Frame fp = make_frame_d(q64[i], pi.x, sp);
 	const int2 bd = bonds[i];
 
 	f3 F = {0, 0, 0}, T = {0, 0, 0};
@@ -407,10 +498,15 @@ __device__ __forceinline__ void particle_forces(int K, int i, int l, int N, cons
 	// bonded neighbours: i is the 5' partner of n3 and the 3' partner of n5
 	if(l == 0 && bd.x >= 0) {
 		int j = bd.x;
-		Frame fq = make_frame(q32[j], pinfo[j].x);
-		f3 r = min_image(P, pos[j], S);
+		Sites64 sq;
+		Frame fq = make_frame_d(q64[j], pinfo[j].x, sq);
+		const d3 rd = min_image_d(P, pos[j], S);
+		const PairGeom g = pair_geom(rd, sp, sq);
+		float stretch;
+		f3 backref;
+		bonded_geom(rd, sp, sq, S, stretch, backref);
 		float e3[3];
-		PairOut o = bonded_pair(S, fp, fq, r, e3, err);
+		PairOut o = bonded_pair(S, fp, fq, g, stretch, backref, e3, err);
 		F -= o.force;
 		T += o.tp;
 		E += 0.5f * o.e;
@@ -422,10 +518,15 @@ __device__ __forceinline__ void particle_forces(int K, int i, int l, int N, cons
 	}
 	if(l == (K > 1 ? 1 : 0) && bd.y >= 0) {
 		int j = bd.y;
-		Frame fq = make_frame(q32[j], pinfo[j].x);
-		f3 r = min_image(pos[j], P, S);
+		Sites64 sq;
+		Frame fq = make_frame_d(q64[j], pinfo[j].x, sq);
+		const d3 rd = min_image_d(pos[j], P, S);
+		const PairGeom g = pair_geom(rd, sq, sp);
+		float stretch;
+		f3 backref;
+		bonded_geom(rd, sq, sp, S, stretch, backref);
 		float e3[3];
-		PairOut o = bonded_pair(S, fq, fp, r, e3, err);
+		PairOut o = bonded_pair(S, fq, fp, g, stretch, backref, e3, err);
 		F += o.force;
 		T += o.tq;
 		E += 0.5f * o.e;
@@ -443,16 +544,18 @@ __device__ __forceinline__ void particle_forces(int K, int i, int l, int N, cons
 	// close sublist: every term
 	for(int k = l; k < nn.x; k += K) {
 		const int j = row[k];
-		const f3 r = min_image(P, pos[j], S);
-		const Frame fq = make_frame(q32[j], pinfo[j].x);
+		Sites64 sq;
+		const Frame fq = make_frame_d(q64[j], pinfo[j].x, sq);
+		const PairGeom g = pair_geom(min_image_d(P, pos[j], S), sp, sq);
+		const f3 r = g.r;
 		f3 fq_ = {0, 0, 0};
-		float edh = debye_huckel(S, r + fq.back - fp.back, cf_i * bpos[j].w, fq_);
+		float edh = debye_huckel(S, g.back, cf_i * bpos[j].w, fq_);
 		f3 tp = -cross(fp.back, fq_);
 		float e = edh;
 		if(SPLIT) es[7] += 0.5f * edh;
 		if(dot(r, r) < R_CLOSE * R_CLOSE) {
 			float e4[4];
-			close_pair(S, fp, fq, r, fq_, tp, e4);
+			close_pair(S, fp, fq, g, fq_, tp, e4);
 			e += e4[0] + e4[1] + e4[2] + e4[3];
 			Ehb += 0.5f * e4[1];
 			if(hbcount && e4[1] < -0.1f) nhb++;

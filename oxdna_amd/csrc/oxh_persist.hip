@@ -45,21 +45,25 @@ struct PersistArgs {
 	const int *cell_count, *cell_items;
 	const float4 *pos32;
 	int *hbcount;
+	const int *limit;
 };
 
-__global__ void __launch_bounds__(PERSIST_BLOCK) k_persist(PersistArgs a) {
+template<int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_persist(PersistArgs a) {
 	__shared__ double sh[5][256];
 	__shared__ double fac[5];
 	__shared__ int need;
 	const int s = blockIdx.x;
 	const oxh_sys &S = a.systems[s];
 	const int first = S.first, n = S.n;
+	const int nsteps = a.limit ? min(a.nsteps, a.limit[s]) : a.nsteps;
+	if(n == 0 || nsteps <= 0) return;   // empty slot, or no steps left for this system
 	const int tid = threadIdx.x;
-	const int K = persist_lanes(n);
+	const int K = persist_lanes(n, BLOCK);
 	const long long step0 = a.pstep[first];
 	if(tid == 0) need = 0;
 	__syncthreads();
-	for(int it = 0; it < a.nsteps; it++) {
+	for(int it = 0; it < nsteps; it++) {
 		// 1. first half kick, drift, rotation, FP32 mirror, skin check
 		if(tid < n && first_step_particle(first + tid, S, a.pos, a.vel, a.Lm, a.q64, a.q32, a.bpos, a.F, a.T, a.rlist, a.bonds, a.pstep))
 			need = 1;
@@ -67,14 +71,14 @@ __global__ void __launch_bounds__(PERSIST_BLOCK) k_persist(PersistArgs a) {
 		// 2. Verlet rebuild (all pairs of the system; such systems use no cells)
 		const bool rebuild = need != 0;
 		if(rebuild) {
-			for(int u = tid; u < n * BUILD_LANES; u += PERSIST_BLOCK)
+			for(int u = tid; u < n * BUILD_LANES; u += BLOCK)
 				build_row<BUILD_LANES>(first + u / BUILD_LANES, u % BUILD_LANES, s, a.pos, a.bonds, a.systems, a.state, a.cell_count,
 						a.cell_items, a.cell_cap, a.nbr, a.nnbr, a.max_neigh, a.rlist, a.pos32);
 			__syncthreads();
 		}
 		// 3. forces and torques, K lanes per nucleotide, far pairs inline
 		if(tid < n * K)
-			particle_forces<false, true>(K, first + tid / K, tid % K, a.N, a.pos, a.q32, a.bpos, a.bonds, a.pinfo, a.nbr, a.nnbr,
+			particle_forces<false, true>(K, first + tid / K, tid % K, a.N, a.pos, a.q64, a.q32, a.bpos, a.bonds, a.pinfo, a.nbr, a.nnbr,
 					a.max_neigh, a.systems, a.state, a.F, a.T, nullptr, a.hbcount);
 		__syncthreads();
 		// 4. second half kick + Langevin / Brownian
@@ -110,13 +114,13 @@ __global__ void __launch_bounds__(PERSIST_BLOCK) k_persist(PersistArgs a) {
 // Forces of the current state with the workgroup engine's lanes per
 // nucleotide (after uploads, list rebuilds and for observables), so that
 // runs continue from exactly the forces the engine itself computes.
-template<bool SPLIT>
-__global__ void __launch_bounds__(PERSIST_BLOCK) k_persist_forces(PersistArgs a, float *esplit) {
+template<bool SPLIT, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_persist_forces(PersistArgs a, float *esplit) {
 	const oxh_sys &S = a.systems[blockIdx.x];
-	const int K = persist_lanes(S.n);
+	const int K = persist_lanes(S.n, BLOCK);
 	const int tid = threadIdx.x;
 	if(tid < S.n * K)
-		particle_forces<SPLIT, true>(K, S.first + tid / K, tid % K, a.N, a.pos, a.q32, a.bpos, a.bonds, a.pinfo, a.nbr, a.nnbr,
+		particle_forces<SPLIT, true>(K, S.first + tid / K, tid % K, a.N, a.pos, a.q64, a.q32, a.bpos, a.bonds, a.pinfo, a.nbr, a.nnbr,
 				a.max_neigh, a.systems, a.state, a.F, a.T, esplit, a.hbcount);
 }
 
@@ -147,20 +151,29 @@ PersistArgs make_args(const Batch &b, int nsteps) {
 	a.cell_items = b.cell_items;
 	a.pos32 = b.pos32;
 	a.hbcount = b.want_hbcount ? b.hbcount : nullptr;
+	a.limit = b.limit;
 	return a;
 }
 
 }  // namespace
 
-void launch_persist(hipStream_t st, const Batch &b, int nsteps) {
+void launch_persist(hipStream_t st, const Batch &b, int nsteps, int block) {
 	if(b.N == 0 || nsteps <= 0) return;
-	k_persist<<<b.nsys, PERSIST_BLOCK, 0, st>>>(make_args(b, nsteps));
+	if(block == 256) k_persist<256><<<b.nsys, 256, 0, st>>>(make_args(b, nsteps));
+	else k_persist<512><<<b.nsys, 512, 0, st>>>(make_args(b, nsteps));
 }
 
-void launch_persist_forces(hipStream_t st, const Batch &b, bool split) {
+void launch_persist_forces(hipStream_t st, const Batch &b, bool split, int block) {
 	if(b.N == 0) return;
-	if(split) k_persist_forces<true><<<b.nsys, PERSIST_BLOCK, 0, st>>>(make_args(b, 0), b.esplit);
-	else k_persist_forces<false><<<b.nsys, PERSIST_BLOCK, 0, st>>>(make_args(b, 0), nullptr);
+	PersistArgs a = make_args(b, 0);
+	if(block == 256) {
+		if(split) k_persist_forces<true, 256><<<b.nsys, 256, 0, st>>>(a, b.esplit);
+		else k_persist_forces<false, 256><<<b.nsys, 256, 0, st>>>(a, nullptr);
+	}
+	else {
+		if(split) k_persist_forces<true, 512><<<b.nsys, 512, 0, st>>>(a, b.esplit);
+		else k_persist_forces<false, 512><<<b.nsys, 512, 0, st>>>(a, nullptr);
+	}
 }
 
 }  // namespace oxh

@@ -64,10 +64,11 @@ def _run(hip, monkeypatch, hs, steps, engine, lanes=None, **opts):
 @pytest.mark.parametrize("thermo", list(THERMO))
 def test_engines_agree(hip, monkeypatch, thermo):
     """Same lanes per nucleotide (8 for 33-64 nt): forces of the same state
-    are bitwise equal, torques equal to FP32 rounding (the compiler may fuse
-    multiply-adds of the torque sums differently in the two kernels), and
-    20 steps later the trajectories still agree to 1e-9 (a 1-ulp torque
-    difference grows by ~10x per 10 steps here)."""
+    are bitwise equal and torques equal to FP32 rounding (the compiler may
+    fuse multiply-adds of the torque sums differently in the two kernels).
+    20 steps later the trajectories agree to 1e-6: the torque rounding
+    moves L by ~dt * ulp(|T|) ~ 1e-9 per step; a wrong term moves it by
+    ~1e-2."""
     hs = _hairpins(33, 64, 12)
     a0 = _run(hip, monkeypatch, hs, 0, "grid", lanes=8, **THERMO[thermo])
     b0 = _run(hip, monkeypatch, hs, 0, "workgroup", **THERMO[thermo])
@@ -79,7 +80,7 @@ def test_engines_agree(hip, monkeypatch, thermo):
     for k, (x, y) in enumerate(zip(a, b)):
         assert x["rebuilds"] == y["rebuilds"], (k, x["rebuilds"], y["rebuilds"])
         for key in ("pos", "a1", "a3", "vel", "L"):
-            assert np.abs(x[key] - y[key]).max() <= 1e-9, (thermo, k, key, np.abs(x[key] - y[key]).max())
+            assert np.abs(x[key] - y[key]).max() <= 1e-6, (thermo, k, key, np.abs(x[key] - y[key]).max())
 
 
 def test_alone_equals_in_batch(hip, monkeypatch):
