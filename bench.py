@@ -255,12 +255,16 @@ def main():
     force_ms = b.time_forces(50)
     E = b.energies().sum(axis=0)
     b.close()
+    # achievable HBM bandwidth of a plain copy kernel on this device (and,
+    # under a counter profiler, the dispatches that calibrate FETCH_SIZE /
+    # WRITE_SIZE: 256 MiB fetched and written per launch)
+    copy_gbps = backend.copy_bench(256 << 20, reps=3)
     nbar = st["mean_neigh"]
     # force kernel, algorithmic bytes per nucleotide: own state read
-    # (pos 32, q32 16, bsite 16, bonds 8, pinfo 8, list length 4), F and T
-    # written (32), one 4-byte list entry per neighbour; neighbour gathers
-    # are counted as cache/LDS-resident
-    b_alg = 116.0 + 4.0 * nbar
+    # (pos 32, FP64 quaternion 32, backbone site 16, bonds 8, pinfo 8, list
+    # lengths 8), F and T written (32), one 4-byte list entry per
+    # neighbour; neighbour gathers are counted as cache-resident
+    b_alg = 136.0 + 4.0 * nbar
     achieved = n_total * b_alg / (force_ms * 1e-3) / 1e9
     rebuild_every = args.steps / max(1, reb1 - reb0)
     log(f"{elapsed:.3f} s for {args.steps} steps; {1e3 * elapsed / args.steps:.4f} ms/step; force kernel "
@@ -293,7 +297,7 @@ def main():
                    "hilbert_sort_every_steps": args.sort_every,
                    "mean_neighbours": round(nbar, 2), "steps_per_rebuild": round(rebuild_every, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "copy_kernel_gbps": copy_gbps,
                      "kernel": "k_forces", "avg_launch_us": force_ms * 1e3, "bytes_per_nt": b_alg,
                      "lanes_per_nt": st["lanes"], "block": st["block"], "far_lanes_per_nt": st["far_lanes"],
                      "far_block": st["far_block"]},

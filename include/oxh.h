@@ -231,6 +231,13 @@ oxh_status oxh_hilbert_keys(int device, int n, const unsigned int *xyzb, unsigne
  * thermostats' generator, for checking against published answers. */
 oxh_status oxh_philox(int device, int n, const unsigned int *ctr4, const unsigned int *key2, unsigned int *out4);
 
+/* Copies `bytes` between two device buffers with a plain float4 copy
+ * kernel (k_copy_probe), reps times, and returns the achieved HBM GB/s
+ * (read + write).  Calibrates HBM traffic counters (a copy of B bytes
+ * fetches and writes B bytes) and measures the achievable bandwidth; no
+ * counterpart in the reference. */
+oxh_status oxh_copy_bench(int device, long long bytes, int reps, double *gbps);
+
 #ifdef __cplusplus
 }
 #endif

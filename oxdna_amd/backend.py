@@ -241,6 +241,13 @@ class HIPPool(HIPBatch):
         return dict(done=done, target=target, n=nn, error=err)
 
 
+def copy_bench(nbytes: int = 1 << 30, reps: int = 5, device: int = 0) -> float:
+    """Achieved HBM GB/s (read + write) of a plain device copy of nbytes."""
+    out = C.c_double(0.0)
+    check(lib().oxh_copy_bench(device, int(nbytes), int(reps), C.byref(out)), "oxh_copy_bench")
+    return out.value
+
+
 def hilbert_keys(xyz: np.ndarray, bits: int, device: int = 0) -> np.ndarray:
     xyzb = np.zeros((len(xyz), 4), np.uint32)
     xyzb[:, :3] = xyz

@@ -202,9 +202,10 @@ void quat_to_axes(const double4 &q, double a1[3], double a3[3]) {
 	a3[2] = -sqx - sqy + sqz + sqw;
 }
 
-__global__ void k_clear_need(int nsys, oxh_sys_state *st, int set_value) {
+__global__ void k_clear_need(int nsys, oxh_sys_state *st, int set_value, int *any) {
 	int s = blockIdx.x * blockDim.x + threadIdx.x;
 	if(s < nsys) st[s].need_rebuild = set_value;
+	if(s == 0) *any = set_value;
 }
 
 oxh_status sync_and_check(oxh_batch *B) {
@@ -222,21 +223,29 @@ oxh_status read_states(oxh_batch *B, std::vector<oxh_sys_state> &st) {
 // list build of the current state (systems flagged, or all)
 void enqueue_lists(oxh_batch *B, bool all) {
 	oxh::Batch &b = B->b;
-	if(all) k_clear_need<<<(b.nsys + 63) / 64, 64, 0, b.stream>>>(b.nsys, b.d_state, 1);
+	if(all) k_clear_need<<<(b.nsys + 63) / 64, 64, 0, b.stream>>>(b.nsys, b.d_state, 1, b.any_rebuild);
 	oxh::launch_cells(b.stream, b, b.pos);
 	oxh::launch_build_list(b.stream, b, 0, b.pos);
 }
 
-// step k of a chunk; step_index = the batch's step count (Bussi alignment)
-void enqueue_step(oxh_batch *B, int k, long long step_index) {
+// Steps 0 .. count - 1 of a chunk; step_index0 = the batch's step count at
+// the chunk's start (Bussi alignment).  The second half of a step and the
+// first half of the next share one kernel, except around Bussi
+// activations, which need all of a system's velocities in between.
+void enqueue_steps(oxh_batch *B, int count, long long step_index0) {
 	oxh::Batch &b = B->b;
-	oxh::launch_first_step(b.stream, b, k);   // also zeroes the cell counters
-	oxh::launch_cells_fill_sort(b.stream, b);
-	oxh::launch_build_list(b.stream, b, 0, b.pos);
-	oxh::launch_forces(b.stream, b, false, k);
-	oxh::launch_second_step(b.stream, b, k);
-	if(B->any_bussi) {
-		if(!B->bussi_aligned || step_index % B->bussi_ns == 0) oxh::launch_bussi(b.stream, b, k);
+	auto bussi_due = [&](int k) { return B->any_bussi && (!B->bussi_aligned || (step_index0 + k) % B->bussi_ns == 0); };
+	oxh::launch_first_step(b.stream, b, 0);   // also zeroes the cell counters
+	for(int k = 0; k < count; k++) {
+		oxh::launch_cells_fill_sort(b.stream, b);
+		oxh::launch_build_list(b.stream, b, 0, b.pos);
+		oxh::launch_forces(b.stream, b, false, k);
+		if(k + 1 < count && !bussi_due(k)) oxh::launch_second_first(b.stream, b, k);
+		else {
+			oxh::launch_second_step(b.stream, b, k);
+			if(bussi_due(k)) oxh::launch_bussi(b.stream, b, k);
+			if(k + 1 < count) oxh::launch_first_step(b.stream, b, k + 1);
+		}
 	}
 }
 
@@ -287,7 +296,7 @@ oxh_status build_all(oxh_batch *B) {
 	for(int attempt = 0; attempt < 12; attempt++) {
 		enqueue_lists(B, true);
 		enqueue_forces(B);
-		k_clear_need<<<(B->b.nsys + 63) / 64, 64, 0, B->b.stream>>>(B->b.nsys, B->b.d_state, 0);
+		k_clear_need<<<(B->b.nsys + 63) / 64, 64, 0, B->b.stream>>>(B->b.nsys, B->b.d_state, 0, B->b.any_rebuild);
 		std::vector<oxh_sys_state> st;
 		TRY(read_states(B, st));
 		bool overflow = false;
@@ -329,7 +338,10 @@ oxh_status restore(oxh_batch *B) {
 	return sync_and_check(B);
 }
 
-__global__ void k_set_need_one(oxh_sys_state *st, int s) { st[s].need_rebuild = 1; }
+__global__ void k_set_need_one(oxh_sys_state *st, int s, int *any) {
+	st[s].need_rebuild = 1;
+	*any = 1;
+}
 
 // Rebuilds every system's list (or only system `only`, if >= 0) from the
 // positions of its last build (rlist): the same pairs in the same order as
@@ -339,11 +351,11 @@ __global__ void k_set_need_one(oxh_sys_state *st, int s) { st[s].need_rebuild = 
 oxh_status rebuild_from_rlist(oxh_batch *B, int only = -1) {
 	oxh::Batch &b = B->b;
 	for(int attempt = 0; attempt < 12; attempt++) {
-		if(only >= 0 && attempt == 0) k_set_need_one<<<1, 1, 0, b.stream>>>(b.d_state, only);
-		else k_clear_need<<<(b.nsys + 63) / 64, 64, 0, b.stream>>>(b.nsys, b.d_state, 1);
+		if(only >= 0 && attempt == 0) k_set_need_one<<<1, 1, 0, b.stream>>>(b.d_state, only, b.any_rebuild);
+		else k_clear_need<<<(b.nsys + 63) / 64, 64, 0, b.stream>>>(b.nsys, b.d_state, 1, b.any_rebuild);
 		oxh::launch_cells(b.stream, b, b.rlist);
 		oxh::launch_build_list(b.stream, b, 0, b.rlist);
-		k_clear_need<<<(b.nsys + 63) / 64, 64, 0, b.stream>>>(b.nsys, b.d_state, 0);
+		k_clear_need<<<(b.nsys + 63) / 64, 64, 0, b.stream>>>(b.nsys, b.d_state, 0, b.any_rebuild);
 		std::vector<oxh_sys_state> st;
 		TRY(read_states(B, st));
 		bool overflow = false;
@@ -661,7 +673,7 @@ oxh_status oxh_batch_destroy(oxh_batch *B) {
 	void *ptrs[] = {b.pos, b.vel, b.Lm, b.q64, b.rlist, b.q32, b.bpos, b.F, b.T, b.bonds, b.pinfo, b.orig, b.pstep, b.nbr, b.nnbr,
 			b.cell_count, b.cell_items, b.cell_owner, b.pos32, b.d_sys, b.d_state, b.red_blocks, b.red_sys_blocks, b.red_partial, b.red_out, b.red_done,
 			b.esplit, b.hbcount, b.keys, b.keys_alt, b.perm, b.perm_alt, b.inv_perm, b.radix_hist, b.scratch, B->bk_pos, B->bk_vel,
-			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state, B->bk_rlist, b.limit};
+			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state, B->bk_rlist, b.limit, b.any_rebuild};
 	for(void *p : ptrs)
 		if(p) (void) hipFree(p);
 	if(b.stream) (void) hipStreamDestroy(b.stream);
@@ -867,6 +879,8 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 	}
 	if(const char *p = getenv("OXH_PERSIST")) B->persist &= atoi(p) != 0;
 	TRYB(dalloc(B, &B->bk_rlist, N));
+	TRYB(dalloc(B, &b.any_rebuild, 1));
+	if(hipMemsetAsync(b.any_rebuild, 0, sizeof(int), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
 	if(B->pool) TRYB(dalloc(B, &b.limit, n_sys));
 	oxh_status s = build_all(B);
 	if(s != OXH_OK) return bail(s);
@@ -904,7 +918,7 @@ oxh_status oxh_batch_update_lists(oxh_batch *B, int force) {
 	HIPCHK(hipSetDevice(B->b.device));
 	for(int attempt = 0; attempt < 12; attempt++) {
 		enqueue_lists(B, force != 0);
-		k_clear_need<<<(B->b.nsys + 63) / 64, 64, 0, B->b.stream>>>(B->b.nsys, B->b.d_state, 0);
+		k_clear_need<<<(B->b.nsys + 63) / 64, 64, 0, B->b.stream>>>(B->b.nsys, B->b.d_state, 0, B->b.any_rebuild);
 		std::vector<oxh_sys_state> st;
 		TRY(read_states(B, st));
 		bool overflow = false;
@@ -958,7 +972,7 @@ oxh_status oxh_batch_run(oxh_batch *B, long long n_steps) {
 			if(!b.graph_exec) {
 				hipGraph_t g;
 				HIPCHK(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
-				for(int k = 0; k < B->chunk; k++) enqueue_step(B, k, k);
+				enqueue_steps(B, B->chunk, 0);
 				HIPCHK(hipStreamEndCapture(b.stream, &g));
 				HIPCHK(hipGraphInstantiate(&b.graph_exec, g, nullptr, nullptr, 0));
 				HIPCHK(hipGraphDestroy(g));
@@ -966,7 +980,7 @@ oxh_status oxh_batch_run(oxh_batch *B, long long n_steps) {
 			HIPCHK(hipGraphLaunch(b.graph_exec, b.stream));
 		}
 		else {
-			for(int k = 0; k < (int) todo; k++) enqueue_step(B, k, b.step + k);
+			enqueue_steps(B, (int) todo, b.step);
 		}
 		std::vector<oxh_sys_state> st;
 		TRY(read_states(B, st));
@@ -1420,5 +1434,40 @@ extern "C" oxh_status oxh_philox(int device, int n, const unsigned int *ctr4, co
 	(void) hipFree(dc);
 	(void) hipFree(dk);
 	(void) hipFree(dout);
+	return OXH_OK;
+}
+
+// STREAM-style copy: each thread moves float4s, grid-stride
+__global__ void __launch_bounds__(256) k_copy_probe(size_t n, const float4 *__restrict__ src, float4 *__restrict__ dst) {
+	for(size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+extern "C" oxh_status oxh_copy_bench(int device, long long bytes, int reps, double *gbps) {
+	if(bytes < 16 || reps <= 0 || !gbps) return fail(OXH_ERR_ARG, "oxh_copy_bench: bad arguments");
+	HIPCHK(hipSetDevice(device));
+	const size_t n = (size_t) bytes / 16;
+	float4 *a = nullptr, *b = nullptr;
+	HIPCHK(hipMalloc(&a, n * 16));
+	if(hipMalloc(&b, n * 16) != hipSuccess) {
+		(void) hipFree(a);
+		return fail(OXH_ERR_ALLOC, "oxh_copy_bench: allocation failed");
+	}
+	hipEvent_t e0, e1;
+	HIPCHK(hipEventCreate(&e0));
+	HIPCHK(hipEventCreate(&e1));
+	const int grid = 256 * 8 * 4;   // 4 workgroups of 256 per SIMD-pair over 256 CUs
+	HIPCHK(hipMemset(a, 0, n * 16));
+	k_copy_probe<<<grid, 256>>>(n, a, b);   // warm
+	HIPCHK(hipEventRecord(e0, 0));
+	for(int r = 0; r < reps; r++) k_copy_probe<<<grid, 256>>>(n, a, b);
+	HIPCHK(hipEventRecord(e1, 0));
+	HIPCHK(hipEventSynchronize(e1));
+	float ms = 0.f;
+	HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+	*gbps = 2.0 * (double) n * 16 * reps / (ms * 1e-3) / 1e9;
+	(void) hipEventDestroy(e0);
+	(void) hipEventDestroy(e1);
+	(void) hipFree(a);
+	(void) hipFree(b);
 	return OXH_OK;
 }

@@ -33,12 +33,17 @@ __global__ void __launch_bounds__(256) k_forces(int N, const double4 *__restrict
 		const float4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
 		const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh, const oxh_sys *__restrict__ systems,
 		oxh_sys_state *__restrict__ sstate, float4 *__restrict__ Fout, float4 *__restrict__ Tout, float *__restrict__ esplit,
-		int *__restrict__ hbcount, int k, const int *__restrict__ limit) {
+		int *__restrict__ hbcount, int k, const int *__restrict__ limit, int *__restrict__ any) {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
 	const int i = t / K;
 	const int l = t % K;
+	if(k >= 0 && t == 0) *any = 0;   // inside a step: this step's list builds are done
 	if(i >= N) return;
 	const int s = pinfo[i].y;
+	if(k >= 0 && l == 0 && i == systems[s].first && sstate[s].need_rebuild) {
+		sstate[s].need_rebuild = 0;
+		sstate[s].n_rebuilds += 1;
+	}
 	if(idle(i, systems[s], s, k, limit)) return;
 
 	particle_forces<SPLIT, FAR>(K, i, l, N, pos, q64, q32, bpos, bonds, pinfo, nbr, nnbr, max_neigh, systems, sstate, Fout, Tout, esplit, hbcount);
@@ -114,18 +119,18 @@ static void launch_k(hipStream_t st, const Batch &b, bool split, int block, bool
 	if(split) {
 		if(far_inline)
 			k_forces<K, true, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-					b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr);
+					b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr, b.any_rebuild);
 		else
 			k_forces<K, true, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-					b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr);
+					b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr, b.any_rebuild);
 	}
 	else {
 		if(far_inline)
 			k_forces<K, false, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-					b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr);
+					b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr, b.any_rebuild);
 		else
 			k_forces<K, false, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-					b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr);
+					b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr, b.any_rebuild);
 	}
 }
 

@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(256) k_first_step(int N, double4 *__restrict__
 		double4 *__restrict__ q64, float4 *__restrict__ q32, float4 *__restrict__ bpos, const float4 *__restrict__ F,
 		const float4 *__restrict__ T, const double4 *__restrict__ rlist, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
 		long long *__restrict__ pstep, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st, int ncell,
-		int *__restrict__ cell_count, int k, const int *__restrict__ limit) {
+		int *__restrict__ cell_count, int k, const int *__restrict__ limit, int *__restrict__ any) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	// cell counters start every step at zero; they are only filled (after
 	// this kernel) in steps that rebuild the lists
@@ -48,13 +48,16 @@ __global__ void __launch_bounds__(256) k_first_step(int N, double4 *__restrict__
 	const int s = pinfo[i].y;
 	const oxh_sys &S = systems[s];
 	if(idle(i, S, s, k, limit)) return;
-	if(first_step_particle(i, S, pos, vel, Lm, q64, q32, bpos, F, T, rlist, bonds, pstep)) st[s].need_rebuild = 1;
+	if(first_step_particle(i, S, pos, vel, Lm, q64, q32, bpos, F, T, rlist, bonds, pstep)) {
+		st[s].need_rebuild = 1;
+		*any = 1;
+	}
 }
 
 void launch_first_step(hipStream_t st, const Batch &b, int k) {
 	int n = std::max(b.N, b.ncell_total);
 	k_first_step<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.F, b.T, b.rlist, b.bonds, b.pinfo,
-			b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit);
+			b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild);
 }
 
 // second half kick + Langevin / Brownian thermostats; the first particle
@@ -69,15 +72,42 @@ __global__ void __launch_bounds__(256) k_second_step(int N, double4 *__restrict_
 	const oxh_sys &S = systems[s];
 	if(idle(i, S, s, k, limit)) return;
 	second_step_particle(i, S, vel, Lm, F, T, orig, pstep);
-	if(i == S.first && st[s].need_rebuild) {
-		st[s].need_rebuild = 0;
-		st[s].n_rebuilds += 1;
-	}
 }
 
 void launch_second_step(hipStream_t st, const Batch &b, int k) {
 	k_second_step<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.vel, b.Lm, b.F, b.T, b.pinfo, b.orig, b.pstep, b.d_sys, b.d_state, k,
 			b.limit);
+}
+
+// The second half of step k and the first half of step k + 1 in one pass
+// over the particles (when no Bussi activation falls in between): the same
+// per-particle functions in the same order as k_second_step followed by
+// k_first_step.
+__global__ void __launch_bounds__(256) k_second_first(int N, double4 *__restrict__ pos, double4 *__restrict__ vel,
+		double4 *__restrict__ Lm, double4 *__restrict__ q64, float4 *__restrict__ q32, float4 *__restrict__ bpos,
+		const float4 *__restrict__ F, const float4 *__restrict__ T, const double4 *__restrict__ rlist, const int2 *__restrict__ bonds,
+		const int2 *__restrict__ pinfo, const int *__restrict__ orig, long long *__restrict__ pstep, const oxh_sys *__restrict__ systems,
+		oxh_sys_state *__restrict__ st, int ncell, int *__restrict__ cell_count, int k, const int *__restrict__ limit,
+		int *__restrict__ any) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i < ncell) cell_count[i] = 0;
+	if(i >= N) return;
+	const int s = pinfo[i].y;
+	const oxh_sys &S = systems[s];
+	if(padding(i, S)) return;
+	if(!(limit && k >= limit[s])) second_step_particle(i, S, vel, Lm, F, T, orig, pstep);
+	if(!(limit && k + 1 >= limit[s])) {
+		if(first_step_particle(i, S, pos, vel, Lm, q64, q32, bpos, F, T, rlist, bonds, pstep)) {
+			st[s].need_rebuild = 1;
+			*any = 1;
+		}
+	}
+}
+
+void launch_second_first(hipStream_t st, const Batch &b, int k) {
+	int n = std::max(b.N, b.ncell_total);
+	k_second_first<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.F, b.T, b.rlist, b.bonds, b.pinfo,
+			b.orig, b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild);
 }
 
 // -------------------------------------------------------- FP64 reductions

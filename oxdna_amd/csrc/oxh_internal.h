@@ -52,6 +52,10 @@ struct Batch {
 	// Particles [first + n, first + capacity) of a slot are padding, which
 	// every kernel skips.
 	int *limit = nullptr;
+	// set when some system's skin check (or the host) asks for a list build
+	// in this step, cleared by the step's second half; the per-step cell and
+	// list kernels return at once while it is clear
+	int *any_rebuild = nullptr;
 
 	// reductions: blocks of at most RED_CHUNK particles, each inside one system
 	int nred = 0;
@@ -113,6 +117,8 @@ void launch_cells_fill_sort(hipStream_t st, const Batch &b);
 void launch_build_list(hipStream_t st, const Batch &b, int force_all, const double4 *src);
 void launch_first_step(hipStream_t st, const Batch &b, int k);
 void launch_second_step(hipStream_t st, const Batch &b, int k);
+// second half of step k + first half of step k + 1
+void launch_second_first(hipStream_t st, const Batch &b, int k);
 void launch_bussi(hipStream_t st, const Batch &b, int k);
 void launch_reduce(hipStream_t st, const Batch &b);
 // velocities ~ N(0, T) for particles [i0, i0 + n)

@@ -32,7 +32,9 @@ __global__ void k_cells_clear(int ncell, const int *__restrict__ owner, const ox
 }
 
 __global__ void k_cells_fill(int N, const double4 *__restrict__ pos, const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems,
-		oxh_sys_state *__restrict__ st, int *__restrict__ count, int *__restrict__ items, int cap, float4 *__restrict__ pos32) {
+		oxh_sys_state *__restrict__ st, int *__restrict__ count, int *__restrict__ items, int cap, float4 *__restrict__ pos32,
+		const int *__restrict__ any) {
+	if(!*any) return;
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= N) return;
 	int s = pinfo[i].y;
@@ -56,8 +58,9 @@ __global__ void k_cells_fill(int N, const double4 *__restrict__ pos, const int2 
 // whole cell staged in LDS (indices are distinct, so rank = number of
 // smaller indices).
 __global__ void __launch_bounds__(64) k_cells_sort(int ncell, const int *__restrict__ owner, const oxh_sys_state *__restrict__ st,
-		const int *__restrict__ count, int *__restrict__ items, int cap) {
+		const int *__restrict__ count, int *__restrict__ items, int cap, const int *__restrict__ any) {
 	extern __shared__ int cell_buf[];
+	if(!*any) return;
 	for(int c = blockIdx.x; c < ncell; c += gridDim.x) {
 		if(!st[owner[c]].need_rebuild) continue;
 		const int n = min(count[c], cap);
@@ -80,9 +83,9 @@ void launch_cells_fill_sort(hipStream_t st, const Batch &b) {
 	if(!b.any_cells) return;
 	int bs = 256;
 	k_cells_fill<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, b.pos, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_items, b.cell_cap,
-			b.pos32);
-	k_cells_sort<<<std::min(b.ncell_total, 4096), 64, (size_t) b.cell_cap * sizeof(int), st>>>(b.ncell_total, b.cell_owner, b.d_state,
-			b.cell_count, b.cell_items, b.cell_cap);
+			b.pos32, b.any_rebuild);
+	k_cells_sort<<<std::min(b.ncell_total, 1024), 64, (size_t) b.cell_cap * sizeof(int), st>>>(b.ncell_total, b.cell_owner, b.d_state,
+			b.cell_count, b.cell_items, b.cell_cap, b.any_rebuild);
 }
 
 void launch_cells(hipStream_t st, const Batch &b, const double4 *src) {
@@ -90,24 +93,29 @@ void launch_cells(hipStream_t st, const Batch &b, const double4 *src) {
 	int bs = 256;
 	k_cells_clear<<<(b.ncell_total + bs - 1) / bs, bs, 0, st>>>(b.ncell_total, b.cell_owner, b.d_state, b.cell_count);
 	k_cells_fill<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, src, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_items, b.cell_cap,
-			b.pos32);
-	k_cells_sort<<<std::min(b.ncell_total, 4096), 64, (size_t) b.cell_cap * sizeof(int), st>>>(b.ncell_total, b.cell_owner, b.d_state,
-			b.cell_count, b.cell_items, b.cell_cap);
+			b.pos32, b.any_rebuild);
+	k_cells_sort<<<std::min(b.ncell_total, 1024), 64, (size_t) b.cell_cap * sizeof(int), st>>>(b.ncell_total, b.cell_owner, b.d_state,
+			b.cell_count, b.cell_items, b.cell_cap, b.any_rebuild);
 }
 
 __global__ void __launch_bounds__(256) k_build_list(int N, const double4 *__restrict__ pos, const int2 *__restrict__ bonds,
 		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st,
 		const int *__restrict__ count, const int *__restrict__ items, int cap, int *__restrict__ nbr, int2 *__restrict__ nnbr,
-		int max_neigh, double4 *__restrict__ rlist, int force_all, const float4 *__restrict__ pos32) {
+		int max_neigh, double4 *__restrict__ rlist, int force_all, const float4 *__restrict__ pos32, const int *__restrict__ any) {
 	constexpr int K = BUILD_LANES;
-	const int t = blockIdx.x * blockDim.x + threadIdx.x;
-	const int i = t / K;
-	const int l = t % K;
-	if(i >= N) return;
-	int s = pinfo[i].y;
-	if(!force_all && !st[s].need_rebuild) return;
-	if(padding(i, systems[s])) return;
-	build_row<K>(i, l, s, pos, bonds, systems, st, count, items, cap, nbr, nnbr, max_neigh, rlist, pos32);
+	if(!force_all && !*any) return;
+	// grid-stride over the K-lane groups: a step without a rebuild only
+	// dispatches a bounded number of workgroups; the stride is a multiple of
+	// the wave, so every group stays whole and aligned
+	const size_t stride = (size_t) gridDim.x * blockDim.x;
+	for(size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x; t < (size_t) N * K; t += stride) {
+		const int i = (int) (t / K);
+		const int l = (int) (t % K);
+		const int s = pinfo[i].y;
+		if(!force_all && !st[s].need_rebuild) continue;
+		if(padding(i, systems[s])) continue;
+		build_row<K>(i, l, s, pos, bonds, systems, st, count, items, cap, nbr, nnbr, max_neigh, rlist, pos32);
+	}
 }
 
 // src = b.pos (the usual build) or b.rlist (the lists of the last build,
@@ -115,8 +123,9 @@ __global__ void __launch_bounds__(256) k_build_list(int N, const double4 *__rest
 void launch_build_list(hipStream_t st, const Batch &b, int force_all, const double4 *src) {
 	int bs = 256;
 	size_t threads = (size_t) b.N * BUILD_LANES;
-	k_build_list<<<(unsigned) ((threads + bs - 1) / bs), bs, 0, st>>>(b.N, src, b.bonds, b.pinfo, b.d_sys, b.d_state, b.cell_count,
-			b.cell_items, b.cell_cap, b.nbr, b.nnbr, b.max_neigh, src == b.rlist ? nullptr : b.rlist, force_all, b.pos32);
+	const unsigned grid = (unsigned) std::min<size_t>((threads + bs - 1) / bs, 2048);
+	k_build_list<<<grid, bs, 0, st>>>(b.N, src, b.bonds, b.pinfo, b.d_sys, b.d_state, b.cell_count,
+			b.cell_items, b.cell_cap, b.nbr, b.nnbr, b.max_neigh, src == b.rlist ? nullptr : b.rlist, force_all, b.pos32, b.any_rebuild);
 }
 
 // ------------------------------------------------------------ Hilbert sort
