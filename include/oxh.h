@@ -158,6 +158,17 @@ oxh_status oxh_batch_energies(oxh_batch *b, double *out4);
  * get_system_energy_split (src/Interactions/BaseInteraction.cpp:61-86). */
 oxh_status oxh_batch_energy_split(oxh_batch *b, int sys, double *E8);
 
+/* Stress tensor of system `sys` at its current state, components xx, yy,
+ * zz, xy, xz, yz: virial6 = sum over interacting pairs of -r (x) F_p (r = q
+ * - p centre separation, F_p the pair's force on p), kinetic6 = sum of
+ * v (x) v.  (virial6 + kinetic6) / V is the reference's stress tensor and
+ * its trace / 3 the pressure (DNAInteraction's _update_stress_tensor,
+ * src/Interactions/BaseInteraction.cpp:159-200,285-352,
+ * src/Observables/Pressure.cpp); the GPU counterpart is
+ * CPU_stress_tensor (src/CUDA/Interactions/CUDABaseInteraction.cu:134-141).
+ * Either pointer may be NULL. */
+oxh_status oxh_batch_stress(oxh_batch *b, int sys, double *virial6, double *kinetic6);
+
 /* Number of base pairs (pairs with E_HB < -0.1, the reference's HB_CUTOFF,
  * src/defs.h:62, src/Observables/HBList.cpp:141-164) of every system. */
 oxh_status oxh_batch_hb_count(oxh_batch *b, int *out);

@@ -913,8 +913,42 @@ void oxo_params_dh(const oxo_params *P, double *out4) {
  *   E_part (out, N, may be NULL): per-particle energy, each pair's energy
  *      split half-half, so that sum(E_part) = total energy.
  * Returns the total potential energy. */
+static double forces_impl(const oxo_params *P, int N, const double *pos, const double *a1, const double *a2, const double *a3,
+		const int *n3, const int *n5, const int *btype, const double *box, double *E8, double *F, double *Tbody, double *E_part,
+		double *W6);
+
 double oxo_forces(const oxo_params *P, int N, const double *pos, const double *a1, const double *a2, const double *a3, const int *n3,
 		const int *n5, const int *btype, const double *box, double *E8, double *F, double *Tbody, double *E_part) {
+	return forces_impl(P, N, pos, a1, a2, a3, n3, n5, btype, box, E8, F, Tbody, E_part, nullptr);
+}
+
+/* Pair virial of the stress tensor that the DNA interactions accumulate
+ * during a force computation (DNAInteraction's _update_stress_tensor
+ * calls, src/Interactions/BaseInteraction.cpp:159-200; the pressure
+ * observable divides by the volume and adds the kinetic term,
+ * BaseInteraction.cpp:285-352): sum over every interacting pair (bonded and
+ * non-bonded, once) of -r (x) F_p, r = q - p the nearest-image centre
+ * separation and F_p the pair's total force on p; components xx, yy, zz,
+ * xy, xz, yz.  Returns the potential energy. */
+double oxo_virial(const oxo_params *P, int N, const double *pos, const double *a1, const double *a2, const double *a3, const int *n3,
+		const int *n5, const int *btype, const double *box, double *W6) {
+	for(int k = 0; k < 6; k++) W6[k] = 0.;
+	return forces_impl(P, N, pos, a1, a2, a3, n3, n5, btype, box, nullptr, nullptr, nullptr, nullptr, W6);
+}
+
+static void add_virial(double *W, const V3 &r, const V3 &dFp) {
+	if(!W) return;
+	W[0] -= r.x * dFp.x;
+	W[1] -= r.y * dFp.y;
+	W[2] -= r.z * dFp.z;
+	W[3] -= r.x * dFp.y;
+	W[4] -= r.x * dFp.z;
+	W[5] -= r.y * dFp.z;
+}
+
+static double forces_impl(const oxo_params *P, int N, const double *pos, const double *a1, const double *a2, const double *a3,
+		const int *n3, const int *n5, const int *btype, const double *box, double *E8, double *F, double *Tbody, double *E_part,
+		double *W6) {
 	std::vector<Nuc> nucs = make_nucs(N, pos, a1, a2, a3, n3, n5, btype);
 	Acc A;
 	A.F.assign(N, V3{0, 0, 0});
@@ -928,9 +962,11 @@ double oxo_forces(const oxo_params *P, int N, const double *pos, const double *a
 		const Nuc &p = nucs[i];
 		const Nuc &q = nucs[p.n3];
 		V3 r = min_image(p.pos, q.pos, box);
+		const V3 F0 = A.F[p.index];
 		double e0 = backbone(P, p, q, r, &A);
 		double e1 = bonded_excluded_volume(p, q, r, &A);
 		double e2 = stacking(P, p, q, r, &A);
+		add_virial(W6, r, A.F[p.index] - F0);
 		A.E[0] += e0;
 		A.E[1] += e1;
 		A.E[2] += e2;
@@ -947,11 +983,13 @@ double oxo_forces(const oxo_params *P, int N, const double *pos, const double *a
 		if(is_bonded(p, q)) continue;
 		V3 r = min_image(p.pos, q.pos, box);
 		if(dot(r, r) >= P->sqr_rcut) continue;
+		const V3 F0 = A.F[p.index];
 		double e3 = nonbonded_excluded_volume(p, q, r, &A);
 		double e4 = hydrogen_bonding(P, p, q, r, &A);
 		double e5 = cross_stacking(P, p, q, r, &A);
 		double e6 = coaxial_stacking(P, p, q, r, &A);
 		double e7 = debye_huckel(P, p, q, r, &A);
+		add_virial(W6, r, A.F[p.index] - F0);
 		A.E[3] += e3;
 		A.E[4] += e4;
 		A.E[5] += e5;

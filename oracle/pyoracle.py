@@ -35,6 +35,8 @@ def lib():
         L.oxo_forces.argtypes = [C.c_void_p, C.c_int, DP, DP, DP, DP, IP, IP, IP, DP, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p]
         L.oxo_forces.restype = C.c_double
+        L.oxo_virial.argtypes = [C.c_void_p, C.c_int, DP, DP, DP, DP, IP, IP, IP, DP, C.c_void_p]
+        L.oxo_virial.restype = C.c_double
         L.oxo_count_hb.argtypes = [C.c_void_p, C.c_int, DP, DP, DP, DP, IP, IP, IP, DP]
         L.oxo_count_hb.restype = C.c_int
         L.oxo_verlet_pairs.argtypes = [C.c_int, DP, IP, IP, DP, C.c_double, C.c_void_p, C.c_longlong]
@@ -90,6 +92,16 @@ def forces(params: Params, s):
                          _c(s.n5, np.int32), _c(s.btype, np.int32), _c(s.box), E8.ctypes.data, F.ctypes.data,
                          T.ctypes.data, Ep.ctypes.data)
     return dict(E8=E8, U=U, F=F, T=T, Epart=Ep)
+
+
+def virial(params: Params, s) -> np.ndarray:
+    """Pair virial of the standard stress tensor: sum over interacting pairs
+    of -r (x) F_p (xx, yy, zz, xy, xz, yz), not divided by the volume."""
+    a2 = np.cross(s.a3, s.a1)
+    W = np.zeros(6)
+    lib().oxo_virial(params.buf, s.N, _c(s.pos), _c(s.a1), _c(a2), _c(s.a3), _c(s.n3, np.int32), _c(s.n5, np.int32),
+                     _c(s.btype, np.int32), _c(s.box), W.ctypes.data)
+    return W
 
 
 def count_hb(params: Params, s) -> int:

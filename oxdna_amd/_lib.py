@@ -22,7 +22,7 @@ EXPORTS = [
     "oxh_batch_hilbert_sort", "oxh_batch_set_sort_interval", "oxh_batch_sync", "oxh_batch_get_state", "oxh_batch_get_forces", "oxh_batch_energies",
     "oxh_batch_energy_split", "oxh_batch_hb_count", "oxh_batch_get_pairs", "oxh_batch_system_info",
     "oxh_batch_set_state", "oxh_batch_stats", "oxh_batch_time_forces", "oxh_hilbert_keys", "oxh_philox",
-    "oxh_pool_create", "oxh_pool_admit", "oxh_pool_release", "oxh_pool_status", "oxh_copy_bench",
+    "oxh_pool_create", "oxh_pool_admit", "oxh_pool_release", "oxh_pool_status", "oxh_copy_bench", "oxh_batch_stress",
 ]
 
 
@@ -81,6 +81,7 @@ def lib():
         L.oxh_pool_release.argtypes = [V, I]
         L.oxh_pool_status.argtypes = [V, P, P, P, P]
         L.oxh_copy_bench.argtypes = [I, LL, I, P]
+        L.oxh_batch_stress.argtypes = [V, I, P, P]
         for name in EXPORTS:
             f = getattr(L, name)
             if name not in ("oxh_last_error", "oxh_system_desc_defaults"):

@@ -161,6 +161,14 @@ class HIPBatch:
         check(lib().oxh_batch_energy_split(self.h, sys, ptr(out)), "oxh_batch_energy_split")
         return out
 
+    def stress(self, sys: int = 0):
+        """(virial, kinetic): 6 components each (xx, yy, zz, xy, xz, yz),
+        not divided by the volume."""
+        w = np.zeros(6)
+        k = np.zeros(6)
+        check(lib().oxh_batch_stress(self.h, sys, ptr(w), ptr(k)), "oxh_batch_stress")
+        return w, k
+
     def hb_count(self) -> np.ndarray:
         out = np.zeros(self.nsys, np.int32)
         check(lib().oxh_batch_hb_count(self.h, ptr(out)), "oxh_batch_hb_count")

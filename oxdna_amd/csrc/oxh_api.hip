@@ -672,7 +672,7 @@ oxh_status oxh_batch_destroy(oxh_batch *B) {
 	if(b.graph_exec) (void) hipGraphExecDestroy(b.graph_exec);
 	void *ptrs[] = {b.pos, b.vel, b.Lm, b.q64, b.rlist, b.q32, b.bpos, b.F, b.T, b.bonds, b.pinfo, b.orig, b.pstep, b.nbr, b.nnbr,
 			b.cell_count, b.cell_items, b.cell_owner, b.pos32, b.d_sys, b.d_state, b.red_blocks, b.red_sys_blocks, b.red_partial, b.red_out, b.red_done,
-			b.esplit, b.hbcount, b.keys, b.keys_alt, b.perm, b.perm_alt, b.inv_perm, b.radix_hist, b.scratch, B->bk_pos, B->bk_vel,
+			b.esplit, b.stress, b.hbcount, b.keys, b.keys_alt, b.perm, b.perm_alt, b.inv_perm, b.radix_hist, b.scratch, B->bk_pos, B->bk_vel,
 			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state, B->bk_rlist, b.limit, b.any_rebuild};
 	for(void *p : ptrs)
 		if(p) (void) hipFree(p);
@@ -779,6 +779,7 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 	TRYB(dalloc(B, &b.d_sys, n_sys));
 	TRYB(dalloc(B, &b.d_state, n_sys));
 	TRYB(dalloc(B, &b.esplit, (size_t) 8 * N));
+	TRYB(dalloc(B, &b.stress, (size_t) 6 * N));
 	TRYB(dalloc(B, &b.hbcount, N));
 	TRYB(dalloc(B, &b.keys, N));
 	TRYB(dalloc(B, &b.keys_alt, N));
@@ -1258,6 +1259,36 @@ oxh_status oxh_batch_energy_split(oxh_batch *B, int sys, double *E8) {
 		double acc = 0;
 		for(int k = 0; k < n; k++) acc += e[(size_t) t * n + k];
 		E8[t] = acc;
+	}
+	return OXH_OK;
+}
+
+oxh_status oxh_batch_stress(oxh_batch *B, int sys, double *virial6, double *kinetic6) {
+	TRY(check_sys(B, sys));
+	enqueue_forces(B, true);
+	const int f = B->first[sys], n = B->n[sys];
+	std::vector<float> w((size_t) 6 * std::max(n, 1));
+	for(int c = 0; c < 6; c++)
+		HIPCHK(hipMemcpyAsync(w.data() + (size_t) c * n, B->b.stress + (size_t) c * B->b.N + f, n * sizeof(float), hipMemcpyDeviceToHost,
+				B->b.stream));
+	std::vector<double4> v;
+	TRY(download(B, v, B->b.vel, f, n));
+	TRY(sync_and_check(B));
+	for(int c = 0; c < 6; c++) {
+		double acc = 0.;
+		for(int k = 0; k < n; k++) acc += w[(size_t) c * n + k];
+		if(virial6) virial6[c] = acc;
+	}
+	if(kinetic6) {
+		for(int c = 0; c < 6; c++) kinetic6[c] = 0.;
+		for(const double4 &x : v) {
+			kinetic6[0] += x.x * x.x;
+			kinetic6[1] += x.y * x.y;
+			kinetic6[2] += x.z * x.z;
+			kinetic6[3] += x.x * x.y;
+			kinetic6[4] += x.x * x.z;
+			kinetic6[5] += x.y * x.z;
+		}
 	}
 	return OXH_OK;
 }

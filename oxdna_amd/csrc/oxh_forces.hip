@@ -33,7 +33,7 @@ __global__ void __launch_bounds__(256) k_forces(int N, const double4 *__restrict
 		const float4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
 		const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh, const oxh_sys *__restrict__ systems,
 		oxh_sys_state *__restrict__ sstate, float4 *__restrict__ Fout, float4 *__restrict__ Tout, float *__restrict__ esplit,
-		int *__restrict__ hbcount, int k, const int *__restrict__ limit, int *__restrict__ any) {
+		int *__restrict__ hbcount, int k, const int *__restrict__ limit, int *__restrict__ any, float *__restrict__ stress) {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
 	const int i = t / K;
 	const int l = t % K;
@@ -46,7 +46,8 @@ __global__ void __launch_bounds__(256) k_forces(int N, const double4 *__restrict
 	}
 	if(idle(i, systems[s], s, k, limit)) return;
 
-	particle_forces<SPLIT, FAR>(K, i, l, N, pos, q64, q32, bpos, bonds, pinfo, nbr, nnbr, max_neigh, systems, sstate, Fout, Tout, esplit, hbcount);
+	particle_forces<SPLIT, FAR>(K, i, l, N, pos, q64, q32, bpos, bonds, pinfo, nbr, nnbr, max_neigh, systems, sstate, Fout, Tout, esplit, hbcount,
+			stress);
 }
 
 // Far pairs on their own: Debye-Hueckel between backbone sites, few
@@ -60,7 +61,7 @@ template<int K, bool SPLIT>
 __global__ void __launch_bounds__(256) k_forces_far(int N, const double4 *__restrict__ pos, const float4 *__restrict__ q32,
 		const float4 *__restrict__ bpos, const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh,
 		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, float4 *__restrict__ Fio, float4 *__restrict__ Tio,
-		float *__restrict__ esplit, int k_step, const int *__restrict__ limit) {
+		float *__restrict__ esplit, int k_step, const int *__restrict__ limit, float *__restrict__ stress) {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
 	const int i = t / K;
 	const int l = t % K;
@@ -73,6 +74,7 @@ __global__ void __launch_bounds__(256) k_forces_far(int N, const double4 *__rest
 	const float4 Bi = bpos[i];
 	f3 Ff = {0, 0, 0};
 	float Ef = 0.f;
+	float sv[6] = {0, 0, 0, 0, 0, 0};
 	Frame fp;
 	if(S.small_box) fp = make_frame(q32[i], 0);
 	for(int k = l; k < nf; k += K) {
@@ -87,12 +89,15 @@ __global__ void __launch_bounds__(256) k_forces_far(int N, const double4 *__rest
 		f3 fq_ = {0, 0, 0};
 		Ef += debye_huckel(S, rb, Bi.w * Bj.w, fq_);
 		Ff += fq_;
+		if(SPLIT) add_virial(sv, min_image(pos[i], pos[j], S), fq_);
 	}
 	if(K > 1) {
 		Ff.x = group_sum(Ff.x, K);
 		Ff.y = group_sum(Ff.y, K);
 		Ff.z = group_sum(Ff.z, K);
 		Ef = group_sum(Ef, K);
+		if(SPLIT)
+			for(int q = 0; q < 6; q++) sv[q] = group_sum(sv[q], K);
 	}
 	if(l != 0) return;
 	f3 a1, a2, a3;
@@ -108,7 +113,10 @@ __global__ void __launch_bounds__(256) k_forces_far(int N, const double4 *__rest
 	T.z += POS_MM_BACK2 * f1b - POS_MM_BACK1 * f2b;
 	Fio[i] = F;
 	Tio[i] = T;
-	if(SPLIT) esplit[(size_t) 7 * N + i] += 0.5f * Ef;
+	if(SPLIT) {
+		esplit[(size_t) 7 * N + i] += 0.5f * Ef;
+		for(int q = 0; q < 6; q++) stress[(size_t) q * N + i] += 0.5f * sv[q];
+	}
 }
 
 template<int K>
@@ -119,18 +127,18 @@ static void launch_k(hipStream_t st, const Batch &b, bool split, int block, bool
 	if(split) {
 		if(far_inline)
 			k_forces<K, true, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-					b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr, b.any_rebuild);
+					b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, b.stress);
 		else
 			k_forces<K, true, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-					b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr, b.any_rebuild);
+					b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, b.stress);
 	}
 	else {
 		if(far_inline)
 			k_forces<K, false, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-					b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr, b.any_rebuild);
+					b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, nullptr);
 		else
 			k_forces<K, false, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-					b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr, b.any_rebuild);
+					b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, nullptr);
 	}
 }
 
@@ -140,10 +148,10 @@ static void launch_far(hipStream_t st, const Batch &b, bool split, int block, in
 	dim3 grid((unsigned) ((threads + block - 1) / block));
 	if(split)
 		k_forces_far<K, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F, b.T,
-				b.esplit, k, k >= 0 ? b.limit : nullptr);
+				b.esplit, k, k >= 0 ? b.limit : nullptr, b.stress);
 	else
 		k_forces_far<K, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F, b.T,
-				nullptr, k, k >= 0 ? b.limit : nullptr);
+				nullptr, k, k >= 0 ? b.limit : nullptr, nullptr);
 }
 
 // close (+ bonded) kernel; the far pairs either inline (far_lanes == 0) or

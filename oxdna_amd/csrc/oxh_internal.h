@@ -67,6 +67,7 @@ struct Batch {
 
 	// optional outputs
 	float *esplit = nullptr;   // 8 * N
+	float *stress = nullptr;   // 6 * N: each particle's half of its pairs' virial (xx yy zz xy xz yz)
 	int *hbcount = nullptr;    // N
 	bool want_hbcount = false;
 

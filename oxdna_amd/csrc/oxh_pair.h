@@ -459,6 +459,17 @@ __device__ __forceinline__ f3 min_image_f(float4 a, float4 b, const oxh_sys &S) 
 	return mk(dx, dy, dz);
 }
 
+// pair virial r_pq (x) F_q (= -r_pq (x) F_p) in the reference's component
+// order xx, yy, zz, xy, xz, yz (src/Interactions/BaseInteraction.cpp:159-200)
+__device__ __forceinline__ void add_virial(float sv[6], f3 r, f3 fq) {
+	sv[0] += r.x * fq.x;
+	sv[1] += r.y * fq.y;
+	sv[2] += r.z * fq.z;
+	sv[3] += r.x * fq.y;
+	sv[4] += r.x * fq.z;
+	sv[5] += r.y * fq.z;
+}
+
 // sum over the K lanes of a group (K a power of two, groups aligned to K
 // inside the wave), fixed butterfly order
 __device__ __forceinline__ float group_sum(float v, int K) {
@@ -481,7 +492,8 @@ __device__ __forceinline__ void particle_forces(int K, int i, int l, int N, cons
 		const double4 *__restrict__ q64, const float4 *__restrict__ q32,
 		const float4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo, const int *__restrict__ nbr,
 		const int2 *__restrict__ nnbr, int max_neigh, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ sstate,
-		float4 *__restrict__ Fout, float4 *__restrict__ Tout, float *__restrict__ esplit, int *__restrict__ hbcount) {
+		float4 *__restrict__ Fout, float4 *__restrict__ Tout, float *__restrict__ esplit, int *__restrict__ hbcount,
+		float *__restrict__ stress) {
 	const int2 pi = pinfo[i];
 	const oxh_sys &S = systems[pi.y];
 	const double4 P = pos[i];
@@ -492,6 +504,7 @@ __device__ __forceinline__ void particle_forces(int K, int i, int l, int N, cons
 	f3 F = {0, 0, 0}, T = {0, 0, 0};
 	float E = 0.f, Ehb = 0.f;
 	float es[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	float sv[6] = {0, 0, 0, 0, 0, 0};   // pair virial, r_pq (x) F_q summed over this particle's pairs
 	int err = 0;
 	int nhb = 0;
 
@@ -514,6 +527,7 @@ __device__ __forceinline__ void particle_forces(int K, int i, int l, int N, cons
 			es[0] += 0.5f * e3[0];
 			es[1] += 0.5f * e3[1];
 			es[2] += 0.5f * e3[2];
+			add_virial(sv, g.r, o.force);
 		}
 	}
 	if(l == (K > 1 ? 1 : 0) && bd.y >= 0) {
@@ -534,6 +548,7 @@ __device__ __forceinline__ void particle_forces(int K, int i, int l, int N, cons
 			es[0] += 0.5f * e3[0];
 			es[1] += 0.5f * e3[1];
 			es[2] += 0.5f * e3[2];
+			add_virial(sv, g.r, o.force);
 		}
 	}
 
@@ -569,6 +584,7 @@ __device__ __forceinline__ void particle_forces(int K, int i, int l, int N, cons
 		F -= fq_;
 		T += tp;
 		E += 0.5f * e;
+		if(SPLIT) add_virial(sv, r, fq_);
 	}
 
 	// far sublist: Debye-Hueckel between backbone sites only (or left to
@@ -591,6 +607,7 @@ __device__ __forceinline__ void particle_forces(int K, int i, int l, int N, cons
 		float edh = debye_huckel(S, rb, cf_i * Bj.w, fq_);
 		Ff += fq_;
 		Ef += edh;
+		if(SPLIT) add_virial(sv, min_image(P, pos[j], S), fq_);
 	}
 	F -= Ff;
 	T -= cross(fp.back, Ff);
@@ -606,8 +623,10 @@ __device__ __forceinline__ void particle_forces(int K, int i, int l, int N, cons
 		T.z = group_sum(T.z, K);
 		E = group_sum(E, K);
 		Ehb = group_sum(Ehb, K);
-		if(SPLIT)
+		if(SPLIT) {
 			for(int q = 0; q < 8; q++) es[q] = group_sum(es[q], K);
+			for(int q = 0; q < 6; q++) sv[q] = group_sum(sv[q], K);
+		}
 		if(hbcount) nhb = (int) group_sum((float) nhb, K);
 		err = (int) group_sum((float) err, K);
 	}
@@ -619,6 +638,8 @@ __device__ __forceinline__ void particle_forces(int K, int i, int l, int N, cons
 	Tout[i] = make_float4(Tb.x, Tb.y, Tb.z, Ehb);
 	if(SPLIT) {
 		for(int q = 0; q < 8; q++) esplit[(size_t) q * N + i] = es[q];
+		// each particle takes half of every pair's virial
+		for(int q = 0; q < 6; q++) stress[(size_t) q * N + i] = 0.5f * sv[q];
 	}
 	if(hbcount) hbcount[i] = nhb;
 	if(err) atomicOr(&sstate[pi.y].error, 1);

@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(BLOCK) k_persist(PersistArgs a) {
 		// 3. forces and torques, K lanes per nucleotide, far pairs inline
 		if(tid < n * K)
 			particle_forces<false, true>(K, first + tid / K, tid % K, a.N, a.pos, a.q64, a.q32, a.bpos, a.bonds, a.pinfo, a.nbr, a.nnbr,
-					a.max_neigh, a.systems, a.state, a.F, a.T, nullptr, a.hbcount);
+					a.max_neigh, a.systems, a.state, a.F, a.T, nullptr, a.hbcount, nullptr);
 		__syncthreads();
 		// 4. second half kick + Langevin / Brownian
 		if(tid < n) second_step_particle(first + tid, S, a.vel, a.Lm, a.F, a.T, a.orig, a.pstep);
@@ -115,13 +115,13 @@ __global__ void __launch_bounds__(BLOCK) k_persist(PersistArgs a) {
 // nucleotide (after uploads, list rebuilds and for observables), so that
 // runs continue from exactly the forces the engine itself computes.
 template<bool SPLIT, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_persist_forces(PersistArgs a, float *esplit) {
+__global__ void __launch_bounds__(BLOCK) k_persist_forces(PersistArgs a, float *esplit, float *stress) {
 	const oxh_sys &S = a.systems[blockIdx.x];
 	const int K = persist_lanes(S.n, BLOCK);
 	const int tid = threadIdx.x;
 	if(tid < S.n * K)
 		particle_forces<SPLIT, true>(K, S.first + tid / K, tid % K, a.N, a.pos, a.q64, a.q32, a.bpos, a.bonds, a.pinfo, a.nbr, a.nnbr,
-				a.max_neigh, a.systems, a.state, a.F, a.T, esplit, a.hbcount);
+				a.max_neigh, a.systems, a.state, a.F, a.T, esplit, a.hbcount, stress);
 }
 
 PersistArgs make_args(const Batch &b, int nsteps) {
@@ -167,12 +167,12 @@ void launch_persist_forces(hipStream_t st, const Batch &b, bool split, int block
 	if(b.N == 0) return;
 	PersistArgs a = make_args(b, 0);
 	if(block == 256) {
-		if(split) k_persist_forces<true, 256><<<b.nsys, 256, 0, st>>>(a, b.esplit);
-		else k_persist_forces<false, 256><<<b.nsys, 256, 0, st>>>(a, nullptr);
+		if(split) k_persist_forces<true, 256><<<b.nsys, 256, 0, st>>>(a, b.esplit, b.stress);
+		else k_persist_forces<false, 256><<<b.nsys, 256, 0, st>>>(a, nullptr, nullptr);
 	}
 	else {
-		if(split) k_persist_forces<true, 512><<<b.nsys, 512, 0, st>>>(a, b.esplit);
-		else k_persist_forces<false, 512><<<b.nsys, 512, 0, st>>>(a, nullptr);
+		if(split) k_persist_forces<true, 512><<<b.nsys, 512, 0, st>>>(a, b.esplit, b.stress);
+		else k_persist_forces<false, 512><<<b.nsys, 512, 0, st>>>(a, nullptr, nullptr);
 	}
 }
 

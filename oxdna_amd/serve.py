@@ -4,7 +4,7 @@ Jobs (one oxDNA2 system each, with its own temperature, seed and step
 count) arrive over time.  The server keeps two kinds of slot pool resident
 in HBM (include/oxh.h, oxh_pool_*):
 
-  - a workgroup pool: slots of 512 nucleotides, one workgroup per job and
+  - a workgroup pool: slots of 128 nucleotides, one workgroup per job and
     many steps per launch;
   - grid pools: slots of a few capacity classes up to the largest job,
     one kernel per phase for all of a pool's jobs together, cell lists.
@@ -34,7 +34,10 @@ import numpy as np
 from .backend import HIPPool
 from .systems import System
 
-SMALL_CAP = 512
+# jobs up to SMALL_CAP nucleotides run in the workgroup pool (one workgroup
+# each, at least 4 lanes per nucleotide); a workgroup engine step costs ~n
+# on one CU, so larger jobs would hold a CU for long: they go to grid pools
+SMALL_CAP = 128
 
 
 @dataclass
@@ -64,13 +67,13 @@ def footprint_bytes(n_nucleotides: int, max_neigh: int = 64) -> int:
 
 def plan_pools(budget_bytes: int, largest: int, small_slots: int = 256):
     """Slot capacities for one workgroup pool and the grid pools, within
-    budget_bytes: the workgroup pool takes small_slots x 512 nt; the rest
-    goes to grid slots in capacity classes x4 apart up to `largest`, about
-    the same bytes per class."""
+    budget_bytes: the workgroup pool takes small_slots x SMALL_CAP nt; the
+    rest goes to grid slots in capacity classes x4 apart (512, 2048, ...)
+    up to `largest`, about the same bytes per class."""
     small = [SMALL_CAP] * small_slots
     left = budget_bytes - footprint_bytes(SMALL_CAP * small_slots)
     classes = []
-    c = 2048
+    c = 512
     while True:
         classes.append(min(c, largest))
         if c >= largest:

@@ -22,7 +22,7 @@ def load_golden(name):
                        T=float(z["T"]), salt=float(z["salt"]), name=name)
     opts = dict(dh_half_charged_ends=bool(z["half_charged_ends"]), dh_lambda=float(z["dh_lambda"]),
                 dh_strength=float(z["dh_strength"]))
-    ref = dict(E8=z["E8"].copy(), F=z["F"].copy(), T=z["Tbody"].copy())
+    ref = dict(E8=z["E8"].copy(), F=z["F"].copy(), T=z["Tbody"].copy(), stress7=z["stress7"].copy())
     return s, opts, ref
 
 

@@ -38,13 +38,21 @@ PLUGIN_DIR = os.path.join(ROOT, "oracle", "_ref")
 
 def run_reference(s: systems.System, T_str: str, salt: float, extra: dict | None = None, interaction="DNA2_nomesh"):
     extra = extra or {}
+    s = s.copy()
+    # zero velocities, so that the pressure observable holds the pair virial
+    # only; a tiny angular momentum, which the reference requires without
+    # refresh_vel (src/Backends/MDBackend.cpp:99-103) and which enters no
+    # observable here
+    s.vel = np.zeros((s.N, 3))
+    s.L = np.zeros((s.N, 3))
+    s.L[:, 0] = 1e-6
     with tempfile.TemporaryDirectory(dir="/tmp") as d:
         io.write_topology(os.path.join(d, "t.top"), s)
         io.write_conf(os.path.join(d, "c.dat"), s)
         inp = {
             "backend": "CPU", "sim_type": "MD", "interaction_type": interaction, "steps": "0", "seed": "1",
             "T": T_str, "salt_concentration": repr(salt), "dt": "0.003", "verlet_skin": "0.05",
-            "thermostat": "no", "refresh_vel": "1", "topology": "t.top", "conf_file": "c.dat",
+            "thermostat": "no", "refresh_vel": "0", "topology": "t.top", "conf_file": "c.dat",
             "trajectory_file": "traj.dat", "energy_file": "energy.dat", "print_energy_every": "1000",
             "print_conf_interval": "100000", "no_stdout_energy": "1", "restart_step_counter": "1",
             "box_type": "orthogonal", "time_scale": "linear", "plugin_search_path": PLUGIN_DIR, "log_file": "log.dat",
@@ -56,18 +64,22 @@ def run_reference(s: systems.System, T_str: str, salt: float, extra: dict | None
             f.write("data_output_1 = {\n name = split.dat\n print_every = 1\n col_1 = {\n type = potential_energy\n"
                     " split = true\n precision = 15\n }\n}\n")
             f.write("data_output_2 = {\n name = forces.dat\n print_every = 1\n col_1 = {\n type = ForceDump\n }\n}\n")
+            f.write("data_output_3 = {\n name = stress.dat\n print_every = 1\n col_1 = {\n type = pressure\n"
+                    " stress_tensor = true\n custom_stress_tensor = true\n }\n}\n")
         r = subprocess.run([REF_BIN, "input"], cwd=d, capture_output=True, text=True)
         if r.returncode != 0:
             log = open(os.path.join(d, "log.dat")).read() if os.path.exists(os.path.join(d, "log.dat")) else ""
             raise RuntimeError(f"reference failed: {r.stdout[-2000:]} {r.stderr[-2000:]} {log[-2000:]}")
         split = np.array([float(x) for x in open(os.path.join(d, "split.dat")).read().split()])
+        # P, then the stress tensor xx yy zz xy xz yz divided by the volume
+        stress = np.array([float(x) for x in open(os.path.join(d, "stress.dat")).read().split()])
         rows = np.loadtxt(os.path.join(d, "forces.dat"), ndmin=2)
     idx = rows[:, 0].astype(int)
     F = np.zeros((s.N, 3))
     T = np.zeros((s.N, 3))
     F[idx] = rows[:, 1:4]
     T[idx] = rows[:, 4:7]
-    return split * s.N, F, T
+    return split * s.N, F, T, stress
 
 
 def tangle(seed: int, n_strands: int = 6, length: int = 8, box: float = 6.0) -> systems.System:
@@ -204,14 +216,14 @@ def main():
     if not os.path.exists(REF_BIN):
         raise SystemExit("build the reference first: make -f oracle/ref.mk -j8")
     for name, s, T_str, salt, extra in cases():
-        E, F, T = run_reference(s, T_str, salt, extra)
+        E, F, T, stress = run_reference(s, T_str, salt, extra)
         a2 = np.cross(s.a3, s.a1)
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), pos=s.pos, a1=s.a1, a2=a2, a3=s.a3, btype=s.btype,
                             n3=s.n3, n5=s.n5, strand=s.strand, box=s.box, T=io.temperature(T_str), salt=salt,
                             half_charged_ends=int(extra.get("dh_half_charged_ends", "true") != "false"),
                             dh_lambda=float(extra.get("dh_lambda", -1.0)),
                             dh_strength=float(extra.get("dh_strength", -1.0)),
-                            E8=E, F=F, Tbody=T)
+                            E8=E, F=F, Tbody=T, stress7=stress)
         print(f"{name}: N={s.N} U/N={E.sum() / s.N:.6f} max|F|={np.abs(F).max():.3g}")
 
 

@@ -43,6 +43,22 @@ def test_forces_vs_reference(hip, name):
     assert abs(Ep.sum() - U) <= 1e-5 * scale
 
 
+@pytest.mark.parametrize("name", golden_names())
+def test_stress_vs_reference(hip, name):
+    """Pair virial against the reference's pressure observable (stress
+    tensor x volume; zero velocities in the fixtures).  FP32 pair forces:
+    each pair term carries ~1e-6 of the largest force, so the bound is
+    1e-6 x N x max|F| (|r| ~ 1 for interacting pairs)."""
+    s, opts, ref = load_golden(name)
+    with hip.HIPBatch(s, **opts) as b:
+        W, _ = b.stress()
+    want = ref["stress7"][1:] * np.prod(s.box)
+    scale = s.N * np.abs(ref["F"]).max()
+    err = np.abs(W - want).max() / scale
+    print(f"{name}: virial err {err:.2e} (|W| {np.abs(want).max():.3g}, scale {scale:.3g})")
+    assert err <= 1e-6
+
+
 @pytest.mark.parametrize("name", ["duplex20_md_20C", "bundle_md_30C", "gas8_25C_0.5M", "tangle2_45C_0.1M"])
 def test_pair_set_exact(hip, name):
     """The GPU Verlet pair set equals the FP64 predicate's, bit for bit."""

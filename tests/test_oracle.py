@@ -38,6 +38,20 @@ def test_oracle_matches_reference_binary(name):
     assert abs(r["Epart"].sum() - r["U"]) <= 1e-12 * s.N
 
 
+@pytest.mark.parametrize("name", golden_names())
+def test_oracle_virial_matches_reference_pressure(name):
+    """Pair virial against the reference's pressure observable with its
+    stress tensor (src/Observables/Pressure.cpp, printed with 8 significant
+    digits; velocities are zero in the fixtures, so the observable holds the
+    virial / V only)."""
+    s, opts, ref = load_golden(name)
+    W = po.virial(oracle_params(s, opts), s) / np.prod(s.box)
+    st = ref["stress7"]
+    scale = np.abs(st[1:]).max()
+    assert np.all(np.abs(W - st[1:]) <= 2e-8 * scale + 1e-14), (W, st[1:])
+    assert abs(W[:3].sum() / 3 - st[0]) <= 2e-8 * scale + 1e-14
+
+
 def test_oracle_matches_reference_fixture_file():
     """reference.dat of the reference's FORCE_FIELD/AVG_SEQ test: DNA2
     (meshed f4), 16 nt, 20 C, 1.0 M, columns per nucleotide."""
