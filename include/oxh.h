@@ -135,6 +135,19 @@ oxh_status oxh_batch_hilbert_sort(oxh_batch *b);
  * chunks. */
 oxh_status oxh_batch_set_sort_interval(oxh_batch *b, long long steps);
 
+/* Moves every strand whose centre of mass has left the box back into it by
+ * whole box lengths (the reference's fix_diffusion, src/Backends/
+ * SimBackend.cpp:788-882): positions and list positions move together;
+ * undone if the potential energy changes by more than 1e-6 of itself.
+ * n_moved (may be NULL) receives the number of strands moved. */
+oxh_status oxh_batch_fix_diffusion(oxh_batch *b, int *n_moved);
+
+/* Runs oxh_batch_fix_diffusion every `steps` MD steps of oxh_batch_run (0 =
+ * never; the reference's fix_diffusion / fix_diffusion_every keys, default
+ * on every 100000 steps, src/Managers/SimManager.cpp:92,151), checked at
+ * chunk boundaries.  Slot pools do not re-box. */
+oxh_status oxh_batch_set_fix_diffusion(oxh_batch *b, long long steps);
+
 /* Blocks until all queued work of the batch has finished. */
 oxh_status oxh_batch_sync(oxh_batch *b);
 

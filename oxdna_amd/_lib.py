@@ -23,6 +23,7 @@ EXPORTS = [
     "oxh_batch_energy_split", "oxh_batch_hb_count", "oxh_batch_get_pairs", "oxh_batch_system_info",
     "oxh_batch_set_state", "oxh_batch_stats", "oxh_batch_time_forces", "oxh_hilbert_keys", "oxh_philox",
     "oxh_pool_create", "oxh_pool_admit", "oxh_pool_release", "oxh_pool_status", "oxh_copy_bench", "oxh_batch_stress",
+    "oxh_batch_fix_diffusion", "oxh_batch_set_fix_diffusion",
 ]
 
 
@@ -82,6 +83,8 @@ def lib():
         L.oxh_pool_status.argtypes = [V, P, P, P, P]
         L.oxh_copy_bench.argtypes = [I, LL, I, P]
         L.oxh_batch_stress.argtypes = [V, I, P, P]
+        L.oxh_batch_fix_diffusion.argtypes = [V, P]
+        L.oxh_batch_set_fix_diffusion.argtypes = [V, LL]
         for name in EXPORTS:
             f = getattr(L, name)
             if name not in ("oxh_last_error", "oxh_system_desc_defaults"):

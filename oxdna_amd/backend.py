@@ -127,6 +127,17 @@ class HIPBatch:
     def hilbert_sort(self):
         check(lib().oxh_batch_hilbert_sort(self.h), "oxh_batch_hilbert_sort")
 
+    def fix_diffusion(self) -> int:
+        """Re-boxes strands whose centre of mass left the box; returns how
+        many moved."""
+        n = C.c_int(0)
+        check(lib().oxh_batch_fix_diffusion(self.h, C.byref(n)), "oxh_batch_fix_diffusion")
+        return n.value
+
+    def set_fix_diffusion(self, every: int):
+        """Re-box strands every `every` steps of run() (0 = never)."""
+        check(lib().oxh_batch_set_fix_diffusion(self.h, int(every)), "oxh_batch_set_fix_diffusion")
+
     def set_sort_interval(self, steps: int):
         """Hilbert re-sort every `steps` MD steps (0 = never)."""
         check(lib().oxh_batch_set_sort_interval(self.h, int(steps)), "oxh_batch_set_sort_interval")

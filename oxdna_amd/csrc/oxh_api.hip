@@ -48,6 +48,11 @@ struct oxh_batch {
 	// step target and steps done per slot, per-chunk step limits
 	bool pool = false;
 	std::vector<int> cap, cell_budget, h_limit;
+	// strands (chains of bonds) of each system, by original particle index;
+	// fix_diffusion_every: interval of the strand re-boxing (0 = off)
+	std::vector<std::vector<int>> strand_of;
+	std::vector<int> n_strands;
+	long long fix_every = 0, since_fix = 0;
 	std::vector<long long> target, done_steps;
 	double4 *bk_rlist = nullptr;
 };
@@ -527,6 +532,25 @@ void stage_system(const oxh_system_desc &d, int s, int f, double4 *hpos, double4
 	}
 }
 
+// strand index of every particle: chains of n3 / n5 bonds, numbered in
+// the order of their first particle (the reference's strand ids come from
+// the topology file, TopologyParser.cpp:15-172; bonds define the same
+// partition)
+int strands_of(const oxh_system_desc &d, std::vector<int> &sid) {
+	sid.assign(d.N, -1);
+	int ns = 0;
+	for(int i = 0; i < d.N; i++) {
+		if(sid[i] >= 0) continue;
+		// walk to the 5' end, then along n3
+		int a = i, guard = 0;
+		while(d.n5[a] >= 0 && d.n5[a] != i && guard++ < d.N) a = d.n5[a];
+		for(int b = a, g = 0; b >= 0 && sid[b] < 0 && g <= d.N; b = d.n3[b], g++) sid[b] = ns;
+		if(sid[i] < 0) sid[i] = ns;   // defensive: inconsistent bonds
+		ns++;
+	}
+	return ns;
+}
+
 void initial_state(const oxh_system_desc &d, oxh_sys_state &st) {
 	std::memset(&st, 0, sizeof(st));
 	st.K_t = 0.5 * 3.0 * (d.N - 1) * d.T;
@@ -561,6 +585,101 @@ oxh_status check_sys(oxh_batch *B, int sys) {
 	HIPCHK(hipSetDevice(B->b.device));
 	return OXH_OK;
 }
+__global__ void k_shift(int N, const double4 *__restrict__ shift, double4 *__restrict__ pos, double4 *__restrict__ rlist) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= N) return;
+	const double4 d = shift[i];
+	if(d.x == 0. && d.y == 0. && d.z == 0.) return;
+	double4 p = pos[i], r = rlist[i];
+	p.x += d.x;
+	p.y += d.y;
+	p.z += d.z;
+	r.x += d.x;
+	r.y += d.y;
+	r.z += d.z;
+	pos[i] = p;
+	rlist[i] = r;   // relative positions are unchanged: lists stay valid
+}
+
+// Moves every strand whose centre of mass left the box back by whole box
+// lengths (SimBackend::fix_diffusion, src/Backends/SimBackend.cpp:788-882,
+// with OrthogonalBox::shift_particle, src/Boxes/OrthogonalBox.cpp:67-74):
+// centres of mass in FP64 in the original particle order, the same shift
+// for every particle of a strand; positions and list positions move
+// together.  Orientations are unit quaternions renormalised every step, so
+// the reference's re-orthonormalisation has nothing to do.  If the
+// potential energy changes by more than 1e-6 of itself (as the reference
+// checks) the shift is undone.
+oxh_status fix_diffusion_impl(oxh_batch *B, int *n_moved) {
+	oxh::Batch &b = B->b;
+	std::vector<double4> p;
+	std::vector<int> o;
+	TRY(download(B, p, b.pos, 0, b.N));
+	TRY(download(B, o, b.orig, 0, b.N));
+	std::vector<double> E0(4 * b.nsys);
+	TRY(sync_and_check(B));
+	std::vector<double4> shift(b.N, make_double4(0, 0, 0, 0));
+	int moved = 0;
+	for(int s = 0; s < b.nsys; s++) {
+		const int f = B->first[s], n = B->n[s], ns = B->n_strands[s];
+		if(n == 0 || ns == 0) continue;
+		std::vector<int> where(n);
+		for(int k = 0; k < n; k++) where[o[f + k]] = f + k;
+		std::vector<double> com(3 * ns, 0.0);
+		std::vector<int> cnt(ns, 0);
+		const std::vector<int> &sid = B->strand_of[s];
+		for(int i = 0; i < n; i++) {
+			const double4 x = p[where[i]];
+			com[3 * sid[i]] += x.x;
+			com[3 * sid[i] + 1] += x.y;
+			com[3 * sid[i] + 2] += x.z;
+			cnt[sid[i]]++;
+		}
+		const oxh_sys &S = b.h_sys[s];
+		std::vector<double> d(3 * ns, 0.0);
+		for(int m = 0; m < ns; m++) {
+			bool any = false;
+			for(int c = 0; c < 3; c++) {
+				const double cm = com[3 * m + c] / cnt[m];
+				d[3 * m + c] = -S.box[c] * std::floor(cm / S.box[c]);
+				any |= d[3 * m + c] != 0.0;
+			}
+			moved += any;
+		}
+		for(int i = 0; i < n; i++) {
+			const int m = sid[i];
+			shift[where[i]] = make_double4(d[3 * m], d[3 * m + 1], d[3 * m + 2], 0.0);
+		}
+	}
+	if(n_moved) *n_moved = moved;
+	if(moved == 0) return OXH_OK;
+	double4 *dshift = nullptr;
+	TRY(dalloc(B, &dshift, b.N));
+	auto apply = [&](double sign) -> oxh_status {
+		if(sign < 0)
+			for(auto &x : shift) x = make_double4(-x.x, -x.y, -x.z, 0.0);
+		HIPCHK(hipMemcpyAsync(dshift, shift.data(), b.N * sizeof(double4), hipMemcpyHostToDevice, b.stream));
+		k_shift<<<(b.N + 255) / 256, 256, 0, b.stream>>>(b.N, dshift, b.pos, b.rlist);
+		oxh::launch_prepare_mirror(b.stream, b);
+		enqueue_forces(B);
+		return sync_and_check(B);
+	};
+	std::vector<double> before(4 * b.nsys), after(4 * b.nsys);
+	oxh_status st = oxh_batch_energies(B, before.data());
+	if(st == OXH_OK) st = apply(1.0);
+	if(st == OXH_OK) st = oxh_batch_energies(B, after.data());
+	bool undo = false;
+	for(int s = 0; st == OXH_OK && s < b.nsys; s++)
+		undo |= std::fabs(after[4 * s] - before[4 * s]) > 1e-6 * std::fabs(before[4 * s]);
+	if(st == OXH_OK && undo) {
+		st = apply(-1.0);
+		if(n_moved) *n_moved = 0;
+	}
+	(void) hipFree(dshift);
+	B->device_bytes -= (size_t) b.N * sizeof(double4);
+	return st;
+}
+
 // Picks the force kernels' launch shapes by timing every candidate on this
 // batch's own lists: threads per nucleotide and block size of the close
 // (+ bonded) kernel, and whether the far pairs run inline or in their own
@@ -818,6 +937,8 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 	std::vector<int2> hbonds(N, make_int2(-1, -1)), hinfo(N);
 	std::vector<int> horig(N), howner(std::max(ncell, 1));
 	std::vector<oxh_sys_state> hst(n_sys);
+	B->strand_of.assign(n_sys, std::vector<int>());
+	B->n_strands.assign(n_sys, 0);
 	for(int s = 0; s < n_sys; s++) {
 		static const oxh_system_desc empty = {};
 		const oxh_system_desc &d = descs ? descs[s] : empty;
@@ -831,6 +952,7 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 		stage_system(d, s, f, hpos.data() + f, hvel.data() + f, hL.data() + f, hq.data() + f, hbonds.data() + f, hinfo.data() + f,
 				horig.data() + f);
 		initial_state(d, hst[s]);
+		B->n_strands[s] = strands_of(d, B->strand_of[s]);
 	}
 	hipStream_t st = b.stream;
 	auto up = [&](void *dst, const void *src, size_t bytes) {
@@ -1003,6 +1125,11 @@ oxh_status oxh_batch_run(oxh_batch *B, long long n_steps) {
 		b.step += todo;
 		done += todo;
 		B->since_sort += todo;
+		B->since_fix += todo;
+		if(B->fix_every > 0 && B->since_fix >= B->fix_every && !B->pool) {
+			TRY(fix_diffusion_impl(B, nullptr));
+			B->since_fix = 0;
+		}
 		if(B->pool) {
 			for(int s = 0; s < b.nsys; s++) {
 				B->done_steps[s] += B->h_limit[s];
@@ -1054,6 +1181,7 @@ oxh_status oxh_pool_admit(oxh_batch *B, int slot, const oxh_system_desc *desc, l
 	stage_system(d, slot, f, hpos.data(), hvel.data(), hL.data(), hq.data(), hbonds.data(), hinfo.data(), horig.data());
 	oxh_sys_state hst;
 	initial_state(d, hst);
+	B->n_strands[slot] = strands_of(d, B->strand_of[slot]);
 	hipStream_t st = b.stream;
 	HIPCHK(hipStreamSynchronize(st));   // staging of an earlier admission may still be in flight
 	HIPCHK(hipMemcpyAsync(b.pos + f, hpos.data(), d.N * sizeof(double4), hipMemcpyHostToDevice, st));
@@ -1154,6 +1282,19 @@ oxh_status oxh_batch_hilbert_sort(oxh_batch *B) {
 	if(oxh::launch_hilbert_sort(B->b.stream, B->b) != 0) return fail(OXH_ERR_ARG, "too many systems for the sort key");
 	TRY(sync_and_check(B));
 	return oxh_batch_update_lists(B, 1);
+}
+
+oxh_status oxh_batch_fix_diffusion(oxh_batch *B, int *n_moved) {
+	if(!B) return fail(OXH_ERR_ARG, "null batch");
+	HIPCHK(hipSetDevice(B->b.device));
+	return fix_diffusion_impl(B, n_moved);
+}
+
+oxh_status oxh_batch_set_fix_diffusion(oxh_batch *B, long long every) {
+	if(!B || every < 0) return fail(OXH_ERR_ARG, "bad arguments");
+	B->fix_every = every;
+	B->since_fix = 0;
+	return OXH_OK;
 }
 
 oxh_status oxh_batch_set_sort_interval(oxh_batch *B, long long steps) {
