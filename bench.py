@@ -212,6 +212,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--sort-every", type=int, default=500, help="Hilbert re-sort interval in steps (0 = never)")
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "double"],
+                    help="backend_precision: mixed (FP32 pair terms, FP64 state) or double (all FP64)")
     ap.add_argument("--jobs", type=int, default=64, help="served: number of jobs")
     ap.add_argument("--step-scale", type=float, default=0.01, help="served: factor on the jobs' step counts")
     ap.add_argument("--rate-mult", type=float, default=4.0, help="served: arrival rate / plain path's job rate")
@@ -235,6 +237,7 @@ def main():
     else:
         systems_ = [s]
         b_args = dict(opts)
+    b_args["backend_precision"] = args.precision
     n_total = sum(x.N for x in systems_)
     log(f"workload {args.workload}: {len(systems_)} system(s), N={n_total}")
     t_up = time.perf_counter()
@@ -264,7 +267,8 @@ def main():
     # (pos 32, FP64 quaternion 32, backbone site 16, bonds 8, pinfo 8, list
     # lengths 8), F and T written (32), one 4-byte list entry per
     # neighbour; neighbour gathers are counted as cache-resident
-    b_alg = 136.0 + 4.0 * nbar
+    # (all-FP64 mode: F and T are written as FP64, 32 more bytes)
+    b_alg = 136.0 + 4.0 * nbar + (32.0 if args.precision == "double" else 0.0)
     achieved = n_total * b_alg / (force_ms * 1e-3) / 1e9
     rebuild_every = args.steps / max(1, reb1 - reb0)
     log(f"{elapsed:.3f} s for {args.steps} steps; {1e3 * elapsed / args.steps:.4f} ms/step; force kernel "
@@ -289,16 +293,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 forces / f64 state",
+        "dtype": "f32 forces / f64 state" if args.precision == "mixed" else "f64 forces / f64 state",
         "data": "synthetic",
         "config": {"workload": desc, "n_nucleotides": n_total, "n_systems": len(systems_),
+                   "backend_precision": args.precision,
                    "form": "plain (one system)" if len(systems_) == 1 else "batched (one launch for all systems)",
                    "parallelism": "none",
                    "hilbert_sort_every_steps": args.sort_every,
                    "mean_neighbours": round(nbar, 2), "steps_per_rebuild": round(rebuild_every, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "copy_kernel_gbps": copy_gbps,
-                     "kernel": "k_forces", "avg_launch_us": force_ms * 1e3, "bytes_per_nt": b_alg,
+                     "kernel": "k_forces" if args.precision == "mixed" else "k_forces_dp", "avg_launch_us": force_ms * 1e3, "bytes_per_nt": b_alg,
                      "lanes_per_nt": st["lanes"], "block": st["block"], "far_lanes_per_nt": st["far_lanes"],
                      "far_block": st["far_block"]},
         "cpu_baseline": cpu,

@@ -85,6 +85,11 @@ typedef struct oxh_system_desc {
 	int bussi_tau;
 	unsigned long long seed;
 	int refresh_vel;         /* 1: draw vel and L from N(0, T) at upload */
+	/* the reference's backend_precision key (src/Backends/BackendFactory.cpp:
+	 * 48-80): 0 "mixed" (FP32 pair terms from FP64 site geometry, FP64
+	 * state; default), 1 "double" (every term in FP64).  All systems of a
+	 * batch use the same; slot pools take mixed only. */
+	int backend_precision;
 } oxh_system_desc;
 
 /* Fills a descriptor with the reference's defaults (no arrays set). */

@@ -40,7 +40,7 @@ class SystemDesc(C.Structure):
         ("seq_hydr_GC", C.c_double),
         ("verlet_skin", C.c_double), ("dt", C.c_double), ("thermostat", C.c_int), ("diff_coeff", C.c_double),
         ("gamma_trans", C.c_double), ("pt", C.c_double), ("newtonian_steps", C.c_int), ("bussi_tau", C.c_int),
-        ("seed", C.c_ulonglong), ("refresh_vel", C.c_int),
+        ("seed", C.c_ulonglong), ("refresh_vel", C.c_int), ("backend_precision", C.c_int),
     ]
 
 

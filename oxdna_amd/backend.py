@@ -62,6 +62,10 @@ def make_desc(s: System, keep: list, **opts) -> SystemDesc:
     d.bussi_tau = int(opts.get("bussi_tau", 1))
     d.seed = int(opts.get("seed", 1)) & 0xFFFFFFFFFFFFFFFF
     d.refresh_vel = int(bool(opts.get("refresh_vel", False)))
+    prec = opts.get("backend_precision", "mixed")
+    if prec not in ("mixed", "double"):
+        raise ValueError(f"backend_precision {prec!r}: use 'mixed' or 'double'")
+    d.backend_precision = 1 if prec == "double" else 0
     if "seq_stck" in opts:
         st = _c(opts["seq_stck"], np.float64).ravel()
         keep.append(st)

@@ -17,10 +17,11 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "liboxh.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
+# compile flags for every source; packed FP32 instructions are switched off
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-ffp-contract=fast", "-munsafe-fp-atomics", "-Wno-unused-result",
+    "-ffp-contract=fast", "-Wno-unused-result",
 ]
 
 
@@ -34,9 +35,7 @@ def build(verbose: bool = True) -> str:
     procs = []
     for src in sources():
         obj = os.path.join(CSRC, os.path.basename(src)[:-4] + ".o")
-        cmd = [HIPCC, *FLAGS[:-0 or None]]
-        cmd = [HIPCC, "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
-               "-packed-fp32-ops", "-ffp-contract=fast", "-Wno-unused-result", "-c", src, "-o", obj]
+        cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         objs.append(obj)
     failed = False

@@ -25,6 +25,7 @@ struct oxh_batch {
 	// backup of the state at the start of a chunk (overflow recovery)
 	double4 *bk_pos = nullptr, *bk_vel = nullptr, *bk_Lm = nullptr, *bk_q64 = nullptr;
 	float4 *bk_F = nullptr, *bk_T = nullptr;
+	double4 *bk_F64 = nullptr, *bk_T64 = nullptr;
 	long long *bk_pstep = nullptr;
 	oxh_sys_state *bk_state = nullptr;
 	size_t device_bytes = 0;
@@ -120,8 +121,10 @@ void model_constants(const oxh_system_desc &d, oxh_sys &S, double &rcut) {
 	for(int t = 0; t < 2; t++)
 		for(int i = 0; i < 5; i++)
 			for(int j = 0; j < 5; j++) {
-				S.f1_eps[t][i][j] = (float) eps[t][i][j];
-				S.f1_shift[t][i][j] = (float) (eps[t][i][j] * (t == 1 ? stck_shift : hydr_shift));
+				S.f1_eps_d[t][i][j] = eps[t][i][j];
+				S.f1_shift_d[t][i][j] = eps[t][i][j] * (t == 1 ? stck_shift : hydr_shift);
+				S.f1_eps[t][i][j] = (float) S.f1_eps_d[t][i][j];
+				S.f1_shift[t][i][j] = (float) S.f1_shift_d[t][i][j];
 			}
 
 	const float salt_f = (float) d.salt;
@@ -133,23 +136,33 @@ void model_constants(const oxh_system_desc &d, oxh_sys &S, double &rcut) {
 	const double x = rhigh;
 	const double B = -(std::exp(-x / l) * q * q * (x + l) * (x + l)) / (-4. * x * x * x * l * l * q);
 	const double RC = x * (q * x + 3. * q * l) / (q * (x + l));
-	S.dh_prefactor = (float) q;
-	S.dh_minus_kappa = (float) (-1.0 / l);
-	S.dh_rhigh = (float) rhigh;
-	S.dh_rc = (float) RC;
-	S.dh_b = (float) B;
+	S.dh_prefactor_d = q;
+	S.dh_minus_kappa_d = -1.0 / l;
+	S.dh_rhigh_d = rhigh;
+	S.dh_rc_d = RC;
+	S.dh_b_d = B;
 	S.dh_half = d.dh_half_charged_ends ? 1 : 0;
-	S.hb_multi = (float) d.hb_multiplier;
-	S.fene_r0 = 0.7564f;
+	S.hb_multi_d = d.hb_multiplier;
+	S.fene_r0_d = (double) 0.7564f;
 	S.use_mbf = 0;
 	if(d.max_backbone_force >= 0) {
 		const double FE = 2.0, FD2 = (double) 0.0625f;
 		double fmax = d.max_backbone_force;
 		S.use_mbf = 1;
-		S.mbf_fmax = (float) fmax;
-		S.mbf_xmax = (float) ((-FE + std::sqrt(FE * FE + 4.f * fmax * fmax * FD2)) / (2.f * fmax));
-		S.mbf_finf = (float) (d.max_backbone_force_far >= 0 ? d.max_backbone_force_far : 0.04f);
+		S.mbf_fmax_d = fmax;
+		S.mbf_xmax_d = (-FE + std::sqrt(FE * FE + 4.f * fmax * fmax * FD2)) / (2.f * fmax);
+		S.mbf_finf_d = d.max_backbone_force_far >= 0 ? d.max_backbone_force_far : (double) 0.04f;
 	}
+	S.dh_prefactor = (float) S.dh_prefactor_d;
+	S.dh_minus_kappa = (float) S.dh_minus_kappa_d;
+	S.dh_rhigh = (float) S.dh_rhigh_d;
+	S.dh_rc = (float) S.dh_rc_d;
+	S.dh_b = (float) S.dh_b_d;
+	S.hb_multi = (float) S.hb_multi_d;
+	S.fene_r0 = (float) S.fene_r0_d;
+	S.mbf_fmax = (float) S.mbf_fmax_d;
+	S.mbf_xmax = (float) S.mbf_xmax_d;
+	S.mbf_finf = (float) S.mbf_finf_d;
 	const double arm = std::sqrt(SQRd((double) -0.34f) + SQRd((double) 0.3408f));
 	rcut = std::max(2 * arm + (double) 0.711879214356f, 2 * (double) 0.4f + (double) 0.783775f);
 	rcut = std::max(rcut, 2.0 * arm + RC);
@@ -321,6 +334,10 @@ oxh_status backup(oxh_batch *B) {
 	HIPCHK(hipMemcpyAsync(B->bk_q64, b.q64, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
 	HIPCHK(hipMemcpyAsync(B->bk_F, b.F, N * sizeof(float4), hipMemcpyDeviceToDevice, b.stream));
 	HIPCHK(hipMemcpyAsync(B->bk_T, b.T, N * sizeof(float4), hipMemcpyDeviceToDevice, b.stream));
+	if(b.fp64) {
+		HIPCHK(hipMemcpyAsync(B->bk_F64, b.F64, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
+		HIPCHK(hipMemcpyAsync(B->bk_T64, b.T64, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
+	}
 	HIPCHK(hipMemcpyAsync(B->bk_pstep, b.pstep, N * sizeof(long long), hipMemcpyDeviceToDevice, b.stream));
 	HIPCHK(hipMemcpyAsync(B->bk_state, b.d_state, b.nsys * sizeof(oxh_sys_state), hipMemcpyDeviceToDevice, b.stream));
 	HIPCHK(hipMemcpyAsync(B->bk_rlist, b.rlist, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
@@ -336,6 +353,10 @@ oxh_status restore(oxh_batch *B) {
 	HIPCHK(hipMemcpyAsync(b.q64, B->bk_q64, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
 	HIPCHK(hipMemcpyAsync(b.F, B->bk_F, N * sizeof(float4), hipMemcpyDeviceToDevice, b.stream));
 	HIPCHK(hipMemcpyAsync(b.T, B->bk_T, N * sizeof(float4), hipMemcpyDeviceToDevice, b.stream));
+	if(b.fp64) {
+		HIPCHK(hipMemcpyAsync(b.F64, B->bk_F64, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
+		HIPCHK(hipMemcpyAsync(b.T64, B->bk_T64, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
+	}
 	HIPCHK(hipMemcpyAsync(b.pstep, B->bk_pstep, N * sizeof(long long), hipMemcpyDeviceToDevice, b.stream));
 	HIPCHK(hipMemcpyAsync(b.d_state, B->bk_state, b.nsys * sizeof(oxh_sys_state), hipMemcpyDeviceToDevice, b.stream));
 	HIPCHK(hipMemcpyAsync(b.rlist, B->bk_rlist, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
@@ -692,8 +713,9 @@ oxh_status autotune_forces(oxh_batch *B) {
 		if(const char *f = getenv("OXH_FAR_LANES")) b.far_lanes = atoi(f);
 		return OXH_OK;
 	}
-	// the workgroup engine fixes its own lanes per system
-	if(B->persist) return OXH_OK;
+	// the workgroup engine fixes its own lanes per system; the all-FP64 mode
+	// runs 8 lanes per nucleotide in blocks of 64
+	if(B->persist || b.fp64) return OXH_OK;
 	hipEvent_t e0, e1;
 	HIPCHK(hipEventCreate(&e0));
 	HIPCHK(hipEventCreate(&e1));
@@ -792,7 +814,7 @@ oxh_status oxh_batch_destroy(oxh_batch *B) {
 	void *ptrs[] = {b.pos, b.vel, b.Lm, b.q64, b.rlist, b.q32, b.bpos, b.F, b.T, b.bonds, b.pinfo, b.orig, b.pstep, b.nbr, b.nnbr,
 			b.cell_count, b.cell_items, b.cell_owner, b.pos32, b.d_sys, b.d_state, b.red_blocks, b.red_sys_blocks, b.red_partial, b.red_out, b.red_done,
 			b.esplit, b.stress, b.hbcount, b.keys, b.keys_alt, b.perm, b.perm_alt, b.inv_perm, b.radix_hist, b.scratch, B->bk_pos, B->bk_vel,
-			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state, B->bk_rlist, b.limit, b.any_rebuild};
+			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state, B->bk_rlist, b.limit, b.any_rebuild, b.F64, b.T64, B->bk_F64, B->bk_T64};
 	for(void *p : ptrs)
 		if(p) (void) hipFree(p);
 	if(b.stream) (void) hipStreamDestroy(b.stream);
@@ -826,6 +848,15 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 		g_err = msg;
 		return s;
 	};
+	if(descs) {
+		b.fp64 = descs[0].backend_precision == 1;
+		for(int s = 0; s < n_sys; s++) {
+			if(descs[s].backend_precision != 0 && descs[s].backend_precision != 1)
+				return bail(fail(OXH_ERR_ARG, "system %d: backend_precision must be 0 (mixed) or 1 (double)", s));
+			if((descs[s].backend_precision == 1) != b.fp64)
+				return bail(fail(OXH_ERR_ARG, "all systems of a batch need the same backend_precision"));
+		}
+	}
 #define TRYB(x)                               \
 	do {                                      \
 		oxh_status s_ = (x);                  \
@@ -914,6 +945,12 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 	TRYB(dalloc(B, &B->bk_q64, N));
 	TRYB(dalloc(B, &B->bk_F, N));
 	TRYB(dalloc(B, &B->bk_T, N));
+	if(b.fp64) {
+		TRYB(dalloc(B, &b.F64, N));
+		TRYB(dalloc(B, &b.T64, N));
+		TRYB(dalloc(B, &B->bk_F64, N));
+		TRYB(dalloc(B, &B->bk_T64, N));
+	}
 	TRYB(dalloc(B, &B->bk_pstep, N));
 	TRYB(dalloc(B, &B->bk_state, n_sys));
 
@@ -970,6 +1007,9 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 	if(hipMemsetAsync(b.cell_count, 0, std::max(ncell, 1) * sizeof(int), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
 	if(hipMemsetAsync(b.F, 0, N * sizeof(float4), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
 	if(hipMemsetAsync(b.T, 0, N * sizeof(float4), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
+	if(b.fp64 && (hipMemsetAsync(b.F64, 0, N * sizeof(double4), st) != hipSuccess ||
+			hipMemsetAsync(b.T64, 0, N * sizeof(double4), st) != hipSuccess))
+		return bail(fail(OXH_ERR_HIP, "memset failed"));
 
 	// refresh_vel systems: draw on the device, keep the others as uploaded
 	bool any_refresh = false;
@@ -1001,6 +1041,7 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 		B->persist &= !S.use_cells && reach <= std::min(oxh::PERSIST_MAX_N, B->persist_block);
 	}
 	if(const char *p = getenv("OXH_PERSIST")) B->persist &= atoi(p) != 0;
+	B->persist &= !b.fp64;   // the workgroup engine computes in the mixed form
 	TRYB(dalloc(B, &B->bk_rlist, N));
 	TRYB(dalloc(B, &b.any_rebuild, 1));
 	if(hipMemsetAsync(b.any_rebuild, 0, sizeof(int), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
@@ -1162,6 +1203,7 @@ oxh_status oxh_pool_admit(oxh_batch *B, int slot, const oxh_system_desc *desc, l
 	if(B->n[slot] > 0) return fail(OXH_ERR_ARG, "slot %d is in use", slot);
 	const oxh_system_desc &d = *desc;
 	if(d.N <= 0 || d.N > B->cap[slot]) return fail(OXH_ERR_ARG, "slot %d holds at most %d nucleotides (job has %d)", slot, B->cap[slot], d.N);
+	if(d.backend_precision != 0) return fail(OXH_ERR_ARG, "slot pools run the mixed precision form only");
 	oxh::Batch &b = B->b;
 	oxh_sys S;
 	double rv = 0;
@@ -1352,10 +1394,22 @@ oxh_status oxh_batch_get_state(oxh_batch *B, int sys, double *pos, double *a1, d
 oxh_status oxh_batch_get_forces(oxh_batch *B, int sys, double *F, double *T, double *Epart) {
 	TRY(check_sys(B, sys));
 	int f = B->first[sys], n = B->n[sys];
-	std::vector<float4> hf, ht;
+	std::vector<double4> hf(n), ht(n);
 	std::vector<int> o;
-	TRY(download(B, hf, B->b.F, f, n));
-	TRY(download(B, ht, B->b.T, f, n));
+	if(B->b.fp64) {
+		TRY(download(B, hf, B->b.F64, f, n));
+		TRY(download(B, ht, B->b.T64, f, n));
+	}
+	else {
+		std::vector<float4> sf, sT;
+		TRY(download(B, sf, B->b.F, f, n));
+		TRY(download(B, sT, B->b.T, f, n));
+		TRY(sync_and_check(B));
+		for(int k = 0; k < n; k++) {
+			hf[k] = make_double4(sf[k].x, sf[k].y, sf[k].z, sf[k].w);
+			ht[k] = make_double4(sT[k].x, sT[k].y, sT[k].z, sT[k].w);
+		}
+	}
 	TRY(download(B, o, B->b.orig, f, n));
 	TRY(sync_and_check(B));
 	for(int k = 0; k < n; k++) {
@@ -1391,9 +1445,9 @@ oxh_status oxh_batch_energy_split(oxh_batch *B, int sys, double *E8) {
 	TRY(check_sys(B, sys));
 	enqueue_forces(B, true);
 	int f = B->first[sys], n = B->n[sys];
-	std::vector<float> e((size_t) 8 * n);
+	std::vector<double> e((size_t) 8 * n);
 	for(int t = 0; t < 8; t++)
-		HIPCHK(hipMemcpyAsync(e.data() + (size_t) t * n, B->b.esplit + (size_t) t * B->b.N + f, n * sizeof(float), hipMemcpyDeviceToHost,
+		HIPCHK(hipMemcpyAsync(e.data() + (size_t) t * n, B->b.esplit + (size_t) t * B->b.N + f, n * sizeof(double), hipMemcpyDeviceToHost,
 				B->b.stream));
 	TRY(sync_and_check(B));
 	for(int t = 0; t < 8; t++) {
@@ -1408,9 +1462,9 @@ oxh_status oxh_batch_stress(oxh_batch *B, int sys, double *virial6, double *kine
 	TRY(check_sys(B, sys));
 	enqueue_forces(B, true);
 	const int f = B->first[sys], n = B->n[sys];
-	std::vector<float> w((size_t) 6 * std::max(n, 1));
+	std::vector<double> w((size_t) 6 * std::max(n, 1));
 	for(int c = 0; c < 6; c++)
-		HIPCHK(hipMemcpyAsync(w.data() + (size_t) c * n, B->b.stress + (size_t) c * B->b.N + f, n * sizeof(float), hipMemcpyDeviceToHost,
+		HIPCHK(hipMemcpyAsync(w.data() + (size_t) c * n, B->b.stress + (size_t) c * B->b.N + f, n * sizeof(double), hipMemcpyDeviceToHost,
 				B->b.stream));
 	std::vector<double4> v;
 	TRY(download(B, v, B->b.vel, f, n));

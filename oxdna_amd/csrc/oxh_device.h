@@ -1,6 +1,7 @@
 // Device-side building blocks of the oxDNA2 hot path for gfx950:
-// model constants, the FP32 radial/angular modulation functions,
-// quaternion algebra and the Philox4x32-10 counter generator.
+// model constants, small FP32 / FP64 vectors, quaternion algebra and the
+// Philox4x32-10 counter generator.  The modulation functions live in
+// oxh_terms.inc with the pair terms.
 //
 // Model constants are the published oxDNA2 parameters as the reference
 // writes them (src/model.h:15-434, src/defs.h:14 for the float PI); the
@@ -100,6 +101,22 @@ __device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y 
 __device__ __forceinline__ f3 cross(f3 a, f3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 __device__ __forceinline__ float SQR(float x) { return x * x; }
 
+// FP64 small vectors, same operations
+struct dv3 {
+	double x, y, z;
+};
+__device__ __forceinline__ dv3 mk(double x, double y, double z) { return {x, y, z}; }
+__device__ __forceinline__ dv3 operator+(dv3 a, dv3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ dv3 operator-(dv3 a, dv3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ dv3 operator-(dv3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ dv3 operator*(dv3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ dv3 operator*(double s, dv3 a) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ dv3 &operator+=(dv3 &a, dv3 b) { a.x += b.x; a.y += b.y; a.z += b.z; return a; }
+__device__ __forceinline__ dv3 &operator-=(dv3 &a, dv3 b) { a.x -= b.x; a.y -= b.y; a.z -= b.z; return a; }
+__device__ __forceinline__ double dot(dv3 a, dv3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ dv3 cross(dv3 a, dv3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ __forceinline__ double SQR(double x) { return x * x; }
+
 // body axes from a unit quaternion (x, y, z, w): a1, a2, a3 are the images
 // of the x, y, z axes (the columns of the rotation matrix)
 __device__ __forceinline__ void quat_axes(float4 q, f3 &a1, f3 &a2, f3 &a3) {
@@ -110,150 +127,14 @@ __device__ __forceinline__ void quat_axes(float4 q, f3 &a1, f3 &a2, f3 &a3) {
 	a3 = {2.f * (xz + yw), 2.f * (yz - xw), -sqx - sqy + sqz + sqw};
 }
 
-// --------------------------------------------------- modulation functions
-
-__device__ __forceinline__ float f1(const oxh_sys &S, float r, int t, int n3, int n5) {
-	float val = 0.f;
-	if(r < F1_RCHIGH[t]) {
-		float eps = S.f1_eps[t][n3][n5];
-		if(r > F1_RHIGH[t]) val = eps * F1_BHIGH[t] * SQR(r - F1_RCHIGH[t]);
-		else if(r > F1_RLOW[t]) {
-			float tmp = 1.f - expf(-(r - F1_R0[t]) * F1_A[t]);
-			val = eps * SQR(tmp) - S.f1_shift[t][n3][n5];
-		}
-		else if(r > F1_RCLOW[t]) val = eps * F1_BLOW[t] * SQR(r - F1_RCLOW[t]);
-	}
-	return val;
-}
-
-__device__ __forceinline__ float f1D(const oxh_sys &S, float r, int t, int n3, int n5) {
-	float val = 0.f;
-	if(r < F1_RCHIGH[t]) {
-		if(r > F1_RHIGH[t]) val = 2.f * F1_BHIGH[t] * (r - F1_RCHIGH[t]);
-		else if(r > F1_RLOW[t]) {
-			float tmp = expf(-(r - F1_R0[t]) * F1_A[t]);
-			val = 2.f * (1.f - tmp) * tmp * F1_A[t];
-		}
-		else if(r > F1_RCLOW[t]) val = 2.f * F1_BLOW[t] * (r - F1_RCLOW[t]);
-	}
-	return S.f1_eps[t][n3][n5] * val;
-}
-
-__device__ __forceinline__ float f2(float r, int t) {
-	float val = 0.f;
-	if(r < F2_RCHIGH[t]) {
-		if(r > F2_RHIGH[t]) val = F2_K[t] * F2_BHIGH[t] * SQR(r - F2_RCHIGH[t]);
-		else if(r > F2_RLOW[t]) val = (F2_K[t] * 0.5f) * (SQR(r - F2_R0[t]) - SQR(F2_RC[t] - F2_R0[t]));
-		else if(r > F2_RCLOW[t]) val = F2_K[t] * F2_BLOW[t] * SQR(r - F2_RCLOW[t]);
-	}
-	return val;
-}
-
-__device__ __forceinline__ float f2D(float r, int t) {
-	float val = 0.f;
-	if(r < F2_RCHIGH[t]) {
-		if(r > F2_RHIGH[t]) val = 2.f * F2_K[t] * F2_BHIGH[t] * (r - F2_RCHIGH[t]);
-		else if(r > F2_RLOW[t]) val = F2_K[t] * (r - F2_R0[t]);
-		else if(r > F2_RCLOW[t]) val = 2.f * F2_K[t] * F2_BLOW[t] * (r - F2_RCLOW[t]);
-	}
-	return val;
-}
-
-// angle from its cosine, clamped into [-1, 1]
-__device__ __forceinline__ float safe_acos(float c) { return acosf(fminf(1.f, fmaxf(-1.f, c))); }
-
-// f4(theta) and (df4/dtheta)/sin(theta) of slot s at the angle whose
-// cosine is c; the sine comes from the same c so theta/sin(theta) stays
-// well conditioned near 0 and pi.  Small-sine guard as the reference's
-// _f4Dsin (src/Interactions/DNAInteraction.cpp:1394-1420).
-struct f4pair {
-	float f, dsin;
-};
-
-__device__ __forceinline__ f4pair f4_both(float c, int s) {
-	c = fminf(1.f, fmaxf(-1.f, c));
-	float t = acosf(c);
-	float tt0 = t - F4_T0[s];
-	float m = 1.f;
-	if(tt0 < 0.f) {
-		tt0 = -tt0;
-		m = -1.f;
-	}
-	f4pair r = {0.f, 0.f};
-	if(tt0 < F4_TC[s]) {
-		float sint2 = (1.f - c) * (1.f + c);
-		float inv_sin = sint2 > 1e-8f ? rsqrtf(sint2) : 1.f;
-		if(tt0 > F4_TS[s]) {
-			r.f = F4_B[s] * SQR(F4_TC[s] - tt0);
-			r.dsin = m * 2.f * F4_B[s] * (tt0 - F4_TC[s]) * rsqrtf(fmaxf(sint2, 1e-30f));
-		}
-		else {
-			r.f = 1.f - F4_A[s] * SQR(tt0);
-			r.dsin = sint2 > 1e-8f ? -m * 2.f * F4_A[s] * tt0 * inv_sin : -m * 2.f * F4_A[s];
-		}
-	}
-	return r;
-}
-
-// custom_f4 / custom_f4D of the DNA2_nomesh interaction: f4(c) and
-// -(df4/dtheta)/sin(theta); coaxial theta1 adds the oxDNA2 harmonic term
-// (src/Interactions/DNA2Interaction.cpp:308-363)
-__device__ __forceinline__ void cf4(float c, int s, float &f, float &fD) {
-	f4pair p = f4_both(c, s);
-	f = p.f;
-	fD = -p.dsin;
-	if(s == S_CXST_T1) {
-		c = fminf(1.f, fmaxf(-1.f, c));
-		float t = acosf(c);
-		float tt0 = t - CXST_THETA1_SB;
-		if(tt0 >= 0.f) {
-			f += CXST_THETA1_SA * SQR(tt0);
-			float sint2 = (1.f - c) * (1.f + c);
-			fD -= sint2 > 1e-8f ? 2.f * CXST_THETA1_SA * tt0 * rsqrtf(sint2) : 2.f * CXST_THETA1_SA;
-		}
-	}
-}
-
-__device__ __forceinline__ float f5(float f) {
-	float val = 0.f;
-	if(f > F5_XC) {
-		if(f < F5_XS) val = F5_B * SQR(F5_XC - f);
-		else if(f < 0.f) val = 1.f - F5_A * SQR(f);
-		else val = 1.f;
-	}
-	return val;
-}
-
-__device__ __forceinline__ float f5D(float f) {
-	float val = 0.f;
-	if(f > F5_XC) {
-		if(f < F5_XS) val = 2.f * F5_B * (f - F5_XC);
-		else if(f < 0.f) val = -2.f * F5_A * f;
-	}
-	return val;
-}
-
-// truncated/smoothed LJ (src/Interactions/DNAInteraction.cpp:1182-1205);
-// returns the energy and sets force = force on the second site
-__device__ __forceinline__ float repulsive_lj(f3 r, f3 &force, float sigma, float rstar, float b, float rc) {
-	float rnorm = dot(r, r);
-	float energy = 0.f;
-	force = {0.f, 0.f, 0.f};
-	if(rnorm < rc * rc) {
-		if(rnorm > rstar * rstar) {
-			float rmod = sqrtf(rnorm);
-			float rrc = rmod - rc;
-			energy = EXCL_EPS * b * SQR(rrc);
-			force = r * (-(2.f * EXCL_EPS * b * rrc / rmod));
-		}
-		else {
-			float tmp = sigma * sigma / rnorm;
-			float lj = tmp * tmp * tmp;
-			energy = 4.f * EXCL_EPS * (SQR(lj) - lj);
-			force = r * (-(24.f * EXCL_EPS * (lj - 2.f * SQR(lj)) / rnorm));
-		}
-	}
-	return energy;
+// the same axes in FP64 from an FP32 quaternion
+__device__ __forceinline__ void quat_axes(float4 qf, dv3 &a1, dv3 &a2, dv3 &a3) {
+	const double qx = qf.x, qy = qf.y, qz = qf.z, qw = qf.w;
+	double sqx = qx * qx, sqy = qy * qy, sqz = qz * qz, sqw = qw * qw;
+	double xy = qx * qy, xz = qx * qz, xw = qx * qw, yz = qy * qz, yw = qy * qw, zw = qz * qw;
+	a1 = {sqx - sqy - sqz + sqw, 2. * (xy + zw), 2. * (xz - yw)};
+	a2 = {2. * (xy - zw), -sqx + sqy - sqz + sqw, 2. * (yz + xw)};
+	a3 = {2. * (xz + yw), 2. * (yz - xw), -sqx - sqy + sqz + sqw};
 }
 
 // ------------------------------------------------------------ Philox4x32-10

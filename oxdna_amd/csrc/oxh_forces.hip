@@ -28,12 +28,12 @@ namespace oxh {
 // n5 bonded pairs go to lanes 0 / 1.  Partial sums are combined by a fixed
 // butterfly of wave shuffles, so results do not depend on timing.
 template<int K, bool SPLIT, bool FAR>
-__global__ void __launch_bounds__(256) k_forces(int N, const double4 *__restrict__ pos, const double4 *__restrict__ q64,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_forces(int N, const double4 *__restrict__ pos, const double4 *__restrict__ q64,
 		const float4 *__restrict__ q32,
 		const float4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
 		const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh, const oxh_sys *__restrict__ systems,
-		oxh_sys_state *__restrict__ sstate, float4 *__restrict__ Fout, float4 *__restrict__ Tout, float *__restrict__ esplit,
-		int *__restrict__ hbcount, int k, const int *__restrict__ limit, int *__restrict__ any, float *__restrict__ stress) {
+		oxh_sys_state *__restrict__ sstate, float4 *__restrict__ Fout, float4 *__restrict__ Tout, double *__restrict__ esplit,
+		int *__restrict__ hbcount, int k, const int *__restrict__ limit, int *__restrict__ any, double *__restrict__ stress) {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
 	const int i = t / K;
 	const int l = t % K;
@@ -61,7 +61,7 @@ template<int K, bool SPLIT>
 __global__ void __launch_bounds__(256) k_forces_far(int N, const double4 *__restrict__ pos, const float4 *__restrict__ q32,
 		const float4 *__restrict__ bpos, const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh,
 		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, float4 *__restrict__ Fio, float4 *__restrict__ Tio,
-		float *__restrict__ esplit, int k_step, const int *__restrict__ limit, float *__restrict__ stress) {
+		double *__restrict__ esplit, int k_step, const int *__restrict__ limit, double *__restrict__ stress) {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
 	const int i = t / K;
 	const int l = t % K;
@@ -119,6 +119,44 @@ __global__ void __launch_bounds__(256) k_forces_far(int N, const double4 *__rest
 	}
 }
 
+// All-FP64 forces (the reference's backend_precision = double): the same
+// terms in FP64 (oxh::dp), far pairs inline, F and T in FP64.  Blocks of
+// 64 threads, so the FP64 terms get the register file they need.
+template<int K, bool SPLIT>
+__global__ void __launch_bounds__(64) k_forces_dp(int N, const double4 *__restrict__ pos, const double4 *__restrict__ q64,
+		const float4 *__restrict__ q32, const float4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
+		const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh, const oxh_sys *__restrict__ systems,
+		oxh_sys_state *__restrict__ sstate, double4 *__restrict__ Fout, double4 *__restrict__ Tout, double *__restrict__ esplit,
+		int *__restrict__ hbcount, int k, const int *__restrict__ limit, int *__restrict__ any, double *__restrict__ stress) {
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	const int i = t / K;
+	const int l = t % K;
+	if(k >= 0 && t == 0) *any = 0;
+	if(i >= N) return;
+	const int s = pinfo[i].y;
+	if(k >= 0 && l == 0 && i == systems[s].first && sstate[s].need_rebuild) {
+		sstate[s].need_rebuild = 0;
+		sstate[s].n_rebuilds += 1;
+	}
+	if(idle(i, systems[s], s, k, limit)) return;
+	dp::particle_forces<SPLIT, true>(K, i, l, N, pos, q64, q32, bpos, bonds, pinfo, nbr, nnbr, max_neigh, systems, sstate, Fout, Tout,
+			esplit, hbcount, stress);
+}
+
+template<int K>
+static void launch_dp(hipStream_t st, const Batch &b, bool split, int k) {
+	const int block = 64;
+	size_t threads = (size_t) b.N * K;
+	dim3 grid((unsigned) ((threads + block - 1) / block));
+	const int *lim = k >= 0 ? b.limit : nullptr;
+	if(split)
+		k_forces_dp<K, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh, b.d_sys,
+				b.d_state, b.F64, b.T64, b.esplit, b.hbcount, k, lim, b.any_rebuild, b.stress);
+	else
+		k_forces_dp<K, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
+				b.d_sys, b.d_state, b.F64, b.T64, nullptr, b.want_hbcount ? b.hbcount : nullptr, k, lim, b.any_rebuild, nullptr);
+}
+
 template<int K>
 static void launch_k(hipStream_t st, const Batch &b, bool split, int block, bool far_inline, int k) {
 	size_t threads = (size_t) b.N * K;
@@ -158,6 +196,17 @@ static void launch_far(hipStream_t st, const Batch &b, bool split, int block, in
 // in k_forces_far right after it
 void launch_forces(hipStream_t st, const Batch &b, bool split, int k) {
 	if(b.N == 0) return;
+	if(b.fp64) {
+		switch(b.lanes) {
+		case 1: launch_dp<1>(st, b, split, k); break;
+		case 2: launch_dp<2>(st, b, split, k); break;
+		case 4: launch_dp<4>(st, b, split, k); break;
+		case 16: launch_dp<16>(st, b, split, k); break;
+		case 32: launch_dp<32>(st, b, split, k); break;
+		default: launch_dp<8>(st, b, split, k); break;
+		}
+		return;
+	}
 	const bool inl = b.far_lanes == 0;
 	int block = b.force_block;
 	switch(b.lanes) {

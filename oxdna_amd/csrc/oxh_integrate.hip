@@ -35,9 +35,10 @@ void launch_prepare_mirror(hipStream_t st, const Batch &b) {
 	k_prepare_mirror<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.q64, b.bonds, b.pinfo, b.d_sys, b.q32, b.bpos);
 }
 
+template<class FT>
 __global__ void __launch_bounds__(256) k_first_step(int N, double4 *__restrict__ pos, double4 *__restrict__ vel, double4 *__restrict__ Lm,
-		double4 *__restrict__ q64, float4 *__restrict__ q32, float4 *__restrict__ bpos, const float4 *__restrict__ F,
-		const float4 *__restrict__ T, const double4 *__restrict__ rlist, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
+		double4 *__restrict__ q64, float4 *__restrict__ q32, float4 *__restrict__ bpos, const FT *__restrict__ F,
+		const FT *__restrict__ T, const double4 *__restrict__ rlist, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
 		long long *__restrict__ pstep, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st, int ncell,
 		int *__restrict__ cell_count, int k, const int *__restrict__ limit, int *__restrict__ any) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -56,14 +57,19 @@ __global__ void __launch_bounds__(256) k_first_step(int N, double4 *__restrict__
 
 void launch_first_step(hipStream_t st, const Batch &b, int k) {
 	int n = std::max(b.N, b.ncell_total);
-	k_first_step<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.F, b.T, b.rlist, b.bonds, b.pinfo,
-			b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild);
+	if(b.fp64)
+		k_first_step<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.F64, b.T64, b.rlist, b.bonds, b.pinfo,
+				b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild);
+	else
+		k_first_step<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.F, b.T, b.rlist, b.bonds, b.pinfo,
+				b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild);
 }
 
 // second half kick + Langevin / Brownian thermostats; the first particle
 // of each system closes the system's list-rebuild flag for this step
+template<class FT>
 __global__ void __launch_bounds__(256) k_second_step(int N, double4 *__restrict__ vel, double4 *__restrict__ Lm,
-		const float4 *__restrict__ F, const float4 *__restrict__ T, const int2 *__restrict__ pinfo, const int *__restrict__ orig,
+		const FT *__restrict__ F, const FT *__restrict__ T, const int2 *__restrict__ pinfo, const int *__restrict__ orig,
 		const long long *__restrict__ pstep, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st, int k,
 		const int *__restrict__ limit) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -75,17 +81,22 @@ __global__ void __launch_bounds__(256) k_second_step(int N, double4 *__restrict_
 }
 
 void launch_second_step(hipStream_t st, const Batch &b, int k) {
-	k_second_step<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.vel, b.Lm, b.F, b.T, b.pinfo, b.orig, b.pstep, b.d_sys, b.d_state, k,
-			b.limit);
+	if(b.fp64)
+		k_second_step<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.vel, b.Lm, b.F64, b.T64, b.pinfo, b.orig, b.pstep, b.d_sys, b.d_state, k,
+				b.limit);
+	else
+		k_second_step<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.vel, b.Lm, b.F, b.T, b.pinfo, b.orig, b.pstep, b.d_sys, b.d_state, k,
+				b.limit);
 }
 
 // The second half of step k and the first half of step k + 1 in one pass
 // over the particles (when no Bussi activation falls in between): the same
 // per-particle functions in the same order as k_second_step followed by
 // k_first_step.
+template<class FT>
 __global__ void __launch_bounds__(256) k_second_first(int N, double4 *__restrict__ pos, double4 *__restrict__ vel,
 		double4 *__restrict__ Lm, double4 *__restrict__ q64, float4 *__restrict__ q32, float4 *__restrict__ bpos,
-		const float4 *__restrict__ F, const float4 *__restrict__ T, const double4 *__restrict__ rlist, const int2 *__restrict__ bonds,
+		const FT *__restrict__ F, const FT *__restrict__ T, const double4 *__restrict__ rlist, const int2 *__restrict__ bonds,
 		const int2 *__restrict__ pinfo, const int *__restrict__ orig, long long *__restrict__ pstep, const oxh_sys *__restrict__ systems,
 		oxh_sys_state *__restrict__ st, int ncell, int *__restrict__ cell_count, int k, const int *__restrict__ limit,
 		int *__restrict__ any) {
@@ -106,22 +117,27 @@ __global__ void __launch_bounds__(256) k_second_first(int N, double4 *__restrict
 
 void launch_second_first(hipStream_t st, const Batch &b, int k) {
 	int n = std::max(b.N, b.ncell_total);
-	k_second_first<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.F, b.T, b.rlist, b.bonds, b.pinfo,
-			b.orig, b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild);
+	if(b.fp64)
+		k_second_first<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.F64, b.T64, b.rlist, b.bonds,
+				b.pinfo, b.orig, b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild);
+	else
+		k_second_first<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.F, b.T, b.rlist, b.bonds, b.pinfo,
+				b.orig, b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild);
 }
 
 // -------------------------------------------------------- FP64 reductions
 // per system: U = sum F.w, U_hb = sum T.w, K_t = 1/2 sum v^2, K_r = 1/2 sum L^2,
 // sum v (3 components), N.  Blocks never straddle systems; the final sum
 // runs over a system's blocks in order, so results are reproducible.
-__global__ void __launch_bounds__(256) k_reduce_partial(const int3 *__restrict__ blocks, const float4 *__restrict__ F,
-		const float4 *__restrict__ T, const double4 *__restrict__ vel, const double4 *__restrict__ Lm, double *__restrict__ partial) {
+template<class FT>
+__global__ void __launch_bounds__(256) k_reduce_partial(const int3 *__restrict__ blocks, const FT *__restrict__ F,
+		const FT *__restrict__ T, const double4 *__restrict__ vel, const double4 *__restrict__ Lm, double *__restrict__ partial) {
 	__shared__ double sh[RED_W][256];
 	int3 bl = blocks[blockIdx.x];
 	if(bl.x < 0) bl.y = bl.z = 0;   // unused block of a slot pool
 	double acc[RED_W] = {0, 0, 0, 0, 0, 0, 0, 0};
 	for(int i = bl.y + threadIdx.x; i < bl.z; i += blockDim.x) {
-		float4 f = F[i], t = T[i];
+		const FT f = F[i], t = T[i];
 		double4 v = vel[i], L = Lm[i];
 		acc[0] += (double) f.w;
 		acc[1] += (double) t.w;
@@ -155,7 +171,8 @@ __global__ void k_reduce_final(int nsys, const int2 *__restrict__ sys_blocks, co
 
 void launch_reduce(hipStream_t st, const Batch &b) {
 	if(b.nred == 0) return;
-	k_reduce_partial<<<b.nred, 256, 0, st>>>(b.red_blocks, b.F, b.T, b.vel, b.Lm, b.red_partial);
+	if(b.fp64) k_reduce_partial<<<b.nred, 256, 0, st>>>(b.red_blocks, b.F64, b.T64, b.vel, b.Lm, b.red_partial);
+	else k_reduce_partial<<<b.nred, 256, 0, st>>>(b.red_blocks, b.F, b.T, b.vel, b.Lm, b.red_partial);
 	k_reduce_final<<<(b.nsys + 63) / 64, 64, 0, st>>>(b.nsys, b.red_sys_blocks, b.red_partial, b.red_out);
 }
 

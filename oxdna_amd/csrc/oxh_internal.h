@@ -22,6 +22,9 @@ struct Batch {
 	// per-particle state (FP64 integration state, FP32 kernel mirror)
 	double4 *pos = nullptr, *vel = nullptr, *Lm = nullptr, *q64 = nullptr, *rlist = nullptr;
 	float4 *q32 = nullptr, *F = nullptr, *T = nullptr;
+	// all-FP64 mode (backend_precision = double): forces and torques in FP64
+	bool fp64 = false;
+	double4 *F64 = nullptr, *T64 = nullptr;
 	float4 *bpos = nullptr;   // backbone-site position wrapped into the box (FP32), .w = Debye-Hueckel charge factor
 	int2 *bonds = nullptr;   // (n3, n5) batch indices, -1 = none
 	int2 *pinfo = nullptr;   // (btype, system)
@@ -66,8 +69,8 @@ struct Batch {
 	int *red_done = nullptr;           // per system: blocks finished (Bussi's last-block reduction)
 
 	// optional outputs
-	float *esplit = nullptr;   // 8 * N
-	float *stress = nullptr;   // 6 * N: each particle's half of its pairs' virial (xx yy zz xy xz yz)
+	double *esplit = nullptr;   // 8 * N
+	double *stress = nullptr;   // 6 * N: each particle's half of its pairs' virial (xx yy zz xy xz yz)
 	int *hbcount = nullptr;    // N
 	bool want_hbcount = false;
 

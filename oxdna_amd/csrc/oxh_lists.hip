@@ -327,6 +327,10 @@ int launch_hilbert_sort(hipStream_t st, Batch &b) {
 	permute(b.q64);
 	permute(b.F);
 	permute(b.T);
+	if(b.fp64) {
+		permute(b.F64);
+		permute(b.T64);
+	}
 	permute(b.pinfo);
 	permute(b.orig);
 	permute(b.pstep);

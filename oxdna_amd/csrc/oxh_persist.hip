@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(BLOCK) k_persist(PersistArgs a) {
 // nucleotide (after uploads, list rebuilds and for observables), so that
 // runs continue from exactly the forces the engine itself computes.
 template<bool SPLIT, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_persist_forces(PersistArgs a, float *esplit, float *stress) {
+__global__ void __launch_bounds__(BLOCK) k_persist_forces(PersistArgs a, double *esplit, double *stress) {
 	const oxh_sys &S = a.systems[blockIdx.x];
 	const int K = persist_lanes(S.n, BLOCK);
 	const int tid = threadIdx.x;

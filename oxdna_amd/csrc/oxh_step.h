@@ -48,13 +48,14 @@ __device__ __forceinline__ void write_mirror(int i, double4 r, double4 q, const 
 // first half kick, drift and rotation of particle i, FP32 mirror, step
 // counter; returns true when the particle moved more than the skin since
 // the last list build (src/CUDA/Backends/CUDA_mixed.cuh:23-69)
+template<class FT>
 __device__ __forceinline__ bool first_step_particle(int i, const oxh_sys &S, double4 *__restrict__ pos, double4 *__restrict__ vel,
 		double4 *__restrict__ Lm, double4 *__restrict__ q64, float4 *__restrict__ q32, float4 *__restrict__ bpos,
-		const float4 *__restrict__ F, const float4 *__restrict__ T, const double4 *__restrict__ rlist, const int2 *__restrict__ bonds,
+		const FT *__restrict__ F, const FT *__restrict__ T, const double4 *__restrict__ rlist, const int2 *__restrict__ bonds,
 		long long *__restrict__ pstep) {
 	const double hdt = 0.5 * S.dt;
 
-	float4 f = F[i];
+	const FT f = F[i];
 	double4 v = vel[i];
 	v.x += (double) f.x * hdt;
 	v.y += (double) f.y * hdt;
@@ -66,7 +67,7 @@ __device__ __forceinline__ bool first_step_particle(int i, const oxh_sys &S, dou
 	r.z += v.z * S.dt;
 	pos[i] = r;
 
-	float4 t = T[i];
+	const FT t = T[i];
 	double4 L = Lm[i];
 	L.x += (double) t.x * hdt;
 	L.y += (double) t.y * hdt;
@@ -96,10 +97,11 @@ __device__ __forceinline__ bool first_step_particle(int i, const oxh_sys &S, dou
 
 // second half kick of particle i and the Langevin / Brownian thermostat
 // kick (src/CUDA/Backends/CUDA_mixed.cuh:71-93)
+template<class FT>
 __device__ __forceinline__ void second_step_particle(int i, const oxh_sys &S, double4 *__restrict__ vel, double4 *__restrict__ Lm,
-		const float4 *__restrict__ F, const float4 *__restrict__ T, const int *__restrict__ orig, const long long *__restrict__ pstep) {
+		const FT *__restrict__ F, const FT *__restrict__ T, const int *__restrict__ orig, const long long *__restrict__ pstep) {
 	const double hdt = 0.5 * S.dt;
-	float4 f = F[i], t = T[i];
+	const FT f = F[i], t = T[i];
 	double4 v = vel[i], L = Lm[i];
 	v.x += (double) f.x * hdt;
 	v.y += (double) f.y * hdt;

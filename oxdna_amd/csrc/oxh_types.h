@@ -41,6 +41,11 @@ typedef struct oxh_sys {
 	int use_mbf;
 	float mbf_xmax, mbf_fmax, mbf_finf;
 	float fene_r0;
+	// the same constants in FP64, for the all-FP64 force mode
+	double f1_eps_d[2][5][5];
+	double f1_shift_d[2][5][5];
+	double dh_prefactor_d, dh_minus_kappa_d, dh_rhigh_d, dh_rc_d, dh_b_d;
+	double hb_multi_d, mbf_xmax_d, mbf_fmax_d, mbf_finf_d, fene_r0_d;
 	// integration, FP64
 	double dt, T;
 	int thermostat;       // 0 none, 1 Langevin, 2 Bussi, 3 Brownian
