@@ -165,6 +165,9 @@ def served_main(args):
     res = srv.run(jobs)
     log(f"served at {rate:.2f} jobs/s ({args.rate_mult}x plain): makespan {res['makespan']:.2f} s, "
         f"{res['nt_steps_per_s']:.3e} nt-steps/s, p50 {res['p50']:.3f} s p99 {res['p99']:.3f} s")
+    for j in sorted(jobs, key=lambda j: -j.finished)[:5]:
+        log(f"  job {j.job_id}: N {j.system.N} steps {j.steps} arrived {j.arrival:.2f} s admitted {j.admitted:.2f} s "
+            f"finished {j.finished:.2f} s ({1e3 * (j.finished - j.admitted) / j.steps:.4f} ms/step in its pool)")
     for j in jobs:
         j.arrival = 0.0
         j.admitted = j.finished = None

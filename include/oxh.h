@@ -207,15 +207,18 @@ oxh_status oxh_batch_set_state(oxh_batch *b, int sys, const double *pos, const d
 		const double *L);
 
 /* Neighbour-list, memory and launch-shape figures of the batch (10
- * doubles): out[0] N, out[1] max_neigh, out[2] mean neighbours per
- * particle, out[3] device bytes held, out[4] threads per nucleotide and
+ * doubles): out[0] N (a pool: its slots' total capacity), out[1]
+ * max_neigh, out[2] mean neighbours per particle of the systems held,
+ * out[3] device bytes held, out[4] threads per nucleotide and
  * out[5] block size of the close-pair force kernel, out[6] threads per
  * nucleotide (0 = inline) and out[7] block size of the far-pair kernel,
  * out[8] engine of oxh_batch_run (0: one kernel per phase, 1: one
  * workgroup per system for many steps per launch, used when every system
  * has at most 512 nucleotides and no cells), out[9] number of systems.
- * Shapes are picked by timing at upload; OXH_FORCE_LANES / OXH_FAR_LANES
- * fix them, OXH_PERSIST=0 disables the workgroup engine. */
+ * Shapes are picked by timing at upload (the all-FP64 mode times the lanes
+ * only, in blocks of 64 with the far pairs inline; pools keep 8 lanes);
+ * OXH_FORCE_LANES / OXH_FAR_LANES fix them, OXH_PERSIST=0 disables the
+ * workgroup engine. */
 oxh_status oxh_batch_stats(oxh_batch *b, double *out10);
 
 /* Average duration in ms of n back-to-back launches of the force kernel on

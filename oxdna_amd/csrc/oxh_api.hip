@@ -713,9 +713,8 @@ oxh_status autotune_forces(oxh_batch *B) {
 		if(const char *f = getenv("OXH_FAR_LANES")) b.far_lanes = atoi(f);
 		return OXH_OK;
 	}
-	// the workgroup engine fixes its own lanes per system; the all-FP64 mode
-	// runs 8 lanes per nucleotide in blocks of 64
-	if(B->persist || b.fp64) return OXH_OK;
+	// the workgroup engine fixes its own lanes per system
+	if(B->persist) return OXH_OK;
 	hipEvent_t e0, e1;
 	HIPCHK(hipEventCreate(&e0));
 	HIPCHK(hipEventCreate(&e1));
@@ -731,6 +730,27 @@ oxh_status autotune_forces(oxh_batch *B) {
 	const int lanes[] = {1, 2, 4, 8, 16, 32};
 	const int blocks[] = {64, 128, 256};
 	float best = 1e30f;
+	if(b.fp64) {
+		// all-FP64 mode: blocks of 64, far pairs inline; only the lanes vary
+		b.force_block = 64;
+		b.far_lanes = 0;
+		int bl64 = 8;
+		for(int L : lanes) {
+			if(L == 1) continue;
+			b.lanes = L;
+			float ms;
+			TRY(timeit(ms));
+			if(ms < best) {
+				best = ms;
+				bl64 = L;
+			}
+		}
+		(void) hipEventDestroy(e0);
+		(void) hipEventDestroy(e1);
+		b.lanes = bl64;
+		enqueue_forces(B);
+		return sync_and_check(B);
+	}
 	int bl = b.lanes, bb = b.force_block, bfl = 0, bfb = 256;
 	// 1) close kernel shape with the far pairs inline
 	b.far_lanes = 0;
@@ -1586,11 +1606,16 @@ oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {  // 10 doubles
 	std::vector<int2> nn;
 	TRY(download(B, nn, B->b.nnbr, 0, B->b.N));
 	TRY(sync_and_check(B));
+	// mean over the systems' particles (a pool's free capacity holds no lists)
 	double acc = 0;
-	for(int2 x : nn) acc += x.x + x.y;
+	long long n_real = 0;
+	for(const oxh_sys &S : B->b.h_sys) {
+		for(int i = S.first; i < S.first + S.n; i++) acc += nn[i].x + nn[i].y;
+		n_real += S.n;
+	}
 	out6[0] = B->b.N;
 	out6[1] = B->b.max_neigh;
-	out6[2] = B->b.N ? acc / B->b.N : 0.0;
+	out6[2] = n_real ? acc / n_real : 0.0;
 	out6[3] = (double) B->device_bytes;
 	out6[4] = B->b.lanes;
 	out6[5] = B->b.force_block;

@@ -1,22 +1,24 @@
 """Served form: a request loop over resident slot pools on one GPU.
 
 Jobs (one oxDNA2 system each, with its own temperature, seed and step
-count) arrive over time.  The server keeps two kinds of slot pool resident
-in HBM (include/oxh.h, oxh_pool_*):
+count) arrive over time.  The server keeps three kinds of slot pool
+resident in HBM (include/oxh.h, oxh_pool_*), one per size class:
 
-  - a workgroup pool: slots of 128 nucleotides, one workgroup per job and
-    many steps per launch;
-  - grid pools: slots of a few capacity classes up to the largest job,
-    one kernel per phase for all of a pool's jobs together, cell lists.
+  - small jobs (up to SMALL_CAP nucleotides): one workgroup pool, one
+    workgroup per job and many steps per launch;
+  - mid-size jobs: one grid pool with slots of two capacities, one kernel
+    per phase for all of its jobs together, cell lists;
+  - large jobs: single-slot grid pools, one per stream, so that the few
+    jobs that hold most of the work (and set the makespan) step as fast as
+    they would alone while the other pools fill the rest of the device.
 
 Admission is by HBM footprint: the pools are sized at start-up from a byte
-budget (oxh_batch_stats reports what each pool holds), and a job is
-admitted when a free slot of large enough capacity exists.  Each pool is
-driven by its own host thread, so the pools' kernels overlap on the device;
-every `quantum` steps a pool's thread releases finished jobs and admits
-waiting ones into the freed slots while the other slots keep their state
-(continuous batching).  A job's result does not depend on what else runs:
-see oxh_pool_create.
+budget (oxh_batch_stats reports what each pool holds).  Each pool is driven
+by its own host thread, so the pools' kernels overlap on the device; every
+`quantum` steps a pool's thread releases finished jobs and takes waiting
+jobs of its class into the freed slots while the other slots keep their
+state (continuous batching).  A job's result does not depend on what else
+runs: see oxh_pool_create.
 
 The reference has no counterpart (one system per process,
 src/Utilities/ConfigInfo.h:47); its input keys per job are honoured through
@@ -34,10 +36,11 @@ import numpy as np
 from .backend import HIPPool
 from .systems import System
 
-# jobs up to SMALL_CAP nucleotides run in the workgroup pool (one workgroup
-# each, at least 4 lanes per nucleotide); a workgroup engine step costs ~n
-# on one CU, so larger jobs would hold a CU for long: they go to grid pools
-SMALL_CAP = 128
+# jobs up to SMALL_CAP nucleotides run in the workgroup pool (one
+# 512-thread workgroup each); MID_CAP bounds the shared grid pool's slots;
+# larger jobs get single-slot pools
+SMALL_CAP = 512
+MID_CAP = 8192
 
 
 @dataclass
@@ -65,43 +68,35 @@ def footprint_bytes(n_nucleotides: int, max_neigh: int = 64) -> int:
     return n_nucleotides * (160 + 168 + 64 + 40 + 4 * max_neigh)
 
 
-def plan_pools(budget_bytes: int, largest: int, small_slots: int = 256):
-    """Slot capacities for one workgroup pool and the grid pools, within
-    budget_bytes: the workgroup pool takes small_slots x SMALL_CAP nt; the
-    rest goes to grid slots in capacity classes x4 apart (512, 2048, ...)
-    up to `largest`, about the same bytes per class."""
+def plan_pools(budget_bytes: int, largest: int, small_slots: int = 256, mid_slots: tuple = (16, 4),
+               large_pools: int = 2):
+    """Pools as lists of slot capacities, one list per pool: the workgroup
+    pool (small_slots x SMALL_CAP), the mid-size grid pool (mid_slots[0]
+    slots of MID_CAP / 4 and mid_slots[1] of MID_CAP) and large_pools
+    single-slot pools of `largest`; fewer mid-size slots if the budget
+    is short.  Returns (pools, size class of each pool)."""
     small = [SMALL_CAP] * small_slots
-    left = budget_bytes - footprint_bytes(SMALL_CAP * small_slots)
-    classes = []
-    c = 512
-    while True:
-        classes.append(min(c, largest))
-        if c >= largest:
-            break
-        c *= 4
-    per_class = max(left // max(1, len(classes)), 0)
-    grid = []
-    for c in classes:
-        # every step of a grid pool launches over all its slots' capacity,
-        # so a class holds at most ~64k nucleotides of slots
-        n = max(1, int(per_class // footprint_bytes(c, 128)))
-        grid.append([c] * max(1, min(n, 65536 // c)))
-    return small, grid
+    n1, n2 = mid_slots
+    mid = [MID_CAP // 4] * n1 + [MID_CAP] * n2
+    large = [[max(largest, MID_CAP)] for _ in range(large_pools)] if largest > MID_CAP else []
+    fixed = footprint_bytes(SMALL_CAP * small_slots) + sum(footprint_bytes(p[0], 128) for p in large)
+    while len(mid) > 1 and fixed + footprint_bytes(sum(mid), 128) > budget_bytes:
+        mid.pop(0)
+    pools = [small, mid] + large
+    classes = [SMALL_CAP, MID_CAP] + [p[0] for p in large]
+    return pools, classes
 
 
 class Server:
     """Resident pools and the request loop."""
 
     def __init__(self, budget_bytes: int = 8 << 30, largest: int = 20000, small_slots: int = 256, device: int = 0,
-                 quantum_small: int = 1024, quantum_grid: int = 256):
-        small, grid = plan_pools(budget_bytes, largest, small_slots)
-        self.pools = [HIPPool(small, device=device)]
-        self.quantum = [quantum_small]
-        for caps in grid:
-            self.pools.append(HIPPool(caps, device=device))
-            self.quantum.append(quantum_grid)
+                 quantum_small: int = 1024, quantum_grid: int = 256, quantum_large: int = 1024):
+        pools, classes = plan_pools(budget_bytes, largest, small_slots)
+        self.pools = [HIPPool(caps, device=device) for caps in pools]
+        self.classes = classes
+        self.quantum = [quantum_small, quantum_grid] + [quantum_large] * (len(pools) - 2)
         self.capacity = [max(p.capacity) for p in self.pools]
-        self._lock = threading.Lock()
 
     def close(self):
         for p in self.pools:
@@ -110,47 +105,62 @@ class Server:
     def hbm_bytes(self) -> int:
         return sum(p.stats()["device_bytes"] for p in self.pools)
 
-    def _pool_for(self, n: int) -> int:
-        for k, c in enumerate(self.capacity):
+    def _class_of(self, n: int) -> int:
+        for c in sorted(set(self.classes)):
             if n <= c:
-                return k
+                return c
         raise ValueError(f"job of {n} nucleotides exceeds every slot")
 
     def run(self, jobs: list[Job], keep_results: bool = False) -> dict:
         """Serves `jobs` (arrival times in seconds from now); returns when
-        all are done.  Each pool's thread admits, runs and releases."""
-        queues = [[] for _ in self.pools]
+        all are done.  Jobs wait in one queue per size class, in arrival
+        order; each pool's thread admits from its class's queue, runs and
+        releases."""
+        queues = {c: [] for c in set(self.classes)}
         for j in sorted(jobs, key=lambda j: j.arrival):
-            queues[self._pool_for(j.system.N)].append(j)
+            queues[self._class_of(j.system.N)].append(j)
+        lock = threading.Lock()
         t0 = time.perf_counter()
         errors = []
 
+        def take(q, fits, now):
+            """Earliest arrived job of q that fits a free slot (under lock)."""
+            with lock:
+                for i, j in enumerate(q):
+                    if j.arrival > now:
+                        return None
+                    s = fits(j.system.N)
+                    if s is not None:
+                        q.pop(i)
+                        return j, s
+            return None
+
         def worker(k: int):
-            pool, q, quantum = self.pools[k], queues[k], self.quantum[k]
+            pool, q, quantum = self.pools[k], queues[self.classes[k]], self.quantum[k]
             slots: list[Job | None] = [None] * pool.nsys
             caps = pool.capacity
+
+            def fits(n):
+                free = [s for s in range(pool.nsys) if slots[s] is None and caps[s] >= n]
+                return min(free, key=lambda s: caps[s]) if free else None
+
             try:
                 while True:
-                    now = time.perf_counter() - t0
                     # admit arrived jobs into free slots (smallest fitting slot)
-                    i = 0
-                    while i < len(q) and q[i].arrival <= now:
-                        j = q[i]
-                        free = [s for s in range(pool.nsys) if slots[s] is None and caps[s] >= j.system.N]
-                        if not free:
-                            i += 1
-                            continue
-                        s = min(free, key=lambda s: caps[s])
+                    while True:
+                        got = take(q, fits, time.perf_counter() - t0)
+                        if got is None:
+                            break
+                        j, s = got
                         pool.admit(s, j.system, j.steps, **j.opts)
                         j.admitted = time.perf_counter() - t0
                         slots[s] = j
-                        q.pop(i)
                     if all(x is None for x in slots):
-                        if not q:
-                            return
-                        wait = q[0].arrival - (time.perf_counter() - t0)
-                        if wait > 0:
-                            time.sleep(min(wait, 0.05))
+                        with lock:
+                            if not q:
+                                return
+                            wait = q[0].arrival - (time.perf_counter() - t0)
+                        time.sleep(min(max(wait, 0.0), 0.05) if wait > 0 else 0.001)
                         continue
                     pool.run(quantum)
                     st = pool.status()

@@ -135,3 +135,23 @@ def test_hairpin_sizes_follow_the_spec():
 def test_bench_refuses_several_gpus():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_served_pool_plan():
+    """The served form's pools: a workgroup pool for small jobs, one grid
+    pool for mid-size jobs, single-slot pools for the largest; within the
+    HBM budget, and every job size from the served distribution has a
+    class."""
+    from oxdna_amd import serve
+    pools, classes = serve.plan_pools(8 << 30, 20000)
+    assert classes[0] == serve.SMALL_CAP and all(c == serve.SMALL_CAP for c in pools[0])
+    assert max(pools[1]) == serve.MID_CAP and len(pools) == 4
+    assert all(len(p) == 1 and p[0] == 20000 for p in pools[2:])
+    total = sum(serve.footprint_bytes(sum(p), 128) for p in pools)
+    assert total <= 8 << 30
+    # a tight budget drops mid-size slots but keeps one
+    pools2, _ = serve.plan_pools(serve.footprint_bytes(512 * 256) + 3 * serve.footprint_bytes(20000, 128), 20000)
+    assert 1 <= len(pools2[1]) < len(pools[1])
+    # nothing above MID_CAP: no large pools
+    pools3, classes3 = serve.plan_pools(8 << 30, 4000)
+    assert len(pools3) == 2 and classes3 == [serve.SMALL_CAP, serve.MID_CAP]
