@@ -214,11 +214,13 @@ oxh_status oxh_batch_set_state(oxh_batch *b, int sys, const double *pos, const d
  * nucleotide (0 = inline) and out[7] block size of the far-pair kernel,
  * out[8] engine of oxh_batch_run (0: one kernel per phase, 1: one
  * workgroup per system for many steps per launch, used when every system
- * has at most 512 nucleotides and no cells), out[9] number of systems.
+ * has at most 512 nucleotides and no cells, 2: one kernel per phase with
+ * the close pairs in half-list form, each pair evaluated once -- the
+ * reference's use_edge), out[9] number of systems.
  * Shapes are picked by timing at upload (the all-FP64 mode times the lanes
  * only, in blocks of 64 with the far pairs inline; pools keep 8 lanes);
  * OXH_FORCE_LANES / OXH_FAR_LANES fix them, OXH_PERSIST=0 disables the
- * workgroup engine. */
+ * workgroup engine, OXH_EDGE=0 / 1 forbids / forces the half-list form. */
 oxh_status oxh_batch_stats(oxh_batch *b, double *out10);
 
 /* Average duration in ms of n back-to-back launches of the force kernel on

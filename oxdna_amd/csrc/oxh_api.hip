@@ -278,14 +278,51 @@ __global__ void k_clear_err(int nsys, oxh_sys_state *st, int mask) {
 	if(s < nsys) st[s].error &= ~mask;
 }
 
-// doubles the neighbour or cell capacity after an overflow and clears the
-// overflow bits (the caller rebuilds)
+// (re)allocates the edge rows and slots for rows of width ecap (0 frees
+// them and switches the half-list form off)
+oxh_status alloc_edges(oxh_batch *B, int ecap) {
+	oxh::Batch &b = B->b;
+	HIPCHK(hipStreamSynchronize(b.stream));
+	const size_t old_rows = (size_t) b.N * b.ecap;
+	void *ptrs[] = {b.eown, b.ein, b.E1, b.E2, b.E3};
+	for(void *p : ptrs)
+		if(p) HIPCHK(hipFree(p));
+	if(b.ecap > 0) B->device_bytes -= old_rows * (2 * sizeof(int) + 3 * sizeof(float4));
+	b.eown = b.ein = nullptr;
+	b.E1 = b.E2 = b.E3 = nullptr;
+	b.ecap = 0;
+	if(ecap <= 0) {
+		b.edge = false;
+		return OXH_OK;
+	}
+	const size_t rows = (size_t) b.N * ecap;
+	TRY(dalloc(B, &b.eown, rows));
+	TRY(dalloc(B, &b.ein, rows));
+	TRY(dalloc(B, &b.E1, rows));
+	TRY(dalloc(B, &b.E2, rows));
+	TRY(dalloc(B, &b.E3, rows));
+	if(!b.ecount) {
+		TRY(dalloc(B, &b.ecount, b.N));
+		TRY(dalloc(B, &b.B1, b.N));
+		TRY(dalloc(B, &b.B2, b.N));
+		TRY(dalloc(B, &b.B3, b.N));
+	}
+	// empty edge rows until the next list build fills them
+	HIPCHK(hipMemsetAsync(b.ecount, 0, (size_t) b.N * sizeof(int2), b.stream));
+	b.ecap = ecap;
+	return OXH_OK;
+}
+
+// doubles the neighbour, cell or edge-row capacity after an overflow and
+// clears the overflow bits (the caller rebuilds)
 oxh_status grow_lists(oxh_batch *B, const std::vector<oxh_sys_state> &st) {
 	oxh::Batch &b = B->b;
-	bool grow_neigh = false, grow_cells = false;
+	bool grow_neigh = false, grow_cells = false, grow_edge = false;
 	for(auto &s : st) {
-		if(s.error & 2) grow_neigh = true;
-		if(s.error & 4) grow_cells = true;
+		if(s.error & oxh::ERR_LIST) grow_neigh = true;
+		if(s.error & oxh::ERR_CELLS) grow_cells = true;
+		// a row overflow empties the row, which its partners report as well
+		if((s.error & oxh::ERR_EDGE) && !(s.error & oxh::ERR_LIST)) grow_edge = true;
 	}
 	HIPCHK(hipStreamSynchronize(b.stream));
 	if(grow_neigh) {
@@ -300,7 +337,8 @@ oxh_status grow_lists(oxh_batch *B, const std::vector<oxh_sys_state> &st) {
 		b.cell_cap = b.cell_cap * 2;
 		TRY(dalloc(B, &b.cell_items, (size_t) b.ncell_total * b.cell_cap));
 	}
-	k_clear_err<<<(b.nsys + 63) / 64, 64, 0, b.stream>>>(b.nsys, b.d_state, 6);
+	if(grow_edge && b.ecap > 0) TRY(alloc_edges(B, std::min(2 * b.ecap, b.max_neigh)));
+	k_clear_err<<<(b.nsys + 63) / 64, 64, 0, b.stream>>>(b.nsys, b.d_state, oxh::ERR_OVERFLOW);
 	if(b.graph_exec) {
 		(void) hipGraphExecDestroy(b.graph_exec);
 		b.graph_exec = nullptr;
@@ -318,7 +356,7 @@ oxh_status build_all(oxh_batch *B) {
 		std::vector<oxh_sys_state> st;
 		TRY(read_states(B, st));
 		bool overflow = false;
-		for(auto &s : st) overflow |= (s.error & 6) != 0;
+		for(auto &s : st) overflow |= (s.error & oxh::ERR_OVERFLOW) != 0;
 		if(!overflow) return OXH_OK;
 		TRY(grow_lists(B, st));
 	}
@@ -385,7 +423,7 @@ oxh_status rebuild_from_rlist(oxh_batch *B, int only = -1) {
 		std::vector<oxh_sys_state> st;
 		TRY(read_states(B, st));
 		bool overflow = false;
-		for(auto &x : st) overflow |= (x.error & 6) != 0;
+		for(auto &x : st) overflow |= (x.error & oxh::ERR_OVERFLOW) != 0;
 		if(!overflow) return OXH_OK;
 		TRY(grow_lists(B, st));
 	}
@@ -708,9 +746,13 @@ oxh_status fix_diffusion_impl(oxh_batch *B, int *n_moved) {
 // to fill 256 CUs) and on the close / far neighbour counts.
 oxh_status autotune_forces(oxh_batch *B) {
 	oxh::Batch &b = B->b;
+	const char *edge_env = getenv("OXH_EDGE");
 	if(const char *env = getenv("OXH_FORCE_LANES")) {
 		b.lanes = atoi(env);
 		if(const char *f = getenv("OXH_FAR_LANES")) b.far_lanes = atoi(f);
+		b.edge = b.ecap > 0 && edge_env && atoi(edge_env) == 1;
+		if(const char *g = getenv("OXH_EDGE_GROUP")) b.edge_group = atoi(g);
+		if(b.ecap > 0 && !b.edge) TRY(alloc_edges(B, 0));
 		return OXH_OK;
 	}
 	// the workgroup engine fixes its own lanes per system
@@ -781,10 +823,49 @@ oxh_status autotune_forces(oxh_batch *B) {
 				bfb = bs;
 			}
 		}
-	(void) hipEventDestroy(e0);
-	(void) hipEventDestroy(e1);
 	b.far_lanes = bfl;
 	b.far_block = bfb;
+	// 3) the half-list form of the close pairs: particles per wave of the
+	// pair kernel x lanes per particle of the gather (far pairs inline, or
+	// in the far kernel picked above)
+	if(b.ecap > 0) {
+		const bool forced = edge_env && atoi(edge_env) == 1;
+		const int groups[] = {4, 8, 16, 32, 64};
+		float best_edge = 1e30f;
+		int el = 8, eg = 16, efl = bfl;
+		b.edge = true;
+		for(int fl : {0, bfl}) {
+			b.far_lanes = fl;
+			for(int G : groups)
+				for(int L : lanes) {
+					b.edge_group = G;
+					b.lanes = L;
+					float ms;
+					TRY(timeit(ms));
+					if(ms < best_edge) {
+						best_edge = ms;
+						el = L;
+						eg = G;
+						efl = fl;
+					}
+				}
+			if(bfl == 0) break;
+		}
+		if(forced || best_edge < best) {
+			b.lanes = el;
+			b.edge_group = eg;
+			b.far_lanes = efl;
+		}
+		else {
+			b.edge = false;
+			b.lanes = bl;
+			b.force_block = bb;
+			b.far_lanes = bfl;
+			TRY(alloc_edges(B, 0));
+		}
+	}
+	(void) hipEventDestroy(e0);
+	(void) hipEventDestroy(e1);
 	// forces of the current state with the chosen shapes
 	enqueue_forces(B);
 	return sync_and_check(B);
@@ -834,7 +915,7 @@ oxh_status oxh_batch_destroy(oxh_batch *B) {
 	void *ptrs[] = {b.pos, b.vel, b.Lm, b.q64, b.rlist, b.q32, b.bpos, b.F, b.T, b.bonds, b.pinfo, b.orig, b.pstep, b.nbr, b.nnbr,
 			b.cell_count, b.cell_items, b.cell_owner, b.pos32, b.d_sys, b.d_state, b.red_blocks, b.red_sys_blocks, b.red_partial, b.red_out, b.red_done,
 			b.esplit, b.stress, b.hbcount, b.keys, b.keys_alt, b.perm, b.perm_alt, b.inv_perm, b.radix_hist, b.scratch, B->bk_pos, B->bk_vel,
-			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state, B->bk_rlist, b.limit, b.any_rebuild, b.F64, b.T64, B->bk_F64, B->bk_T64};
+			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state, B->bk_rlist, b.limit, b.any_rebuild, b.F64, b.T64, B->bk_F64, B->bk_T64, b.eown, b.ein, b.ecount, b.E1, b.E2, b.E3, b.B1, b.B2, b.B3};
 	for(void *p : ptrs)
 		if(p) (void) hipFree(p);
 	if(b.stream) (void) hipStreamDestroy(b.stream);
@@ -1062,6 +1143,13 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 	}
 	if(const char *p = getenv("OXH_PERSIST")) B->persist &= atoi(p) != 0;
 	B->persist &= !b.fp64;   // the workgroup engine computes in the mixed form
+	// half-list form of the step's close pairs: grid-engine batches in the
+	// mixed form, timed against the full list at upload (OXH_EDGE=0 / 1
+	// forbids / forces it)
+	{
+		const char *e = getenv("OXH_EDGE");
+		if(!B->pool && !B->persist && !b.fp64 && !(e && atoi(e) == 0)) TRYB(alloc_edges(B, std::min(b.max_neigh, 64)));
+	}
 	TRYB(dalloc(B, &B->bk_rlist, N));
 	TRYB(dalloc(B, &b.any_rebuild, 1));
 	if(hipMemsetAsync(b.any_rebuild, 0, sizeof(int), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
@@ -1106,7 +1194,7 @@ oxh_status oxh_batch_update_lists(oxh_batch *B, int force) {
 		std::vector<oxh_sys_state> st;
 		TRY(read_states(B, st));
 		bool overflow = false;
-		for(auto &x : st) overflow |= (x.error & 6) != 0;
+		for(auto &x : st) overflow |= (x.error & oxh::ERR_OVERFLOW) != 0;
 		if(!overflow) return OXH_OK;
 		TRY(grow_lists(B, st));
 		force = 1;
@@ -1171,7 +1259,7 @@ oxh_status oxh_batch_run(oxh_batch *B, long long n_steps) {
 		bool overflow = false;
 		int bad = -1;
 		for(int s = 0; s < b.nsys; s++) {
-			if(st[s].error & 6) overflow = true;
+			if(st[s].error & oxh::ERR_OVERFLOW) overflow = true;
 			if((st[s].error & 1) && bad < 0) bad = s;
 		}
 		if(overflow) {
@@ -1621,7 +1709,7 @@ oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {  // 10 doubles
 	out6[5] = B->b.force_block;
 	out6[6] = B->b.far_lanes;
 	out6[7] = B->b.far_block;
-	out6[8] = B->persist ? 1.0 : 0.0;
+	out6[8] = B->persist ? 1.0 : (B->b.edge ? 2.0 : 0.0);
 	out6[9] = B->b.nsys;
 	return OXH_OK;
 }

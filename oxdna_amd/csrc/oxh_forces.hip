@@ -50,6 +50,172 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 			stress);
 }
 
+// Half-list form (b.edge), first half: every bond and every close pair
+// once, packed one item per lane.  Wave w takes particles [w P, w P + P):
+// lane prefix sums (shuffles) over the P particles' bond flags and owned
+// pair counts number the wave's items, bonds first, then the owned pairs in
+// particle and row order; each lane finds its item's particle by a binary
+// search over the prefix sums and evaluates it (edge_item).  Bond of
+// particle i -> B1..B3[i], owned pair m of i -> E1..E3[i * ecap + m].  Same
+// bookkeeping as k_forces inside a step.
+template<int P>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_edge_pairs(int N, const double4 *__restrict__ pos,
+		const double4 *__restrict__ q64, const float4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
+		const int *__restrict__ eown, const int2 *__restrict__ ecount, int ecap, const oxh_sys *__restrict__ systems,
+		oxh_sys_state *__restrict__ sstate, float4 *__restrict__ E1, float4 *__restrict__ E2, float4 *__restrict__ E3,
+		float4 *__restrict__ B1, float4 *__restrict__ B2, float4 *__restrict__ B3, int k, int *__restrict__ any) {
+	static_assert(P >= 1 && P <= 64 && (64 % P) == 0, "P must divide the wave");
+	const int lane = threadIdx.x & 63;
+	const int i0 = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * P;
+	if(k >= 0 && blockIdx.x == 0 && threadIdx.x == 0) *any = 0;
+	if(i0 >= N) return;   // whole waves
+	const int ii = i0 + lane;
+	int cb = 0, cp = 0;
+	if(lane < P && ii < N) {
+		const int s = pinfo[ii].y;
+		if(k >= 0 && ii == systems[s].first && sstate[s].need_rebuild) {
+			sstate[s].need_rebuild = 0;
+			sstate[s].n_rebuilds += 1;
+		}
+		cb = bonds[ii].x >= 0;
+		cp = ecount[ii].x;
+	}
+	int xb = cb, xp = cp;
+	for(int off = 1; off < P; off <<= 1) {
+		const int vb = __shfl_up(xb, off), vp = __shfl_up(xp, off);
+		if(lane >= off) {
+			xb += vb;
+			xp += vp;
+		}
+	}
+	const int eb = xb - cb, ep = xp - cp;   // exclusive prefix sums (lanes < P)
+	const int nb = __shfl(xb, P - 1), np = __shfl(xp, P - 1);
+	for(int g0 = 0; g0 < nb + np; g0 += 64) {
+		const int g = g0 + lane;
+		const int gp = g - nb;
+		// largest particle index with prefix <= item index (wave-uniform loop)
+		int lb = 0, lp = 0;
+		for(int step = P >> 1; step > 0; step >>= 1) {
+			if(__shfl(eb, lb + step) <= g) lb += step;
+			if(__shfl(ep, lp + step) <= gp) lp += step;
+		}
+		const int mp = gp - __shfl(ep, lp);
+		if(g < nb) {
+			const int i = i0 + lb;
+			const oxh_sys &S = systems[pinfo[i].y];
+			int err = 0;
+			float4 o1, o2, o3;
+			edge_item<true>(i, bonds[i].x, pos, q64, bpos, pinfo, S, o1, o2, o3, err);
+			B1[i] = o1;
+			B2[i] = o2;
+			B3[i] = o3;
+			if(err) atomicOr(&sstate[pinfo[i].y].error, ERR_BOND);
+		}
+		else if(g < nb + np) {
+			const int i = i0 + lp;
+			const size_t slot = (size_t) i * ecap + mp;
+			const oxh_sys &S = systems[pinfo[i].y];
+			int err = 0;
+			float4 o1, o2, o3;
+			edge_item<false>(i, eown[slot], pos, q64, bpos, pinfo, S, o1, o2, o3, err);
+			E1[slot] = o1;
+			E2[slot] = o2;
+			E3[slot] = o3;
+		}
+	}
+}
+
+// Half-list form, second half: particle i (K lanes) sums its side of its
+// owned pairs and of its bond to n3 and the partner side of its pairs with
+// j < i (slots in row ein[i]) and of its bond to n5, in a fixed order, adds
+// the far pairs (Debye-Hueckel, both directions) when FAR, and writes F
+// (.w energy share) and the body-frame torque (.w HB share).
+template<int K, bool FAR>
+__global__ void __launch_bounds__(256) k_edge_gather(int N, const double4 *__restrict__ pos, const double4 *__restrict__ q64,
+		const float4 *__restrict__ q32, const float4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
+		const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh, const oxh_sys *__restrict__ systems,
+		const int *__restrict__ ein, const int2 *__restrict__ ecount, int ecap, const float4 *__restrict__ E1, const float4 *__restrict__ E2,
+		const float4 *__restrict__ E3, const float4 *__restrict__ B1, const float4 *__restrict__ B2, const float4 *__restrict__ B3,
+		float4 *__restrict__ Fout, float4 *__restrict__ Tout) {
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	const int i = t / K;
+	const int l = t % K;
+	if(i >= N) return;
+	const int2 pi = pinfo[i];
+	const oxh_sys &S = systems[pi.y];
+	const int2 ec = ecount[i];
+	const size_t row = (size_t) i * ecap;
+	f3 F = {0, 0, 0}, T = {0, 0, 0};
+	float E = 0.f, Ehb = 0.f;
+	for(int m = l; m < ec.x; m += K) {
+		const float4 a = E1[row + m], b = E2[row + m];
+		F -= mk(a.x, a.y, a.z);
+		E += a.w;
+		T += mk(b.x, b.y, b.z);
+		Ehb += b.w;
+	}
+	for(int m = l; m < ec.y; m += K) {
+		const int slot = ein[row + m];
+		const float4 a = E1[slot], b = E2[slot], c = E3[slot];
+		F += mk(a.x, a.y, a.z);
+		E += a.w;
+		T += mk(c.x, c.y, c.z);
+		Ehb += b.w;
+	}
+	const int2 bd = bonds[i];
+	if(l == 0 && bd.x >= 0) {
+		const float4 a = B1[i], b = B2[i];
+		F -= mk(a.x, a.y, a.z);
+		E += a.w;
+		T += mk(b.x, b.y, b.z);
+	}
+	if(l == (K > 1 ? 1 : 0) && bd.y >= 0) {
+		const float4 a = B1[bd.y], c = B3[bd.y];
+		F += mk(a.x, a.y, a.z);
+		E += a.w;
+		T += mk(c.x, c.y, c.z);
+	}
+	Sites64 st;
+	const Frame fp = make_frame_d(q64[i], 0, st);
+	if(FAR) {
+		const double4 Pi = pos[i];
+		const float4 Bi = bpos[i];
+		const int nf = nnbr[i].y;
+		const int *rowf = nbr + (size_t) i * max_neigh + max_neigh - 1;
+		f3 Ff = {0, 0, 0};
+		float Ef = 0.f;
+		for(int m = l; m < nf; m += K) {
+			const int j = rowf[-m];
+			const float4 Bj = bpos[j];
+			f3 rb;
+			if(!S.small_box) rb = min_image_f(Bi, Bj, S);
+			else {
+				const Frame fq = make_frame(q32[j], 0);
+				rb = min_image(Pi, pos[j], S) + fq.back - fp.back;
+			}
+			f3 fq_ = {0, 0, 0};
+			Ef += debye_huckel(S, rb, Bi.w * Bj.w, fq_);
+			Ff += fq_;
+		}
+		F -= Ff;
+		T -= cross(fp.back, Ff);
+		E += 0.5f * Ef;
+	}
+	if(K > 1) {
+		F.x = group_sum(F.x, K);
+		F.y = group_sum(F.y, K);
+		F.z = group_sum(F.z, K);
+		T.x = group_sum(T.x, K);
+		T.y = group_sum(T.y, K);
+		T.z = group_sum(T.z, K);
+		E = group_sum(E, K);
+		Ehb = group_sum(Ehb, K);
+	}
+	if(l != 0) return;
+	Fout[i] = make_float4(F.x, F.y, F.z, E);
+	Tout[i] = make_float4(dot(fp.a1, T), dot(fp.a2, T), dot(fp.a3, T), Ehb);
+}
+
 // Far pairs on their own: Debye-Hueckel between backbone sites, few
 // registers, so many more waves stay resident to hide the gather latency.
 // Adds onto the F / T written by k_forces (same thread owns the
@@ -192,8 +358,48 @@ static void launch_far(hipStream_t st, const Batch &b, bool split, int block, in
 				nullptr, k, k >= 0 ? b.limit : nullptr, nullptr);
 }
 
+template<int P>
+static void launch_edge_pairs(hipStream_t st, const Batch &b, int k) {
+	const size_t waves = ((size_t) b.N + P - 1) / P;
+	dim3 grid((unsigned) ((waves + 3) / 4));
+	k_edge_pairs<P><<<grid, 256, 0, st>>>(b.N, b.pos, b.q64, b.bpos, b.bonds, b.pinfo, b.eown, b.ecount, b.ecap, b.d_sys, b.d_state, b.E1,
+			b.E2, b.E3, b.B1, b.B2, b.B3, k, b.any_rebuild);
+}
+
+template<int K>
+static void launch_edge_gather(hipStream_t st, const Batch &b, bool far_inline) {
+	dim3 grid((unsigned) (((size_t) b.N * K + 255) / 256));
+	if(far_inline)
+		k_edge_gather<K, true><<<grid, 256, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh, b.d_sys,
+				b.ein, b.ecount, b.ecap, b.E1, b.E2, b.E3, b.B1, b.B2, b.B3, b.F, b.T);
+	else
+		k_edge_gather<K, false><<<grid, 256, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh, b.d_sys,
+				b.ein, b.ecount, b.ecap, b.E1, b.E2, b.E3, b.B1, b.B2, b.B3, b.F, b.T);
+}
+
+// half-list form: b.edge_group particles per wave in the pair kernel,
+// b.lanes lanes per particle in the gather
+static void launch_edge(hipStream_t st, const Batch &b, bool far_inline, int k) {
+	switch(b.edge_group) {
+	case 4: launch_edge_pairs<4>(st, b, k); break;
+	case 8: launch_edge_pairs<8>(st, b, k); break;
+	case 32: launch_edge_pairs<32>(st, b, k); break;
+	case 64: launch_edge_pairs<64>(st, b, k); break;
+	default: launch_edge_pairs<16>(st, b, k); break;
+	}
+	switch(b.lanes) {
+	case 1: launch_edge_gather<1>(st, b, far_inline); break;
+	case 2: launch_edge_gather<2>(st, b, far_inline); break;
+	case 4: launch_edge_gather<4>(st, b, far_inline); break;
+	case 16: launch_edge_gather<16>(st, b, far_inline); break;
+	case 32: launch_edge_gather<32>(st, b, far_inline); break;
+	default: launch_edge_gather<8>(st, b, far_inline); break;
+	}
+}
+
 // close (+ bonded) kernel; the far pairs either inline (far_lanes == 0) or
-// in k_forces_far right after it
+// in k_forces_far right after it.  The step's forces (split = false) take
+// the half-list form when b.edge is set.
 void launch_forces(hipStream_t st, const Batch &b, bool split, int k) {
 	if(b.N == 0) return;
 	if(b.fp64) {
@@ -209,14 +415,17 @@ void launch_forces(hipStream_t st, const Batch &b, bool split, int k) {
 	}
 	const bool inl = b.far_lanes == 0;
 	int block = b.force_block;
-	switch(b.lanes) {
-	case 1: launch_k<1>(st, b, split, block, inl, k); break;
-	case 2: launch_k<2>(st, b, split, block, inl, k); break;
-	case 4: launch_k<4>(st, b, split, block, inl, k); break;
-	case 8: launch_k<8>(st, b, split, block, inl, k); break;
-	case 16: launch_k<16>(st, b, split, block, inl, k); break;
-	case 32: launch_k<32>(st, b, split, block, inl, k); break;
-	default: launch_k<64>(st, b, split, block, inl, k); break;
+	if(b.edge && !split) launch_edge(st, b, inl, k);
+	else {
+		switch(b.lanes) {
+		case 1: launch_k<1>(st, b, split, block, inl, k); break;
+		case 2: launch_k<2>(st, b, split, block, inl, k); break;
+		case 4: launch_k<4>(st, b, split, block, inl, k); break;
+		case 8: launch_k<8>(st, b, split, block, inl, k); break;
+		case 16: launch_k<16>(st, b, split, block, inl, k); break;
+		case 32: launch_k<32>(st, b, split, block, inl, k); break;
+		default: launch_k<64>(st, b, split, block, inl, k); break;
+		}
 	}
 	if(inl) return;
 	block = b.far_block;

@@ -41,6 +41,20 @@ struct Batch {
 	int far_lanes = 0;         // threads per nucleotide of the separate far-pair kernel (0 = far pairs inline)
 	int far_block = 256;
 
+	// edge (half-list) form of the close pairs, rebuilt with the lists when
+	// ecap > 0 and used by the step's force evaluation when edge is set.
+	// Row i (width ecap) of eown holds i's close neighbours j > i in list
+	// order: the pair (i, eown[i][m]) is evaluated once, into slot
+	// i * ecap + m of E1 (force on j, half energy), E2 (lab torque on i, half
+	// HB energy), E3 (lab torque on j).  Row i of ein holds the slots of i's
+	// pairs with j < i.  ecount[i] = (owned, incoming).  B1..B3[i]: the bond
+	// i -> n3 likewise.  edge_group: particles per wave of the pair kernel.
+	bool edge = false;
+	int ecap = 0, edge_group = 16;
+	int *eown = nullptr, *ein = nullptr;
+	int2 *ecount = nullptr;
+	float4 *E1 = nullptr, *E2 = nullptr, *E3 = nullptr, *B1 = nullptr, *B2 = nullptr, *B3 = nullptr;
+
 	// cells (systems with use_cells), fixed capacity per cell
 	int ncell_total = 0, cell_cap = 0;
 	int *cell_count = nullptr, *cell_items = nullptr, *cell_owner = nullptr;
@@ -92,6 +106,10 @@ struct Batch {
 	std::string last_error;
 };
 
+// oxh_sys_state.error bits
+constexpr int ERR_BOND = 1, ERR_LIST = 2, ERR_CELLS = 4, ERR_EDGE = 8;
+constexpr int ERR_OVERFLOW = ERR_LIST | ERR_CELLS | ERR_EDGE;   // fixed by growing a buffer and replaying
+
 constexpr int RED_CHUNK = 2048;
 constexpr int RED_W = 8;   // reduction width: U, Uhb, Kt, Kr, vx, vy, vz, count
 
@@ -118,6 +136,7 @@ void launch_forces(hipStream_t st, const Batch &b, bool split, int k = -1);
 // rebuild exactly the lists of the last build)
 void launch_cells(hipStream_t st, const Batch &b, const double4 *src);
 void launch_cells_fill_sort(hipStream_t st, const Batch &b);
+// (and the edge rows of rebuilt systems, when b.ecap > 0)
 void launch_build_list(hipStream_t st, const Batch &b, int force_all, const double4 *src);
 void launch_first_step(hipStream_t st, const Batch &b, int k);
 void launch_second_step(hipStream_t st, const Batch &b, int k);

@@ -50,7 +50,7 @@ __global__ void k_cells_fill(int N, const double4 *__restrict__ pos, const int2 
 			(float) (p.z - S.box[2] * floor(p.z * S.inv_box[2])), 0.f);
 	int slot = atomicAdd(&count[c], 1);
 	if(slot < cap) items[(size_t) c * cap + slot] = i;
-	else atomicOr(&st[s].error, 4);
+	else atomicOr(&st[s].error, ERR_CELLS);
 }
 
 // Sorts every rebuilt cell's slots by particle index: one wavefront per
@@ -118,6 +118,74 @@ __global__ void __launch_bounds__(256) k_build_list(int N, const double4 *__rest
 	}
 }
 
+// Edge rows of particle i, from its finished list row (the half-list form
+// of the close pairs; the reference's edge lists, src/CUDA/Lists/
+// CUDA_simple_verlet.cuh:97-175, hold each unordered pair once as well).
+// Owned pairs (j > i) are compacted in row order; for each pair with j < i
+// the slot that j writes is i's rank among j's owned pairs, i.e. the number
+// of j's close neighbours above j that come before i in j's row (rows are
+// symmetric: the pair predicate gives the same result from both ends).
+// Ballot compaction in row order, so the rows do not depend on timing.
+// Rows longer than ecap, or a pair missing from its partner's row, set
+// ERR_EDGE and leave the particle's edge rows empty.
+__global__ void __launch_bounds__(256) k_edge_rows(int N, const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems,
+		oxh_sys_state *__restrict__ st, const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh, int ecap,
+		int *__restrict__ eown, int *__restrict__ ein, int2 *__restrict__ ecount, int force_all, const int *__restrict__ any) {
+	constexpr int K = BUILD_LANES;
+	if(!force_all && !*any) return;
+	const size_t stride = (size_t) gridDim.x * blockDim.x;
+	const int g0 = (threadIdx.x & 63) & ~(K - 1);
+	const uint64_t gmask = ((1ull << K) - 1ull) << g0;
+	const uint64_t below = ((1ull << (threadIdx.x & 63)) - 1ull) & gmask;
+	for(size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x; t < (size_t) N * K; t += stride) {
+		const int i = (int) (t / K);
+		const int l = (int) (t % K);
+		const int s = pinfo[i].y;
+		if(!force_all && !st[s].need_rebuild) continue;
+		if(padding(i, systems[s])) continue;
+		const int nc = nnbr[i].x;
+		const int *row = nbr + (size_t) i * max_neigh;
+		int no = 0, ni = 0;
+		bool bad = false;
+		for(int m0 = 0; m0 < nc; m0 += K) {
+			const int m = m0 + l;
+			const int j = m < nc ? row[m] : -1;
+			const bool own = j > i, inc = j >= 0 && j < i;
+			int slot = -1;
+			if(inc) {
+				const int *rj = nbr + (size_t) j * max_neigh;
+				const int ncj = nnbr[j].x;
+				int rank = 0, q = 0;
+				for(; q < ncj; q++) {
+					const int x = rj[q];
+					if(x == i) break;
+					rank += x > j;
+				}
+				if(q < ncj && rank < ecap) slot = j * ecap + rank;
+				else bad = true;
+			}
+			const uint64_t bo = __ballot(own) & gmask, bi = __ballot(inc) & gmask;
+			if(own) {
+				const int k = no + __popcll(bo & below);
+				if(k < ecap) eown[(size_t) i * ecap + k] = j;
+			}
+			if(inc) {
+				const int k = ni + __popcll(bi & below);
+				if(k < ecap) ein[(size_t) i * ecap + k] = slot;
+			}
+			no += __popcll(bo);
+			ni += __popcll(bi);
+		}
+		bad = __ballot(bad) & gmask;
+		if(l != 0) continue;
+		if(bad || no > ecap || ni > ecap) {
+			atomicOr(&st[s].error, ERR_EDGE);
+			no = ni = 0;
+		}
+		ecount[i] = make_int2(no, ni);
+	}
+}
+
 // src = b.pos (the usual build) or b.rlist (the lists of the last build,
 // rebuilt exactly, e.g. after a capacity change); rlist is then left as is
 void launch_build_list(hipStream_t st, const Batch &b, int force_all, const double4 *src) {
@@ -126,6 +194,9 @@ void launch_build_list(hipStream_t st, const Batch &b, int force_all, const doub
 	const unsigned grid = (unsigned) std::min<size_t>((threads + bs - 1) / bs, 2048);
 	k_build_list<<<grid, bs, 0, st>>>(b.N, src, b.bonds, b.pinfo, b.d_sys, b.d_state, b.cell_count,
 			b.cell_items, b.cell_cap, b.nbr, b.nnbr, b.max_neigh, src == b.rlist ? nullptr : b.rlist, force_all, b.pos32, b.any_rebuild);
+	if(b.ecap > 0)
+		k_edge_rows<<<grid, bs, 0, st>>>(b.N, b.pinfo, b.d_sys, b.d_state, b.nbr, b.nnbr, b.max_neigh, b.ecap, b.eown, b.ein, b.ecount,
+				force_all, b.any_rebuild);
 }
 
 // ------------------------------------------------------------ Hilbert sort

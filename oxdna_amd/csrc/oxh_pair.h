@@ -54,5 +54,7 @@ using sp::make_frame;
 using sp::min_image;
 using sp::min_image_f;
 using sp::particle_forces;
+using sp::edge_item;
+using sp::make_frame_d;
 
 }  // namespace oxh

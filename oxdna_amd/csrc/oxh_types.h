@@ -61,7 +61,7 @@ typedef struct oxh_sys {
 typedef struct oxh_sys_state {
 	double K_t, K_r;          // Bussi thermostat internal kinetic energies
 	int need_rebuild;         // skin trigger
-	int error;                // bit 0: FENE bond broken, bit 1: list overflow, bit 2: cell overflow
+	int error;                // bit 0: FENE bond broken, bit 1: list overflow, bit 2: cell overflow, bit 3: edge-row overflow
 	long long n_rebuilds;
 } oxh_sys_state;
 

@@ -85,11 +85,14 @@ def test_deterministic(hip):
         assert np.array_equal(a, c)
 
 
-@pytest.mark.parametrize("lanes,far_lanes", [(1, 0), (8, 0), (4, 4), (8, 16), (32, 1)])
+@pytest.mark.parametrize("lanes,far_lanes,edge", [(1, 0, 0), (8, 0, 0), (4, 4, 0), (8, 16, 0), (32, 1, 0), (2, 0, 1), (8, 4, 1)])
 @pytest.mark.parametrize("name", ["bundle_md_30C", "tangle3_nohalf_dh", "gas8_25C_0.5M"])
-def test_launch_shapes(hip, monkeypatch, name, lanes, far_lanes):
+def test_launch_shapes(hip, monkeypatch, name, lanes, far_lanes, edge):
     """Every launch shape the autotuner may pick gives the same physics:
-    far pairs inline or in their own kernel, any lanes per nucleotide."""
+    far pairs inline or in their own kernel, any lanes per nucleotide, the
+    close pairs in full-list or half-list form (grid engine)."""
+    monkeypatch.setenv("OXH_PERSIST", "0")
+    monkeypatch.setenv("OXH_EDGE", str(edge))
     monkeypatch.setenv("OXH_FORCE_LANES", str(lanes))
     monkeypatch.setenv("OXH_FAR_LANES", str(far_lanes))
     s, opts, ref = load_golden(name)
@@ -99,7 +102,76 @@ def test_launch_shapes(hip, monkeypatch, name, lanes, far_lanes):
         e8 = b.energy_split()
         st = b.stats()
     assert st["lanes"] == lanes and st["far_lanes"] == far_lanes
+    assert st["engine"] == ("grid-edge" if edge else "grid")
     assert vec_rel_err(F, ref["F"])[0] <= F_TOL
     assert vec_rel_err(T, ref["T"])[0] <= F_TOL
     scale = np.abs(ref["E8"]).sum() + 1e-6 * s.N
     assert np.all(np.abs(e8 - ref["E8"]) <= 1e-5 * scale)
+
+
+@pytest.mark.parametrize("lanes,group,far_lanes", [(1, 4, 0), (4, 16, 0), (16, 64, 0), (2, 8, 4)])
+@pytest.mark.parametrize("name", golden_names())
+def test_edge_forces_vs_reference(hip, monkeypatch, name, lanes, group, far_lanes):
+    """Half-list form of the step's forces (each close pair and bond
+    evaluated once, one per lane, `group` particles per wave; the partner's
+    torque from t_q = -t_p - r x F_q; partner shares gathered in a fixed
+    order with `lanes` lanes per particle; far pairs in the gather or in
+    their own kernel): the same bar as the full list, and the same
+    per-particle energies."""
+    monkeypatch.setenv("OXH_PERSIST", "0")
+    monkeypatch.setenv("OXH_EDGE", "1")
+    monkeypatch.setenv("OXH_EDGE_GROUP", str(group))
+    monkeypatch.setenv("OXH_FORCE_LANES", str(lanes))
+    monkeypatch.setenv("OXH_FAR_LANES", str(far_lanes))
+    s, opts, ref = load_golden(name)
+    with hip.HIPBatch(s, **opts) as b:
+        assert b.stats()["engine"] == "grid-edge"
+        b.compute_forces()
+        F, T, Ep = b.forces()
+    monkeypatch.setenv("OXH_EDGE", "0")
+    with hip.HIPBatch(s, **opts) as b:
+        assert b.stats()["engine"] == "grid"
+        _, _, Ep_full = b.forces()
+    ef_big, ef_pp = vec_rel_err(F, ref["F"])
+    et_big, et_pp = vec_rel_err(T, ref["T"])
+    print(f"{name} K={lanes} P={group} far={far_lanes}: edge F {ef_big:.2e}/{ef_pp:.2e} T {et_big:.2e}/{et_pp:.2e}")
+    assert ef_big <= F_TOL and et_big <= F_TOL
+    assert ef_pp <= F_TOL_PER_PARTICLE and et_pp <= F_TOL_PER_PARTICLE
+    scale = np.abs(ref["E8"]).sum() + 1e-6 * s.N
+    assert abs(Ep.sum() - ref["E8"].sum()) <= 1e-5 * scale
+    assert np.abs(Ep - Ep_full).max() <= 1e-5 * (np.abs(Ep_full).max() + 1e-6)
+
+
+def test_edge_run_with_rebuilds(hip, monkeypatch):
+    """A 1,536-nt duplex gas (cell lists) run 400 Langevin steps in the
+    half-list form, with list rebuilds and Hilbert re-sorts in between (the
+    edge rows are rebuilt with the lists): forces of the final state match
+    the oracle, and the run matches the full-list run's energy to the
+    thermal noise."""
+    from oracle import pyoracle as po
+    from oxdna_amd import systems
+    monkeypatch.setenv("OXH_PERSIST", "0")
+    s = systems.duplex_gas(3, n_duplex=64, bp=12, density=2e-3)
+    out = {}
+    for edge in (1, 0):
+        monkeypatch.setenv("OXH_EDGE", str(edge))
+        with hip.HIPBatch(s, thermostat="langevin", diff_coeff=2.5, dt=0.003, verlet_skin=0.05, seed=9) as b:
+            assert b.stats()["engine"] == ("grid-edge" if edge else "grid")
+            b.set_sort_interval(100)
+            b.run(400)
+            st = b.state()
+            F, T, Ep = b.forces()
+            nreb, err = b.info()
+        out[edge] = (st, F, T, Ep, nreb, err)
+    st, F, T, Ep, nreb, err = out[1]
+    assert err == 0 and nreb >= 5
+    t = s.copy()
+    t.pos, t.a1, t.a3 = st["pos"], st["a1"], st["a3"]
+    ref = po.forces(po.Params(T=s.T, salt=s.salt), t)
+    ef, et = vec_rel_err(F, ref["F"])[0], vec_rel_err(T, ref["T"])[0]
+    print(f"edge run: {nreb} rebuilds, F {ef:.2e} T {et:.2e}")
+    assert ef <= F_TOL and et <= F_TOL
+    assert abs(Ep.sum() - ref["U"]) <= 1e-5 * np.abs(ref["Epart"]).sum()
+    # both forms sample the same ensemble: U/N within a few thermal widths
+    u_edge, u_full = Ep.sum() / s.N, out[0][3].sum() / s.N
+    assert abs(u_edge - u_full) <= 0.05, (u_edge, u_full)
