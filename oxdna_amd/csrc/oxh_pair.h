@@ -32,6 +32,15 @@ using out4 = float4;
 constexpr bool FP64_TERMS = false;
 __device__ __forceinline__ real pick(float f, double) { return f; }
 __device__ __forceinline__ out4 make_out4(real x, real y, real z, real w) { return make_float4(x, y, z, w); }
+// |r| and 1 / |r| from r^2 by the hardware reciprocal square root (~1 ulp),
+// and e^x by the hardware base-2 exponential (relative error ~|x| ulp):
+// Debye-Hueckel's far pairs outnumber the close ones tenfold in dense
+// systems, and these replace a correctly rounded sqrt, divide and exp
+__device__ __forceinline__ void norm_inv(real r2, real &r, real &inv) {
+	inv = rsqrtf(r2);
+	r = r2 * inv;
+}
+__device__ __forceinline__ real exp_dh(real x) { return __expf(x); }
 #include "oxh_terms.inc"
 }  // namespace sp
 
@@ -42,6 +51,11 @@ using out4 = double4;
 constexpr bool FP64_TERMS = true;
 __device__ __forceinline__ real pick(float, double d) { return d; }
 __device__ __forceinline__ out4 make_out4(real x, real y, real z, real w) { return make_double4(x, y, z, w); }
+__device__ __forceinline__ void norm_inv(real r2, real &r, real &inv) {
+	r = sqrt(r2);
+	inv = real(1.) / r;
+}
+__device__ __forceinline__ real exp_dh(real x) { return exp(x); }
 #include "oxh_terms.inc"
 }  // namespace dp
 
