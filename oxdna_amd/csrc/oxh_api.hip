@@ -313,6 +313,24 @@ oxh_status alloc_edges(oxh_batch *B, int ecap) {
 	return OXH_OK;
 }
 
+// (re)allocates both cell slot buffers (binning order and sorted) for
+// `cap` slots per cell: every kernel indexes both with b.cell_cap
+oxh_status alloc_cell_slots(oxh_batch *B, int cap) {
+	oxh::Batch &b = B->b;
+	HIPCHK(hipStreamSynchronize(b.stream));
+	const size_t ncell = (size_t) std::max(b.ncell_total, 1);
+	if(b.cell_items) {
+		HIPCHK(hipFree(b.cell_items));
+		HIPCHK(hipFree(b.cell_raw));
+		B->device_bytes -= 2 * ncell * b.cell_cap * sizeof(int);
+		b.cell_items = b.cell_raw = nullptr;
+	}
+	b.cell_cap = cap;
+	TRY(dalloc(B, &b.cell_items, ncell * cap));
+	TRY(dalloc(B, &b.cell_raw, ncell * cap));
+	return OXH_OK;
+}
+
 // doubles the neighbour, cell or edge-row capacity after an overflow and
 // clears the overflow bits (the caller rebuilds)
 oxh_status grow_lists(oxh_batch *B, const std::vector<oxh_sys_state> &st) {
@@ -331,12 +349,7 @@ oxh_status grow_lists(oxh_batch *B, const std::vector<oxh_sys_state> &st) {
 		b.max_neigh = std::min(std::max(b.max_neigh * 2, 8), std::max(b.N, 1));
 		TRY(dalloc(B, &b.nbr, (size_t) b.nstride * b.max_neigh));
 	}
-	if(grow_cells && b.ncell_total) {
-		HIPCHK(hipFree(b.cell_items));
-		B->device_bytes -= (size_t) b.ncell_total * b.cell_cap * sizeof(int);
-		b.cell_cap = b.cell_cap * 2;
-		TRY(dalloc(B, &b.cell_items, (size_t) b.ncell_total * b.cell_cap));
-	}
+	if(grow_cells && b.ncell_total) TRY(alloc_cell_slots(B, b.cell_cap * 2));
 	if(grow_edge && b.ecap > 0) TRY(alloc_edges(B, std::min(2 * b.ecap, b.max_neigh)));
 	k_clear_err<<<(b.nsys + 63) / 64, 64, 0, b.stream>>>(b.nsys, b.d_state, oxh::ERR_OVERFLOW);
 	if(b.graph_exec) {
@@ -915,7 +928,7 @@ oxh_status oxh_batch_destroy(oxh_batch *B) {
 	void *ptrs[] = {b.pos, b.vel, b.Lm, b.q64, b.rlist, b.q32, b.bpos, b.F, b.T, b.bonds, b.pinfo, b.orig, b.pstep, b.nbr, b.nnbr,
 			b.cell_count, b.cell_items, b.cell_owner, b.pos32, b.d_sys, b.d_state, b.red_blocks, b.red_sys_blocks, b.red_partial, b.red_out, b.red_done,
 			b.esplit, b.stress, b.hbcount, b.keys, b.keys_alt, b.perm, b.perm_alt, b.inv_perm, b.radix_hist, b.scratch, B->bk_pos, B->bk_vel,
-			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state, B->bk_rlist, b.limit, b.any_rebuild, b.F64, b.T64, B->bk_F64, B->bk_T64, b.eown, b.ein, b.ecount, b.E1, b.E2, b.E3, b.B1, b.B2, b.B3};
+			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state, B->bk_rlist, b.limit, b.any_rebuild, b.F64, b.T64, B->bk_F64, B->bk_T64, b.eown, b.ein, b.ecount, b.E1, b.E2, b.E3, b.B1, b.B2, b.B3, b.cell_raw, b.cell_of};
 	for(void *p : ptrs)
 		if(p) (void) hipFree(p);
 	if(b.stream) (void) hipStreamDestroy(b.stream);
@@ -1026,7 +1039,8 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 	TRYB(dalloc(B, &b.cell_count, std::max(ncell, 1)));
 	TRYB(dalloc(B, &b.cell_owner, std::max(ncell, 1)));
 	TRYB(dalloc(B, &b.pos32, N));
-	TRYB(dalloc(B, &b.cell_items, (size_t) std::max(ncell, 1) * b.cell_cap));
+	TRYB(alloc_cell_slots(B, b.cell_cap));
+	TRYB(dalloc(B, &b.cell_of, N));
 	TRYB(dalloc(B, &b.d_sys, n_sys));
 	TRYB(dalloc(B, &b.d_state, n_sys));
 	TRYB(dalloc(B, &b.esplit, (size_t) 8 * N));
@@ -1363,11 +1377,7 @@ oxh_status oxh_pool_admit(oxh_batch *B, int slot, const oxh_system_desc *desc, l
 	// other slots' lists exactly (from their last build positions)
 	bool regrow = false;
 	if(S.use_cells && 3 * max_occ > b.cell_cap) {
-		HIPCHK(hipStreamSynchronize(st));
-		HIPCHK(hipFree(b.cell_items));
-		B->device_bytes -= (size_t) b.ncell_total * b.cell_cap * sizeof(int);
-		b.cell_cap = 3 * max_occ;
-		TRY(dalloc(B, &b.cell_items, (size_t) b.ncell_total * b.cell_cap));
+		TRY(alloc_cell_slots(B, 3 * max_occ));
 		regrow = true;
 	}
 	if((int) std::ceil(est) > b.max_neigh) {

@@ -57,7 +57,9 @@ struct Batch {
 
 	// cells (systems with use_cells), fixed capacity per cell
 	int ncell_total = 0, cell_cap = 0;
-	int *cell_count = nullptr, *cell_items = nullptr, *cell_owner = nullptr;
+	// (cell_raw: slots in binning order; cell_items: the same sorted by
+	// particle index; cell_of: a binned particle's cell)
+	int *cell_count = nullptr, *cell_items = nullptr, *cell_owner = nullptr, *cell_raw = nullptr, *cell_of = nullptr;
 	float4 *pos32 = nullptr;   // wrapped FP32 centres, written by the cell fill of a rebuild
 
 	// per-system parameters and device-written state

@@ -23,7 +23,7 @@ __device__ __forceinline__ double sqr_dist(double4 a, double4 b, const oxh_sys &
 	return fma(dz, dz, fma(dy, dy, dx * dx));
 }
 
-constexpr int BUILD_LANES = 16;
+constexpr int BUILD_LANES = 32;
 
 // BUILD_LANES threads per particle share the candidate scan: in every chunk
 // each lane tests one candidate, a wave ballot gives every passing lane its
@@ -32,6 +32,13 @@ constexpr int BUILD_LANES = 16;
 // the back.  A pair cannot come closer than R_CLOSE before the next rebuild
 // unless it is within R_CLOSE + 2 skin now, so far pairs only ever need
 // Debye-Hueckel.
+// With cells and K >= 27 lanes, lane k < 27 takes neighbour cell k (dz, dy,
+// dx = k / 9 - 1, k / 3 % 3 - 1, k % 3 - 1), a lane prefix sum of the
+// cells' occupancies numbers the candidates (cells in that order, each cell
+// in slot order, its items sorted by particle index), and the group scans
+// them in chunks of K, each lane finding its candidate's cell by a binary
+// search over the prefix sums: the 27 cells' loads overlap instead of
+// following each other.
 // Every lane of the group must call it; lane 0 writes the row's counts and
 // the list position.
 template<int K>
@@ -48,78 +55,95 @@ __device__ __forceinline__ void build_row(int i, int l, int s, const double4 *__
 	const uint64_t gmask = (K == 64) ? ~0ull : (((1ull << K) - 1ull) << g0);
 	const uint64_t below = ((1ull << (threadIdx.x & 63)) - 1ull) & gmask;
 	int nc = 0, nf = 0;
-	// tests candidates base[0 .. n) in chunks of K
-	auto scan = [&](const int *cand, int first, int n) {
-		for(int m0 = 0; m0 < n; m0 += K) {
-			const int m = m0 + l;
-			bool close = false, far = false;
-			int j = -1;
-			if(m < n) {
-				j = cand ? cand[m] : first + m;
-				if(j != i && j != bd.x && j != bd.y) {
-					if(cand) {
-						// FP32 decides unless within its error bound of a threshold
-						const float4 q = pos32[j];
-						float dx = q.x - p32.x, dy = q.y - p32.y, dz = q.z - p32.z;
-						dx -= S.boxf[0] * rintf(dx * S.inv_boxf[0]);
-						dy -= S.boxf[1] * rintf(dy * S.inv_boxf[1]);
-						dz -= S.boxf[2] * rintf(dz * S.inv_boxf[2]);
-						const float r2f = dx * dx + dy * dy + dz * dz;
-						if(fabsf(r2f - S.rv2f) < S.eps2 || fabsf(r2f - S.rclose2f) < S.eps2) {
-							double r2 = sqr_dist(p, pos[j], S);
-							if(r2 < S.rv2) {
-								close = r2 < S.rclose2;
-								far = !close;
-							}
-						}
-						else if(r2f < S.rv2f) {
-							close = r2f < S.rclose2f;
-							far = !close;
-						}
-					}
-					else {
-						double r2 = sqr_dist(p, pos[j], S);
-						if(r2 < S.rv2) {
-							close = r2 < S.rclose2;
-							far = !close;
-						}
+	// one chunk: lane tests candidate j (-1: none); from_cells selects the
+	// FP32 prefilter on wrapped centres
+	auto test = [&](int j, bool from_cells) {
+		bool close = false, far = false;
+		if(j >= 0 && j != i && j != bd.x && j != bd.y) {
+			if(from_cells) {
+				// FP32 decides unless within its error bound of a threshold
+				const float4 q = pos32[j];
+				float dx = q.x - p32.x, dy = q.y - p32.y, dz = q.z - p32.z;
+				dx -= S.boxf[0] * rintf(dx * S.inv_boxf[0]);
+				dy -= S.boxf[1] * rintf(dy * S.inv_boxf[1]);
+				dz -= S.boxf[2] * rintf(dz * S.inv_boxf[2]);
+				const float r2f = dx * dx + dy * dy + dz * dz;
+				if(fabsf(r2f - S.rv2f) < S.eps2 || fabsf(r2f - S.rclose2f) < S.eps2) {
+					double r2 = sqr_dist(p, pos[j], S);
+					if(r2 < S.rv2) {
+						close = r2 < S.rclose2;
+						far = !close;
 					}
 				}
+				else if(r2f < S.rv2f) {
+					close = r2f < S.rclose2f;
+					far = !close;
+				}
 			}
-			const uint64_t bc = __ballot(close) & gmask, bf = __ballot(far) & gmask;
-			if(close) {
-				int k = nc + __popcll(bc & below);
-				if(k + nf < max_neigh) row[k] = j;
+			else {
+				double r2 = sqr_dist(p, pos[j], S);
+				if(r2 < S.rv2) {
+					close = r2 < S.rclose2;
+					far = !close;
+				}
 			}
-			if(far) {
-				int k = nf + __popcll(bf & below);
-				if(nc + k < max_neigh) row[max_neigh - 1 - k] = j;
-			}
-			nc += __popcll(bc);
-			nf += __popcll(bf);
 		}
+		const uint64_t bc = __ballot(close) & gmask, bf = __ballot(far) & gmask;
+		if(close) {
+			int k = nc + __popcll(bc & below);
+			if(k + nf < max_neigh) row[k] = j;
+		}
+		if(far) {
+			int k = nf + __popcll(bf & below);
+			if(nc + k < max_neigh) row[max_neigh - 1 - k] = j;
+		}
+		nc += __popcll(bc);
+		nf += __popcll(bf);
 	};
 	if(S.use_cells) {
-		int cx = cell_coord(p.x, S.inv_box[0], S.nc[0]);
-		int cy = cell_coord(p.y, S.inv_box[1], S.nc[1]);
-		int cz = cell_coord(p.z, S.inv_box[2], S.nc[2]);
-		for(int dz = -1; dz <= 1; dz++) {
-			int z = cz + dz;
+		const int cx = cell_coord(p.x, S.inv_box[0], S.nc[0]);
+		const int cy = cell_coord(p.y, S.inv_box[1], S.nc[1]);
+		const int cz = cell_coord(p.z, S.inv_box[2], S.nc[2]);
+		auto cell_of = [&](int k) {
+			int z = cz + k / 9 - 1, y = cy + (k / 3) % 3 - 1, x = cx + k % 3 - 1;
 			z += (z < 0) ? S.nc[2] : (z >= S.nc[2] ? -S.nc[2] : 0);
-			for(int dy = -1; dy <= 1; dy++) {
-				int y = cy + dy;
-				y += (y < 0) ? S.nc[1] : (y >= S.nc[1] ? -S.nc[1] : 0);
-				for(int dx = -1; dx <= 1; dx++) {
-					int x = cx + dx;
-					x += (x < 0) ? S.nc[0] : (x >= S.nc[0] ? -S.nc[0] : 0);
-					int c = S.cell_first + (z * S.nc[1] + y) * S.nc[0] + x;
-					scan(items + (size_t) c * cap, 0, min(count[c], cap));
-				}
+			y += (y < 0) ? S.nc[1] : (y >= S.nc[1] ? -S.nc[1] : 0);
+			x += (x < 0) ? S.nc[0] : (x >= S.nc[0] ? -S.nc[0] : 0);
+			return S.cell_first + (z * S.nc[1] + y) * S.nc[0] + x;
+		};
+		if constexpr(K >= 27) {
+			int cell = 0, cnt = 0;
+			if(l < 27) {
+				cell = cell_of(l);
+				cnt = min(count[cell], cap);
+			}
+			int incl = cnt;
+			for(int off = 1; off < K; off <<= 1) {
+				const int v = __shfl_up(incl, off, K);
+				if(l >= off) incl += v;
+			}
+			const int excl = incl - cnt;
+			const int total = __shfl(incl, K - 1, K);
+			for(int m0 = 0; m0 < total; m0 += K) {
+				const int g = m0 + l;
+				int lo = 0;
+				for(int step = K >> 1; step > 0; step >>= 1)
+					if(__shfl(excl, lo + step, K) <= g) lo += step;
+				const int c = __shfl(cell, lo, K);
+				const int slot = g - __shfl(excl, lo, K);
+				test(g < total ? items[(size_t) c * cap + slot] : -1, true);
+			}
+		}
+		else {
+			for(int k = 0; k < 27; k++) {
+				const int c = cell_of(k);
+				const int n = min(count[c], cap);
+				for(int m0 = 0; m0 < n; m0 += K) test(m0 + l < n ? items[(size_t) c * cap + m0 + l] : -1, true);
 			}
 		}
 	}
 	else {
-		scan(nullptr, S.first, S.n);
+		for(int m0 = 0; m0 < S.n; m0 += K) test(m0 + l < S.n ? S.first + m0 + l : -1, false);
 	}
 	if(l != 0) return;
 	if(nc + nf > max_neigh) {

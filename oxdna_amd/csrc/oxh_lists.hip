@@ -31,9 +31,11 @@ __global__ void k_cells_clear(int ncell, const int *__restrict__ owner, const ox
 	if(st[owner[c]].need_rebuild) count[c] = 0;
 }
 
+// Bins flagged systems' particles: cell slots in arrival order (raw), the
+// particle's cell in cell_of, its wrapped FP32 centre in pos32.
 __global__ void k_cells_fill(int N, const double4 *__restrict__ pos, const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems,
-		oxh_sys_state *__restrict__ st, int *__restrict__ count, int *__restrict__ items, int cap, float4 *__restrict__ pos32,
-		const int *__restrict__ any) {
+		oxh_sys_state *__restrict__ st, int *__restrict__ count, int *__restrict__ raw, int *__restrict__ cell_of, int cap,
+		float4 *__restrict__ pos32, const int *__restrict__ any) {
 	if(!*any) return;
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= N) return;
@@ -48,54 +50,56 @@ __global__ void k_cells_fill(int N, const double4 *__restrict__ pos, const int2 
 	int c = S.cell_first + (cz * S.nc[1] + cy) * S.nc[0] + cx;
 	pos32[i] = make_float4((float) (p.x - S.box[0] * floor(p.x * S.inv_box[0])), (float) (p.y - S.box[1] * floor(p.y * S.inv_box[1])),
 			(float) (p.z - S.box[2] * floor(p.z * S.inv_box[2])), 0.f);
+	cell_of[i] = c;
 	int slot = atomicAdd(&count[c], 1);
-	if(slot < cap) items[(size_t) c * cap + slot] = i;
+	if(slot < cap) raw[(size_t) c * cap + slot] = i;
 	else atomicOr(&st[s].error, ERR_CELLS);
 }
 
-// Sorts every rebuilt cell's slots by particle index: one wavefront per
-// cell (grid-stride over cells), each lane ranks its slots against the
-// whole cell staged in LDS (indices are distinct, so rank = number of
-// smaller indices).
-__global__ void __launch_bounds__(64) k_cells_sort(int ncell, const int *__restrict__ owner, const oxh_sys_state *__restrict__ st,
-		const int *__restrict__ count, int *__restrict__ items, int cap, const int *__restrict__ any) {
-	extern __shared__ int cell_buf[];
+// Puts every binned particle at its rank in its cell: items[c][rank] with
+// rank = number of particles of the cell with a smaller index, so a cell's
+// items are sorted by particle index whatever order the atomics gave.
+__global__ void k_cells_rank(int N, const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems,
+		const oxh_sys_state *__restrict__ st, const int *__restrict__ count, const int *__restrict__ raw, const int *__restrict__ cell_of,
+		int *__restrict__ items, int cap, const int *__restrict__ any) {
 	if(!*any) return;
-	for(int c = blockIdx.x; c < ncell; c += gridDim.x) {
-		if(!st[owner[c]].need_rebuild) continue;
-		const int n = min(count[c], cap);
-		if(n <= 1) continue;
-		int *it = items + (size_t) c * cap;
-		for(int k = threadIdx.x; k < n; k += 64) cell_buf[k] = it[k];
-		__syncthreads();
-		for(int k = threadIdx.x; k < n; k += 64) {
-			const int v = cell_buf[k];
-			int rank = 0;
-			for(int m = 0; m < n; m++) rank += cell_buf[m] < v;
-			it[rank] = v;
-		}
-		__syncthreads();
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= N) return;
+	int s = pinfo[i].y;
+	if(!st[s].need_rebuild) return;
+	const oxh_sys &S = systems[s];
+	if(!S.use_cells || padding(i, S)) return;
+	const int c = cell_of[i];
+	const int n = min(count[c], cap);
+	const int *r = raw + (size_t) c * cap;
+	int rank = 0;
+	bool stored = false;
+	for(int m = 0; m < n; m++) {
+		const int v = r[m];
+		rank += v < i;
+		stored |= v == i;
 	}
+	if(stored) items[(size_t) c * cap + rank] = i;
 }
 
-// fill + sort of flagged systems' cells; the counters must be zero
+// fill + rank of flagged systems' cells; the counters must be zero
 void launch_cells_fill_sort(hipStream_t st, const Batch &b) {
 	if(!b.any_cells) return;
 	int bs = 256;
-	k_cells_fill<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, b.pos, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_items, b.cell_cap,
-			b.pos32, b.any_rebuild);
-	k_cells_sort<<<std::min(b.ncell_total, 1024), 64, (size_t) b.cell_cap * sizeof(int), st>>>(b.ncell_total, b.cell_owner, b.d_state,
-			b.cell_count, b.cell_items, b.cell_cap, b.any_rebuild);
+	k_cells_fill<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, b.pos, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_raw, b.cell_of,
+			b.cell_cap, b.pos32, b.any_rebuild);
+	k_cells_rank<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_raw, b.cell_of, b.cell_items,
+			b.cell_cap, b.any_rebuild);
 }
 
 void launch_cells(hipStream_t st, const Batch &b, const double4 *src) {
 	if(!b.any_cells) return;
 	int bs = 256;
 	k_cells_clear<<<(b.ncell_total + bs - 1) / bs, bs, 0, st>>>(b.ncell_total, b.cell_owner, b.d_state, b.cell_count);
-	k_cells_fill<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, src, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_items, b.cell_cap,
-			b.pos32, b.any_rebuild);
-	k_cells_sort<<<std::min(b.ncell_total, 1024), 64, (size_t) b.cell_cap * sizeof(int), st>>>(b.ncell_total, b.cell_owner, b.d_state,
-			b.cell_count, b.cell_items, b.cell_cap, b.any_rebuild);
+	k_cells_fill<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, src, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_raw, b.cell_of,
+			b.cell_cap, b.pos32, b.any_rebuild);
+	k_cells_rank<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_raw, b.cell_of, b.cell_items,
+			b.cell_cap, b.any_rebuild);
 }
 
 __global__ void __launch_bounds__(256) k_build_list(int N, const double4 *__restrict__ pos, const int2 *__restrict__ bonds,
