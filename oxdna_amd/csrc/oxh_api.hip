@@ -543,27 +543,46 @@ oxh_status sys_setup(oxh_batch *B, const oxh_system_desc &d, int s, oxh_sys &S, 
 void sys_cells(const oxh_system_desc &d, oxh_sys &S, double rv, int cell_first, int max_cells, int &max_occ,
 		double &max_neigh_est) {
 	double V = d.box[0] * d.box[1] * d.box[2];
-	double maxf = std::pow(2.0 * d.N / V, 1.0 / 3.0);
-	int nc[3];
-	bool cells_ok = d.N > 1024;
-	for(int k = 0; k < 3; k++) {
-		nc[k] = (int) std::floor(d.box[k] / rv);
-		nc[k] = std::min(nc[k], (int) std::ceil(maxf * d.box[k]));
-	}
-	if(max_cells > 0) {
-		while((long long) nc[0] * nc[1] * nc[2] > max_cells) {
-			int kmax = (nc[0] >= nc[1] && nc[0] >= nc[2]) ? 0 : (nc[1] >= nc[2] ? 1 : 2);
-			nc[kmax]--;
+	// half-size cells (edges >= r_verlet / 2, stencil +-2, at most 16 N
+	// cells, at least 5 per side) or full-size ones (edges >= r_verlet,
+	// stencil +-1, at most 2 N cells, at least 3 per side): half-size cells
+	// where they hold 2 or more particles on average (fewer candidates per
+	// particle), or where full-size ones do not fit
+	int ncs[3][3];
+	bool ok[3] = {false, false, false};
+	long long ncell_of[3] = {0, 0, 0};
+	for(int st = 1; st <= 2 && d.N > 1024; st++) {
+		const double maxf = std::pow((st == 2 ? 16.0 : 2.0) * d.N / V, 1.0 / 3.0);
+		int *nc = ncs[st];
+		for(int k = 0; k < 3; k++) {
+			nc[k] = (int) std::floor(d.box[k] * st / rv);
+			nc[k] = std::min(nc[k], (int) std::ceil(maxf * d.box[k]));
 		}
+		if(max_cells > 0) {
+			while((long long) nc[0] * nc[1] * nc[2] > max_cells) {
+				int kmax = (nc[0] >= nc[1] && nc[0] >= nc[2]) ? 0 : (nc[1] >= nc[2] ? 1 : 2);
+				nc[kmax]--;
+			}
+		}
+		ok[st] = nc[0] >= 2 * st + 1 && nc[1] >= 2 * st + 1 && nc[2] >= 2 * st + 1;
+		ncell_of[st] = (long long) nc[0] * nc[1] * nc[2];
 	}
-	for(int k = 0; k < 3; k++)
-		if(nc[k] < 3) cells_ok = false;
+	int stencil = 0;
+	if(ok[2] && (!ok[1] || d.N >= 2 * ncell_of[2])) stencil = 2;
+	else if(ok[1]) stencil = 1;
+	const bool cells_ok = stencil > 0;
+	const int *nc = ncs[stencil];
 	S.nc[0] = S.nc[1] = S.nc[2] = 0;
 	S.cell_first = cell_first;
 	S.ncell = 0;
+	S.stencil = 0;
 	if(cells_ok) {
 		S.use_cells = 1;
-		for(int k = 0; k < 3; k++) S.nc[k] = nc[k];
+		S.stencil = stencil;
+		for(int k = 0; k < 3; k++) {
+			S.nc[k] = nc[k];
+			S.cell_w[k] = (float) (d.box[k] / nc[k]);
+		}
 		S.ncell = nc[0] * nc[1] * nc[2];
 		// occupancy of the initial configuration
 		std::vector<int> occ(S.ncell, 0);
@@ -577,8 +596,11 @@ void sys_cells(const oxh_system_desc &d, oxh_sys &S, double rv, int cell_first, 
 			int idx = (c[2] * nc[1] + c[1]) * nc[0] + c[0];
 			max_occ = std::max(max_occ, ++occ[idx]);
 		}
+		// neighbours: the list sphere at a multiple of the fullest cell's
+		// density (3x for full-size cells, 1.5x for the noisier half-size
+		// ones); an underestimate only costs a regrow
 		double cellvol = V / S.ncell;
-		double est = 4.0 / 3.0 * M_PI * rv * rv * rv * (3.0 * max_occ / cellvol);
+		double est = 4.0 / 3.0 * M_PI * rv * rv * rv * ((stencil == 2 ? 1.5 : 3.0) * max_occ / cellvol);
 		max_neigh_est = std::max(max_neigh_est, std::min(est, (double) d.N - 1));
 	}
 	else {

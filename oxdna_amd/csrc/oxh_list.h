@@ -3,6 +3,8 @@
 // engine (oxh_persist.hip).
 #pragma once
 
+#include <type_traits>
+
 #include "oxh_device.h"
 
 namespace oxh {
@@ -32,13 +34,15 @@ constexpr int BUILD_LANES = 32;
 // the back.  A pair cannot come closer than R_CLOSE before the next rebuild
 // unless it is within R_CLOSE + 2 skin now, so far pairs only ever need
 // Debye-Hueckel.
-// With cells and K >= 27 lanes, lane k < 27 takes neighbour cell k (dz, dy,
-// dx = k / 9 - 1, k / 3 % 3 - 1, k % 3 - 1), a lane prefix sum of the
-// cells' occupancies numbers the candidates (cells in that order, each cell
-// in slot order, its items sorted by particle index), and the group scans
-// them in chunks of K, each lane finding its candidate's cell by a binary
-// search over the prefix sums: the 27 cells' loads overlap instead of
-// following each other.
+// With cells, the candidates are the particles of the cells within
+// +-stencil of the particle's cell (27 or 125 cells; k = (oz + R) W^2 +
+// (oy + R) W + ox + R), minus the cells whose nearest point is out of
+// range.  With K = 32 lanes, lane l takes cells l, l + 32, ...; lane prefix
+// sums of the cells' occupancies number the candidates (cells in k order,
+// each cell's items sorted by particle index), and the group scans them in
+// chunks of K, each lane finding its candidate's cell by a binary search
+// over the prefix sums: the cells' loads overlap instead of following each
+// other.
 // Every lane of the group must call it; lane 0 writes the row's counts and
 // the list position.
 template<int K>
@@ -104,40 +108,76 @@ __device__ __forceinline__ void build_row(int i, int l, int s, const double4 *__
 		const int cx = cell_coord(p.x, S.inv_box[0], S.nc[0]);
 		const int cy = cell_coord(p.y, S.inv_box[1], S.nc[1]);
 		const int cz = cell_coord(p.z, S.inv_box[2], S.nc[2]);
-		auto cell_of = [&](int k) {
-			int z = cz + k / 9 - 1, y = cy + (k / 3) % 3 - 1, x = cx + k % 3 - 1;
+		const int R = S.stencil, W = 2 * R + 1, NS = W * W * W;
+		// position inside its cell, for the distance to other cells
+		const float ux = p32.x - cx * S.cell_w[0], uy = p32.y - cy * S.cell_w[1], uz = p32.z - cz * S.cell_w[2];
+		// a cell is culled when its nearest point is farther than r_verlet
+		// by a margin far above the FP32 error of these distances
+		const float cull = (sqrtf(S.rv2f) + 1e-3f) * (sqrtf(S.rv2f) + 1e-3f);
+		auto gap = [](int o, float u, float w) { return o > 0 ? o * w - u : (o < 0 ? u + (-o - 1) * w : 0.f); };
+		auto cell_of = [&](int k, bool &in_range) {
+			const int oz = k / (W * W) - R, oy = (k / W) % W - R, ox = k % W - R;
+			const float gx = gap(ox, ux, S.cell_w[0]), gy = gap(oy, uy, S.cell_w[1]), gz = gap(oz, uz, S.cell_w[2]);
+			in_range = gx * gx + gy * gy + gz * gz <= cull;
+			int z = cz + oz, y = cy + oy, x = cx + ox;
 			z += (z < 0) ? S.nc[2] : (z >= S.nc[2] ? -S.nc[2] : 0);
 			y += (y < 0) ? S.nc[1] : (y >= S.nc[1] ? -S.nc[1] : 0);
 			x += (x < 0) ? S.nc[0] : (x >= S.nc[0] ? -S.nc[0] : 0);
 			return S.cell_first + (z * S.nc[1] + y) * S.nc[0] + x;
 		};
-		if constexpr(K >= 27) {
-			int cell = 0, cnt = 0;
-			if(l < 27) {
-				cell = cell_of(l);
-				cnt = min(count[cell], cap);
-			}
-			int incl = cnt;
-			for(int off = 1; off < K; off <<= 1) {
-				const int v = __shfl_up(incl, off, K);
-				if(l >= off) incl += v;
-			}
-			const int excl = incl - cnt;
-			const int total = __shfl(incl, K - 1, K);
-			for(int m0 = 0; m0 < total; m0 += K) {
-				const int g = m0 + l;
-				int lo = 0;
-				for(int step = K >> 1; step > 0; step >>= 1)
-					if(__shfl(excl, lo + step, K) <= g) lo += step;
-				const int c = __shfl(cell, lo, K);
-				const int slot = g - __shfl(excl, lo, K);
-				test(g < total ? items[(size_t) c * cap + slot] : -1, true);
-			}
+		if constexpr(K == 32) {
+			// lane l takes cells l, l + 32, ... (1 round for 27 cells, 4 for
+			// 125); the stencil is a compile-time constant of each branch
+			auto stream = [&](auto rounds_tag) {
+				constexpr int ROUNDS = decltype(rounds_tag)::value;
+				int cell[ROUNDS], excl[ROUNDS], base[ROUNDS + 1];
+				base[0] = 0;
+#pragma unroll
+				for(int r = 0; r < ROUNDS; r++) {
+					const int k = l + K * r;
+					int cnt = 0;
+					cell[r] = 0;
+					if(k < NS) {
+						bool in_range;
+						cell[r] = cell_of(k, in_range);
+						if(in_range) cnt = min(count[cell[r]], cap);
+					}
+					int incl = cnt;
+					for(int off = 1; off < K; off <<= 1) {
+						const int v = __shfl_up(incl, off, K);
+						if(l >= off) incl += v;
+					}
+					excl[r] = incl - cnt;
+					base[r + 1] = base[r] + __shfl(incl, K - 1, K);
+				}
+				const int total = base[ROUNDS];
+				for(int m0 = 0; m0 < total; m0 += K) {
+					const int g = m0 + l;
+					int c = 0, slot = 0;
+#pragma unroll
+					for(int r = 0; r < ROUNDS; r++) {
+						// candidates base[r] .. base[r + 1] - 1 come from round r
+						const int gr = g - base[r];
+						int lo = 0;
+						for(int step = K >> 1; step > 0; step >>= 1)
+							if(__shfl(excl[r], lo + step, K) <= gr) lo += step;
+						const int cr = __shfl(cell[r], lo, K), sr = gr - __shfl(excl[r], lo, K);
+						if(g >= base[r] && g < base[r + 1]) {
+							c = cr;
+							slot = sr;
+						}
+					}
+					test(g < total ? items[(size_t) c * cap + slot] : -1, true);
+				}
+			};
+			if(R == 1) stream(std::integral_constant<int, 1>());
+			else stream(std::integral_constant<int, 4>());
 		}
 		else {
-			for(int k = 0; k < 27; k++) {
-				const int c = cell_of(k);
-				const int n = min(count[c], cap);
+			for(int k = 0; k < NS; k++) {
+				bool in_range;
+				const int c = cell_of(k, in_range);
+				const int n = in_range ? min(count[c], cap) : 0;
 				for(int m0 = 0; m0 < n; m0 += K) test(m0 + l < n ? items[(size_t) c * cap + m0 + l] : -1, true);
 			}
 		}

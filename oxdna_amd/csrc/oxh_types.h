@@ -22,6 +22,11 @@ typedef struct oxh_sys {
 	// cell arrays; use_cells = 0 selects the all-pairs list build
 	int nc[3];
 	int cell_first, ncell, use_cells;
+	// neighbour-cell stencil: cells within +-stencil of a particle's cell
+	// (1: cell edges >= r_verlet, 27 cells; 2: edges >= r_verlet / 2, 125
+	// cells); cell edge lengths (FP32), for culling cells out of range
+	int stencil;
+	float cell_w[3];
 	// 1 when a box side is shorter than 2 (r_verlet + 2 backbone arms): the
 	// nearest image of two backbone sites may then differ from the image the
 	// centres select, so far pairs take the centres' image

@@ -72,6 +72,29 @@ def test_pair_set_exact(hip, name):
     assert got.shape == want.shape and np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("kind", ["bundle", "gas"])
+def test_pair_set_exact_cells(hip, kind):
+    """Cell lists (systems above 1,024 nt), with per-particle culling of
+    cells out of range: half-size cells and a 125-cell stencil in a dense
+    bundle (0.9 nt per unit volume), full-size cells and 27 in a dilute gas
+    (half-size cells there would hold under 2 nt on average); the pair set
+    equals the brute-force FP64 predicate's."""
+    from oracle import pyoracle as po
+    from oxdna_amd import systems
+    if kind == "bundle":
+        s = systems.duplex_bundle(2, nx=6, ny=6, bp=32, spacing=2.6, lz=14.0, tc=30.0, salt=0.2)
+    else:
+        s = systems.duplex_gas(4, n_duplex=64, bp=12, density=2e-3)
+    assert s.N > 1024
+    P = po.Params(T=s.T, salt=s.salt)
+    rv = P.rcut + 2 * 0.05
+    with hip.HIPBatch(s, verlet_skin=0.05, thermostat="no", salt_concentration=s.salt) as b:
+        got = b.pairs()
+    want = po.verlet_pairs(s.pos, s.n3, s.n5, s.box, rv * rv)
+    print(f"{kind}: N={s.N}, {len(want)} pairs")
+    assert got.shape == want.shape and np.array_equal(got, want)
+
+
 def test_deterministic(hip):
     s, opts, _ = load_golden("bundle_md_30C")
     outs = []
