@@ -243,19 +243,31 @@ __global__ void __launch_bounds__(256) k_forces_far(int N, const double4 *__rest
 	float sv[6] = {0, 0, 0, 0, 0, 0};
 	Frame fp;
 	if(S.small_box) fp = make_frame(q32[i], 0);
-	for(int k = l; k < nf; k += K) {
-		const int j = row[-k];
-		const float4 Bj = bpos[j];
-		f3 rb;
-		if(!S.small_box) rb = min_image_f(Bi, Bj, S);
-		else {
-			const Frame fq = make_frame(q32[j], 0);
-			rb = min_image(pos[i], pos[j], S) + fq.back - fp.back;
+	// four pairs per iteration, their loads issued together; each lane still
+	// adds its pairs in list order
+	constexpr int U = 4;
+	for(int k0 = l; k0 < nf; k0 += U * K) {
+		int jj[U];
+		float4 Bj[U];
+#pragma unroll
+		for(int u = 0; u < U; u++) jj[u] = k0 + u * K < nf ? row[-(k0 + u * K)] : -1;
+#pragma unroll
+		for(int u = 0; u < U; u++) Bj[u] = bpos[jj[u] >= 0 ? jj[u] : i];
+#pragma unroll
+		for(int u = 0; u < U; u++) {
+			if(jj[u] < 0) continue;
+			const int j = jj[u];
+			f3 rb;
+			if(!S.small_box) rb = min_image_f(Bi, Bj[u], S);
+			else {
+				const Frame fq = make_frame(q32[j], 0);
+				rb = min_image(pos[i], pos[j], S) + fq.back - fp.back;
+			}
+			f3 fq_ = {0, 0, 0};
+			Ef += debye_huckel(S, rb, Bi.w * Bj[u].w, fq_);
+			Ff += fq_;
+			if(SPLIT) add_virial(sv, min_image(pos[i], pos[j], S), fq_);
 		}
-		f3 fq_ = {0, 0, 0};
-		Ef += debye_huckel(S, rb, Bi.w * Bj.w, fq_);
-		Ff += fq_;
-		if(SPLIT) add_virial(sv, min_image(pos[i], pos[j], S), fq_);
 	}
 	if(K > 1) {
 		Ff.x = group_sum(Ff.x, K);
