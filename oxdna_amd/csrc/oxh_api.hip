@@ -808,10 +808,11 @@ oxh_status autotune_forces(oxh_batch *B) {
 	const int blocks[] = {64, 128, 256};
 	float best = 1e30f;
 	if(b.fp64) {
-		// all-FP64 mode: blocks of 64, far pairs inline; only the lanes vary
+		// all-FP64 mode: close kernel in blocks of 64 with far pairs inline,
+		// then far pairs in their own kernel (blocks of 256)
 		b.force_block = 64;
 		b.far_lanes = 0;
-		int bl64 = 8;
+		int bl64 = 8, bf64 = 0;
 		for(int L : lanes) {
 			if(L == 1) continue;
 			b.lanes = L;
@@ -822,9 +823,24 @@ oxh_status autotune_forces(oxh_batch *B) {
 				bl64 = L;
 			}
 		}
+		b.lanes = bl64;
+		for(int FL : {4, 8, 16, 32}) {
+			for(int L : {2, 4, 8}) {
+				b.lanes = L;
+				b.far_lanes = FL;
+				float ms;
+				TRY(timeit(ms));
+				if(ms < best) {
+					best = ms;
+					bl64 = L;
+					bf64 = FL;
+				}
+			}
+		}
 		(void) hipEventDestroy(e0);
 		(void) hipEventDestroy(e1);
 		b.lanes = bl64;
+		b.far_lanes = bf64;
 		enqueue_forces(B);
 		return sync_and_check(B);
 	}

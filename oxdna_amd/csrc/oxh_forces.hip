@@ -298,9 +298,10 @@ __global__ void __launch_bounds__(256) k_forces_far(int N, const double4 *__rest
 }
 
 // All-FP64 forces (the reference's backend_precision = double): the same
-// terms in FP64 (oxh::dp), far pairs inline, F and T in FP64.  Blocks of
-// 64 threads, so the FP64 terms get the register file they need.
-template<int K, bool SPLIT>
+// terms in FP64 (oxh::dp), F and T in FP64, far pairs inline (FAR) or in
+// k_forces_far_dp.  Blocks of 64 threads, so the FP64 terms get the
+// register file they need.
+template<int K, bool SPLIT, bool FAR>
 __global__ void __launch_bounds__(64) k_forces_dp(int N, const double4 *__restrict__ pos, const double4 *__restrict__ q64,
 		const float4 *__restrict__ q32, const float4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
 		const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh, const oxh_sys *__restrict__ systems,
@@ -317,22 +318,115 @@ __global__ void __launch_bounds__(64) k_forces_dp(int N, const double4 *__restri
 		sstate[s].n_rebuilds += 1;
 	}
 	if(idle(i, systems[s], s, k, limit)) return;
-	dp::particle_forces<SPLIT, true>(K, i, l, N, pos, q64, q32, bpos, bonds, pinfo, nbr, nnbr, max_neigh, systems, sstate, Fout, Tout,
+	dp::particle_forces<SPLIT, FAR>(K, i, l, N, pos, q64, q32, bpos, bonds, pinfo, nbr, nnbr, max_neigh, systems, sstate, Fout, Tout,
 			esplit, hbcount, stress);
 }
 
-template<int K>
-static void launch_dp(hipStream_t st, const Batch &b, bool split, int k) {
+// All-FP64 far pairs on their own: Debye-Hueckel between FP64 backbone
+// sites (centres' nearest image, then the site offsets), with the few
+// registers that let many waves hide the gathers, next to the
+// register-heavy k_forces_dp.  Adds onto its F / T (the same thread owns a
+// particle's outputs in both kernels; fixed order, no atomics).
+template<int K, bool SPLIT>
+__global__ void __launch_bounds__(256) k_forces_far_dp(int N, const double4 *__restrict__ pos, const double4 *__restrict__ q64,
+		const float4 *__restrict__ bpos, const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh,
+		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, double4 *__restrict__ Fio, double4 *__restrict__ Tio,
+		double *__restrict__ esplit, int k_step, const int *__restrict__ limit, double *__restrict__ stress) {
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	const int i = t / K;
+	const int l = t % K;
+	if(i >= N) return;
+	const int s = pinfo[i].y;
+	const oxh_sys &S = systems[s];
+	if(idle(i, S, s, k_step, limit)) return;
+	const int nf = nnbr[i].y;
+	const int *row = nbr + (size_t) i * max_neigh + max_neigh - 1;
+	const double4 Pi = pos[i];
+	Sites64 si;
+	const dp::Frame fp = dp::make_frame_d(q64[i], 0, si);
+	const double cfi = bpos[i].w;
+	dv3 Ff = {0, 0, 0};
+	double Ef = 0.;
+	double sv[6] = {0, 0, 0, 0, 0, 0};
+	for(int k = l; k < nf; k += K) {
+		const int j = row[-k];
+		Sites64 sj;
+		sites_d(q64[j], sj);
+		const dv3 rc = dp::min_image_d(Pi, pos[j], S);
+		dv3 fq_ = {0, 0, 0};
+		Ef += dp::debye_huckel(S, dp::site_sep(rc, sj.back, si.back), cfi * (double) bpos[j].w, fq_);
+		Ff += fq_;
+		if(SPLIT) dp::add_virial(sv, rc, fq_);
+	}
+	if(K > 1) {
+		Ff.x = dp::group_sum(Ff.x, K);
+		Ff.y = dp::group_sum(Ff.y, K);
+		Ff.z = dp::group_sum(Ff.z, K);
+		Ef = dp::group_sum(Ef, K);
+		if(SPLIT)
+			for(int q = 0; q < 6; q++) sv[q] = dp::group_sum(sv[q], K);
+	}
+	if(l != 0) return;
+	const dv3 Tl = -cross(fp.back, Ff);
+	double4 F = Fio[i], T = Tio[i];
+	F.x -= Ff.x;
+	F.y -= Ff.y;
+	F.z -= Ff.z;
+	F.w += 0.5 * Ef;
+	T.x += dot(fp.a1, Tl);
+	T.y += dot(fp.a2, Tl);
+	T.z += dot(fp.a3, Tl);
+	Fio[i] = F;
+	Tio[i] = T;
+	if(SPLIT) {
+		esplit[(size_t) 7 * N + i] += 0.5 * Ef;
+		for(int q = 0; q < 6; q++) stress[(size_t) q * N + i] += 0.5 * sv[q];
+	}
+}
+
+template<int K, bool FAR>
+static void launch_dp_close(hipStream_t st, const Batch &b, bool split, int k) {
 	const int block = 64;
 	size_t threads = (size_t) b.N * K;
 	dim3 grid((unsigned) ((threads + block - 1) / block));
 	const int *lim = k >= 0 ? b.limit : nullptr;
 	if(split)
-		k_forces_dp<K, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh, b.d_sys,
-				b.d_state, b.F64, b.T64, b.esplit, b.hbcount, k, lim, b.any_rebuild, b.stress);
+		k_forces_dp<K, true, FAR><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
+				b.d_sys, b.d_state, b.F64, b.T64, b.esplit, b.hbcount, k, lim, b.any_rebuild, b.stress);
 	else
-		k_forces_dp<K, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
+		k_forces_dp<K, false, FAR><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
 				b.d_sys, b.d_state, b.F64, b.T64, nullptr, b.want_hbcount ? b.hbcount : nullptr, k, lim, b.any_rebuild, nullptr);
+}
+
+template<int K>
+static void launch_far_dp(hipStream_t st, const Batch &b, bool split, int k) {
+	const int block = 256;
+	size_t threads = (size_t) b.N * K;
+	dim3 grid((unsigned) ((threads + block - 1) / block));
+	const int *lim = k >= 0 ? b.limit : nullptr;
+	if(split)
+		k_forces_far_dp<K, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F64,
+				b.T64, b.esplit, k, lim, b.stress);
+	else
+		k_forces_far_dp<K, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F64,
+				b.T64, nullptr, k, lim, nullptr);
+}
+
+// all-FP64 forces: close (+ bonded) kernel with b.lanes, far pairs inline
+// (b.far_lanes == 0) or in k_forces_far_dp with b.far_lanes
+template<int K>
+static void launch_dp(hipStream_t st, const Batch &b, bool split, int k) {
+	if(b.far_lanes == 0) {
+		launch_dp_close<K, true>(st, b, split, k);
+		return;
+	}
+	launch_dp_close<K, false>(st, b, split, k);
+	switch(b.far_lanes) {
+	case 4: launch_far_dp<4>(st, b, split, k); break;
+	case 16: launch_far_dp<16>(st, b, split, k); break;
+	case 32: launch_far_dp<32>(st, b, split, k); break;
+	default: launch_far_dp<8>(st, b, split, k); break;
+	}
 }
 
 template<int K>

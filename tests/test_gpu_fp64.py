@@ -42,6 +42,28 @@ def test_fp64_forces_vs_reference(hip, name):
     assert np.all(np.abs(W - want) <= 2e-8 * np.abs(want).max() + 1e-12)
 
 
+@pytest.mark.parametrize("lanes,far_lanes", [(8, 0), (4, 8), (2, 16)])
+@pytest.mark.parametrize("name", ["bundle_md_30C", "gas8_25C_0.5M", "tangle3_nohalf_dh"])
+def test_fp64_launch_shapes(hip, monkeypatch, name, lanes, far_lanes):
+    """All-FP64 far pairs inline or in their own kernel (k_forces_far_dp),
+    any lanes: the same FP64 bar, energy split and virial included."""
+    monkeypatch.setenv("OXH_FORCE_LANES", str(lanes))
+    monkeypatch.setenv("OXH_FAR_LANES", str(far_lanes))
+    s, opts, ref = load_golden(name)
+    with hip.HIPBatch(s, backend_precision="double", **opts) as b:
+        assert b.stats()["far_lanes"] == far_lanes
+        b.compute_forces()
+        F, T, Ep = b.forces()
+        e8 = b.energy_split()
+        W, _ = b.stress()
+    assert vec_rel_err(F, ref["F"])[0] <= 1e-9 and vec_rel_err(T, ref["T"])[0] <= 1e-9
+    scale = np.abs(ref["E8"]).sum() + 1e-6 * s.N
+    assert np.all(np.abs(e8 - ref["E8"]) <= 1e-10 * scale)
+    assert abs(Ep.sum() - ref["E8"].sum()) <= 1e-10 * scale
+    want = ref["stress7"][1:] * np.prod(s.box)
+    assert np.all(np.abs(W - want) <= 2e-8 * np.abs(want).max() + 1e-12)
+
+
 @pytest.mark.parametrize("precision,tol", [("double", 1e-9), ("mixed", 1e-5)])
 def test_nve_trajectory_follows_oracle(hip, precision, tol):
     from oracle import pyoracle as po
