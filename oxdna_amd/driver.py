@@ -1,0 +1,215 @@
+"""Host driver: runs an oxDNA input file (sim_type = MD, interaction DNA2)
+on the HIP batch, writing the reference's output files.
+
+    python -m oxdna_amd.driver input [key=value ...]
+
+The run loop follows SimManager::run (src/Managers/SimManager.cpp:142-201):
+at every step the observables due at that step are printed before the step
+is taken (the energy file every print_energy_every steps, step 0 included),
+the trajectory gets a configuration every print_conf_interval steps (not at
+step 0), strands are re-boxed every fix_diffusion_every steps, and after the
+last step the observables due then and the last configuration are printed.
+Between output points the batch runs uninterrupted on the device (HIP graph
+replay), so a run with sparse output never returns to the host.
+
+Output formats are the reference's:
+  - energy file: the step observable in MD units (`%14.4lf` of step * dt,
+    src/Observables/Step.cpp:20-23), then potential, kinetic and total
+    energy per particle (`%10.6lf`, src/Observables/TotalEnergy.cpp:32-38,
+    src/Observables/BaseObservable.cpp:32-43), space separated;
+  - trajectory and last configuration: `t = `, `b = `, `E = ` headers and
+    one line of position, a1, a3, velocity, angular momentum per nucleotide
+    (oxdna_amd/io.py).
+
+Input keys read (reference key names): topology, conf_file, steps, T, dt,
+salt_concentration, verlet_skin, thermostat, newtonian_steps, diff_coeff,
+gamma_trans, pt, bussi_tau, refresh_vel, seed, restart_step_counter,
+backend_precision, dh_lambda, dh_strength, debye_huckel_rhigh,
+dh_half_charged_ends, hb_multiplier, max_backbone_force,
+max_backbone_force_far, energy_file, print_energy_every, trajectory_file,
+print_conf_interval, lastconf_file, fix_diffusion, fix_diffusion_every,
+CUDA_sort_every, CUDA_device.  sim_type must be MD and interaction_type
+DNA2 or DNA2_nomesh (the GPU interaction is the analytic DNA2_nomesh form);
+use_average_seq = false is refused (sequence-dependent parameter files are
+not read here).
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+from . import io
+
+_BOOL_TRUE = {"1", "true", "yes", "on"}
+
+
+def _bool(v) -> bool:
+    return str(v).strip().lower() in _BOOL_TRUE
+
+
+def _num(inp: dict, key: str, cast=float, default=None):
+    if key not in inp:
+        if default is None:
+            raise KeyError(f"input key '{key}' is required")
+        return default
+    return cast(float(inp[key])) if cast is int else cast(inp[key])
+
+
+def batch_options(inp: dict) -> dict:
+    """HIPBatch options (reference key names) from an input dict."""
+    st = inp.get("sim_type", "MD")
+    if st != "MD":
+        raise ValueError(f"sim_type = {st}: this driver runs molecular dynamics only")
+    it = inp.get("interaction_type", "DNA")
+    if it not in ("DNA2", "DNA2_nomesh"):
+        raise ValueError(f"interaction_type = {it}: this driver runs the oxDNA2 interaction (DNA2)")
+    if not _bool(inp.get("use_average_seq", "true")):
+        raise ValueError("use_average_seq = false: sequence-dependent parameters are not supported by this driver")
+    th = inp.get("thermostat", "no")
+    if th == "john":
+        th = "brownian"
+    opts = dict(
+        T=io.temperature(inp["T"]),
+        dt=_num(inp, "dt"),
+        salt_concentration=_num(inp, "salt_concentration"),   # required by DNA2, as in the reference
+        verlet_skin=_num(inp, "verlet_skin", float, 0.05),
+        thermostat=th,
+        refresh_vel=_bool(inp.get("refresh_vel", "false")),
+        seed=_num(inp, "seed", int, 1),
+        backend_precision=inp.get("backend_precision", "mixed"),
+        dh_half_charged_ends=_bool(inp.get("dh_half_charged_ends", "true")),
+    )
+    for key, cast in (("newtonian_steps", int), ("bussi_tau", int), ("diff_coeff", float), ("gamma_trans", float),
+                      ("pt", float), ("dh_lambda", float), ("dh_strength", float), ("debye_huckel_rhigh", float),
+                      ("hb_multiplier", float), ("max_backbone_force", float), ("max_backbone_force_far", float)):
+        if key in inp:
+            opts[key] = _num(inp, key, cast)
+    return opts
+
+
+def output_steps(start: int, steps: int, energy_every: int, conf_every: int) -> list:
+    """(step, print_energy, print_conf) for the steps of a run of `steps`
+    steps from step `start` at which something is printed, in order, as
+    SimManager::run does: energy at every multiple of energy_every from
+    start to start + steps inclusive; configurations at multiples of
+    conf_every after start, up to start + steps inclusive."""
+    out = {}
+    end = start + steps
+    if energy_every > 0:
+        first = start + (-start) % energy_every
+        for t in range(first, end + 1, energy_every):
+            out.setdefault(t, [False, False])[0] = True
+    if conf_every > 0:
+        first = start + (-start) % conf_every
+        if first == start:
+            first += conf_every
+        for t in range(first, end + 1, conf_every):
+            out.setdefault(t, [False, False])[1] = True
+    return [(t, e, c) for t, (e, c) in sorted(out.items())]
+
+
+def energy_line(step: int, dt: float, U: float, K: float) -> str:
+    return f"{step * dt:14.4f} {U:10.6f} {K:10.6f} {U + K:10.6f}"
+
+
+def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
+    """Runs the input file; returns the energy lines and the final state.
+    Paths in the input are relative to workdir (default: the input file's
+    directory)."""
+    from .backend import HIPBatch
+
+    inp = io.read_input(input_path, overrides)
+    wd = workdir or os.path.dirname(os.path.abspath(input_path))
+    path = lambda p: p if os.path.isabs(p) else os.path.join(wd, p)  # noqa: E731
+    opts = batch_options(inp)
+    s = io.load_system(path(inp["topology"]), path(inp["conf_file"]), T=opts["T"], salt=opts["salt_concentration"])
+    conf_step, _, *_ = io.read_conf(path(inp["conf_file"]), s.N)
+    start = 0 if _bool(inp.get("restart_step_counter", "false")) else conf_step
+    steps = _num(inp, "steps", int)
+    dt = opts["dt"]
+    energy_every = _num(inp, "print_energy_every", int, 0)
+    conf_every = _num(inp, "print_conf_interval", int, 0)
+    fix_every = _num(inp, "fix_diffusion_every", int, 100000) if _bool(inp.get("fix_diffusion", "true")) else 0
+    device = _num(inp, "CUDA_device", int, 0)
+    energy_path = path(inp.get("energy_file", "energy.dat"))
+    traj_path = path(inp["trajectory_file"]) if "trajectory_file" in inp else None
+    last_path = path(inp.get("lastconf_file", "last_conf.dat"))
+
+    lines = []
+    with HIPBatch(s, device=device, **opts) as b:
+        b.set_sort_interval(_num(inp, "CUDA_sort_every", int, 0))
+        b.set_fix_diffusion(0)   # re-boxing is done here, at the reference's steps
+
+        def snapshot(step):
+            E = b.energies()[0]
+            U, K = E[0] / s.N, (E[2] + E[3]) / s.N
+            st = b.state(0)
+            t = s.copy()
+            t.pos, t.a1, t.a3, t.vel, t.L = st["pos"], st["a1"], st["a3"], st["vel"], st["L"]
+            return U, K, t
+
+        energy_f = open(energy_path, "w")
+        traj_f = open(traj_path, "w") if traj_path else None
+        try:
+            cur = start
+            plan = output_steps(start, steps, energy_every, conf_every)
+            fixes = [t for t in range(start + fix_every, start + steps + 1, fix_every)] if fix_every > 0 else []
+            events = sorted(set([t for t, _, _ in plan] + fixes))
+            todo = {t: (e, c) for t, e, c in plan}
+            end = start + steps
+            for t in events:
+                if t > cur:
+                    b.run(t - cur)
+                    cur = t
+                e, c = todo.get(t, (False, False))
+                fix = t in fixes
+                # inside the loop the reference prints a due configuration,
+                # then re-boxes, then prints the observables; after the last
+                # step it re-boxes first
+                if fix and t == end:
+                    b.fix_diffusion()
+                state = None
+                if c and traj_f:
+                    U, K, state = snapshot(t)
+                    traj_f.write(_conf_text(state, t, U, K))
+                    traj_f.flush()
+                if fix and t < end:
+                    b.fix_diffusion()
+                if e:
+                    U, K, state = snapshot(t)
+                    line = energy_line(t, dt, U, K)
+                    energy_f.write(line + "\n")
+                    energy_f.flush()
+                    lines.append(line)
+            if cur < start + steps:
+                b.run(start + steps - cur)
+                cur = start + steps
+            U, K, state = snapshot(cur)
+            io.write_conf(last_path, state, step=cur, energies=(U + K, U, K))
+        finally:
+            energy_f.close()
+            if traj_f:
+                traj_f.close()
+    return dict(energy_lines=lines, final=state, step=cur, U=U, K=K)
+
+
+def _conf_text(s, step, U, K) -> str:
+    rows = [f"t = {step}", "b = {:.17g} {:.17g} {:.17g}".format(*s.box), f"E = {U + K:.15g} {U:.15g} {K:.15g}"]
+    for i in range(s.N):
+        r = list(s.pos[i]) + list(s.a1[i]) + list(s.a3[i]) + list(s.vel[i]) + list(s.L[i])
+        rows.append(" ".join(f"{x:.17g}" for x in r))
+    return "\n".join(rows) + "\n"
+
+
+def main(argv=None) -> int:
+    argv = list(sys.argv[1:] if argv is None else argv)
+    if not argv:
+        print(__doc__.split("\n\n")[1], file=sys.stderr)
+        return 2
+    res = run(argv[0], argv[1:])
+    print(f"{res['step']} steps done: U/N {res['U']:.6f} K/N {res['K']:.6f}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,68 @@
+"""Writes tests/golden/driver_nve/ with the reference binary: an input file,
+topology and configuration, and the reference's own output files for them.
+
+Run in the build container (needs oracle/_ref/oxDNA, built from the
+reference's sources by `make -f oracle/ref.mk`):
+
+    python tests/golden/make_driver_golden.py
+
+The run: a 20-bp duplex with seeded thermal velocities and angular momenta,
+300 NVE steps (thermostat = no) of the reference's CPU backend with
+interaction_type = DNA2_nomesh, energy every 50 steps, a configuration every
+100 steps, the last configuration at the end.  The fixture keeps the input
+files and the reference's energy file, trajectory and last configuration;
+the driver test runs the same input through oxdna_amd.driver and compares.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oxdna_amd import io, systems  # noqa: E402
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "oxDNA")
+OUT = os.path.join(HERE, "driver_nve")
+
+INPUT = {
+    "backend": "CPU", "sim_type": "MD", "interaction_type": "DNA2_nomesh", "steps": "300", "seed": "7",
+    "T": "20C", "salt_concentration": "0.5", "dt": "0.003", "verlet_skin": "0.05", "thermostat": "no",
+    "refresh_vel": "0", "restart_step_counter": "1", "topology": "t.top", "conf_file": "c.dat",
+    "trajectory_file": "trajectory.dat", "energy_file": "energy.dat", "lastconf_file": "last_conf.dat",
+    "print_energy_every": "50", "print_conf_interval": "100", "no_stdout_energy": "1", "time_scale": "linear",
+    "box_type": "orthogonal", "log_file": "log.dat",
+}
+
+
+def main():
+    s = systems.duplex_20bp(3)
+    rng = np.random.default_rng(11)
+    s.vel = rng.normal(scale=np.sqrt(s.T), size=(s.N, 3))
+    s.vel -= s.vel.mean(axis=0)
+    s.L = rng.normal(scale=np.sqrt(s.T), size=(s.N, 3))
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        io.write_topology(os.path.join(d, "t.top"), s)
+        io.write_conf(os.path.join(d, "c.dat"), s)
+        with open(os.path.join(d, "input"), "w") as f:
+            for k, v in INPUT.items():
+                f.write(f"{k} = {v}\n")
+        r = subprocess.run([REF_BIN, "input"], cwd=d, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference failed: {r.stdout[-2000:]} {r.stderr[-2000:]}")
+        os.makedirs(OUT, exist_ok=True)
+        for src, dst in (("input", "input"), ("t.top", "t.top"), ("c.dat", "c.dat"), ("energy.dat", "energy_ref.dat"),
+                         ("trajectory.dat", "trajectory_ref.dat"), ("last_conf.dat", "last_conf_ref.dat")):
+            shutil.copy(os.path.join(d, src), os.path.join(OUT, dst))
+    print("wrote", OUT)
+
+
+if __name__ == "__main__":
+    main()

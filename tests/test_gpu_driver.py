@@ -1,9 +1,15 @@
 """GPU tests of the host-driver features around the step, through the C ABI:
 strand re-boxing (fix_diffusion), a broken bond reported as an error
-instead of a crash, and the reference's default of re-boxing during runs.
+instead of a crash, and the input-file driver run end to end against the
+reference binary's own output files for the same input.
 """
+import os
+import shutil
+
 import numpy as np
 import pytest
+
+from _common import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
@@ -55,3 +61,33 @@ def test_broken_bond_is_reported(hip):
             b.run(10)
         nr, err = b.info(0)
         assert err & 1
+
+
+@pytest.mark.parametrize("precision,e_tol,x_tol", [("double", 2e-6, 1e-10), ("mixed", 5e-6, 1e-5)])
+def test_driver_follows_reference_nve(hip, tmp_path, precision, e_tol, x_tol):
+    """The input file of tests/golden/driver_nve (300 NVE steps of a 20-bp
+    duplex, thermal velocities) run by oxdna_amd.driver: the energy file
+    (printed with 6 decimals), the trajectory and the last configuration
+    follow the reference CPU binary's files for the same input.  Measured:
+    FP64 forces give the same energy file and positions within 5e-14 after
+    300 steps (bound 1e-10); FP32 forces 1e-6 and 2e-7 (bounds 5e-6,
+    1e-5)."""
+    from oxdna_amd import driver
+    d = os.path.join(GOLDEN, "driver_nve")
+    for f in ("input", "t.top", "c.dat"):
+        shutil.copy(os.path.join(d, f), tmp_path / f)
+    res = driver.run(str(tmp_path / "input"), [f"backend_precision = {precision}"])
+    assert res["step"] == 300
+    mine = np.loadtxt(tmp_path / "energy.dat", ndmin=2)
+    ref = np.loadtxt(os.path.join(d, "energy_ref.dat"), ndmin=2)
+    assert mine.shape == ref.shape
+    de = np.abs(mine - ref).max()
+    from test_driver import _confs
+    traj, traj_ref = _confs(tmp_path / "trajectory.dat"), _confs(os.path.join(d, "trajectory_ref.dat"))
+    last, last_ref = _confs(tmp_path / "last_conf.dat"), _confs(os.path.join(d, "last_conf_ref.dat"))
+    assert [t for t, _ in traj] == [t for t, _ in traj_ref] and last[0][0] == last_ref[0][0] == 300
+    dx = max(np.abs(a[:, :9] - b[:, :9]).max() for (_, a), (_, b) in zip(traj + last, traj_ref + last_ref))
+    dv = max(np.abs(a[:, 9:] - b[:, 9:]).max() for (_, a), (_, b) in zip(traj + last, traj_ref + last_ref))
+    print(f"{precision}: energy file {de:.1e}, positions/axes {dx:.1e}, velocities/L {dv:.1e}")
+    assert de <= e_tol
+    assert dx <= x_tol and dv <= 20 * x_tol
