@@ -17,8 +17,8 @@ budget (oxh_batch_stats reports what each pool holds).  Each pool is driven
 by its own host thread, so the pools' kernels overlap on the device; every
 `quantum` steps a pool's thread releases finished jobs and takes waiting
 jobs of its class into the freed slots while the other slots keep their
-state (continuous batching).  A job's result does not depend on what else
-runs: see oxh_pool_create.
+state (continuous batching), longest waiting job first.  A job's result does not
+depend on what else runs: see oxh_pool_create.
 
 The reference has no counterpart (one system per process,
 src/Utilities/ConfigInfo.h:47); its input keys per job are honoured through
@@ -124,16 +124,21 @@ class Server:
         errors = []
 
         def take(q, fits, now):
-            """Earliest arrived job of q that fits a free slot (under lock)."""
+            """Of the jobs of q that have arrived, the one with the most work
+            (nucleotides x steps) that fits a free slot, under lock: the
+            longest jobs set the makespan, so they start first."""
             with lock:
+                best = None
                 for i, j in enumerate(q):
                     if j.arrival > now:
-                        return None
+                        break
                     s = fits(j.system.N)
-                    if s is not None:
-                        q.pop(i)
-                        return j, s
-            return None
+                    if s is not None and (best is None or j.work > q[best[0]].work):
+                        best = (i, s)
+                if best is None:
+                    return None
+                j = q.pop(best[0])
+                return j, best[1]
 
         def worker(k: int):
             pool, q, quantum = self.pools[k], queues[self.classes[k]], self.quantum[k]
