@@ -133,11 +133,31 @@ oxh_status oxh_batch_run(oxh_batch *b, long long n_steps);
  * with CUDA_sort_every).  Downloads keep the original particle order. */
 oxh_status oxh_batch_hilbert_sort(oxh_batch *b);
 
+/* Checkpoints: the batch's whole dynamical state -- FP64 positions,
+ * velocities, angular momenta and quaternions, the positions of the last
+ * list build, the forces of the current state, the per-particle RNG step
+ * counters, the particle order, Bussi's internal kinetic energies, the step
+ * counters and the force launch shape -- as one opaque byte buffer.
+ * Restored into a batch created from the same systems (same descs), the
+ * run continues bitwise as if it had not been interrupted: the lists are
+ * rebuilt exactly from their build positions and the Philox streams
+ * continue from the saved counters.  The reference's checkpoints
+ * (checkpoint_every / checkpoint_file, src/Backends/SimBackend.cpp:250-266)
+ * hold a configuration and the thermostat's RNG state; its CUDA backend
+ * cannot resume a trajectory exactly (src/CUDA/Backends/CUDABaseBackend.cu:
+ * 137-140).  Batches only, not slot pools. */
+oxh_status oxh_batch_checkpoint_bytes(oxh_batch *b, size_t *bytes);
+oxh_status oxh_batch_save(oxh_batch *b, void *buf, size_t bytes);
+oxh_status oxh_batch_restore(oxh_batch *b, const void *buf, size_t bytes);
+
 /* Re-sorts along the Hilbert curve every `steps` MD steps (0 = never),
  * starting before the next step; the reference's CUDA_sort_every
  * (src/CUDA/Backends/MD_CUDABackend.cu:575-577) counts list updates, this
  * counts steps so that the decision stays on the host between graph
- * chunks. */
+ * chunks.  The interval is rounded up to whole chunks of the batch (64
+ * steps, or a multiple of newtonian_steps with Bussi), and sorts fall on the
+ * same steps however a run is split into oxh_batch_run calls (so that a
+ * restored checkpoint continues bitwise). */
 oxh_status oxh_batch_set_sort_interval(oxh_batch *b, long long steps);
 
 /* Moves every strand whose centre of mass has left the box back into it by
@@ -149,8 +169,9 @@ oxh_status oxh_batch_fix_diffusion(oxh_batch *b, int *n_moved);
 
 /* Runs oxh_batch_fix_diffusion every `steps` MD steps of oxh_batch_run (0 =
  * never; the reference's fix_diffusion / fix_diffusion_every keys, default
- * on every 100000 steps, src/Managers/SimManager.cpp:92,151), checked at
- * chunk boundaries.  Slot pools do not re-box. */
+ * on every 100000 steps, src/Managers/SimManager.cpp:92,151), rounded up to
+ * whole chunks and applied at the same steps however a run is split into
+ * calls.  Slot pools do not re-box. */
 oxh_status oxh_batch_set_fix_diffusion(oxh_batch *b, long long steps);
 
 /* Blocks until all queued work of the batch has finished. */

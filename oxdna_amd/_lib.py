@@ -23,7 +23,8 @@ EXPORTS = [
     "oxh_batch_energy_split", "oxh_batch_hb_count", "oxh_batch_get_pairs", "oxh_batch_system_info",
     "oxh_batch_set_state", "oxh_batch_stats", "oxh_batch_time_forces", "oxh_hilbert_keys", "oxh_philox",
     "oxh_pool_create", "oxh_pool_admit", "oxh_pool_release", "oxh_pool_status", "oxh_copy_bench", "oxh_batch_stress",
-    "oxh_batch_fix_diffusion", "oxh_batch_set_fix_diffusion",
+    "oxh_batch_fix_diffusion", "oxh_batch_set_fix_diffusion", "oxh_batch_checkpoint_bytes", "oxh_batch_save",
+    "oxh_batch_restore",
 ]
 
 
@@ -85,6 +86,9 @@ def lib():
         L.oxh_batch_stress.argtypes = [V, I, P, P]
         L.oxh_batch_fix_diffusion.argtypes = [V, P]
         L.oxh_batch_set_fix_diffusion.argtypes = [V, LL]
+        L.oxh_batch_checkpoint_bytes.argtypes = [V, P]
+        L.oxh_batch_save.argtypes = [V, P, C.c_size_t]
+        L.oxh_batch_restore.argtypes = [V, P, C.c_size_t]
         for name in EXPORTS:
             f = getattr(L, name)
             if name not in ("oxh_last_error", "oxh_system_desc_defaults"):

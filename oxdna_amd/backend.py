@@ -142,6 +142,20 @@ class HIPBatch:
         """Re-box strands every `every` steps of run() (0 = never)."""
         check(lib().oxh_batch_set_fix_diffusion(self.h, int(every)), "oxh_batch_set_fix_diffusion")
 
+    def checkpoint(self) -> bytes:
+        """The whole dynamical state as bytes (oxh_batch_save)."""
+        n = C.c_size_t(0)
+        check(lib().oxh_batch_checkpoint_bytes(self.h, C.byref(n)), "oxh_batch_checkpoint_bytes")
+        buf = C.create_string_buffer(n.value)
+        check(lib().oxh_batch_save(self.h, buf, n.value), "oxh_batch_save")
+        return buf.raw
+
+    def restore(self, data: bytes):
+        """Continues from a checkpoint of a batch built from the same
+        systems (oxh_batch_restore)."""
+        buf = C.create_string_buffer(data, len(data))
+        check(lib().oxh_batch_restore(self.h, buf, len(data)), "oxh_batch_restore")
+
     def set_sort_interval(self, steps: int):
         """Hilbert re-sort every `steps` MD steps (0 = never)."""
         check(lib().oxh_batch_set_sort_interval(self.h, int(steps)), "oxh_batch_set_sort_interval")

@@ -1275,6 +1275,10 @@ oxh_status oxh_batch_run(oxh_batch *B, long long n_steps) {
 			B->since_sort = 0;
 		}
 		long long todo = std::min<long long>(B->persist ? B->persist_chunk : B->chunk, n_steps - done);
+		// chunks end where a sort or a re-boxing is due, so that these
+		// happen at the same steps whatever the calls' lengths
+		if(!B->persist && !B->pool && B->sort_interval > 0) todo = std::min(todo, B->sort_interval - B->since_sort);
+		if(!B->pool && B->fix_every > 0) todo = std::min(todo, std::max(1LL, B->fix_every - B->since_fix));
 		bool aligned = !B->any_bussi || !B->bussi_aligned || (b.step % B->bussi_ns == 0);
 		if(B->pool) {
 			// every slot advances by at most its remaining steps; slots
@@ -1488,17 +1492,24 @@ oxh_status oxh_batch_fix_diffusion(oxh_batch *B, int *n_moved) {
 	return fix_diffusion_impl(B, n_moved);
 }
 
+// intervals of the host-side events are whole chunks, so that they fall on
+// the same steps however a run is split into oxh_batch_run calls
+static long long round_to_chunks(const oxh_batch *B, long long steps) {
+	const long long c = std::max(1, B->chunk);
+	return steps > 0 ? (steps + c - 1) / c * c : 0;
+}
+
 oxh_status oxh_batch_set_fix_diffusion(oxh_batch *B, long long every) {
 	if(!B || every < 0) return fail(OXH_ERR_ARG, "bad arguments");
-	B->fix_every = every;
+	B->fix_every = round_to_chunks(B, every);
 	B->since_fix = 0;
 	return OXH_OK;
 }
 
 oxh_status oxh_batch_set_sort_interval(oxh_batch *B, long long steps) {
 	if(!B || steps < 0) return fail(OXH_ERR_ARG, "bad arguments");
-	B->sort_interval = steps;
-	B->since_sort = steps;   // sort before the next chunk
+	B->sort_interval = round_to_chunks(B, steps);
+	B->since_sort = B->sort_interval;   // sort before the next chunk
 	return OXH_OK;
 }
 
@@ -1760,6 +1771,128 @@ oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {  // 10 doubles
 	out6[8] = B->persist ? 1.0 : (B->b.edge ? 2.0 : 0.0);
 	out6[9] = B->b.nsys;
 	return OXH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- checkpoints
+// Layout: a header, then the device arrays in the batch's internal
+// (possibly Hilbert-sorted) order, so that a run restored into a batch
+// built from the same systems continues bitwise as if never interrupted.
+namespace {
+struct CheckpointHeader {
+	char magic[8];            // "OXHCKPT1"
+	int N, nsys, fp64, persist;
+	long long step, since_sort, since_fix;
+	int lanes, force_block, far_lanes, far_block, edge, edge_group, ecap, pad_;
+};
+
+// every device array a checkpoint holds, with its byte size
+std::vector<std::pair<void *, size_t>> ckpt_layout(oxh_batch *B) {
+	oxh::Batch &b = B->b;
+	const size_t N = b.N;
+	std::vector<std::pair<void *, size_t>> a = {
+			{b.pos, N * sizeof(double4)}, {b.vel, N * sizeof(double4)}, {b.Lm, N * sizeof(double4)}, {b.q64, N * sizeof(double4)},
+			{b.rlist, N * sizeof(double4)}, {b.F, N * sizeof(float4)}, {b.T, N * sizeof(float4)}, {b.pstep, N * sizeof(long long)},
+			{b.orig, N * sizeof(int)}, {b.bonds, N * sizeof(int2)}, {b.pinfo, N * sizeof(int2)},
+			{b.d_state, (size_t) b.nsys * sizeof(oxh_sys_state)}};
+	if(b.fp64) {
+		a.push_back({b.F64, N * sizeof(double4)});
+		a.push_back({b.T64, N * sizeof(double4)});
+	}
+	return a;
+}
+}  // namespace
+
+extern "C" {
+
+oxh_status oxh_batch_checkpoint_bytes(oxh_batch *B, size_t *bytes) {
+	if(!B || !bytes) return fail(OXH_ERR_ARG, "bad arguments");
+	if(B->pool) return fail(OXH_ERR_ARG, "checkpoints are taken of batches, not slot pools");
+	size_t n = sizeof(CheckpointHeader);
+	for(auto &x : ckpt_layout(B)) n += x.second;
+	*bytes = n;
+	return OXH_OK;
+}
+
+oxh_status oxh_batch_save(oxh_batch *B, void *buf, size_t bytes) {
+	size_t need = 0;
+	TRY(oxh_batch_checkpoint_bytes(B, &need));
+	if(!buf || bytes < need) return fail(OXH_ERR_ARG, "checkpoint buffer of %zu bytes, %zu needed", bytes, need);
+	oxh::Batch &b = B->b;
+	HIPCHK(hipSetDevice(b.device));
+	HIPCHK(hipStreamSynchronize(b.stream));
+	CheckpointHeader h;
+	std::memset(&h, 0, sizeof(h));
+	std::memcpy(h.magic, "OXHCKPT1", 8);
+	h.N = b.N;
+	h.nsys = b.nsys;
+	h.fp64 = b.fp64;
+	h.persist = B->persist;
+	h.step = b.step;
+	h.since_sort = B->since_sort;
+	h.since_fix = B->since_fix;
+	h.lanes = b.lanes;
+	h.force_block = b.force_block;
+	h.far_lanes = b.far_lanes;
+	h.far_block = b.far_block;
+	h.edge = b.edge;
+	h.edge_group = b.edge_group;
+	h.ecap = b.ecap;
+	char *p = (char *) buf;
+	std::memcpy(p, &h, sizeof(h));
+	p += sizeof(h);
+	for(auto &x : ckpt_layout(B)) {
+		HIPCHK(hipMemcpyAsync(p, x.first, x.second, hipMemcpyDeviceToHost, b.stream));
+		p += x.second;
+	}
+	return sync_and_check(B);
+}
+
+oxh_status oxh_batch_restore(oxh_batch *B, const void *buf, size_t bytes) {
+	size_t need = 0;
+	TRY(oxh_batch_checkpoint_bytes(B, &need));
+	if(!buf || bytes < need) return fail(OXH_ERR_ARG, "checkpoint of %zu bytes, %zu expected", bytes, need);
+	CheckpointHeader h;
+	std::memcpy(&h, buf, sizeof(h));
+	oxh::Batch &b = B->b;
+	if(std::memcmp(h.magic, "OXHCKPT1", 8) != 0) return fail(OXH_ERR_ARG, "not a checkpoint");
+	if(h.N != b.N || h.nsys != b.nsys || h.fp64 != (int) b.fp64 || h.persist != (int) B->persist)
+		return fail(OXH_ERR_ARG, "checkpoint of another batch (N %d, %d systems) than this one (N %d, %d systems)", h.N, h.nsys, b.N,
+				b.nsys);
+	HIPCHK(hipSetDevice(b.device));
+	HIPCHK(hipStreamSynchronize(b.stream));
+	// the checkpointed launch shape, so that forces round as they did
+	if(h.edge && b.ecap == 0) TRY(alloc_edges(B, h.ecap > 0 ? h.ecap : std::min(b.max_neigh, 64)));
+	if(!h.edge && b.ecap > 0) TRY(alloc_edges(B, 0));
+	b.lanes = h.lanes;
+	b.force_block = h.force_block;
+	b.far_lanes = h.far_lanes;
+	b.far_block = h.far_block;
+	b.edge = h.edge;
+	b.edge_group = h.edge_group;
+	const char *p = (const char *) buf + sizeof(h);
+	for(auto &x : ckpt_layout(B)) {
+		HIPCHK(hipMemcpyAsync(x.first, p, x.second, hipMemcpyHostToDevice, b.stream));
+		p += x.second;
+	}
+	HIPCHK(hipStreamSynchronize(b.stream));
+	b.step = h.step;
+	B->since_sort = h.since_sort;
+	B->since_fix = h.since_fix;
+	if(b.graph_exec) {
+		(void) hipGraphExecDestroy(b.graph_exec);
+		b.graph_exec = nullptr;
+	}
+	oxh::launch_prepare_mirror(b.stream, b);
+	// the lists of the checkpointed run, rebuilt exactly from their build
+	// positions; the rebuild flags and counters are the checkpoint's
+	std::vector<oxh_sys_state> saved(b.nsys);
+	HIPCHK(hipMemcpyAsync(saved.data(), b.d_state, b.nsys * sizeof(oxh_sys_state), hipMemcpyDeviceToHost, b.stream));
+	HIPCHK(hipStreamSynchronize(b.stream));
+	TRY(rebuild_from_rlist(B));
+	HIPCHK(hipMemcpyAsync(b.d_state, saved.data(), b.nsys * sizeof(oxh_sys_state), hipMemcpyHostToDevice, b.stream));
+	return sync_and_check(B);
 }
 
 oxh_status oxh_batch_time_forces(oxh_batch *B, int n_launches, double *avg_ms) {

@@ -28,7 +28,12 @@ backend_precision, dh_lambda, dh_strength, debye_huckel_rhigh,
 dh_half_charged_ends, hb_multiplier, max_backbone_force,
 max_backbone_force_far, energy_file, print_energy_every, trajectory_file,
 print_conf_interval, lastconf_file, fix_diffusion, fix_diffusion_every,
-CUDA_sort_every, CUDA_device.  sim_type must be MD and interaction_type
+CUDA_sort_every, CUDA_device, checkpoint_every, checkpoint_file,
+reload_from.  A checkpoint file holds the step (one JSON line) and the
+batch's whole state (oxh_batch_save); reload_from continues from it
+bitwise, the batch being built from topology and conf_file first (the
+reference forbids conf_file beside reload_from; here it only shapes the
+batch).  sim_type must be MD and interaction_type
 DNA2 or DNA2_nomesh (the GPU interaction is the analytic DNA2_nomesh form);
 use_average_seq = false is refused (sequence-dependent parameter files are
 not read here).
@@ -118,6 +123,8 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
     directory)."""
     from .backend import HIPBatch
 
+    import json
+
     inp = io.read_input(input_path, overrides)
     wd = workdir or os.path.dirname(os.path.abspath(input_path))
     path = lambda p: p if os.path.isabs(p) else os.path.join(wd, p)  # noqa: E731
@@ -134,11 +141,21 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
     energy_path = path(inp.get("energy_file", "energy.dat"))
     traj_path = path(inp["trajectory_file"]) if "trajectory_file" in inp else None
     last_path = path(inp.get("lastconf_file", "last_conf.dat"))
+    ckpt_every = _num(inp, "checkpoint_every", int, 0)
+    ckpt_path = path(inp["checkpoint_file"]) if "checkpoint_file" in inp else None
+    reload = None
+    if "reload_from" in inp:
+        with open(path(inp["reload_from"]), "rb") as f:
+            head = json.loads(f.readline())
+            reload = f.read()
+        start = int(head["step"])
 
     lines = []
     with HIPBatch(s, device=device, **opts) as b:
         b.set_sort_interval(_num(inp, "CUDA_sort_every", int, 0))
         b.set_fix_diffusion(0)   # re-boxing is done here, at the reference's steps
+        if reload is not None:
+            b.restore(reload)
 
         def snapshot(step):
             E = b.energies()[0]
@@ -154,7 +171,9 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
             cur = start
             plan = output_steps(start, steps, energy_every, conf_every)
             fixes = [t for t in range(start + fix_every, start + steps + 1, fix_every)] if fix_every > 0 else []
-            events = sorted(set([t for t, _, _ in plan] + fixes))
+            ckpts = ([t for t in range(start + ckpt_every, start + steps + 1, ckpt_every)]
+                     if ckpt_every > 0 and ckpt_path else [])
+            events = sorted(set([t for t, _, _ in plan] + fixes + ckpts))
             todo = {t: (e, c) for t, e, c in plan}
             end = start + steps
             for t in events:
@@ -181,6 +200,11 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
                     energy_f.write(line + "\n")
                     energy_f.flush()
                     lines.append(line)
+                if t in ckpts:
+                    data = b.checkpoint()
+                    with open(ckpt_path, "wb") as f:
+                        f.write((json.dumps({"step": t}) + "\n").encode())
+                        f.write(data)
             if cur < start + steps:
                 b.run(start + steps - cur)
                 cur = start + steps

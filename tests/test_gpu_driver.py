@@ -91,3 +91,93 @@ def test_driver_follows_reference_nve(hip, tmp_path, precision, e_tol, x_tol):
     print(f"{precision}: energy file {de:.1e}, positions/axes {dx:.1e}, velocities/L {dv:.1e}")
     assert de <= e_tol
     assert dx <= x_tol and dv <= 20 * x_tol
+
+
+@pytest.mark.parametrize("case", ["grid_langevin_sorted", "grid_bussi", "workgroup", "fp64"])
+def test_checkpoint_resumes_bitwise(hip, monkeypatch, case):
+    """300 steps in one call equal 150 steps, a checkpoint, a new batch of
+    the same systems restored from it and 150 more steps, bit for bit:
+    positions, orientations, velocities, angular momenta, forces.  Covers
+    Langevin noise (Philox counters), Bussi's internal kinetic energies and
+    its step alignment, Hilbert re-sorts (the particle order is part of
+    the state, sorts fall on the same steps), the workgroup engine and the
+    FP64 form."""
+    from oxdna_amd import systems
+    opts = dict(dt=0.003, verlet_skin=0.05, seed=3)
+    sort = 0
+    if case == "workgroup":
+        s = [h for h in systems.hairpin_set(5, n=8)][:4]
+        opts.update(thermostat="langevin", diff_coeff=2.5)
+    else:
+        monkeypatch.setenv("OXH_PERSIST", "0")
+        s = [systems.duplex_gas(5, n_duplex=64, bp=12, density=2e-3)]
+        if case == "grid_bussi":
+            opts.update(thermostat="bussi", newtonian_steps=10, bussi_tau=100)
+        else:
+            opts.update(thermostat="langevin", diff_coeff=2.5)
+        if case == "grid_langevin_sorted":
+            sort = 100
+        if case == "fp64":
+            opts.update(backend_precision="double")
+
+    def final(b):
+        out = []
+        for k in range(len(s)):
+            st = b.state(k)
+            F, T, E = b.forces(k)
+            out.append(dict(st, F=F, T=T, E=E))
+        return out
+
+    with hip.HIPBatch(s, **opts) as a:
+        a.set_sort_interval(sort)
+        a.run(300)
+        want = final(a)
+        engine = a.stats()["engine"]
+    with hip.HIPBatch(s, **opts) as b:
+        b.set_sort_interval(sort)
+        b.run(150)
+        ck = b.checkpoint()
+    with hip.HIPBatch(s, **opts) as c:
+        c.set_sort_interval(sort)
+        c.restore(ck)
+        c.run(150)
+        got = final(c)
+    assert engine == ("workgroup" if case == "workgroup" else "grid")
+    for w, g in zip(want, got):
+        for key in ("pos", "a1", "a3", "vel", "L", "F", "T", "E"):
+            assert np.array_equal(w[key], g[key]), (case, key, np.abs(w[key] - g[key]).max())
+
+
+def test_checkpoint_rejects_another_batch(hip):
+    from oxdna_amd import _lib, systems
+    with hip.HIPBatch(systems.duplex_20bp(0)) as a:
+        ck = a.checkpoint()
+    with hip.HIPBatch(systems.duplex_20bp(0), backend_precision="double") as b:
+        with pytest.raises(_lib.OxhError, match="another batch|checkpoint"):
+            b.restore(ck)
+
+
+def test_driver_checkpoint_reload(hip, tmp_path):
+    """The driver with checkpoint_every / checkpoint_file and reload_from
+    (the reference's keys): 150 steps, a checkpoint, 150 more steps reloaded
+    from it give the same last configuration, digit for digit, as 300
+    steps in one run (Langevin thermostat, so the random streams resume
+    too)."""
+    from oxdna_amd import driver
+    d = os.path.join(GOLDEN, "driver_nve")
+    for f in ("input", "t.top", "c.dat"):
+        shutil.copy(os.path.join(d, f), tmp_path / f)
+    common = ["thermostat = langevin", "diff_coeff = 2.5", "print_conf_interval = 0", "trajectory_file = traj.dat"]
+    driver.run(str(tmp_path / "input"), common + ["steps = 300", "lastconf_file = last_a.dat", "energy_file = e_a.dat"])
+    driver.run(str(tmp_path / "input"), common + ["steps = 150", "lastconf_file = last_b1.dat", "energy_file = e_b1.dat",
+                                                    "checkpoint_every = 150", "checkpoint_file = ck.bin"])
+    res = driver.run(str(tmp_path / "input"), common + ["steps = 150", "lastconf_file = last_b2.dat", "energy_file = e_b2.dat",
+                                                          "reload_from = ck.bin"])
+    assert res["step"] == 300
+    a = (tmp_path / "last_a.dat").read_text()
+    b = (tmp_path / "last_b2.dat").read_text()
+    assert a == b
+    # the energy lines of the second half continue the first run's
+    ea = (tmp_path / "e_a.dat").read_text().splitlines()
+    eb = (tmp_path / "e_b1.dat").read_text().splitlines() + (tmp_path / "e_b2.dat").read_text().splitlines()[1:]
+    assert ea == eb
