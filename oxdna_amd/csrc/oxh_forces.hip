@@ -223,7 +223,7 @@ __global__ void __launch_bounds__(256) k_edge_gather(int N, const double4 *__res
 // i receives -Ff at its backbone site, i.e. the torque back x (-Ff); in the
 // body frame back = (b1, b2, 0), so with Ff_body = (f1, f2, f3) that torque
 // is (-b2 f3, b1 f3, b2 f1 - b1 f2).
-template<int K, bool SPLIT>
+template<int K, bool SPLIT, bool SMALL>
 __global__ void __launch_bounds__(256) k_forces_far(int N, const double4 *__restrict__ pos, const float4 *__restrict__ q32,
 		const float4 *__restrict__ bpos, const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh,
 		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, float4 *__restrict__ Fio, float4 *__restrict__ Tio,
@@ -241,11 +241,15 @@ __global__ void __launch_bounds__(256) k_forces_far(int N, const double4 *__rest
 	f3 Ff = {0, 0, 0};
 	float Ef = 0.f;
 	float sv[6] = {0, 0, 0, 0, 0, 0};
+	// SMALL: some system's box is small enough that the nearest image of two
+	// backbone sites may differ from the centres' one (then the centres'
+	// image plus the site offsets, as the reference takes it); compiled out
+	// otherwise, which frees registers for more waves
 	Frame fp;
-	if(S.small_box) fp = make_frame(q32[i], 0);
-	// four pairs per iteration, their loads issued together; each lane still
+	if(SMALL && S.small_box) fp = make_frame(q32[i], 0);
+	// eight pairs per iteration, their loads issued together; each lane still
 	// adds its pairs in list order
-	constexpr int U = 4;
+	constexpr int U = 8;
 	for(int k0 = l; k0 < nf; k0 += U * K) {
 		int jj[U];
 		float4 Bj[U];
@@ -258,7 +262,7 @@ __global__ void __launch_bounds__(256) k_forces_far(int N, const double4 *__rest
 			if(jj[u] < 0) continue;
 			const int j = jj[u];
 			f3 rb;
-			if(!S.small_box) rb = min_image_f(Bi, Bj[u], S);
+			if(!(SMALL && S.small_box)) rb = min_image_f(Bi, Bj[u], S);
 			else {
 				const Frame fq = make_frame(q32[j], 0);
 				rb = min_image(pos[i], pos[j], S) + fq.back - fp.back;
@@ -452,16 +456,25 @@ static void launch_k(hipStream_t st, const Batch &b, bool split, int block, bool
 	}
 }
 
-template<int K>
-static void launch_far(hipStream_t st, const Batch &b, bool split, int block, int k) {
+template<int K, bool SMALL>
+static void launch_far_k(hipStream_t st, const Batch &b, bool split, int block, int k) {
 	size_t threads = (size_t) b.N * K;
 	dim3 grid((unsigned) ((threads + block - 1) / block));
 	if(split)
-		k_forces_far<K, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F, b.T,
-				b.esplit, k, k >= 0 ? b.limit : nullptr, b.stress);
+		k_forces_far<K, true, SMALL><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F,
+				b.T, b.esplit, k, k >= 0 ? b.limit : nullptr, b.stress);
 	else
-		k_forces_far<K, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F, b.T,
-				nullptr, k, k >= 0 ? b.limit : nullptr, nullptr);
+		k_forces_far<K, false, SMALL><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F,
+				b.T, nullptr, k, k >= 0 ? b.limit : nullptr, nullptr);
+}
+
+// slot pools take any system later, so they keep the small-box path
+template<int K>
+static void launch_far(hipStream_t st, const Batch &b, bool split, int block, int k) {
+	bool small = b.limit != nullptr;
+	for(const oxh_sys &S : b.h_sys) small |= S.small_box != 0;
+	if(small) launch_far_k<K, true>(st, b, split, block, k);
+	else launch_far_k<K, false>(st, b, split, block, k);
 }
 
 template<int P>
