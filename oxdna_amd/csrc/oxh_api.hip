@@ -966,7 +966,7 @@ oxh_status oxh_batch_destroy(oxh_batch *B) {
 	void *ptrs[] = {b.pos, b.vel, b.Lm, b.q64, b.rlist, b.q32, b.bpos, b.F, b.T, b.bonds, b.pinfo, b.orig, b.pstep, b.nbr, b.nnbr,
 			b.cell_count, b.cell_items, b.cell_owner, b.pos32, b.d_sys, b.d_state, b.red_blocks, b.red_sys_blocks, b.red_partial, b.red_out, b.red_done,
 			b.esplit, b.stress, b.hbcount, b.keys, b.keys_alt, b.perm, b.perm_alt, b.inv_perm, b.radix_hist, b.scratch, B->bk_pos, B->bk_vel,
-			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state, B->bk_rlist, b.limit, b.any_rebuild, b.F64, b.T64, B->bk_F64, B->bk_T64, b.eown, b.ein, b.ecount, b.E1, b.E2, b.E3, b.B1, b.B2, b.B3, b.cell_raw, b.cell_of};
+			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state, B->bk_rlist, b.limit, b.any_rebuild, b.F64, b.T64, B->bk_F64, B->bk_T64, b.eown, b.ein, b.ecount, b.E1, b.E2, b.E3, b.B1, b.B2, b.B3, b.cell_raw, b.cell_of, b.back64};
 	for(void *p : ptrs)
 		if(p) (void) hipFree(p);
 	if(b.stream) (void) hipStreamDestroy(b.stream);
@@ -1101,6 +1101,7 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 	if(b.fp64) {
 		TRYB(dalloc(B, &b.F64, N));
 		TRYB(dalloc(B, &b.T64, N));
+		TRYB(dalloc(B, &b.back64, N));
 		TRYB(dalloc(B, &B->bk_F64, N));
 		TRYB(dalloc(B, &B->bk_T64, N));
 	}

@@ -327,13 +327,14 @@ __global__ void __launch_bounds__(64) k_forces_dp(int N, const double4 *__restri
 }
 
 // All-FP64 far pairs on their own: Debye-Hueckel between FP64 backbone
-// sites (centres' nearest image, then the site offsets), with the few
+// sites (centres' nearest image, then the site offsets, read from back64
+// as the state mirror wrote them this step), with the few
 // registers that let many waves hide the gathers, next to the
 // register-heavy k_forces_dp.  Adds onto its F / T (the same thread owns a
 // particle's outputs in both kernels; fixed order, no atomics).
 template<int K, bool SPLIT>
 __global__ void __launch_bounds__(256) k_forces_far_dp(int N, const double4 *__restrict__ pos, const double4 *__restrict__ q64,
-		const float4 *__restrict__ bpos, const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh,
+		const double4 *__restrict__ back64, const float4 *__restrict__ bpos, const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh,
 		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, double4 *__restrict__ Fio, double4 *__restrict__ Tio,
 		double *__restrict__ esplit, int k_step, const int *__restrict__ limit, double *__restrict__ stress) {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -354,11 +355,10 @@ __global__ void __launch_bounds__(256) k_forces_far_dp(int N, const double4 *__r
 	double sv[6] = {0, 0, 0, 0, 0, 0};
 	for(int k = l; k < nf; k += K) {
 		const int j = row[-k];
-		Sites64 sj;
-		sites_d(q64[j], sj);
+		const double4 bj = back64[j];
 		const dv3 rc = dp::min_image_d(Pi, pos[j], S);
 		dv3 fq_ = {0, 0, 0};
-		Ef += dp::debye_huckel(S, dp::site_sep(rc, sj.back, si.back), cfi * (double) bpos[j].w, fq_);
+		Ef += dp::debye_huckel(S, dp::site_sep(rc, dv3{bj.x, bj.y, bj.z}, si.back), cfi * (double) bpos[j].w, fq_);
 		Ff += fq_;
 		if(SPLIT) dp::add_virial(sv, rc, fq_);
 	}
@@ -409,10 +409,10 @@ static void launch_far_dp(hipStream_t st, const Batch &b, bool split, int k) {
 	dim3 grid((unsigned) ((threads + block - 1) / block));
 	const int *lim = k >= 0 ? b.limit : nullptr;
 	if(split)
-		k_forces_far_dp<K, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F64,
+		k_forces_far_dp<K, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.back64, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F64,
 				b.T64, b.esplit, k, lim, b.stress);
 	else
-		k_forces_far_dp<K, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F64,
+		k_forces_far_dp<K, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.back64, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F64,
 				b.T64, nullptr, k, lim, nullptr);
 }
 

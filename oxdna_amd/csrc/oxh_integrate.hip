@@ -22,23 +22,24 @@
 namespace oxh {
 
 __global__ void k_prepare_mirror(int N, const double4 *__restrict__ pos, const double4 *__restrict__ q64, const int2 *__restrict__ bonds,
-		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, float4 *__restrict__ q32, float4 *__restrict__ bpos) {
+		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, float4 *__restrict__ q32, float4 *__restrict__ bpos,
+		double4 *__restrict__ back64) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= N) return;
 	const oxh_sys &S = systems[pinfo[i].y];
 	if(padding(i, S)) return;
-	write_mirror(i, pos[i], q64[i], S, bonds[i], q32, bpos);
+	write_mirror(i, pos[i], q64[i], S, bonds[i], q32, bpos, back64);
 }
 
 void launch_prepare_mirror(hipStream_t st, const Batch &b) {
 	if(b.N == 0) return;
-	k_prepare_mirror<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.q64, b.bonds, b.pinfo, b.d_sys, b.q32, b.bpos);
+	k_prepare_mirror<<<(b.N + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.q64, b.bonds, b.pinfo, b.d_sys, b.q32, b.bpos, b.back64);
 }
 
 template<class FT>
 __global__ void __launch_bounds__(256) k_first_step(int N, double4 *__restrict__ pos, double4 *__restrict__ vel, double4 *__restrict__ Lm,
-		double4 *__restrict__ q64, float4 *__restrict__ q32, float4 *__restrict__ bpos, const FT *__restrict__ F,
-		const FT *__restrict__ T, const double4 *__restrict__ rlist, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
+		double4 *__restrict__ q64, float4 *__restrict__ q32, float4 *__restrict__ bpos, double4 *__restrict__ back64,
+		const FT *__restrict__ F, const FT *__restrict__ T, const double4 *__restrict__ rlist, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
 		long long *__restrict__ pstep, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st, int ncell,
 		int *__restrict__ cell_count, int k, const int *__restrict__ limit, int *__restrict__ any) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -49,7 +50,7 @@ __global__ void __launch_bounds__(256) k_first_step(int N, double4 *__restrict__
 	const int s = pinfo[i].y;
 	const oxh_sys &S = systems[s];
 	if(idle(i, S, s, k, limit)) return;
-	if(first_step_particle(i, S, pos, vel, Lm, q64, q32, bpos, F, T, rlist, bonds, pstep)) {
+	if(first_step_particle(i, S, pos, vel, Lm, q64, q32, bpos, back64, F, T, rlist, bonds, pstep)) {
 		st[s].need_rebuild = 1;
 		*any = 1;
 	}
@@ -58,10 +59,10 @@ __global__ void __launch_bounds__(256) k_first_step(int N, double4 *__restrict__
 void launch_first_step(hipStream_t st, const Batch &b, int k) {
 	int n = std::max(b.N, b.ncell_total);
 	if(b.fp64)
-		k_first_step<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.F64, b.T64, b.rlist, b.bonds, b.pinfo,
+		k_first_step<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.back64, b.F64, b.T64, b.rlist, b.bonds, b.pinfo,
 				b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild);
 	else
-		k_first_step<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.F, b.T, b.rlist, b.bonds, b.pinfo,
+		k_first_step<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.back64, b.F, b.T, b.rlist, b.bonds, b.pinfo,
 				b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild);
 }
 
@@ -96,8 +97,8 @@ void launch_second_step(hipStream_t st, const Batch &b, int k) {
 template<class FT>
 __global__ void __launch_bounds__(256) k_second_first(int N, double4 *__restrict__ pos, double4 *__restrict__ vel,
 		double4 *__restrict__ Lm, double4 *__restrict__ q64, float4 *__restrict__ q32, float4 *__restrict__ bpos,
-		const FT *__restrict__ F, const FT *__restrict__ T, const double4 *__restrict__ rlist, const int2 *__restrict__ bonds,
-		const int2 *__restrict__ pinfo, const int *__restrict__ orig, long long *__restrict__ pstep, const oxh_sys *__restrict__ systems,
+		double4 *__restrict__ back64, const FT *__restrict__ F, const FT *__restrict__ T, const double4 *__restrict__ rlist,
+		const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo, const int *__restrict__ orig, long long *__restrict__ pstep, const oxh_sys *__restrict__ systems,
 		oxh_sys_state *__restrict__ st, int ncell, int *__restrict__ cell_count, int k, const int *__restrict__ limit,
 		int *__restrict__ any) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -108,7 +109,7 @@ __global__ void __launch_bounds__(256) k_second_first(int N, double4 *__restrict
 	if(padding(i, S)) return;
 	if(!(limit && k >= limit[s])) second_step_particle(i, S, vel, Lm, F, T, orig, pstep);
 	if(!(limit && k + 1 >= limit[s])) {
-		if(first_step_particle(i, S, pos, vel, Lm, q64, q32, bpos, F, T, rlist, bonds, pstep)) {
+		if(first_step_particle(i, S, pos, vel, Lm, q64, q32, bpos, back64, F, T, rlist, bonds, pstep)) {
 			st[s].need_rebuild = 1;
 			*any = 1;
 		}
@@ -118,10 +119,10 @@ __global__ void __launch_bounds__(256) k_second_first(int N, double4 *__restrict
 void launch_second_first(hipStream_t st, const Batch &b, int k) {
 	int n = std::max(b.N, b.ncell_total);
 	if(b.fp64)
-		k_second_first<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.F64, b.T64, b.rlist, b.bonds,
+		k_second_first<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.back64, b.F64, b.T64, b.rlist, b.bonds,
 				b.pinfo, b.orig, b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild);
 	else
-		k_second_first<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.F, b.T, b.rlist, b.bonds, b.pinfo,
+		k_second_first<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.back64, b.F, b.T, b.rlist, b.bonds, b.pinfo,
 				b.orig, b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild);
 }
 

@@ -25,6 +25,7 @@ struct Batch {
 	// all-FP64 mode (backend_precision = double): forces and torques in FP64
 	bool fp64 = false;
 	double4 *F64 = nullptr, *T64 = nullptr;
+	double4 *back64 = nullptr;   // FP64 backbone offsets (POS_MM_BACK1 a1 + POS_MM_BACK2 a2), written with the mirror
 	float4 *bpos = nullptr;   // backbone-site position wrapped into the box (FP32), .w = Debye-Hueckel charge factor
 	int2 *bonds = nullptr;   // (n3, n5) batch indices, -1 = none
 	int2 *pinfo = nullptr;   // (btype, system)

@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(BLOCK) k_persist(PersistArgs a) {
 	__syncthreads();
 	for(int it = 0; it < nsteps; it++) {
 		// 1. first half kick, drift, rotation, FP32 mirror, skin check
-		if(tid < n && first_step_particle(first + tid, S, a.pos, a.vel, a.Lm, a.q64, a.q32, a.bpos, a.F, a.T, a.rlist, a.bonds, a.pstep))
+		if(tid < n && first_step_particle(first + tid, S, a.pos, a.vel, a.Lm, a.q64, a.q32, a.bpos, nullptr, a.F, a.T, a.rlist, a.bonds, a.pstep))
 			need = 1;
 		__syncthreads();
 		// 2. Verlet rebuild (all pairs of the system; such systems use no cells)

@@ -30,7 +30,10 @@ __device__ __forceinline__ double4 qmul(double4 a, double4 b) {
 // into the box in FP64 and rounded to FP32, with the particle's
 // Debye-Hueckel charge factor in .w (0.5 at strand ends when
 // dh_half_charged_ends, src/Interactions/DNA2Interaction.cpp:162-165)
-__device__ __forceinline__ void write_mirror(int i, double4 r, double4 q, const oxh_sys &S, int2 bd, float4 *q32, float4 *bpos) {
+// back64 (all-FP64 mode, else nullptr): the FP64 backbone offset
+// POS_MM_BACK1 a1 + POS_MM_BACK2 a2 of the particle, for the FP64 far pairs
+__device__ __forceinline__ void write_mirror(int i, double4 r, double4 q, const oxh_sys &S, int2 bd, float4 *q32, float4 *bpos,
+		double4 *back64) {
 	q32[i] = make_float4((float) q.x, (float) q.y, (float) q.z, (float) q.w);
 	double sqx = q.x * q.x, sqy = q.y * q.y, sqz = q.z * q.z, sqw = q.w * q.w;
 	double xy = q.x * q.y, xz = q.x * q.z, xw = q.x * q.w, yz = q.y * q.z, yw = q.y * q.w, zw = q.z * q.w;
@@ -43,6 +46,7 @@ __device__ __forceinline__ void write_mirror(int i, double4 r, double4 q, const 
 	by -= S.box[1] * floor(by * S.inv_box[1]);
 	bz -= S.box[2] * floor(bz * S.inv_box[2]);
 	bpos[i] = make_float4((float) bx, (float) by, (float) bz, cf);
+	if(back64) back64[i] = make_double4(b1 * a1x + b2 * a2x, b1 * a1y + b2 * a2y, b1 * a1z + b2 * a2z, 0.0);
 }
 
 // first half kick, drift and rotation of particle i, FP32 mirror, step
@@ -51,8 +55,8 @@ __device__ __forceinline__ void write_mirror(int i, double4 r, double4 q, const 
 template<class FT>
 __device__ __forceinline__ bool first_step_particle(int i, const oxh_sys &S, double4 *__restrict__ pos, double4 *__restrict__ vel,
 		double4 *__restrict__ Lm, double4 *__restrict__ q64, float4 *__restrict__ q32, float4 *__restrict__ bpos,
-		const FT *__restrict__ F, const FT *__restrict__ T, const double4 *__restrict__ rlist, const int2 *__restrict__ bonds,
-		long long *__restrict__ pstep) {
+		double4 *__restrict__ back64, const FT *__restrict__ F, const FT *__restrict__ T, const double4 *__restrict__ rlist,
+		const int2 *__restrict__ bonds, long long *__restrict__ pstep) {
 	const double hdt = 0.5 * S.dt;
 
 	const FT f = F[i];
@@ -87,7 +91,7 @@ __device__ __forceinline__ bool first_step_particle(int i, const oxh_sys &S, dou
 		q.w *= qn;
 		q64[i] = q;
 	}
-	write_mirror(i, r, q, S, bonds[i], q32, bpos);
+	write_mirror(i, r, q, S, bonds[i], q32, bpos, back64);
 	pstep[i] += 1;
 
 	double4 rl = rlist[i];
