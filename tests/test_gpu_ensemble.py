@@ -1,0 +1,89 @@
+"""Ensemble parity with the reference binary: the 20-bp duplex (random ACGT
+sequence, cubic box of 20, 0.5 M, dt 0.003, Langevin with diff_coeff 2.5)
+sampled by the HIP path and by the reference's CPU backend under the same
+protocol: independent runs of 3e5 steps from the same ideal helix with
+fresh thermal velocities, a sample every 1000 steps, the first 5e4 steps
+dropped.
+
+The reference's series (tests/golden/ensemble_duplex20.npz, written by
+tests/golden/make_ensemble_golden.py: 48 runs per temperature, 20 C and
+50 C) give the means of the potential energy per nucleotide, the kinetic
+energy per nucleotide and the base-pair fraction (hydrogen-bonded pairs
+with energy below -0.1, the reference's hb_list observable with only_count,
+src/Observables/HBList.cpp:64-164, over 20 possible pairs).  The HIP path
+runs 64 replicas of the same system in one batch, each with its own seed.
+The duplex's slow modes (fraying ends) decorrelate over 1e5 steps or more,
+so each run contributes one mean and the standard error comes from the
+spread of those independent run means; the two codes' means agree within
+two combined standard errors.
+
+The CPU test pins the fixture's statistics on a closed form: Langevin
+equipartition gives <K/N> = 3T (translational and rotational, 3T/2 each).
+"""
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ensemble_duplex20.npz")
+BURN = 50      # samples (of 1000 steps) dropped at the start of each run
+REPLICAS = 64
+
+
+def run_stats(x):
+    """Mean over independent runs of x (runs, samples) after the burn-in,
+    and its standard error from the spread of the run means."""
+    m = np.asarray(x, np.float64)[:, BURN:].mean(axis=1)
+    return float(m.mean()), float(m.std(ddof=1) / np.sqrt(m.size))
+
+
+def reference():
+    z = np.load(GOLDEN)
+    return z, [float(t) for t in z["temps_c"]]
+
+
+def test_reference_ensemble_equipartition():
+    from oxdna_amd import systems
+    z, temps = reference()
+    for ti, tc in enumerate(temps):
+        T = systems.celsius(tc)
+        m, se = run_stats(z["K"][ti])
+        print(f"reference {tc:.0f} C: <K/N> {m:.5f} +- {se:.5f}, 3T = {3 * T:.5f}")
+        assert abs(m - 3 * T) <= 3 * se + 1e-6 * T, (tc, m, se, 3 * T)
+        # the fixture is an equilibrium series of a bound duplex
+        assert z["hb"][ti].min() >= 0 and z["hb"][ti].max() <= 20
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ti", [0, 1], ids=["20C", "50C"])
+def test_ensemble_matches_reference(ti):
+    from oxdna_amd import _lib, backend, systems
+    assert _lib.device_count() >= 1, "no GPU visible"
+    z, temps = reference()
+    tc = temps[ti]
+    s = systems.duplex_20bp(int(z["system_seed"]))
+    s.T = systems.celsius(tc)
+    every = int(z["every"])
+    n = int(z["steps"]) // every + 1
+    U = np.zeros((REPLICAS, n))
+    K = np.zeros((REPLICAS, n))
+    hb = np.zeros((REPLICAS, n))
+    opts = [dict(seed=9000 + 37 * ti + r) for r in range(REPLICAS)]
+    with backend.HIPBatch([s] * REPLICAS, options=opts, thermostat="langevin", diff_coeff=2.5, dt=0.003,
+                          verlet_skin=0.05, refresh_vel=True, salt_concentration=0.5) as b:
+        for k in range(n):
+            if k:
+                b.run(every)
+            E = b.energies()
+            U[:, k] = E[:, 0] / s.N
+            K[:, k] = (E[:, 2] + E[:, 3]) / s.N
+            hb[:, k] = b.hb_count() / 20.0
+        nreb, err = b.info()
+        assert err == 0
+    for name, gpu, ref in (("U/N", U, z["U"][ti]), ("K/N", K, z["K"][ti]), ("bp fraction", hb, z["hb"][ti] / 20.0)):
+        mg, sg = run_stats(gpu)
+        mr, sr = run_stats(ref)
+        sig = np.hypot(sg, sr)
+        print(f"{tc:.0f} C {name}: HIP {mg:.5f} +- {sg:.5f}, reference {mr:.5f} +- {sr:.5f}, "
+              f"z = {(mg - mr) / sig:+.2f}")
+        assert abs(mg - mr) <= 2 * sig, (name, mg, sg, mr, sr)
