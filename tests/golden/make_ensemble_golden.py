@@ -1,4 +1,4 @@
-"""Writes tests/golden/ensemble_duplex20.npz with the reference binary:
+"""Writes tests/golden/ensemble/duplex20.npz with the reference binary:
 equilibrium time series of the 20-bp duplex (the reference's CPU-runnable
 case: random ACGT sequence, cubic box of 20, 0.5 M, dt 0.003, Langevin with
 diff_coeff 2.5) at 20 C and at 50 C.
@@ -36,7 +36,7 @@ sys.path.insert(0, ROOT)
 from oxdna_amd import io, systems  # noqa: E402
 
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "oxDNA_release")
-OUT = os.path.join(HERE, "ensemble_duplex20.npz")
+OUT = os.path.join(HERE, "ensemble", "duplex20.npz")
 
 SYSTEM_SEED = 0
 TEMPS_C = (20.0, 50.0)

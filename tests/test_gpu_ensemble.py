@@ -5,7 +5,7 @@ protocol: independent runs of 3e5 steps from the same ideal helix with
 fresh thermal velocities, a sample every 1000 steps, the first 5e4 steps
 dropped.
 
-The reference's series (tests/golden/ensemble_duplex20.npz, written by
+The reference's series (tests/golden/ensemble/duplex20.npz, written by
 tests/golden/make_ensemble_golden.py: 48 runs per temperature, 20 C and
 50 C) give the means of the potential energy per nucleotide, the kinetic
 energy per nucleotide and the base-pair fraction (hydrogen-bonded pairs
@@ -25,7 +25,7 @@ import os
 import numpy as np
 import pytest
 
-GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ensemble_duplex20.npz")
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ensemble", "duplex20.npz")
 BURN = 50      # samples (of 1000 steps) dropped at the start of each run
 REPLICAS = 64
 
