@@ -237,12 +237,17 @@ oxh_status oxh_batch_set_state(oxh_batch *b, int sys, const double *pos, const d
  * workgroup per system for many steps per launch, used when every system
  * has at most 512 nucleotides and no cells, 2: one kernel per phase with
  * the close pairs in half-list form, each pair evaluated once -- the
- * reference's use_edge), out[9] number of systems.
+ * reference's use_edge), out[9] number of systems, out[10] 1 when the
+ * close kernel takes the packed form (HB, cross and coaxial stacking of a
+ * workgroup's close pairs evaluated from work lists in LDS, one pair per
+ * thread; same results bit for bit), else 0.
  * Shapes are picked by timing at upload (the all-FP64 mode times the lanes
  * only, in blocks of 64 with the far pairs inline; pools keep 8 lanes);
  * OXH_FORCE_LANES / OXH_FAR_LANES fix them, OXH_PERSIST=0 disables the
- * workgroup engine, OXH_EDGE=0 / 1 forbids / forces the half-list form. */
-oxh_status oxh_batch_stats(oxh_batch *b, double *out10);
+ * workgroup engine, OXH_EDGE=0 / 1 forbids / forces the half-list form,
+ * OXH_PACK=0 / 1 forbids / forces the packed form (never used with one
+ * lane per nucleotide). */
+oxh_status oxh_batch_stats(oxh_batch *b, double *out11);
 
 /* Average duration in ms of n back-to-back launches of the force kernel on
  * the batch's stream, timed with HIP events (the reference times its steps

@@ -231,11 +231,12 @@ class HIPBatch:
         return ms.value
 
     def stats(self):
-        out = np.zeros(10)
+        out = np.zeros(11)
         check(lib().oxh_batch_stats(self.h, ptr(out)), "oxh_batch_stats")
         return dict(N=int(out[0]), max_neigh=int(out[1]), mean_neigh=out[2], device_bytes=int(out[3]),
                     lanes=int(out[4]), block=int(out[5]), far_lanes=int(out[6]), far_block=int(out[7]),
-                    engine={0: "grid", 1: "workgroup", 2: "grid-edge"}[int(out[8])], n_systems=int(out[9]))
+                    engine={0: "grid", 1: "workgroup", 2: "grid-edge"}[int(out[8])], n_systems=int(out[9]),
+                    pack=bool(out[10]))
 
 
 class HIPPool(HIPBatch):

@@ -782,8 +782,10 @@ oxh_status fix_diffusion_impl(oxh_batch *B, int *n_moved) {
 oxh_status autotune_forces(oxh_batch *B) {
 	oxh::Batch &b = B->b;
 	const char *edge_env = getenv("OXH_EDGE");
+	const char *pack_env = getenv("OXH_PACK");
 	if(const char *env = getenv("OXH_FORCE_LANES")) {
 		b.lanes = atoi(env);
+		b.pack = !b.fp64 && b.lanes > 1 && pack_env && atoi(pack_env) == 1;
 		if(const char *f = getenv("OXH_FAR_LANES")) b.far_lanes = atoi(f);
 		b.edge = b.ecap > 0 && edge_env && atoi(edge_env) == 1;
 		if(const char *g = getenv("OXH_EDGE_GROUP")) b.edge_group = atoi(g);
@@ -845,23 +847,38 @@ oxh_status autotune_forces(oxh_batch *B) {
 		return sync_and_check(B);
 	}
 	int bl = b.lanes, bb = b.force_block, bfl = 0, bfb = 256;
+	bool bp = false;
+	// close kernel shapes: lanes x workgroup size, and lanes with the
+	// packed form (OXH_PACK=0/1 keeps one form), at the current far shape
+	auto scan_close = [&]() -> oxh_status {
+		for(int pk = 0; pk < 2; pk++) {
+			if(pack_env && atoi(pack_env) != pk) continue;
+			b.pack = pk == 1;
+			for(int L : lanes)
+				for(int bs : blocks) {
+					if(b.pack && (bs != oxh::PACK_BS || L == 1)) continue;
+					b.lanes = L;
+					b.force_block = bs;
+					float ms;
+					TRY(timeit(ms));
+					if(ms < best) {
+						best = ms;
+						bl = L;
+						bb = bs;
+						bp = b.pack;
+					}
+				}
+		}
+		b.lanes = bl;
+		b.force_block = bb;
+		b.pack = bp;
+		return OXH_OK;
+	};
 	// 1) close kernel shape with the far pairs inline
 	b.far_lanes = 0;
-	for(int L : lanes)
-		for(int bs : blocks) {
-			b.lanes = L;
-			b.force_block = bs;
-			float ms;
-			TRY(timeit(ms));
-			if(ms < best) {
-				best = ms;
-				bl = L;
-				bb = bs;
-			}
-		}
-	// 2) far pairs in their own kernel, close kernel at its best shape
-	b.lanes = bl;
-	b.force_block = bb;
+	TRY(scan_close());
+	// 2) far pairs in their own kernel, close kernel at its best shape; if
+	// that wins, the close shape again next to the far kernel
 	for(int L : lanes)
 		for(int bs : blocks) {
 			b.far_lanes = L;
@@ -876,6 +893,7 @@ oxh_status autotune_forces(oxh_batch *B) {
 		}
 	b.far_lanes = bfl;
 	b.far_block = bfb;
+	if(bfl != 0) TRY(scan_close());
 	// 3) the half-list form of the close pairs: particles per wave of the
 	// pair kernel x lanes per particle of the gather (far pairs inline, or
 	// in the far kernel picked above)
@@ -1748,7 +1766,7 @@ oxh_status oxh_batch_set_state(oxh_batch *B, int sys, const double *pos, const d
 	return oxh_batch_compute_forces(B);
 }
 
-oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {  // 10 doubles
+oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {  // 11 doubles
 	if(!B || !out6) return fail(OXH_ERR_ARG, "bad arguments");
 	HIPCHK(hipSetDevice(B->b.device));
 	std::vector<int2> nn;
@@ -1771,6 +1789,7 @@ oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {  // 10 doubles
 	out6[7] = B->b.far_block;
 	out6[8] = B->persist ? 1.0 : (B->b.edge ? 2.0 : 0.0);
 	out6[9] = B->b.nsys;
+	out6[10] = B->b.pack ? 1.0 : 0.0;
 	return OXH_OK;
 }
 

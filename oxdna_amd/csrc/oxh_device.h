@@ -84,6 +84,8 @@ constexpr float F5_A = 2.0f, F5_B = 10.9032f, F5_XC = -0.769231f, F5_XS = -0.65f
 // site-pair range (back-back excluded volume, 0.7119) plus twice the
 // backbone-site arm |(-0.34, 0.3408)| = 0.4814
 constexpr float R_CLOSE = 1.676f;
+// workgroup size of the packed form of the close kernel
+constexpr int PACK_BS = 256;
 
 // ------------------------------------------------------------ small vectors
 struct f3 {

@@ -26,8 +26,11 @@ namespace oxh {
 // K threads per nucleotide: lane l of nucleotide i takes list entries
 // l, l + K, ... of i's close sublist, then of its far sublist, and the n3 /
 // n5 bonded pairs go to lanes 0 / 1.  Partial sums are combined by a fixed
-// butterfly of wave shuffles, so results do not depend on timing.
-template<int K, bool SPLIT, bool FAR>
+// butterfly of wave shuffles, so results do not depend on timing.  PACK
+// (K >= 2): workgroups of PACK_BS threads evaluate HB, CRST and CXST from
+// LDS work lists (the packed form in particle_forces); same results bit
+// for bit.
+template<int K, bool SPLIT, bool FAR, bool PACK>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_forces(int N, const double4 *__restrict__ pos, const double4 *__restrict__ q64,
 		const float4 *__restrict__ q32,
 		const float4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
@@ -38,16 +41,31 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 	const int i = t / K;
 	const int l = t % K;
 	if(k >= 0 && t == 0) *any = 0;   // inside a step: this step's list builds are done
-	if(i >= N) return;
-	const int s = pinfo[i].y;
-	if(k >= 0 && l == 0 && i == systems[s].first && sstate[s].need_rebuild) {
-		sstate[s].need_rebuild = 0;
-		sstate[s].n_rebuilds += 1;
+	if constexpr(PACK) {
+		// every thread of the workgroup takes part in the work lists'
+		// barriers; threads past N stand in for particle N - 1, inactive
+		__shared__ PackShared sh;
+		const int ic = i < N ? i : N - 1;
+		const int s = pinfo[ic].y;
+		if(i < N && k >= 0 && l == 0 && i == systems[s].first && sstate[s].need_rebuild) {
+			sstate[s].need_rebuild = 0;
+			sstate[s].n_rebuilds += 1;
+		}
+		const bool act = i < N && !idle(i, systems[s], s, k, limit);
+		particle_forces<SPLIT, FAR, true>(K, ic, l, N, pos, q64, q32, bpos, bonds, pinfo, nbr, nnbr, max_neigh, systems, sstate, Fout, Tout,
+				esplit, hbcount, stress, act, &sh);
 	}
-	if(idle(i, systems[s], s, k, limit)) return;
-
-	particle_forces<SPLIT, FAR>(K, i, l, N, pos, q64, q32, bpos, bonds, pinfo, nbr, nnbr, max_neigh, systems, sstate, Fout, Tout, esplit, hbcount,
-			stress);
+	else {
+		if(i >= N) return;
+		const int s = pinfo[i].y;
+		if(k >= 0 && l == 0 && i == systems[s].first && sstate[s].need_rebuild) {
+			sstate[s].need_rebuild = 0;
+			sstate[s].n_rebuilds += 1;
+		}
+		if(idle(i, systems[s], s, k, limit)) return;
+		particle_forces<SPLIT, FAR>(K, i, l, N, pos, q64, q32, bpos, bonds, pinfo, nbr, nnbr, max_neigh, systems, sstate, Fout, Tout, esplit,
+				hbcount, stress);
+	}
 }
 
 // Half-list form (b.edge), first half: every bond and every close pair
@@ -433,27 +451,34 @@ static void launch_dp(hipStream_t st, const Batch &b, bool split, int k) {
 	}
 }
 
-template<int K>
-static void launch_k(hipStream_t st, const Batch &b, bool split, int block, bool far_inline, int k) {
+template<int K, bool PACK>
+static void launch_kp(hipStream_t st, const Batch &b, bool split, int block, bool far_inline, int k) {
 	size_t threads = (size_t) b.N * K;
 	dim3 grid((unsigned) ((threads + block - 1) / block));
 	int *hbc = b.want_hbcount ? b.hbcount : nullptr;
 	if(split) {
 		if(far_inline)
-			k_forces<K, true, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-					b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, b.stress);
+			k_forces<K, true, true, PACK><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr,
+					b.max_neigh, b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, b.stress);
 		else
-			k_forces<K, true, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-					b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, b.stress);
+			k_forces<K, true, false, PACK><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr,
+					b.max_neigh, b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, b.stress);
 	}
 	else {
 		if(far_inline)
-			k_forces<K, false, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-					b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, nullptr);
+			k_forces<K, false, true, PACK><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr,
+					b.max_neigh, b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, nullptr);
 		else
-			k_forces<K, false, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-					b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, nullptr);
+			k_forces<K, false, false, PACK><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr,
+					b.max_neigh, b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, nullptr);
 	}
+}
+
+// b.pack: the packed form, in workgroups of PACK_BS threads
+template<int K>
+static void launch_k(hipStream_t st, const Batch &b, bool split, int block, bool far_inline, int k) {
+	if(b.pack) launch_kp<K, true>(st, b, split, PACK_BS, far_inline, k);
+	else launch_kp<K, false>(st, b, split, block, far_inline, k);
 }
 
 template<int K, bool SMALL>

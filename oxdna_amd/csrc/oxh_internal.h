@@ -99,6 +99,7 @@ struct Batch {
 	size_t scratch_bytes = 0;
 
 	int force_block = 256;
+	bool pack = false;   // close pairs' HB/CRST/CXST through LDS work lists (k_forces PACK)
 	int any_cells = 0;
 	int thermostat_mask = 0;     // bit t set if some system uses thermostat t
 	int min_newtonian = 1;

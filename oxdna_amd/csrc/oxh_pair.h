@@ -9,6 +9,22 @@
 
 namespace oxh {
 
+// Work lists of the packed form of k_forces (one per workgroup of
+// PACK_BS threads, in LDS): the pairs of one round whose base-base
+// separation reaches HB or CRST (h) and those whose stack-stack separation
+// reaches CXST (c), one item per slot, fields as separate arrays so that
+// consecutive slots are conflict-free.  Fields of h: p's a1, a3, q's a1, a3,
+// p's base offset, base-base separation (18 floats), p's and q's base type
+// and the system (ints).  Fields of c: p's a1, a3, q's a1, a3, p's stack
+// offset, stack-stack separation.  Evaluating an item overwrites its first
+// fields with the results: force on q, lab torque on p, energies.
+constexpr int PACK_H = 21, PACK_C = 18;
+struct PackShared {
+	float h[PACK_H][PACK_BS];
+	float c[PACK_C][PACK_BS];
+	int nh[PACK_BS / 64], nc[PACK_BS / 64], rounds[PACK_BS / 64];
+};
+
 // a nucleotide's FP64 a1 axis and backbone offset (the stacking and base
 // offsets are POS_STACK a1 and POS_BASE a1)
 struct Sites64 {

@@ -132,6 +132,49 @@ def test_launch_shapes(hip, monkeypatch, name, lanes, far_lanes, edge):
     assert np.all(np.abs(e8 - ref["E8"]) <= 1e-5 * scale)
 
 
+@pytest.mark.parametrize("lanes,far_lanes", [(2, 0), (4, 16), (8, 0), (32, 4)])
+def test_packed_form_bitwise(hip, monkeypatch, lanes, far_lanes):
+    """The packed form of the close kernel (HB, CRST and CXST of a
+    workgroup's close pairs evaluated from LDS work lists) gives the same
+    bits as the in-place form: forces, torques, per-particle energies, the
+    energy split, the stress tensor and the HB count, on a batch of
+    fixtures of different temperatures, salts and Debye-Hueckel options
+    (items carry their system), and the same bits for each system as in a
+    batch of its own.  (With one lane per nucleotide the packed form is not
+    used: OXH_PACK=1 leaves the in-place form.)"""
+    monkeypatch.setenv("OXH_PERSIST", "0")
+    monkeypatch.setenv("OXH_EDGE", "0")
+    monkeypatch.setenv("OXH_FORCE_LANES", str(lanes))
+    monkeypatch.setenv("OXH_FAR_LANES", str(far_lanes))
+    names = ["bundle_md_30C", "tangle3_nohalf_dh", "gas8_25C_0.5M", "duplex20_md_20C"]
+    loaded = [load_golden(n) for n in names]
+    systems = [x[0] for x in loaded]
+    options = [x[1] for x in loaded]
+
+    def run(pack, idx):
+        monkeypatch.setenv("OXH_PACK", str(pack))
+        with hip.HIPBatch([systems[k] for k in idx], options=[options[k] for k in idx]) as b:
+            assert b.stats()["pack"] == bool(pack) and b.stats()["lanes"] == lanes
+            b.compute_forces()
+            out = []
+            hb = b.hb_count()
+            for m in range(len(idx)):
+                F, T, Ep = b.forces(m)
+                w, _ = b.stress(m)
+                out.append((F, T, Ep, b.energy_split(m), w, hb[m]))
+        return out
+
+    allidx = list(range(len(names)))
+    ref = run(0, allidx)
+    got = run(1, allidx)
+    for m, n in enumerate(names):
+        for a, c in zip(ref[m], got[m]):
+            assert np.array_equal(np.asarray(a), np.asarray(c)), n
+    alone = run(1, [2])[0]
+    for a, c in zip(alone, got[2]):
+        assert np.array_equal(np.asarray(a), np.asarray(c))
+
+
 @pytest.mark.parametrize("lanes,group,far_lanes", [(1, 4, 0), (4, 16, 0), (16, 64, 0), (2, 8, 4)])
 @pytest.mark.parametrize("name", golden_names())
 def test_edge_forces_vs_reference(hip, monkeypatch, name, lanes, group, far_lanes):
