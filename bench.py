@@ -308,7 +308,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "copy_kernel_gbps": copy_gbps,
                      "kernel": "k_forces" if args.precision == "mixed" else "k_forces_dp", "avg_launch_us": force_ms * 1e3, "bytes_per_nt": b_alg,
                      "lanes_per_nt": st["lanes"], "block": st["block"], "far_lanes_per_nt": st["far_lanes"],
-                     "far_block": st["far_block"]},
+                     "far_block": st["far_block"], "packed": st["pack"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
