@@ -227,7 +227,7 @@ oxh_status oxh_batch_system_info(oxh_batch *b, int sys, long long *n_rebuilds, i
 oxh_status oxh_batch_set_state(oxh_batch *b, int sys, const double *pos, const double *a1, const double *a3, const double *vel,
 		const double *L);
 
-/* Neighbour-list, memory and launch-shape figures of the batch (10
+/* Neighbour-list, memory and launch-shape figures of the batch (11
  * doubles): out[0] N (a pool: its slots' total capacity), out[1]
  * max_neigh, out[2] mean neighbours per particle of the systems held,
  * out[3] device bytes held, out[4] threads per nucleotide and
