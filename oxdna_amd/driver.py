@@ -27,13 +27,16 @@ gamma_trans, pt, bussi_tau, refresh_vel, seed, restart_step_counter,
 backend_precision, dh_lambda, dh_strength, debye_huckel_rhigh,
 dh_half_charged_ends, hb_multiplier, max_backbone_force,
 max_backbone_force_far, energy_file, print_energy_every, trajectory_file,
-print_conf_interval, lastconf_file, fix_diffusion, fix_diffusion_every,
-CUDA_sort_every, CUDA_device, checkpoint_every, checkpoint_file,
-reload_from.  A checkpoint file holds the step (one JSON line) and the
-batch's whole state (oxh_batch_save); reload_from continues from it
-bitwise, the batch being built from topology and conf_file first (the
-reference forbids conf_file beside reload_from; here it only shapes the
-batch).  sim_type must be MD and interaction_type
+print_conf_interval, lastconf_file, lastconf_file_bin, binary_initial_conf,
+fix_diffusion, fix_diffusion_every, CUDA_sort_every, CUDA_device,
+checkpoint_every, checkpoint_file, reload_from.  Binary configurations are
+the reference's (oxdna_amd/io.py: write_conf_binary, read_conf_binary).
+A checkpoint file holds the step (one JSON line) and the batch's whole
+state (oxh_batch_save); reload_from continues from it bitwise, the batch
+being built from topology and conf_file first (here conf_file only shapes
+the batch).  reload_from also takes the reference's own reload file, a
+binary configuration, with the reference's rules: no conf_file, no seed,
+restart_step_counter false.  sim_type must be MD and interaction_type
 DNA2 or DNA2_nomesh (the GPU interaction is the analytic DNA2_nomesh form);
 use_average_seq = false is refused (sequence-dependent parameter files are
 not read here).
@@ -129,9 +132,35 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
     wd = workdir or os.path.dirname(os.path.abspath(input_path))
     path = lambda p: p if os.path.isabs(p) else os.path.join(wd, p)  # noqa: E731
     opts = batch_options(inp)
-    s = io.load_system(path(inp["topology"]), path(inp["conf_file"]), T=opts["T"], salt=opts["salt_concentration"])
-    conf_step, _, *_ = io.read_conf(path(inp["conf_file"]), s.N)
-    start = 0 if _bool(inp.get("restart_step_counter", "false")) else conf_step
+    restart = _bool(inp.get("restart_step_counter", "false"))
+    # reload_from: this library's checkpoint (a JSON line, then the batch's
+    # state), or the reference's own form, a binary configuration
+    # (src/Backends/SimBackend.cpp:127-150): conf_file and seed are then not
+    # given and the step counter is kept; the configuration holds positions,
+    # orientations and momenta, and its stored generator words seed Philox
+    reload_path = path(inp["reload_from"]) if "reload_from" in inp else None
+    ckpt_reload = False
+    if reload_path:
+        with open(reload_path, "rb") as f:
+            ckpt_reload = f.read(1) == b"{"
+    if reload_path and not ckpt_reload:
+        for key in ("conf_file", "seed"):
+            if key in inp:
+                raise ValueError(f"{key} cannot be specified together with reload_from (a binary configuration)")
+        if restart:
+            raise ValueError("restart_step_counter must be false when reload_from is specified")
+        conf_path, binary = reload_path, True
+        opts["refresh_vel"] = False
+    else:
+        conf_path, binary = path(inp["conf_file"]), _bool(inp.get("binary_initial_conf", "false"))
+    s = io.load_system(path(inp["topology"]), conf_path, T=opts["T"], salt=opts["salt_concentration"], binary=binary)
+    if binary:
+        conf_step, *_, words = io.read_conf_binary(conf_path, s.N)
+        if reload_path and not ckpt_reload:
+            opts["seed"] = words[0] | (words[1] << 16) | (words[2] << 32)
+    else:
+        conf_step, _, *_ = io.read_conf(conf_path, s.N)
+    start = 0 if restart else conf_step
     steps = _num(inp, "steps", int)
     dt = opts["dt"]
     energy_every = _num(inp, "print_energy_every", int, 0)
@@ -144,8 +173,8 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
     ckpt_every = _num(inp, "checkpoint_every", int, 0)
     ckpt_path = path(inp["checkpoint_file"]) if "checkpoint_file" in inp else None
     reload = None
-    if "reload_from" in inp:
-        with open(path(inp["reload_from"]), "rb") as f:
+    if ckpt_reload:
+        with open(reload_path, "rb") as f:
             head = json.loads(f.readline())
             reload = f.read()
         start = int(head["step"])
@@ -210,6 +239,8 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
                 cur = start + steps
             U, K, state = snapshot(cur)
             io.write_conf(last_path, state, step=cur, energies=(U + K, U, K))
+            if "lastconf_file_bin" in inp:
+                io.write_conf_binary(path(inp["lastconf_file_bin"]), state, step=cur, energies=(U + K, U, K))
         finally:
             energy_f.close()
             if traj_f:

@@ -197,8 +197,60 @@ def read_conf(path: str, N: int):
     return step, box, pos, a1, a2, a3, vel, L
 
 
-def load_system(topology: str, conf: str, T: float = 0.1, salt: float = 0.5, name: str = "") -> System:
+# the reference's binary configuration (src/Observables/Configurations/
+# BinaryConfiguration.cpp:17-91, read by src/Backends/SimBackend.cpp:521-553,
+# 636-648), little-endian: the step (int64), the drand48 seed words (3 x
+# uint16), the box (3 doubles), total, potential and kinetic energy per
+# nucleotide (3 doubles), then per nucleotide the position (3 doubles), its
+# image shift (3 int32), the rows of the orientation's transpose a1, a2, a3
+# (9 doubles), velocity and angular momentum (6 doubles); the observable
+# output ends the configuration with a newline
+_BIN_HEAD = np.dtype([("step", "<i8"), ("seed", "<u2", 3), ("box", "<f8", 3), ("E", "<f8", 3)])
+_BIN_PART = np.dtype([("pos", "<f8", 3), ("shift", "<i4", 3), ("a", "<f8", 9), ("vel", "<f8", 3), ("L", "<f8", 3)])
+
+
+def write_conf_binary(path: str, s: System, step: int = 0, energies=(0.0, 0.0, 0.0), seed=(0, 0, 0)) -> None:
+    """Writes the reference's binary configuration.  `energies` = (total,
+    potential, kinetic) per nucleotide; `seed` = the three drand48 state
+    words the reference stores (its generator's state; this library draws
+    from Philox, so they carry nothing here).  Positions are unwrapped, so
+    the image shifts are 0."""
+    vel = s.vel if s.vel is not None else np.zeros((s.N, 3))
+    L = s.L if s.L is not None else np.zeros((s.N, 3))
+    head = np.zeros(1, _BIN_HEAD)
+    head["step"], head["seed"], head["box"], head["E"] = step, seed, s.box, energies
+    rec = np.zeros(s.N, _BIN_PART)
+    rec["pos"] = s.pos
+    rec["a"] = np.concatenate([s.a1, np.cross(s.a3, s.a1), s.a3], axis=1)
+    rec["vel"] = vel
+    rec["L"] = L
+    with open(path, "wb") as f:
+        f.write(head.tobytes())
+        f.write(rec.tobytes())
+        f.write(b"\n")
+
+
+def read_conf_binary(path: str, N: int):
+    """Returns (step, box, pos, a1, a2, a3, vel, L, seed) of the reference's
+    binary configuration; the axes are taken as stored (the reference does
+    not re-orthonormalise them here)."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    need = _BIN_HEAD.itemsize + N * _BIN_PART.itemsize
+    if len(raw) < need:
+        raise ValueError(f"binary configuration of {len(raw)} bytes, {need} expected for {N} particles")
+    head = np.frombuffer(raw, _BIN_HEAD, 1)[0]
+    rec = np.frombuffer(raw, _BIN_PART, N, _BIN_HEAD.itemsize)
+    a = rec["a"]
+    return (int(head["step"]), head["box"].copy(), rec["pos"].copy(), a[:, 0:3].copy(), a[:, 3:6].copy(), a[:, 6:9].copy(),
+            rec["vel"].copy(), rec["L"].copy(), tuple(int(x) for x in head["seed"]))
+
+
+def load_system(topology: str, conf: str, T: float = 0.1, salt: float = 0.5, name: str = "", binary: bool = False) -> System:
     btype, n3, n5, strand = read_topology(topology)
-    step, box, pos, a1, a2, a3, vel, L = read_conf(conf, len(btype))
+    if binary:
+        step, box, pos, a1, a2, a3, vel, L, _ = read_conf_binary(conf, len(btype))
+    else:
+        step, box, pos, a1, a2, a3, vel, L = read_conf(conf, len(btype))
     return System(pos=pos, a1=a1, a3=a3, btype=btype, n3=n3, n5=n5, strand=strand, box=box, T=T, salt=salt,
                   name=name, vel=vel, L=L)

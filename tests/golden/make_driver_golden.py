@@ -9,8 +9,9 @@ reference's sources by `make -f oracle/ref.mk`):
 The run: a 20-bp duplex with seeded thermal velocities and angular momenta,
 300 NVE steps (thermostat = no) of the reference's CPU backend with
 interaction_type = DNA2_nomesh, energy every 50 steps, a configuration every
-100 steps, the last configuration at the end.  The fixture keeps the input
-files and the reference's energy file, trajectory and last configuration;
+100 steps, the last configuration at the end, in ASCII and in the
+reference's binary format (lastconf_file_bin).  The fixture keeps the input
+files and the reference's energy file, trajectory and last configurations;
 the driver test runs the same input through oxdna_amd.driver and compares.
 """
 from __future__ import annotations
@@ -38,7 +39,7 @@ INPUT = {
     "refresh_vel": "0", "restart_step_counter": "1", "topology": "t.top", "conf_file": "c.dat",
     "trajectory_file": "trajectory.dat", "energy_file": "energy.dat", "lastconf_file": "last_conf.dat",
     "print_energy_every": "50", "print_conf_interval": "100", "no_stdout_energy": "1", "time_scale": "linear",
-    "box_type": "orthogonal", "log_file": "log.dat",
+    "box_type": "orthogonal", "log_file": "log.dat", "lastconf_file_bin": "last_conf.bin",
 }
 
 
@@ -59,7 +60,8 @@ def main():
             raise RuntimeError(f"reference failed: {r.stdout[-2000:]} {r.stderr[-2000:]}")
         os.makedirs(OUT, exist_ok=True)
         for src, dst in (("input", "input"), ("t.top", "t.top"), ("c.dat", "c.dat"), ("energy.dat", "energy_ref.dat"),
-                         ("trajectory.dat", "trajectory_ref.dat"), ("last_conf.dat", "last_conf_ref.dat")):
+                         ("trajectory.dat", "trajectory_ref.dat"), ("last_conf.dat", "last_conf_ref.dat"),
+                         ("last_conf.bin", "last_conf_ref.bin")):
             shutil.copy(os.path.join(d, src), os.path.join(OUT, dst))
     print("wrote", OUT)
 

@@ -91,6 +91,43 @@ def test_driver_follows_reference_nve(hip, tmp_path, precision, e_tol, x_tol):
     print(f"{precision}: energy file {de:.1e}, positions/axes {dx:.1e}, velocities/L {dv:.1e}")
     assert de <= e_tol
     assert dx <= x_tol and dv <= 20 * x_tol
+    # the binary last configuration (lastconf_file_bin) against the
+    # reference's, in its binary layout
+    from oxdna_amd import io
+    got = io.read_conf_binary(str(tmp_path / "last_conf.bin"), 40)
+    want = io.read_conf_binary(os.path.join(d, "last_conf_ref.bin"), 40)
+    assert got[0] == want[0] == 300 and np.array_equal(got[1], want[1])
+    assert max(np.abs(a - b).max() for a, b in zip(got[2:6], want[2:6])) <= x_tol
+    assert max(np.abs(a - b).max() for a, b in zip(got[6:8], want[6:8])) <= 20 * x_tol
+
+
+def test_driver_reloads_reference_binary_configuration(hip, tmp_path):
+    """reload_from with the reference's own reload file, a binary
+    configuration (here its step-300 last configuration), under the
+    reference's rules (no conf_file or seed, the step counter kept): the
+    driver starts at step 300 with that state and prints the reference's
+    step-300 energy line; 150 more NVE steps conserve the total energy."""
+    from oxdna_amd import driver, io
+    d = os.path.join(GOLDEN, "driver_nve")
+    shutil.copy(os.path.join(d, "t.top"), tmp_path / "t.top")
+    shutil.copy(os.path.join(d, "last_conf_ref.bin"), tmp_path / "reload.bin")
+    inp = io.read_input(os.path.join(d, "input"))
+    for key in ("conf_file", "seed", "lastconf_file_bin"):
+        inp.pop(key, None)
+    inp.update(reload_from="reload.bin", restart_step_counter="0", steps="150", backend_precision="double")
+    with open(tmp_path / "input", "w") as f:
+        for k, v in inp.items():
+            f.write(f"{k} = {v}\n")
+    res = driver.run(str(tmp_path / "input"))
+    assert res["step"] == 450
+    mine = np.loadtxt(tmp_path / "energy.dat", ndmin=2)
+    ref = np.loadtxt(os.path.join(d, "energy_ref.dat"), ndmin=2)
+    assert np.allclose(mine[0, 0], 0.9) and np.abs(mine[0] - ref[-1]).max() <= 2e-6
+    assert np.abs(mine[:, 3] - mine[0, 3]).max() <= 1e-4
+    with open(tmp_path / "input", "a") as f:
+        f.write("seed = 3\n")
+    with pytest.raises(ValueError, match="seed"):
+        driver.run(str(tmp_path / "input"))
 
 
 @pytest.mark.parametrize("case", ["grid_langevin_sorted", "grid_bussi", "workgroup", "fp64"])
