@@ -1072,6 +1072,15 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 	b.nstride = N;
 	b.ncell_total = ncell;
 	b.any_cells = ncell > 0;
+	// cell-centric list build where cells hold several particles each
+	// (every system with cells, not a pool; the rows are the same either
+	// way; OXH_CELL_BUILD=0 / 1 forbids / forces it)
+	{
+		bool all_cells = !B->pool && ncell > 0;
+		for(int s = 0; s < n_sys; s++) all_cells &= b.h_sys[s].use_cells != 0;
+		const char *e = getenv("OXH_CELL_BUILD");
+		b.cell_build = all_cells && (e ? atoi(e) == 1 : N >= 4 * ncell);
+	}
 	b.cell_cap = std::max(8, 3 * max_occ);
 	b.max_neigh = std::max(8, (int) std::ceil(max_neigh_est));
 	if(b.max_neigh > N) b.max_neigh = std::max(1, N);

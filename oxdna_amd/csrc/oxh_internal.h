@@ -58,6 +58,7 @@ struct Batch {
 
 	// cells (systems with use_cells), fixed capacity per cell
 	int ncell_total = 0, cell_cap = 0;
+	bool cell_build = false;   // cell-centric list build (k_build_cells)
 	// (cell_raw: slots in binning order; cell_items: the same sorted by
 	// particle index; cell_of: a binned particle's cell)
 	int *cell_count = nullptr, *cell_items = nullptr, *cell_owner = nullptr, *cell_raw = nullptr, *cell_of = nullptr;

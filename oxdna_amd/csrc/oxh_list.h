@@ -27,6 +27,40 @@ __device__ __forceinline__ double sqr_dist(double4 a, double4 b, const oxh_sys &
 
 constexpr int BUILD_LANES = 32;
 
+// The pair predicate of the lists: candidate j of particle i (FP64 centres
+// p, pj; FP32 wrapped centres p32, q for the prefilter when from_cells) is a
+// close pair (closer than sqrt(rclose2)) or a far pair (closer than r_verlet).
+// FP32 decides unless within its error bound of a threshold, FP64 then.
+__device__ __forceinline__ void classify_pair(const oxh_sys &S, double4 p, float4 p32, float4 q, const double4 *__restrict__ pos,
+		int j, bool from_cells, bool &close, bool &far) {
+	close = far = false;
+	if(from_cells) {
+		float dx = q.x - p32.x, dy = q.y - p32.y, dz = q.z - p32.z;
+		dx -= S.boxf[0] * rintf(dx * S.inv_boxf[0]);
+		dy -= S.boxf[1] * rintf(dy * S.inv_boxf[1]);
+		dz -= S.boxf[2] * rintf(dz * S.inv_boxf[2]);
+		const float r2f = dx * dx + dy * dy + dz * dz;
+		if(fabsf(r2f - S.rv2f) < S.eps2 || fabsf(r2f - S.rclose2f) < S.eps2) {
+			double r2 = sqr_dist(p, pos[j], S);
+			if(r2 < S.rv2) {
+				close = r2 < S.rclose2;
+				far = !close;
+			}
+		}
+		else if(r2f < S.rv2f) {
+			close = r2f < S.rclose2f;
+			far = !close;
+		}
+	}
+	else {
+		double r2 = sqr_dist(p, pos[j], S);
+		if(r2 < S.rv2) {
+			close = r2 < S.rclose2;
+			far = !close;
+		}
+	}
+}
+
 // BUILD_LANES threads per particle share the candidate scan: in every chunk
 // each lane tests one candidate, a wave ballot gives every passing lane its
 // slot, so the row is written in candidate order whatever the timing.
@@ -63,35 +97,8 @@ __device__ __forceinline__ void build_row(int i, int l, int s, const double4 *__
 	// FP32 prefilter on wrapped centres
 	auto test = [&](int j, bool from_cells) {
 		bool close = false, far = false;
-		if(j >= 0 && j != i && j != bd.x && j != bd.y) {
-			if(from_cells) {
-				// FP32 decides unless within its error bound of a threshold
-				const float4 q = pos32[j];
-				float dx = q.x - p32.x, dy = q.y - p32.y, dz = q.z - p32.z;
-				dx -= S.boxf[0] * rintf(dx * S.inv_boxf[0]);
-				dy -= S.boxf[1] * rintf(dy * S.inv_boxf[1]);
-				dz -= S.boxf[2] * rintf(dz * S.inv_boxf[2]);
-				const float r2f = dx * dx + dy * dy + dz * dz;
-				if(fabsf(r2f - S.rv2f) < S.eps2 || fabsf(r2f - S.rclose2f) < S.eps2) {
-					double r2 = sqr_dist(p, pos[j], S);
-					if(r2 < S.rv2) {
-						close = r2 < S.rclose2;
-						far = !close;
-					}
-				}
-				else if(r2f < S.rv2f) {
-					close = r2f < S.rclose2f;
-					far = !close;
-				}
-			}
-			else {
-				double r2 = sqr_dist(p, pos[j], S);
-				if(r2 < S.rv2) {
-					close = r2 < S.rclose2;
-					far = !close;
-				}
-			}
-		}
+		if(j >= 0 && j != i && j != bd.x && j != bd.y)
+			classify_pair(S, p, p32, from_cells ? pos32[j] : make_float4(0.f, 0.f, 0.f, 0.f), pos, j, from_cells, close, far);
 		const uint64_t bc = __ballot(close) & gmask, bf = __ballot(far) & gmask;
 		if(close) {
 			int k = nc + __popcll(bc & below);

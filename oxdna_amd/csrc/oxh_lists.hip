@@ -190,14 +190,148 @@ __global__ void __launch_bounds__(256) k_edge_rows(int N, const int2 *__restrict
 	}
 }
 
+// Cell-centric list build (b.cell_build: every system with cells, not a
+// pool, several particles per cell): one workgroup per home cell
+// (grid-stride over the cells) stages the candidates of the stencil cells
+// (k order, each cell's items in slot order, cells out of range of the whole
+// home cell culled) in LDS once, and each wave then builds one home
+// particle's row from them, 64 candidates per ballot.  Culled cells hold no
+// pair within r_verlet, so the rows are those of build_row entry for entry,
+// in the same order.  Home cells with more than CELL_CMAX candidates fall
+// back to build_row.
+constexpr int CELL_CMAX = 2048;
+
+__global__ void __launch_bounds__(256) k_build_cells(int ncell_total, const double4 *__restrict__ pos, const int2 *__restrict__ bonds,
+		const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st, const int *__restrict__ cell_owner,
+		const int *__restrict__ count, const int *__restrict__ items, int cap, int *__restrict__ nbr, int2 *__restrict__ nnbr,
+		int max_neigh, double4 *__restrict__ rlist, int force_all, const float4 *__restrict__ pos32, const int *__restrict__ any) {
+	__shared__ int cid[CELL_CMAX];
+	__shared__ float4 cpos[CELL_CMAX];
+	__shared__ int ccell[128], cbase[129];
+	__shared__ int nkept;
+	if(!force_all && !*any) return;
+	const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+	const uint64_t below = (1ull << lane) - 1ull;
+	for(int c = blockIdx.x; c < ncell_total; c += gridDim.x) {
+		// every test below reads values that are the same for the whole
+		// workgroup, so its threads take the same barriers
+		const int s = cell_owner[c];
+		if(!force_all && !st[s].need_rebuild) continue;
+		const int nh = min(count[c], cap);
+		if(nh == 0) continue;
+		const oxh_sys &S = systems[s];
+		if(tid < 64) {
+			// lane l takes stencil cells 2l and 2l + 1 (k order); culled and
+			// missing cells count 0 candidates; one wave scan numbers them
+			const int lc = c - S.cell_first;
+			const int cx = lc % S.nc[0], cy = (lc / S.nc[0]) % S.nc[1], cz = lc / (S.nc[0] * S.nc[1]);
+			const int R = S.stencil, W = 2 * R + 1, NS = W * W * W;
+			const float cull = (sqrtf(S.rv2f) + 1e-3f) * (sqrtf(S.rv2f) + 1e-3f);
+			auto gap = [](int o, float wd) { return o > 0 ? (o - 1) * wd : (o < 0 ? (-o - 1) * wd : 0.f); };
+			int cc[2] = {0, 0}, cnt[2] = {0, 0};
+			for(int q = 0; q < 2; q++) {
+				const int k = 2 * lane + q;
+				if(k >= NS) continue;
+				const int oz = k / (W * W) - R, oy = (k / W) % W - R, ox = k % W - R;
+				const float gx = gap(ox, S.cell_w[0]), gy = gap(oy, S.cell_w[1]), gz = gap(oz, S.cell_w[2]);
+				if(gx * gx + gy * gy + gz * gz > cull) continue;
+				int z = cz + oz, y = cy + oy, x = cx + ox;
+				z += (z < 0) ? S.nc[2] : (z >= S.nc[2] ? -S.nc[2] : 0);
+				y += (y < 0) ? S.nc[1] : (y >= S.nc[1] ? -S.nc[1] : 0);
+				x += (x < 0) ? S.nc[0] : (x >= S.nc[0] ? -S.nc[0] : 0);
+				cc[q] = S.cell_first + (z * S.nc[1] + y) * S.nc[0] + x;
+				cnt[q] = min(count[cc[q]], cap);
+			}
+			const int v = cnt[0] + cnt[1];
+			int incl = v;
+			for(int off = 1; off < 64; off <<= 1) {
+				const int u = __shfl_up(incl, off);
+				if(lane >= off) incl += u;
+			}
+			const int excl = incl - v;
+			ccell[2 * lane] = cc[0];
+			ccell[2 * lane + 1] = cc[1];
+			cbase[2 * lane] = excl;
+			cbase[2 * lane + 1] = excl + cnt[0];
+			if(lane == 63) {
+				cbase[128] = incl;
+				nkept = 128;
+			}
+		}
+		__syncthreads();
+		const int nk = nkept, M = cbase[nk];
+		if(M > CELL_CMAX) {
+			for(int h = tid / BUILD_LANES; h < nh; h += blockDim.x / BUILD_LANES)
+				build_row<BUILD_LANES>(items[(size_t) c * cap + h], tid % BUILD_LANES, s, pos, bonds, systems, st, count, items, cap, nbr,
+						nnbr, max_neigh, rlist, pos32);
+			__syncthreads();
+			continue;
+		}
+		for(int m = tid; m < M; m += blockDim.x) {
+			int lo = 0, hi = nk;   // the stencil cell holding candidate m
+			while(hi - lo > 1) {
+				const int mid = (lo + hi) >> 1;
+				if(cbase[mid] <= m) lo = mid;
+				else hi = mid;
+			}
+			const int j = items[(size_t) ccell[lo] * cap + (m - cbase[lo])];
+			cid[m] = j;
+			cpos[m] = pos32[j];
+		}
+		__syncthreads();
+		for(int h = w; h < nh; h += blockDim.x / 64) {
+			const int i = items[(size_t) c * cap + h];
+			const double4 p = pos[i];
+			const float4 p32 = pos32[i];
+			const int2 bd = bonds[i];
+			int *row = nbr + (size_t) i * max_neigh;
+			int ncl = 0, nfa = 0;
+			for(int m0 = 0; m0 < M; m0 += 64) {
+				const int m = m0 + lane;
+				bool close = false, far = false;
+				int j = -1;
+				if(m < M) {
+					j = cid[m];
+					if(j != i && j != bd.x && j != bd.y) classify_pair(S, p, p32, cpos[m], pos, j, true, close, far);
+				}
+				const uint64_t bc = __ballot(close), bf = __ballot(far);
+				if(close) {
+					const int k = ncl + __popcll(bc & below);
+					if(k + nfa < max_neigh) row[k] = j;
+				}
+				if(far) {
+					const int k = nfa + __popcll(bf & below);
+					if(ncl + k < max_neigh) row[max_neigh - 1 - k] = j;
+				}
+				ncl += __popcll(bc);
+				nfa += __popcll(bf);
+			}
+			if(lane == 0) {
+				if(ncl + nfa > max_neigh) {
+					atomicOr(&st[s].error, 2);
+					ncl = nfa = 0;
+				}
+				nnbr[i] = make_int2(ncl, nfa);
+				if(rlist) rlist[i] = p;
+			}
+		}
+		__syncthreads();
+	}
+}
+
 // src = b.pos (the usual build) or b.rlist (the lists of the last build,
 // rebuilt exactly, e.g. after a capacity change); rlist is then left as is
 void launch_build_list(hipStream_t st, const Batch &b, int force_all, const double4 *src) {
 	int bs = 256;
 	size_t threads = (size_t) b.N * BUILD_LANES;
 	const unsigned grid = (unsigned) std::min<size_t>((threads + bs - 1) / bs, 2048);
-	k_build_list<<<grid, bs, 0, st>>>(b.N, src, b.bonds, b.pinfo, b.d_sys, b.d_state, b.cell_count,
-			b.cell_items, b.cell_cap, b.nbr, b.nnbr, b.max_neigh, src == b.rlist ? nullptr : b.rlist, force_all, b.pos32, b.any_rebuild);
+	if(b.cell_build)
+		k_build_cells<<<(unsigned) std::min(b.ncell_total, 4096), 256, 0, st>>>(b.ncell_total, src, b.bonds, b.d_sys, b.d_state,
+				b.cell_owner, b.cell_count, b.cell_items, b.cell_cap, b.nbr, b.nnbr, b.max_neigh, src == b.rlist ? nullptr : b.rlist,
+				force_all, b.pos32, b.any_rebuild);
+	else
+		k_build_list<<<grid, bs, 0, st>>>(b.N, src, b.bonds, b.pinfo, b.d_sys, b.d_state, b.cell_count,
+				b.cell_items, b.cell_cap, b.nbr, b.nnbr, b.max_neigh, src == b.rlist ? nullptr : b.rlist, force_all, b.pos32, b.any_rebuild);
 	if(b.ecap > 0)
 		k_edge_rows<<<grid, bs, 0, st>>>(b.N, b.pinfo, b.d_sys, b.d_state, b.nbr, b.nnbr, b.max_neigh, b.ecap, b.eown, b.ein, b.ecount,
 				force_all, b.any_rebuild);

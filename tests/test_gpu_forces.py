@@ -208,6 +208,43 @@ def test_edge_forces_vs_reference(hip, monkeypatch, name, lanes, group, far_lane
     assert np.abs(Ep - Ep_full).max() <= 1e-5 * (np.abs(Ep_full).max() + 1e-6)
 
 
+@pytest.mark.parametrize("kind", ["bundle", "bundle_batch", "gas"])
+def test_cell_centric_build_same_rows(hip, monkeypatch, kind):
+    """The cell-centric list build (one workgroup per home cell, the
+    stencil's candidates staged in LDS once for all the cell's particles)
+    writes the same rows as the per-particle build, entry for entry and in
+    the same order: the same pair set, and forces bitwise equal (their sums
+    follow row order), also after 200 Langevin steps with rebuilds."""
+    from oxdna_amd import systems
+    monkeypatch.setenv("OXH_PERSIST", "0")
+    monkeypatch.setenv("OXH_EDGE", "0")
+    monkeypatch.setenv("OXH_FORCE_LANES", "4")
+    monkeypatch.setenv("OXH_FAR_LANES", "16")
+    b0 = systems.duplex_bundle(4, nx=8, ny=10, bp=36, lz=36 * systems.BASE_BASE + 1.0)
+    if kind == "bundle":
+        ss = [b0]
+    elif kind == "bundle_batch":
+        ss = [b0, systems.duplex_bundle(7, nx=8, ny=10, bp=36, lz=36 * systems.BASE_BASE + 1.0, tc=45.0)]
+    else:
+        ss = [systems.duplex_gas(5, n_duplex=64, bp=12, density=2e-3)]
+    out = {}
+    for cb in ("0", "1"):
+        monkeypatch.setenv("OXH_CELL_BUILD", cb)
+        with hip.HIPBatch(ss, thermostat="langevin", diff_coeff=2.5, seed=3) as b:
+            b.compute_forces()
+            first = [b.forces(k) for k in range(len(ss))]
+            pairs = [b.pairs(k) for k in range(len(ss))]
+            b.run(200)
+            nreb, err = b.info()
+            assert err == 0 and nreb >= 2
+            last = [b.forces(k) for k in range(len(ss))]
+        out[cb] = (first, pairs, last)
+    for k in range(len(ss)):
+        assert np.array_equal(out["0"][1][k], out["1"][1][k])
+        for a, c in zip(out["0"][0][k] + out["0"][2][k], out["1"][0][k] + out["1"][2][k]):
+            assert np.array_equal(a, c)
+
+
 def test_edge_run_with_rebuilds(hip, monkeypatch):
     """A 1,536-nt duplex gas (cell lists) run 400 Langevin steps in the
     half-list form, with list rebuilds and Hilbert re-sorts in between (the
