@@ -3,8 +3,9 @@ equilibrium time series of the 20-bp duplex (the reference's CPU-runnable
 case: random ACGT sequence, cubic box of 20, 0.5 M, dt 0.003, Langevin with
 diff_coeff 2.5) at 20 C and at 50 C.
 
-Run in the build container (needs oracle/_ref/oxDNA_release, built from the
-reference's sources by `make -f oracle/ref.mk`):
+Run in the build container (needs oracle/_ref/oxDNA, the strict-IEEE FP64
+build of the reference's sources that pins the oracle, built by
+`make -f oracle/ref.mk`):
 
     python tests/golden/make_ensemble_golden.py
 
@@ -19,6 +20,10 @@ series (float32 energies, int8 counts) and the run parameters; the GPU test
 runs the same protocol and compares the means within two standard errors.
 The duplex's slow modes (fraying ends) decorrelate over ~1e5 steps or more,
 so the statistics use the means of whole independent runs.
+
+The strict binary is the reference here: at 50 C the release build
+(-O3 -ffast-math) gave U/N lower by 0.005 (about two standard errors of 144
+runs) than the strict build and this library, which agree.
 """
 from __future__ import annotations
 
@@ -35,13 +40,13 @@ sys.path.insert(0, ROOT)
 
 from oxdna_amd import io, systems  # noqa: E402
 
-REF_BIN = os.path.join(ROOT, "oracle", "_ref", "oxDNA_release")
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "oxDNA")
 OUT = os.path.join(HERE, "ensemble", "duplex20.npz")
 
 SYSTEM_SEED = 0
 TEMPS_C = (20.0, 50.0)
-RUNS = 48
-SEED0 = 101
+RUNS = 192
+SEED0 = 2001
 STEPS = 300_000
 EVERY = 1000
 
