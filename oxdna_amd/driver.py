@@ -29,7 +29,9 @@ dh_half_charged_ends, hb_multiplier, max_backbone_force,
 max_backbone_force_far, energy_file, print_energy_every, trajectory_file,
 print_conf_interval, lastconf_file, lastconf_file_bin, binary_initial_conf,
 fix_diffusion, fix_diffusion_every, CUDA_sort_every, CUDA_device,
-checkpoint_every, checkpoint_file, reload_from.  Binary configurations are
+checkpoint_every, checkpoint_file, reload_from, data_output_N (class
+DataOutput: step, potential_energy, kinetic_energy and hb_list
+observables).  Binary configurations are
 the reference's (oxdna_amd/io.py: write_conf_binary, read_conf_binary).
 A checkpoint file holds the step (one JSON line) and the batch's whole
 state (oxh_batch_save); reload_from continues from it bitwise, the batch
@@ -120,6 +122,86 @@ def energy_line(step: int, dt: float, U: float, K: float) -> str:
     return f"{step * dt:14.4f} {U:10.6f} {K:10.6f} {U + K:10.6f}"
 
 
+def _block(text: str) -> dict:
+    """key = value pairs inside one { ... } block of an input file."""
+    body = text.strip()
+    if not (body.startswith("{") and body.endswith("}")):
+        raise ValueError(f"malformed block: {text[:60]!r}")
+    return io.parse_input(body[1:-1])
+
+
+class DataOutput:
+    """One data_output_N block (src/Observables/ObservableOutput.cpp:23-80,
+    205-241): columns printed on one line, separated by a space, at every
+    multiple of print_every in [start_from, stop_at].  Observables:
+    step (src/Observables/Step.cpp: "%12lld", or "%14.4lf" of step * dt
+    with units = MD), potential_energy (per nucleotide, or the 8-term split
+    with split = true, each term preceded by a space;
+    src/Observables/PotentialEnergy.cpp:21-55), kinetic_energy
+    (src/Observables/KineticEnergy.cpp:37-50) and hb_list with only_count
+    (pairs with HB energy below -0.1; src/Observables/HBList.cpp:141-164).
+    Numbers print with "%10.<precision>lf", or "%.<precision>g" with
+    general_format (src/Observables/BaseObservable.cpp:32-43)."""
+
+    TYPES = ("step", "potential_energy", "kinetic_energy", "hb_list")
+
+    def __init__(self, text: str, dt: float, path):
+        d = _block(text)
+        if "name" not in d:
+            raise ValueError("data_output block without a name")
+        self.name = path(d["name"])
+        self.every = int(float(d.get("print_every", "0")))
+        self.start_from = int(float(d.get("start_from", "0")))
+        self.stop_at = int(float(d.get("stop_at", "-1")))
+        if not _bool(d.get("linear", "true")):
+            raise ValueError("data_output with linear = false (log-spaced output) is not supported")
+        self.dt = dt
+        self.cols = []
+        k = 1
+        while f"col_{k}" in d:
+            c = _block(d[f"col_{k}"])
+            t = c.get("type", "")
+            if t not in self.TYPES:
+                raise ValueError(f"observable type {t!r} is not supported (supported: {', '.join(self.TYPES)})")
+            if t == "hb_list" and not _bool(c.get("only_count", "false")):
+                raise ValueError("hb_list is supported with only_count = true")
+            prec = int(float(c.get("precision", "6")))
+            fmt = f"{{:.{prec}g}}" if _bool(c.get("general_format", "false")) else f"{{:10.{prec}f}}"
+            self.cols.append(dict(type=t, fmt=fmt, split=_bool(c.get("split", "false")),
+                                  md_units=c.get("units", "steps") == "MD"))
+            k += 1
+        self.f = open(self.name, "w")
+
+    def steps(self, start: int, end: int) -> list:
+        if self.every < 1:
+            return []
+        first = max(start, self.start_from)
+        first += (-first) % self.every
+        last = end if self.stop_at < 0 else min(end, self.stop_at)
+        return list(range(first, last + 1, self.every))
+
+    def line(self, step: int, b, N: int) -> str:
+        out = []
+        for c in self.cols:
+            t = c["type"]
+            if t == "step":
+                out.append(f"{step * self.dt:14.4f}" if c["md_units"] else f"{step:12d}")
+            elif t == "potential_energy":
+                if c["split"]:
+                    out.append("".join(" " + c["fmt"].format(e / N) for e in b.energy_split(0)))
+                else:
+                    out.append(c["fmt"].format(b.energies()[0, 0] / N))
+            elif t == "kinetic_energy":
+                E = b.energies()[0]
+                out.append(c["fmt"].format((E[2] + E[3]) / N))
+            else:
+                out.append(str(int(b.hb_count()[0])))
+        return " ".join(out)
+
+    def close(self):
+        self.f.close()
+
+
 def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
     """Runs the input file; returns the energy lines and the final state.
     Paths in the input are relative to workdir (default: the input file's
@@ -202,7 +284,10 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
             fixes = [t for t in range(start + fix_every, start + steps + 1, fix_every)] if fix_every > 0 else []
             ckpts = ([t for t in range(start + ckpt_every, start + steps + 1, ckpt_every)]
                      if ckpt_every > 0 and ckpt_path else [])
-            events = sorted(set([t for t, _, _ in plan] + fixes + ckpts))
+            blocks = [k for k in inp if k.startswith("data_output_") and k.rsplit("_", 1)[1].isdigit()]
+            outputs = [DataOutput(inp[k], dt, path) for k in sorted(blocks, key=lambda x: int(x.rsplit("_", 1)[1]))]
+            out_steps = [set(o.steps(start, start + steps)) for o in outputs]
+            events = sorted(set([t for t, _, _ in plan] + fixes + ckpts + [t for o in out_steps for t in o]))
             todo = {t: (e, c) for t, e, c in plan}
             end = start + steps
             for t in events:
@@ -229,6 +314,10 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
                     energy_f.write(line + "\n")
                     energy_f.flush()
                     lines.append(line)
+                for o, ts in zip(outputs, out_steps):
+                    if t in ts:
+                        o.f.write(o.line(t, b, s.N) + "\n")
+                        o.f.flush()
                 if t in ckpts:
                     data = b.checkpoint()
                     with open(ckpt_path, "wb") as f:
@@ -245,6 +334,8 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
             energy_f.close()
             if traj_f:
                 traj_f.close()
+            for o in locals().get("outputs", []):
+                o.close()
     return dict(energy_lines=lines, final=state, step=cur, U=U, K=K)
 
 

@@ -10,7 +10,9 @@ The run: a 20-bp duplex with seeded thermal velocities and angular momenta,
 300 NVE steps (thermostat = no) of the reference's CPU backend with
 interaction_type = DNA2_nomesh, energy every 50 steps, a configuration every
 100 steps, the last configuration at the end, in ASCII and in the
-reference's binary format (lastconf_file_bin).  The fixture keeps the input
+reference's binary format (lastconf_file_bin), and a data_output file of
+observables (step, energy split, kinetic energy, hydrogen-bond count,
+potential energy) every 50 steps.  The fixture keeps the input
 files and the reference's energy file, trajectory and last configurations;
 the driver test runs the same input through oxdna_amd.driver and compares.
 """
@@ -43,6 +45,35 @@ INPUT = {
 }
 
 
+# observables of the reference (src/Observables/Step.cpp, PotentialEnergy.cpp,
+# KineticEnergy.cpp, HBList.cpp) in one data_output file
+OBSERVABLES = """data_output_1 = {
+	name = obs.dat
+	print_every = 50
+	col_1 = {
+		type = step
+	}
+	col_2 = {
+		type = potential_energy
+		split = true
+	}
+	col_3 = {
+		type = kinetic_energy
+		precision = 8
+	}
+	col_4 = {
+		type = hb_list
+		only_count = true
+	}
+	col_5 = {
+		type = potential_energy
+		general_format = true
+		precision = 10
+	}
+}
+"""
+
+
 def main():
     s = systems.duplex_20bp(3)
     rng = np.random.default_rng(11)
@@ -55,13 +86,14 @@ def main():
         with open(os.path.join(d, "input"), "w") as f:
             for k, v in INPUT.items():
                 f.write(f"{k} = {v}\n")
+            f.write(OBSERVABLES)
         r = subprocess.run([REF_BIN, "input"], cwd=d, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"reference failed: {r.stdout[-2000:]} {r.stderr[-2000:]}")
         os.makedirs(OUT, exist_ok=True)
         for src, dst in (("input", "input"), ("t.top", "t.top"), ("c.dat", "c.dat"), ("energy.dat", "energy_ref.dat"),
                          ("trajectory.dat", "trajectory_ref.dat"), ("last_conf.dat", "last_conf_ref.dat"),
-                         ("last_conf.bin", "last_conf_ref.bin")):
+                         ("last_conf.bin", "last_conf_ref.bin"), ("obs.dat", "obs_ref.dat")):
             shutil.copy(os.path.join(d, src), os.path.join(OUT, dst))
     print("wrote", OUT)
 

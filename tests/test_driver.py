@@ -101,3 +101,22 @@ def test_binary_configuration_writer_layout():
     assert all(mine[k] == ref[k] for k in range(len(ref)) if k not in a2_bytes)
     for x, y in zip(back[2:8], (pos, a1, np.cross(a3, a1), a3, vel, L)):
         assert np.array_equal(x, y)
+
+
+def test_data_output_block_parsed_like_the_reference(tmp_path):
+    """The reference's data_output block of the golden input: the file name,
+    the schedule (every 50 steps, step 0 and the last step included, as the
+    reference's obs_ref.dat) and the columns with their formats."""
+    inp = io.read_input(os.path.join(D, "input"))
+    o = driver.DataOutput(inp["data_output_1"], 0.003, lambda p: str(tmp_path / p))
+    try:
+        ref = np.loadtxt(os.path.join(D, "obs_ref.dat"), ndmin=2)
+        assert o.steps(0, 300) == [int(x) for x in ref[:, 0]]
+        assert [c["type"] for c in o.cols] == ["step", "potential_energy", "kinetic_energy", "hb_list", "potential_energy"]
+        assert o.cols[1]["split"] and o.cols[2]["fmt"] == "{:10.8f}" and o.cols[4]["fmt"] == "{:.10g}"
+        # the reference's line layout: a step column, then the split terms
+        # each preceded by a space
+        first = open(os.path.join(D, "obs_ref.dat")).readline()
+        assert first.startswith(f"{0:12d}  " + f"{ref[0, 1]:10.6f}")
+    finally:
+        o.close()

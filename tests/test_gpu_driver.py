@@ -91,6 +91,14 @@ def test_driver_follows_reference_nve(hip, tmp_path, precision, e_tol, x_tol):
     print(f"{precision}: energy file {de:.1e}, positions/axes {dx:.1e}, velocities/L {dv:.1e}")
     assert de <= e_tol
     assert dx <= x_tol and dv <= 20 * x_tol
+    # the data_output file (step, energy split, kinetic energy, HB count,
+    # potential energy) against the reference's, column by column
+    obs = np.loadtxt(tmp_path / "obs.dat", ndmin=2)
+    obs_ref = np.loadtxt(os.path.join(d, "obs_ref.dat"), ndmin=2)
+    assert obs.shape == obs_ref.shape and np.array_equal(obs[:, 0], obs_ref[:, 0])
+    assert np.array_equal(obs[:, 10], obs_ref[:, 10])
+    assert np.abs(obs[:, 1:10] - obs_ref[:, 1:10]).max() <= e_tol
+    assert np.abs(obs[:, 11] - obs_ref[:, 11]).max() <= e_tol
     # the binary last configuration (lastconf_file_bin) against the
     # reference's, in its binary layout
     from oxdna_amd import io
