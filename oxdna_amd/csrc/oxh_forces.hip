@@ -321,10 +321,11 @@ __global__ void __launch_bounds__(256) k_forces_far(int N, const double4 *__rest
 
 // All-FP64 forces (the reference's backend_precision = double): the same
 // terms in FP64 (oxh::dp), F and T in FP64, far pairs inline (FAR) or in
-// k_forces_far_dp.  Blocks of 64 threads, so the FP64 terms get the
-// register file they need.
+// k_forces_far_dp.  Blocks of 64 threads; two waves per SIMD (256 VGPRs,
+// a few spilled), which measured faster than one wave with every value in
+// registers.
 template<int K, bool SPLIT, bool FAR>
-__global__ void __launch_bounds__(64) k_forces_dp(int N, const double4 *__restrict__ pos, const double4 *__restrict__ q64,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_forces_dp(int N, const double4 *__restrict__ pos, const double4 *__restrict__ q64,
 		const float4 *__restrict__ q32, const float4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
 		const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh, const oxh_sys *__restrict__ systems,
 		oxh_sys_state *__restrict__ sstate, double4 *__restrict__ Fout, double4 *__restrict__ Tout, double *__restrict__ esplit,
