@@ -75,6 +75,11 @@ def batch_options(inp: dict) -> dict:
         raise ValueError(f"interaction_type = {it}: this driver runs the oxDNA2 interaction (DNA2)")
     if not _bool(inp.get("use_average_seq", "true")):
         raise ValueError("use_average_seq = false: sequence-dependent parameters are not supported by this driver")
+    if _bool(inp.get("external_forces", "false")):
+        # src/Backends/SimBackend.cpp:282 reads the key; with it set the reference
+        # applies the forces of external_forces_file, and running without them
+        # would silently simulate another system
+        raise ValueError("external_forces = true: external forces are not supported by this driver")
     th = inp.get("thermostat", "no")
     if th == "john":
         th = "brownian"

@@ -37,6 +37,18 @@ def test_driver_reads_reference_input():
     assert abs(opts["T"] - io.temperature("20C")) < 1e-15
 
 
+def test_driver_refuses_what_it_does_not_simulate():
+    """Inputs the driver cannot run as the reference would are refused,
+    never run without the missing part."""
+    import pytest
+    base = io.read_input(os.path.join(D, "input"))
+    for key, val in (("external_forces", "1"), ("sim_type", "MC"), ("interaction_type", "DNA"),
+                     ("use_average_seq", "false")):
+        with pytest.raises(ValueError, match=key):
+            driver.batch_options(dict(base, **{key: val}))
+    assert driver.batch_options(dict(base, external_forces="false"))["dt"] == 0.003
+
+
 def test_output_schedule_matches_reference_files():
     energy = np.loadtxt(os.path.join(D, "energy_ref.dat"), ndmin=2)
     traj = _confs(os.path.join(D, "trajectory_ref.dat"))
