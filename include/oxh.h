@@ -182,6 +182,14 @@ oxh_status oxh_batch_sync(oxh_batch *b);
  * (src/CUDA/Backends/MD_CUDABackend.cu:315-394). */
 oxh_status oxh_batch_get_state(oxh_batch *b, int sys, double *pos, double *a1, double *a3, double *vel, double *L);
 
+/* The current device order of system `sys`: order[k] (n entries) is the
+ * original index of the particle held in slot k.  It is the identity until
+ * a Hilbert sort, then the stable order of the particles' (cell) Hilbert
+ * keys.  Replaces the host index maps the reference rebuilds after a sort
+ * (_h_particles_hindex, src/CUDA/Backends/MD_CUDABackend.cu:323-327;
+ * get_inverted_sorted_hindex, src/CUDA/CUDA_sort.cu:183-193). */
+oxh_status oxh_batch_get_order(oxh_batch *b, int sys, int *order);
+
 /* Forces (lab) and torques (body frame) of the last force evaluation, and
  * each particle's share of the potential energy (Epart, N doubles). */
 oxh_status oxh_batch_get_forces(oxh_batch *b, int sys, double *F, double *T, double *Epart);

@@ -1211,16 +1211,4 @@ double oxo_md_step(const oxo_params *P, int N, double *pos, double *a1, double *
 	return U;
 }
 
-/* Langevin thermostat update of one particle, given its 6 normal variates
- * (src/Backends/Thermostats/LangevinThermostat.cpp:78-85). */
-void oxo_langevin_apply(int N, double *vel, double *L, const double *noise6, double dt, double gamma_t, double resc_t, double gamma_r,
-		double resc_r) {
-	for(int i = 0; i < N; i++) {
-		for(int d = 0; d < 3; d++) {
-			vel[3 * i + d] += dt * (-gamma_t * vel[3 * i + d] + noise6[6 * i + d] * resc_t);
-			L[3 * i + d] += dt * (-gamma_r * L[3 * i + d] + noise6[6 * i + 3 + d] * resc_r);
-		}
-	}
-}
-
 }  // extern "C"

@@ -46,8 +46,6 @@ def lib():
         L.oxo_philox4x32_10.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.oxo_md_step.argtypes = [C.c_void_p, C.c_int, DP, DP, DP, DP, DP, DP, IP, IP, IP, DP, C.c_double, DP, DP]
         L.oxo_md_step.restype = C.c_double
-        L.oxo_langevin_apply.argtypes = [C.c_int, DP, DP, DP, C.c_double, C.c_double, C.c_double, C.c_double,
-                                         C.c_double]
         _LIB = L
     return _LIB
 

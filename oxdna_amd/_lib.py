@@ -24,7 +24,7 @@ EXPORTS = [
     "oxh_batch_set_state", "oxh_batch_stats", "oxh_batch_time_forces", "oxh_hilbert_keys", "oxh_philox",
     "oxh_pool_create", "oxh_pool_admit", "oxh_pool_release", "oxh_pool_status", "oxh_copy_bench", "oxh_batch_stress",
     "oxh_batch_fix_diffusion", "oxh_batch_set_fix_diffusion", "oxh_batch_checkpoint_bytes", "oxh_batch_save",
-    "oxh_batch_restore",
+    "oxh_batch_restore", "oxh_batch_get_order",
 ]
 
 
@@ -68,6 +68,7 @@ def lib():
         L.oxh_batch_set_sort_interval.argtypes = [V, LL]
         L.oxh_batch_get_state.argtypes = [V, I, P, P, P, P, P]
         L.oxh_batch_get_forces.argtypes = [V, I, P, P, P]
+        L.oxh_batch_get_order.argtypes = [V, I, P]
         L.oxh_batch_energies.argtypes = [V, P]
         L.oxh_batch_energy_split.argtypes = [V, I, P]
         L.oxh_batch_hb_count.argtypes = [V, P]

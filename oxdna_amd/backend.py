@@ -171,6 +171,13 @@ class HIPBatch:
               "oxh_batch_get_state")
         return out
 
+    def order(self, sys: int = 0) -> np.ndarray:
+        """Original index of the particle in each device slot of system
+        `sys` (the identity until a Hilbert sort)."""
+        out = np.zeros(self.systems[sys].N, np.int32)
+        check(lib().oxh_batch_get_order(self.h, sys, ptr(out)), "oxh_batch_get_order")
+        return out
+
     def forces(self, sys: int = 0):
         n = self.systems[sys].N
         F = np.zeros((n, 3))

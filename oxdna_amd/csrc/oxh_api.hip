@@ -1586,6 +1586,16 @@ oxh_status oxh_batch_get_state(oxh_batch *B, int sys, double *pos, double *a1, d
 	return OXH_OK;
 }
 
+oxh_status oxh_batch_get_order(oxh_batch *B, int sys, int *order) {
+	TRY(check_sys(B, sys));
+	if(!order) return fail(OXH_ERR_ARG, "null order buffer");
+	std::vector<int> o;
+	TRY(download(B, o, B->b.orig, B->first[sys], B->n[sys]));
+	TRY(sync_and_check(B));
+	std::copy(o.begin(), o.end(), order);
+	return OXH_OK;
+}
+
 oxh_status oxh_batch_get_forces(oxh_batch *B, int sys, double *F, double *T, double *Epart) {
 	TRY(check_sys(B, sys));
 	int f = B->first[sys], n = B->n[sys];

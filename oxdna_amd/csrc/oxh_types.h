@@ -60,7 +60,31 @@ typedef struct oxh_sys {
 	double bussi_c;                             // exp(-newtonian_steps / tau)
 	uint32_t seed_lo, seed_hi;
 	int pad_;
+	// external forces: the reference's step counter at this system's first
+	// step (forces of step k use ext_step0 + k), and 1 when any record acts
+	int has_ext;
+	long long ext_step0;
 } oxh_sys;
+
+// One external force acting on one particle (the reference's
+// external_forces_file entries, src/Forces/*.cpp, expanded to one record
+// per particle and stored sorted by particle, in the file's order).
+// Parameters by type (OXH_EXT_* in include/oxh.h):
+//   string:          a = F0, b = rate, v1 = dir (unit, or the centre when
+//                    flags & 2: dir_as_centre)
+//   trap:            a = stiff, b = rate, v1 = dir (unit), v2 = pos0
+//   mutual_trap:     a = stiff, b = stiff_rate, c = r0, d = rate, ref
+//   constant_trap:   a = stiff, c = r0, ref
+//   repulsion_plane: a = stiff, b = position, c = end_position, d = v,
+//                    v1 = dir (unit)
+//   sphere:          a = stiff, b = rate, c = r0, d = r_ext, v1 = center
+// ref is the partner's index inside the system (upload order); flags & 1:
+// nearest image between the two particles (PBC).
+typedef struct oxh_ext_rec {
+	int type, flags, ref, pad_;
+	double a, b, c, d;
+	double v1[3], v2[3];
+} oxh_ext_rec;
 
 // per-system device-side state written by the kernels
 typedef struct oxh_sys_state {

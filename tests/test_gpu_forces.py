@@ -3,9 +3,11 @@ reference's FP64 CPU backend (fixtures written by the reference binary)
 and the FP64 oracle.
 
 Tolerance (FP32 forces from FP64 positions): max |dF_i| over particles
-<= 1e-5 of the largest force in the system, for forces and torques; the
-per-particle error with a floor at 0.1 x RMS is reported and bounded by
-1e-4.  Energies: each term within 1e-5 x sum|E_terms| + 1e-6 N.
+<= 1e-5 of the largest force in the system, for forces and torques, and
+the per-particle error (each particle's |dF_i| over its own |F_i|, with a
+floor at 0.1 x RMS) <= 1e-5 as well.  Energies: each term within 1e-5 x
+sum|E_terms| + 1e-6 N.  The base-pair count (H-bond energy below -0.1)
+equals the reference's hb_list count.
 """
 import numpy as np
 import pytest
@@ -15,7 +17,7 @@ from _common import golden_names, load_golden, oracle_params, vec_rel_err
 pytestmark = pytest.mark.gpu
 
 F_TOL = 1e-5
-F_TOL_PER_PARTICLE = 1e-4
+F_TOL_PER_PARTICLE = 1e-5
 
 
 @pytest.fixture(scope="module")
@@ -32,6 +34,8 @@ def test_forces_vs_reference(hip, name):
         F, T, Ep = b.forces()
         e8 = b.energy_split()
         U = b.energies()[0, 0]
+        hb = int(b.hb_count()[0])
+    assert hb == ref["hb_count"], (hb, ref["hb_count"])
     ef_big, ef_pp = vec_rel_err(F, ref["F"])
     et_big, et_pp = vec_rel_err(T, ref["T"])
     print(f"{name}: N={s.N} F {ef_big:.2e}/{ef_pp:.2e} T {et_big:.2e}/{et_pp:.2e}")

@@ -31,6 +31,8 @@ def test_fp64_forces_vs_reference(hip, name):
         F, T, Ep = b.forces()
         e8 = b.energy_split()
         W, _ = b.stress()
+        hb = int(b.hb_count()[0])
+    assert hb == ref["hb_count"], (hb, ref["hb_count"])
     ef = vec_rel_err(F, ref["F"])[0]
     et = vec_rel_err(T, ref["T"])[0]
     print(f"{name}: FP64 F {ef:.2e} T {et:.2e}")

@@ -77,27 +77,52 @@ def _rotate_particle(s, i, axis, ang):
     return out
 
 
-@pytest.mark.parametrize("name", ["hairpin_md_50C", "nicked_md_30C", "tangle3_nohalf_dh", "bundle_md_30C"])
+@pytest.mark.parametrize("name", golden_names())
+def test_oracle_hb_count_matches_reference(name):
+    """Base pairs (H-bond energy below HB_CUTOFF = -0.1, src/defs.h:62)
+    against the reference's hb_list observable with only_count = true
+    (src/Observables/HBList.cpp:141-164)."""
+    s, opts, ref = load_golden(name)
+    assert po.count_hb(oracle_params(s, opts), s) == ref["hb_count"]
+
+
+def test_fixtures_cover_every_term():
+    """Every term of the energy split is nonzero in at least three
+    fixtures, bonded excluded volume and coaxial stacking included."""
+    nz = np.array([np.abs(load_golden(n)[2]["E8"]) > 1e-9 for n in golden_names()])
+    assert np.all(nz.sum(axis=0) >= 3), nz.sum(axis=0)
+
+
+@pytest.mark.parametrize("name", ["hairpin_md_50C", "nicked_md_30C", "tangle3_nohalf_dh", "bundle_md_30C",
+                                  "compressed_20C", "tangle4_dense_30C", "hot_md_bexcl_95C", "hot_md_cxst_95C",
+                                  "mbf_md_20C", "hbmult_custom_25C", "seqdep_hairpin_45C"])
 def test_forces_and_torques_are_energy_gradients(name):
     s, opts, _ = load_golden(name)
     P = oracle_params(s, opts)
     r = po.forces(P, s)
     rng = np.random.default_rng(0)
     h = 1e-5
-    for i in rng.choice(s.N, size=min(6, s.N), replace=False):
+    # central differences err by h^2/6 times the third derivative, which is
+    # large in the Lennard-Jones cores of the packed fixtures
+    tol = 1e-5 if np.abs(r["F"]).max() < 100 else 1e-4
+    # a random sample plus the four most strongly pushed particles (those
+    # in excluded-volume contact in the packed fixtures)
+    picks = set(rng.choice(s.N, size=min(6, s.N), replace=False).tolist())
+    picks |= set(np.argsort(-np.linalg.norm(r["F"], axis=1))[:4].tolist())
+    for i in sorted(picks):
         for d in range(3):
             sp = s.copy()
             sm = s.copy()
             sp.pos[i, d] += h
             sm.pos[i, d] -= h
             fd = -(_energy(P, sp) - _energy(P, sm)) / (2 * h)
-            assert abs(fd - r["F"][i, d]) <= 1e-5 * max(1.0, abs(r["F"][i, d])), (i, d, fd, r["F"][i, d])
+            assert abs(fd - r["F"][i, d]) <= tol * max(1.0, abs(r["F"][i, d])), (i, d, fd, r["F"][i, d])
         axes = (s.a1[i], np.cross(s.a3[i], s.a1[i]), s.a3[i])
         for k in range(3):
             up = _energy(P, _rotate_particle(s, i, axes[k], h))
             um = _energy(P, _rotate_particle(s, i, axes[k], -h))
             fd = -(up - um) / (2 * h)
-            assert abs(fd - r["T"][i, k]) <= 1e-5 * max(1.0, abs(r["T"][i, k])), (i, k, fd, r["T"][i, k])
+            assert abs(fd - r["T"][i, k]) <= tol * max(1.0, abs(r["T"][i, k])), (i, k, fd, r["T"][i, k])
 
 
 @pytest.mark.parametrize("name", ["duplex20_md_20C", "hairpin_md_50C", "nicked_md_30C"])
