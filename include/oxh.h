@@ -190,6 +190,60 @@ oxh_status oxh_batch_get_state(oxh_batch *b, int sys, double *pos, double *a1, d
  * get_inverted_sorted_hindex, src/CUDA/CUDA_sort.cu:183-193). */
 oxh_status oxh_batch_get_order(oxh_batch *b, int sys, int *order);
 
+/* ---- external forces (the reference's external_forces = true with an
+ * external_forces_file; ForceFactory::add_force, src/Forces/
+ * ForceFactory.cpp:66-110, the CPU classes in src/Forces/*.cpp and the
+ * CUDA_trap family set by MD_CUDABackend::_set_external_forces,
+ * src/CUDA/Backends/MD_CUDABackend.cu:528-533, CUDA_MD.cuh:97-554).
+ * Types, by the reference's `type` key: */
+#define OXH_EXT_STRING 1            /* "string": ConstantRateForce */
+#define OXH_EXT_TRAP 2              /* "trap": MovingTrap */
+#define OXH_EXT_MUTUAL_TRAP 3       /* "mutual_trap": MutualTrap */
+#define OXH_EXT_CONSTANT_TRAP 4     /* "constant_trap": ConstantTrap */
+#define OXH_EXT_REPULSION_PLANE 5   /* "repulsion_plane": RepulsionPlane */
+#define OXH_EXT_SPHERE 6            /* "sphere": RepulsiveSphere */
+
+/* One force acting on one particle, with the reference's keys:
+ *   string:          F0, rate, dir (normalised here unless dir_as_centre,
+ *                    then dir is the centre)
+ *   trap:            stiff, rate, dir (normalised here), pos0
+ *   mutual_trap:     stiff, stiff_rate, r0, rate, ref_particle, PBC
+ *   constant_trap:   stiff, r0, ref_particle, PBC
+ *   repulsion_plane: stiff, position, end_position, v, dir (normalised)
+ *   sphere:          stiff, r0, rate, r_ext, center (in dir)
+ * A force the reference applies to a list of particles (particle = "0-9,
+ * 12" or -1 for all) is one record per particle, in the file's order. */
+typedef struct oxh_ext_force {
+	int type;
+	int particle;        /* index in the system, upload order */
+	int ref_particle;    /* mutual_trap, constant_trap */
+	int pbc;             /* mutual_trap, constant_trap: 1 = nearest image */
+	int dir_as_centre;   /* string */
+	double F0, rate, stiff, stiff_rate, r0;
+	double position, end_position, v, r_ext;
+	double dir[3];
+	double pos0[3];
+} oxh_ext_force;
+
+/* Replaces the external forces of system `sys` (n = 0 removes them).
+ * step0 is the reference's step counter at the system's next force
+ * evaluation (the step read from the configuration, or 0 with
+ * restart_step_counter): the forces of MD step k of later oxh_batch_run
+ * calls use step0 + k, counted from the system's first step.  The forces
+ * act from the current state on: F of the current state is recomputed.
+ * Batches only (not slot pools). */
+oxh_status oxh_batch_set_external_forces(oxh_batch *b, int sys, int n, const oxh_ext_force *forces, long long step0);
+
+/* Periodic image counts of system `sys` (3 ints per particle, original
+ * order): the absolute position of a particle is its position plus
+ * image * box, as the reference's OrthogonalBox::get_abs_pos
+ * (src/Boxes/OrthogonalBox.cpp:63-74); oxh_batch_fix_diffusion moves
+ * positions by whole boxes and counts the moves here.  Set: e.g. from the
+ * shift fields of a binary configuration (src/Backends/SimBackend.cpp:
+ * 630-645). */
+oxh_status oxh_batch_get_images(oxh_batch *b, int sys, int *img3);
+oxh_status oxh_batch_set_images(oxh_batch *b, int sys, const int *img3);
+
 /* Forces (lab) and torques (body frame) of the last force evaluation, and
  * each particle's share of the potential energy (Epart, N doubles). */
 oxh_status oxh_batch_get_forces(oxh_batch *b, int sys, double *F, double *T, double *Epart);

@@ -24,8 +24,12 @@ EXPORTS = [
     "oxh_batch_set_state", "oxh_batch_stats", "oxh_batch_time_forces", "oxh_hilbert_keys", "oxh_philox",
     "oxh_pool_create", "oxh_pool_admit", "oxh_pool_release", "oxh_pool_status", "oxh_copy_bench", "oxh_batch_stress",
     "oxh_batch_fix_diffusion", "oxh_batch_set_fix_diffusion", "oxh_batch_checkpoint_bytes", "oxh_batch_save",
-    "oxh_batch_restore", "oxh_batch_get_order",
+    "oxh_batch_restore", "oxh_batch_get_order", "oxh_batch_set_external_forces", "oxh_batch_get_images",
+    "oxh_batch_set_images",
 ]
+
+# external force types (include/oxh.h OXH_EXT_*), by the reference's `type` key
+OXH_EXT = {"string": 1, "trap": 2, "mutual_trap": 3, "constant_trap": 4, "repulsion_plane": 5, "sphere": 6}
 
 
 class SystemDesc(C.Structure):
@@ -42,6 +46,16 @@ class SystemDesc(C.Structure):
         ("verlet_skin", C.c_double), ("dt", C.c_double), ("thermostat", C.c_int), ("diff_coeff", C.c_double),
         ("gamma_trans", C.c_double), ("pt", C.c_double), ("newtonian_steps", C.c_int), ("bussi_tau", C.c_int),
         ("seed", C.c_ulonglong), ("refresh_vel", C.c_int), ("backend_precision", C.c_int),
+    ]
+
+
+class ExtForce(C.Structure):
+    _fields_ = [
+        ("type", C.c_int), ("particle", C.c_int), ("ref_particle", C.c_int), ("pbc", C.c_int),
+        ("dir_as_centre", C.c_int),
+        ("F0", C.c_double), ("rate", C.c_double), ("stiff", C.c_double), ("stiff_rate", C.c_double),
+        ("r0", C.c_double), ("position", C.c_double), ("end_position", C.c_double), ("v", C.c_double),
+        ("r_ext", C.c_double), ("dir", C.c_double * 3), ("pos0", C.c_double * 3),
     ]
 
 
@@ -69,6 +83,9 @@ def lib():
         L.oxh_batch_get_state.argtypes = [V, I, P, P, P, P, P]
         L.oxh_batch_get_forces.argtypes = [V, I, P, P, P]
         L.oxh_batch_get_order.argtypes = [V, I, P]
+        L.oxh_batch_set_external_forces.argtypes = [V, I, I, P, LL]
+        L.oxh_batch_get_images.argtypes = [V, I, P]
+        L.oxh_batch_set_images.argtypes = [V, I, P]
         L.oxh_batch_energies.argtypes = [V, P]
         L.oxh_batch_energy_split.argtypes = [V, I, P]
         L.oxh_batch_hb_count.argtypes = [V, P]

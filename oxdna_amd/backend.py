@@ -171,6 +171,40 @@ class HIPBatch:
               "oxh_batch_get_state")
         return out
 
+    def set_external_forces(self, forces, sys: int = 0, step0: int = 0):
+        """External forces of system `sys`: a list of dicts with the
+        reference's keys (type, particle, F0, rate, stiff, dir, ...; see
+        oxdna_amd.io.read_external_forces), one record per particle, in the
+        file's order.  step0 is the reference's step counter at the next
+        force evaluation."""
+        arr = (_lib.ExtForce * max(1, len(forces)))()
+        for k, f in enumerate(forces):
+            e = arr[k]
+            e.type = _lib.OXH_EXT[f["type"]]
+            e.particle = int(f["particle"])
+            e.ref_particle = int(f.get("ref_particle", -1))
+            e.pbc = int(bool(f.get("PBC", False)))
+            e.dir_as_centre = int(bool(f.get("dir_as_centre", False)))
+            for key, default in (("F0", 0.0), ("rate", 0.0), ("stiff", 0.0), ("stiff_rate", 0.0), ("r0", 0.0),
+                                 ("position", 0.0), ("end_position", 1e6), ("v", 0.0), ("r_ext", 1e10)):
+                setattr(e, key, float(f.get(key, default)))
+            d = f.get("dir", f.get("center", (0.0, 0.0, 0.0)))
+            for c in range(3):
+                e.dir[c] = float(d[c])
+                e.pos0[c] = float(f.get("pos0", (0.0, 0.0, 0.0))[c])
+        check(lib().oxh_batch_set_external_forces(self.h, sys, len(forces), C.byref(arr), int(step0)),
+              "oxh_batch_set_external_forces")
+
+    def images(self, sys: int = 0) -> np.ndarray:
+        """(N, 3) periodic image counts: absolute position = pos + img * box."""
+        out = np.zeros((self.systems[sys].N, 3), np.int32)
+        check(lib().oxh_batch_get_images(self.h, sys, ptr(out)), "oxh_batch_get_images")
+        return out
+
+    def set_images(self, img, sys: int = 0):
+        a = _c(img, np.int32)
+        check(lib().oxh_batch_set_images(self.h, sys, ptr(a)), "oxh_batch_set_images")
+
     def order(self, sys: int = 0) -> np.ndarray:
         """Original index of the particle in each device slot of system
         `sys` (the identity until a Hilbert sort)."""

@@ -56,6 +56,9 @@ struct oxh_batch {
 	long long fix_every = 0, since_fix = 0;
 	std::vector<long long> target, done_steps;
 	double4 *bk_rlist = nullptr;
+	// external forces of each system: (particle, record) in the caller's
+	// order
+	std::vector<std::vector<std::pair<int, oxh_ext_rec>>> ext_of;
 };
 
 namespace {
@@ -679,20 +682,31 @@ oxh_status check_sys(oxh_batch *B, int sys) {
 	HIPCHK(hipSetDevice(B->b.device));
 	return OXH_OK;
 }
-__global__ void k_shift(int N, const double4 *__restrict__ shift, double4 *__restrict__ pos, double4 *__restrict__ rlist) {
+// moves particle i by -c * box (c: whole boxes per axis) and counts the
+// move in its image counts, as OrthogonalBox::shift_particle does
+// (src/Boxes/OrthogonalBox.cpp:67-74)
+__global__ void k_shift(int N, const int4 *__restrict__ shift, const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems,
+		double4 *__restrict__ pos, double4 *__restrict__ rlist, int4 *__restrict__ img) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= N) return;
-	const double4 d = shift[i];
-	if(d.x == 0. && d.y == 0. && d.z == 0.) return;
+	const int4 c = shift[i];
+	if(c.x == 0 && c.y == 0 && c.z == 0) return;
+	const oxh_sys &S = systems[pinfo[i].y];
+	const double dx = S.box[0] * (double) c.x, dy = S.box[1] * (double) c.y, dz = S.box[2] * (double) c.z;
 	double4 p = pos[i], r = rlist[i];
-	p.x += d.x;
-	p.y += d.y;
-	p.z += d.z;
-	r.x += d.x;
-	r.y += d.y;
-	r.z += d.z;
+	p.x -= dx;
+	p.y -= dy;
+	p.z -= dz;
+	r.x -= dx;
+	r.y -= dy;
+	r.z -= dz;
 	pos[i] = p;
 	rlist[i] = r;   // relative positions are unchanged: lists stay valid
+	int4 m = img[i];
+	m.x += c.x;
+	m.y += c.y;
+	m.z += c.z;
+	img[i] = m;
 }
 
 // Moves every strand whose centre of mass left the box back by whole box
@@ -712,7 +726,7 @@ oxh_status fix_diffusion_impl(oxh_batch *B, int *n_moved) {
 	TRY(download(B, o, b.orig, 0, b.N));
 	std::vector<double> E0(4 * b.nsys);
 	TRY(sync_and_check(B));
-	std::vector<double4> shift(b.N, make_double4(0, 0, 0, 0));
+	std::vector<int4> shift(b.N, make_int4(0, 0, 0, 0));
 	int moved = 0;
 	for(int s = 0; s < b.nsys; s++) {
 		const int f = B->first[s], n = B->n[s], ns = B->n_strands[s];
@@ -730,30 +744,30 @@ oxh_status fix_diffusion_impl(oxh_batch *B, int *n_moved) {
 			cnt[sid[i]]++;
 		}
 		const oxh_sys &S = b.h_sys[s];
-		std::vector<double> d(3 * ns, 0.0);
+		std::vector<int> d(3 * ns, 0);
 		for(int m = 0; m < ns; m++) {
 			bool any = false;
 			for(int c = 0; c < 3; c++) {
 				const double cm = com[3 * m + c] / cnt[m];
-				d[3 * m + c] = -S.box[c] * std::floor(cm / S.box[c]);
-				any |= d[3 * m + c] != 0.0;
+				d[3 * m + c] = (int) std::floor(cm / S.box[c]);
+				any |= d[3 * m + c] != 0;
 			}
 			moved += any;
 		}
 		for(int i = 0; i < n; i++) {
 			const int m = sid[i];
-			shift[where[i]] = make_double4(d[3 * m], d[3 * m + 1], d[3 * m + 2], 0.0);
+			shift[where[i]] = make_int4(d[3 * m], d[3 * m + 1], d[3 * m + 2], 0);
 		}
 	}
 	if(n_moved) *n_moved = moved;
 	if(moved == 0) return OXH_OK;
-	double4 *dshift = nullptr;
+	int4 *dshift = nullptr;
 	TRY(dalloc(B, &dshift, b.N));
 	auto apply = [&](double sign) -> oxh_status {
 		if(sign < 0)
-			for(auto &x : shift) x = make_double4(-x.x, -x.y, -x.z, 0.0);
-		HIPCHK(hipMemcpyAsync(dshift, shift.data(), b.N * sizeof(double4), hipMemcpyHostToDevice, b.stream));
-		k_shift<<<(b.N + 255) / 256, 256, 0, b.stream>>>(b.N, dshift, b.pos, b.rlist);
+			for(auto &x : shift) x = make_int4(-x.x, -x.y, -x.z, 0);
+		HIPCHK(hipMemcpyAsync(dshift, shift.data(), b.N * sizeof(int4), hipMemcpyHostToDevice, b.stream));
+		k_shift<<<(b.N + 255) / 256, 256, 0, b.stream>>>(b.N, dshift, b.pinfo, b.d_sys, b.pos, b.rlist, b.img);
 		oxh::launch_prepare_mirror(b.stream, b);
 		enqueue_forces(B);
 		return sync_and_check(B);
@@ -770,7 +784,7 @@ oxh_status fix_diffusion_impl(oxh_batch *B, int *n_moved) {
 		if(n_moved) *n_moved = 0;
 	}
 	(void) hipFree(dshift);
-	B->device_bytes -= (size_t) b.N * sizeof(double4);
+	B->device_bytes -= (size_t) b.N * sizeof(int4);
 	return st;
 }
 
@@ -984,7 +998,7 @@ oxh_status oxh_batch_destroy(oxh_batch *B) {
 	void *ptrs[] = {b.pos, b.vel, b.Lm, b.q64, b.rlist, b.q32, b.bpos, b.F, b.T, b.bonds, b.pinfo, b.orig, b.pstep, b.nbr, b.nnbr,
 			b.cell_count, b.cell_items, b.cell_owner, b.pos32, b.d_sys, b.d_state, b.red_blocks, b.red_sys_blocks, b.red_partial, b.red_out, b.red_done,
 			b.esplit, b.stress, b.hbcount, b.keys, b.keys_alt, b.perm, b.perm_alt, b.inv_perm, b.radix_hist, b.scratch, B->bk_pos, B->bk_vel,
-			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state, B->bk_rlist, b.limit, b.any_rebuild, b.F64, b.T64, B->bk_F64, B->bk_T64, b.eown, b.ein, b.ecount, b.E1, b.E2, b.E3, b.B1, b.B2, b.B3, b.cell_raw, b.cell_of, b.back64};
+			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state, B->bk_rlist, b.limit, b.any_rebuild, b.F64, b.T64, B->bk_F64, B->bk_T64, b.eown, b.ein, b.ecount, b.E1, b.E2, b.E3, b.B1, b.B2, b.B3, b.cell_raw, b.cell_of, b.back64, b.img, b.slot_of, b.ext_off, b.ext_rec};
 	for(void *p : ptrs)
 		if(p) (void) hipFree(p);
 	if(b.stream) (void) hipStreamDestroy(b.stream);
@@ -1098,6 +1112,8 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 	TRYB(dalloc(B, &b.bonds, N));
 	TRYB(dalloc(B, &b.pinfo, N));
 	TRYB(dalloc(B, &b.orig, N));
+	TRYB(dalloc(B, &b.slot_of, N));
+	TRYB(dalloc(B, &b.img, N));
 	TRYB(dalloc(B, &b.pstep, N));
 	TRYB(dalloc(B, &b.nnbr, N));
 	TRYB(dalloc(B, &b.nbr, (size_t) N * b.max_neigh));
@@ -1184,6 +1200,7 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 			up(b.red_blocks, rb.data(), rb.size() * sizeof(int3)) && up(b.red_sys_blocks, rsb.data(), n_sys * sizeof(int2));
 	if(!ok) return bail(fail(OXH_ERR_HIP, "upload failed"));
 	if(hipMemsetAsync(b.pstep, 0, N * sizeof(long long), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
+	if(hipMemsetAsync(b.img, 0, N * sizeof(int4), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
 	if(hipMemsetAsync(b.red_done, 0, n_sys * sizeof(int), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
 	if(hipMemsetAsync(b.cell_count, 0, std::max(ncell, 1) * sizeof(int), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
 	if(hipMemsetAsync(b.F, 0, N * sizeof(float4), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
@@ -1213,6 +1230,7 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 		(void) up(b.Lm, nL.data(), N * sizeof(double4));
 	}
 	oxh::launch_prepare_mirror(st, b);
+	oxh::launch_slot_of(st, b);
 	// the workgroup engine takes systems of at most one particle per thread
 	if(const char *p = getenv("OXH_PERSIST_BLOCK")) B->persist_block = atoi(p) == 256 ? 256 : 512;
 	B->persist = true;
@@ -1427,6 +1445,7 @@ oxh_status oxh_pool_admit(oxh_batch *B, int slot, const oxh_system_desc *desc, l
 	HIPCHK(hipMemcpyAsync(b.pinfo + f, hinfo.data(), d.N * sizeof(int2), hipMemcpyHostToDevice, st));
 	HIPCHK(hipMemcpyAsync(b.orig + f, horig.data(), d.N * sizeof(int), hipMemcpyHostToDevice, st));
 	HIPCHK(hipMemsetAsync(b.pstep + f, 0, d.N * sizeof(long long), st));
+	HIPCHK(hipMemsetAsync(b.img + f, 0, d.N * sizeof(int4), st));
 	HIPCHK(hipMemsetAsync(b.F + f, 0, d.N * sizeof(float4), st));
 	HIPCHK(hipMemsetAsync(b.T + f, 0, d.N * sizeof(float4), st));
 	b.h_sys[slot] = S;
@@ -1437,6 +1456,7 @@ oxh_status oxh_pool_admit(oxh_batch *B, int slot, const oxh_system_desc *desc, l
 	B->done_steps[slot] = 0;
 	if(d.refresh_vel) oxh::launch_refresh_vel(st, b, f, d.N);
 	oxh::launch_prepare_mirror(st, b);
+	oxh::launch_slot_of(st, b);
 	// reduction blocks follow the slots' sizes
 	std::vector<int3> rb;
 	std::vector<int2> rsb;
@@ -1584,6 +1604,150 @@ oxh_status oxh_batch_get_state(oxh_batch *B, int sys, double *pos, double *a1, d
 		}
 	}
 	return OXH_OK;
+}
+
+oxh_status oxh_batch_set_external_forces(oxh_batch *B, int sys, int n, const oxh_ext_force *forces, long long step0) {
+	TRY(check_sys(B, sys));
+	if(B->pool) return fail(OXH_ERR_ARG, "external forces are set on batches, not slot pools");
+	if(n < 0 || (n > 0 && !forces)) return fail(OXH_ERR_ARG, "oxh_batch_set_external_forces: bad arguments");
+	oxh::Batch &b = B->b;
+	const int N = B->n[sys];
+	std::vector<std::pair<int, oxh_ext_rec>> recs;
+	for(int k = 0; k < n; k++) {
+		const oxh_ext_force &f = forces[k];
+		if(f.particle < 0 || f.particle >= N) return fail(OXH_ERR_ARG, "external force %d: no particle %d in system %d", k, f.particle, sys);
+		oxh_ext_rec r;
+		std::memset(&r, 0, sizeof(r));
+		r.type = f.type;
+		auto unit = [&](double *out) -> bool {
+			const double m = std::sqrt(f.dir[0] * f.dir[0] + f.dir[1] * f.dir[1] + f.dir[2] * f.dir[2]);
+			if(!(m > 0)) return false;
+			for(int c = 0; c < 3; c++) out[c] = f.dir[c] / m;
+			return true;
+		};
+		switch(f.type) {
+		case OXH_EXT_STRING:
+			r.a = f.F0;
+			r.b = f.rate;
+			if(f.dir_as_centre) {
+				r.flags = 2;
+				for(int c = 0; c < 3; c++) r.v1[c] = f.dir[c];
+			}
+			else if(!unit(r.v1)) return fail(OXH_ERR_ARG, "external force %d: zero dir", k);
+			break;
+		case OXH_EXT_TRAP:
+			r.a = f.stiff;
+			r.b = f.rate;
+			if(!unit(r.v1)) return fail(OXH_ERR_ARG, "external force %d: zero dir", k);
+			for(int c = 0; c < 3; c++) r.v2[c] = f.pos0[c];
+			break;
+		case OXH_EXT_MUTUAL_TRAP:
+		case OXH_EXT_CONSTANT_TRAP:
+			if(f.ref_particle < 0 || f.ref_particle >= N)
+				return fail(OXH_ERR_ARG, "external force %d: no reference particle %d in system %d", k, f.ref_particle, sys);
+			r.ref = f.ref_particle;
+			r.flags = f.pbc ? 1 : 0;
+			r.a = f.stiff;
+			r.b = f.type == OXH_EXT_MUTUAL_TRAP ? f.stiff_rate : 0.0;
+			r.c = f.r0;
+			r.d = f.type == OXH_EXT_MUTUAL_TRAP ? f.rate : 0.0;
+			break;
+		case OXH_EXT_REPULSION_PLANE:
+			r.a = f.stiff;
+			r.b = f.position;
+			r.c = f.end_position;
+			r.d = f.v;
+			if(!unit(r.v1)) return fail(OXH_ERR_ARG, "external force %d: zero dir", k);
+			break;
+		case OXH_EXT_SPHERE:
+			r.a = f.stiff;
+			r.b = f.rate;
+			r.c = f.r0;
+			r.d = f.r_ext;
+			for(int c = 0; c < 3; c++) r.v1[c] = f.dir[c];
+			break;
+		default: return fail(OXH_ERR_ARG, "external force %d: unknown type %d", k, f.type);
+		}
+		recs.push_back({f.particle, r});
+	}
+	HIPCHK(hipStreamSynchronize(b.stream));
+	if(B->ext_of.empty()) B->ext_of.resize(b.nsys);
+	std::stable_sort(recs.begin(), recs.end(), [](const std::pair<int, oxh_ext_rec> &x, const std::pair<int, oxh_ext_rec> &y) {
+		return x.first < y.first;
+	});
+	B->ext_of[sys] = recs;
+	// CSR over every system's particles: offsets per (first + particle)
+	std::vector<int> off(b.N + 1, 0);
+	std::vector<oxh_ext_rec> all;
+	for(int s = 0; s < b.nsys; s++)
+		for(auto &x : B->ext_of[s]) off[B->first[s] + x.first + 1]++;
+	for(int i = 0; i < b.N; i++) off[i + 1] += off[i];
+	all.reserve(off[b.N]);
+	for(int s = 0; s < b.nsys; s++)
+		for(auto &x : B->ext_of[s]) all.push_back(x.second);   // systems in order, particles sorted inside each
+	if(b.ext_off) HIPCHK(hipFree(b.ext_off));
+	if(b.ext_rec) HIPCHK(hipFree(b.ext_rec));
+	b.ext_off = nullptr;
+	b.ext_rec = nullptr;
+	b.n_ext = (int) all.size();
+	if(b.n_ext > 0) {
+		TRY(dalloc(B, &b.ext_off, b.N + 1));
+		TRY(dalloc(B, &b.ext_rec, all.size()));
+		HIPCHK(hipMemcpyAsync(b.ext_off, off.data(), (b.N + 1) * sizeof(int), hipMemcpyHostToDevice, b.stream));
+		HIPCHK(hipMemcpyAsync(b.ext_rec, all.data(), all.size() * sizeof(oxh_ext_rec), hipMemcpyHostToDevice, b.stream));
+	}
+	oxh_sys &S = b.h_sys[sys];
+	S.has_ext = recs.empty() ? 0 : 1;
+	// the kernels count a particle's steps in pstep (the step being
+	// completed in a step's force phase is pstep - 1): after `done` steps
+	// the next force evaluation is pstep - 1 = done, which must read step0
+	std::vector<long long> ps;
+	TRY(download(B, ps, b.pstep, B->first[sys], 1));
+	TRY(sync_and_check(B));
+	const long long done = ps.empty() ? 0 : ps[0];
+	S.ext_step0 = step0 - done;
+	HIPCHK(hipMemcpyAsync(b.d_sys + sys, &S, sizeof(oxh_sys), hipMemcpyHostToDevice, b.stream));
+	// the workgroup engine has no external-force phase: such batches step
+	// with the grid engine
+	if(b.n_ext > 0 && B->persist) B->persist = false;
+	if(b.graph_exec) {
+		(void) hipGraphExecDestroy(b.graph_exec);
+		b.graph_exec = nullptr;
+	}
+	enqueue_forces(B);
+	return sync_and_check(B);
+}
+
+oxh_status oxh_batch_get_images(oxh_batch *B, int sys, int *img3) {
+	TRY(check_sys(B, sys));
+	if(!img3) return fail(OXH_ERR_ARG, "null image buffer");
+	const int f = B->first[sys], n = B->n[sys];
+	std::vector<int4> m;
+	std::vector<int> o;
+	TRY(download(B, m, B->b.img, f, n));
+	TRY(download(B, o, B->b.orig, f, n));
+	TRY(sync_and_check(B));
+	for(int k = 0; k < n; k++) {
+		img3[3 * o[k]] = m[k].x;
+		img3[3 * o[k] + 1] = m[k].y;
+		img3[3 * o[k] + 2] = m[k].z;
+	}
+	return OXH_OK;
+}
+
+oxh_status oxh_batch_set_images(oxh_batch *B, int sys, const int *img3) {
+	TRY(check_sys(B, sys));
+	if(!img3) return fail(OXH_ERR_ARG, "null image buffer");
+	const int f = B->first[sys], n = B->n[sys];
+	std::vector<int> o;
+	TRY(download(B, o, B->b.orig, f, n));
+	TRY(sync_and_check(B));
+	std::vector<int4> m(n);
+	for(int k = 0; k < n; k++) m[k] = make_int4(img3[3 * o[k]], img3[3 * o[k] + 1], img3[3 * o[k] + 2], 0);
+	HIPCHK(hipMemcpyAsync(B->b.img + f, m.data(), n * sizeof(int4), hipMemcpyHostToDevice, B->b.stream));
+	// forces that depend on absolute positions follow the new images
+	if(B->b.n_ext > 0) enqueue_forces(B);
+	return sync_and_check(B);
 }
 
 oxh_status oxh_batch_get_order(oxh_batch *B, int sys, int *order) {
@@ -1820,11 +1984,30 @@ oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {  // 11 doubles
 // built from the same systems continues bitwise as if never interrupted.
 namespace {
 struct CheckpointHeader {
-	char magic[8];            // "OXHCKPT1"
+	char magic[8];            // "OXHCKPT2"
 	int N, nsys, fp64, persist;
 	long long step, since_sort, since_fix;
 	int lanes, force_block, far_lanes, far_block, edge, edge_group, ecap, pad_;
+	// FNV-1a of every system's parameters (range, box, cells, model
+	// constants, integrator, thermostat and seed): a checkpoint restores
+	// only into a batch built from the same systems
+	unsigned long long sys_hash;
 };
+
+unsigned long long systems_fingerprint(const oxh_batch *B) {
+	unsigned long long h = 1469598103934665603ull;
+	for(const oxh_sys &S0 : B->b.h_sys) {
+		oxh_sys S = S0;
+		S.has_ext = 0;        // external forces are set again after a restore
+		S.ext_step0 = 0;
+		const unsigned char *c = (const unsigned char *) &S;
+		for(size_t k = 0; k < sizeof(S); k++) {
+			h ^= c[k];
+			h *= 1099511628211ull;
+		}
+	}
+	return h;
+}
 
 // every device array a checkpoint holds, with its byte size
 std::vector<std::pair<void *, size_t>> ckpt_layout(oxh_batch *B) {
@@ -1834,7 +2017,7 @@ std::vector<std::pair<void *, size_t>> ckpt_layout(oxh_batch *B) {
 			{b.pos, N * sizeof(double4)}, {b.vel, N * sizeof(double4)}, {b.Lm, N * sizeof(double4)}, {b.q64, N * sizeof(double4)},
 			{b.rlist, N * sizeof(double4)}, {b.F, N * sizeof(float4)}, {b.T, N * sizeof(float4)}, {b.pstep, N * sizeof(long long)},
 			{b.orig, N * sizeof(int)}, {b.bonds, N * sizeof(int2)}, {b.pinfo, N * sizeof(int2)},
-			{b.d_state, (size_t) b.nsys * sizeof(oxh_sys_state)}};
+			{b.img, N * sizeof(int4)}, {b.d_state, (size_t) b.nsys * sizeof(oxh_sys_state)}};
 	if(b.fp64) {
 		a.push_back({b.F64, N * sizeof(double4)});
 		a.push_back({b.T64, N * sizeof(double4)});
@@ -1863,7 +2046,8 @@ oxh_status oxh_batch_save(oxh_batch *B, void *buf, size_t bytes) {
 	HIPCHK(hipStreamSynchronize(b.stream));
 	CheckpointHeader h;
 	std::memset(&h, 0, sizeof(h));
-	std::memcpy(h.magic, "OXHCKPT1", 8);
+	std::memcpy(h.magic, "OXHCKPT2", 8);
+	h.sys_hash = systems_fingerprint(B);
 	h.N = b.N;
 	h.nsys = b.nsys;
 	h.fp64 = b.fp64;
@@ -1895,10 +2079,12 @@ oxh_status oxh_batch_restore(oxh_batch *B, const void *buf, size_t bytes) {
 	CheckpointHeader h;
 	std::memcpy(&h, buf, sizeof(h));
 	oxh::Batch &b = B->b;
-	if(std::memcmp(h.magic, "OXHCKPT1", 8) != 0) return fail(OXH_ERR_ARG, "not a checkpoint");
+	if(std::memcmp(h.magic, "OXHCKPT2", 8) != 0) return fail(OXH_ERR_ARG, "not a checkpoint of this library version");
 	if(h.N != b.N || h.nsys != b.nsys || h.fp64 != (int) b.fp64 || h.persist != (int) B->persist)
 		return fail(OXH_ERR_ARG, "checkpoint of another batch (N %d, %d systems) than this one (N %d, %d systems)", h.N, h.nsys, b.N,
 				b.nsys);
+	if(h.sys_hash != systems_fingerprint(B))
+		return fail(OXH_ERR_ARG, "checkpoint of systems with other sizes, boxes or parameters than this batch's");
 	HIPCHK(hipSetDevice(b.device));
 	HIPCHK(hipStreamSynchronize(b.stream));
 	// the checkpointed launch shape, so that forces round as they did
@@ -1924,6 +2110,7 @@ oxh_status oxh_batch_restore(oxh_batch *B, const void *buf, size_t bytes) {
 		b.graph_exec = nullptr;
 	}
 	oxh::launch_prepare_mirror(b.stream, b);
+	oxh::launch_slot_of(b.stream, b);
 	// the lists of the checkpointed run, rebuilt exactly from their build
 	// positions; the rebuild flags and counters are the checkpoint's
 	std::vector<oxh_sys_state> saved(b.nsys);

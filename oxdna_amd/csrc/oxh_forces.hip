@@ -545,7 +545,8 @@ static void launch_edge(hipStream_t st, const Batch &b, bool far_inline, int k) 
 // close (+ bonded) kernel; the far pairs either inline (far_lanes == 0) or
 // in k_forces_far right after it.  The step's forces (split = false) take
 // the half-list form when b.edge is set.
-void launch_forces(hipStream_t st, const Batch &b, bool split, int k) {
+namespace {
+void launch_pair_forces(hipStream_t st, const Batch &b, bool split, int k) {
 	if(b.N == 0) return;
 	if(b.fp64) {
 		switch(b.lanes) {
@@ -583,6 +584,14 @@ void launch_forces(hipStream_t st, const Batch &b, bool split, int k) {
 	case 32: launch_far<32>(st, b, split, block, k); break;
 	default: launch_far<64>(st, b, split, block, k); break;
 	}
+}
+}  // namespace
+
+// pair forces (written, not accumulated), then the external forces added
+// on top
+void launch_forces(hipStream_t st, const Batch &b, bool split, int k) {
+	launch_pair_forces(st, b, split, k);
+	launch_ext_forces(st, b);
 }
 
 }  // namespace oxh

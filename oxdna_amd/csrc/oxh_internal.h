@@ -30,6 +30,8 @@ struct Batch {
 	int2 *bonds = nullptr;   // (n3, n5) batch indices, -1 = none
 	int2 *pinfo = nullptr;   // (btype, system)
 	int *orig = nullptr;     // index of the particle inside its system as uploaded
+	int *slot_of = nullptr;  // inverse of orig: slot_of[first + orig[i]] = i
+	int4 *img = nullptr;     // periodic image counts (absolute position = pos + img * box)
 	long long *pstep = nullptr;  // per-particle step counter (RNG counter)
 
 	// neighbour list, row major: particle i owns nbr[i * max_neigh + k];
@@ -92,6 +94,12 @@ struct Batch {
 	int *hbcount = nullptr;    // N
 	bool want_hbcount = false;
 
+	// external forces: CSR over (system first + original index), N + 1
+	// offsets, records in the order the caller gave them per particle
+	int n_ext = 0;
+	int *ext_off = nullptr;
+	oxh_ext_rec *ext_rec = nullptr;
+
 	// Hilbert sort scratch
 	unsigned int *keys = nullptr, *keys_alt = nullptr;
 	int *perm = nullptr, *perm_alt = nullptr, *inv_perm = nullptr;
@@ -153,5 +161,9 @@ void launch_reduce(hipStream_t st, const Batch &b);
 void launch_refresh_vel(hipStream_t st, const Batch &b, int i0, int n);
 void launch_prepare_mirror(hipStream_t st, const Batch &b);
 int launch_hilbert_sort(hipStream_t st, Batch &b);
+// adds every particle's external forces to F (F64) of the current state
+void launch_ext_forces(hipStream_t st, const Batch &b);
+// slot_of from orig (after an upload or a sort)
+void launch_slot_of(hipStream_t st, const Batch &b);
 
 }  // namespace oxh

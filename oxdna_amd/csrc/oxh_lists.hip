@@ -543,12 +543,14 @@ int launch_hilbert_sort(hipStream_t st, Batch &b) {
 	permute(b.pinfo);
 	permute(b.orig);
 	permute(b.pstep);
+	permute(b.img);
 	{
 		int2 *tmp = (int2 *) b.scratch;
 		k_gather_bonds<<<g, bs, 0, st>>>(N, perm, b.inv_perm, b.bonds, tmp);
 		(void) hipMemcpyAsync(b.bonds, tmp, sizeof(int2) * N, hipMemcpyDeviceToDevice, st);
 	}
 	launch_prepare_mirror(st, b);
+	launch_slot_of(st, b);
 	k_mark_rebuild<<<(b.nsys + 63) / 64, 64, 0, st>>>(b.nsys, b.d_state);
 	return 0;
 }

@@ -29,24 +29,32 @@ dh_half_charged_ends, hb_multiplier, max_backbone_force,
 max_backbone_force_far, energy_file, print_energy_every, trajectory_file,
 print_conf_interval, lastconf_file, lastconf_file_bin, binary_initial_conf,
 fix_diffusion, fix_diffusion_every, CUDA_sort_every, CUDA_device,
-checkpoint_every, checkpoint_file, reload_from, data_output_N (class
+checkpoint_every, checkpoint_file, reload_from, back_in_box,
+use_average_seq with seq_dep_file, external_forces with
+external_forces_file and external_forces_as_JSON, data_output_N (class
 DataOutput: step, potential_energy, kinetic_energy and hb_list
 observables).  Binary configurations are
 the reference's (oxdna_amd/io.py: write_conf_binary, read_conf_binary).
+Configurations are written with absolute (unwrapped) positions, the stored
+position plus its periodic image count times the box, unless back_in_box
+(src/Observables/Configurations/Configuration.cpp:110-135); binary ones
+carry the stored position and the image counts.  Without a seed (or with
+seed = 0) the seed comes from the clock and /dev/urandom, as in the
+reference (src/Managers/SimManager.cpp:78-90), and is printed.
 A checkpoint file holds the step (one JSON line) and the batch's whole
 state (oxh_batch_save); reload_from continues from it bitwise, the batch
 being built from topology and conf_file first (here conf_file only shapes
 the batch).  reload_from also takes the reference's own reload file, a
 binary configuration, with the reference's rules: no conf_file, no seed,
 restart_step_counter false.  sim_type must be MD and interaction_type
-DNA2 or DNA2_nomesh (the GPU interaction is the analytic DNA2_nomesh form);
-use_average_seq = false is refused (sequence-dependent parameter files are
-not read here).
+DNA2 or DNA2_nomesh (the GPU interaction is the analytic DNA2_nomesh form).
 """
 from __future__ import annotations
 
 import os
 import sys
+
+import numpy as np
 
 from . import io
 
@@ -65,21 +73,47 @@ def _num(inp: dict, key: str, cast=float, default=None):
     return cast(float(inp[key])) if cast is int else cast(inp[key])
 
 
-def batch_options(inp: dict) -> dict:
-    """HIPBatch options (reference key names) from an input dict."""
+def random_seed() -> int:
+    """The reference's seed when none is given: the clock plus an int from
+    /dev/urandom (src/Managers/SimManager.cpp:78-90)."""
+    import time
+    seed = int(time.time())
+    try:
+        seed += int.from_bytes(os.urandom(4), "little", signed=True)
+    except (NotImplementedError, OSError):
+        pass
+    return seed & 0xFFFFFFFF
+
+
+def seq_dep_options(path: str) -> dict:
+    """Sequence-dependent well depths of a seq_dep_file, read into FP32 as
+    DNAInteraction::init reads them (getInputFloat,
+    src/Interactions/DNAInteraction.cpp:328-376): STCK_X_Y for the 16 base
+    pairs, STCK_FACT_EPS, HYDR_A_T (or HYDR_T_A) and HYDR_G_C (or
+    HYDR_C_G)."""
+    import numpy as np
+    with open(path) as f:
+        kv = {k: float(np.float32(float(v))) for k, v in io.parse_input(f.read()).items()}
+    names = "AGCT"
+    try:
+        stck = [[kv[f"STCK_{names[i]}_{names[j]}"] for j in range(4)] for i in range(4)]
+        at = kv["HYDR_A_T"] if "HYDR_A_T" in kv else kv["HYDR_T_A"]
+        gc = kv["HYDR_G_C"] if "HYDR_G_C" in kv else kv["HYDR_C_G"]
+        fact = kv["STCK_FACT_EPS"]
+    except KeyError as e:
+        raise ValueError(f"seq_dep_file {path}: missing key {e}") from None
+    return dict(seq_stck=stck, seq_stck_fact_eps=fact, seq_hydr_AT=at, seq_hydr_GC=gc)
+
+
+def batch_options(inp: dict, path=lambda p: p) -> dict:
+    """HIPBatch options (reference key names) from an input dict; `path`
+    resolves the file names the input gives (seq_dep_file)."""
     st = inp.get("sim_type", "MD")
     if st != "MD":
         raise ValueError(f"sim_type = {st}: this driver runs molecular dynamics only")
     it = inp.get("interaction_type", "DNA")
     if it not in ("DNA2", "DNA2_nomesh"):
         raise ValueError(f"interaction_type = {it}: this driver runs the oxDNA2 interaction (DNA2)")
-    if not _bool(inp.get("use_average_seq", "true")):
-        raise ValueError("use_average_seq = false: sequence-dependent parameters are not supported by this driver")
-    if _bool(inp.get("external_forces", "false")):
-        # src/Backends/SimBackend.cpp:282 reads the key; with it set the reference
-        # applies the forces of external_forces_file, and running without them
-        # would silently simulate another system
-        raise ValueError("external_forces = true: external forces are not supported by this driver")
     th = inp.get("thermostat", "no")
     if th == "john":
         th = "brownian"
@@ -90,7 +124,7 @@ def batch_options(inp: dict) -> dict:
         verlet_skin=_num(inp, "verlet_skin", float, 0.05),
         thermostat=th,
         refresh_vel=_bool(inp.get("refresh_vel", "false")),
-        seed=_num(inp, "seed", int, 1),
+        seed=_num(inp, "seed", int, 0),
         backend_precision=inp.get("backend_precision", "mixed"),
         dh_half_charged_ends=_bool(inp.get("dh_half_charged_ends", "true")),
     )
@@ -99,6 +133,12 @@ def batch_options(inp: dict) -> dict:
                       ("hb_multiplier", float), ("max_backbone_force", float), ("max_backbone_force_far", float)):
         if key in inp:
             opts[key] = _num(inp, key, cast)
+    if not _bool(inp.get("use_average_seq", "true")):
+        if "seq_dep_file" not in inp:
+            raise ValueError("use_average_seq = false needs seq_dep_file")
+        opts.update(seq_dep_options(path(inp["seq_dep_file"])))
+    if opts["seed"] == 0:
+        opts["seed"] = random_seed()
     return opts
 
 
@@ -218,7 +258,9 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
     inp = io.read_input(input_path, overrides)
     wd = workdir or os.path.dirname(os.path.abspath(input_path))
     path = lambda p: p if os.path.isabs(p) else os.path.join(wd, p)  # noqa: E731
-    opts = batch_options(inp)
+    opts = batch_options(inp, path)
+    if "seed" not in inp or _num(inp, "seed", int) == 0:
+        print(f"seeding the RNG with {opts['seed']}", file=sys.stderr)
     restart = _bool(inp.get("restart_step_counter", "false"))
     # reload_from: this library's checkpoint (a JSON line, then the batch's
     # state), or the reference's own form, a binary configuration
@@ -241,8 +283,9 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
     else:
         conf_path, binary = path(inp["conf_file"]), _bool(inp.get("binary_initial_conf", "false"))
     s = io.load_system(path(inp["topology"]), conf_path, T=opts["T"], salt=opts["salt_concentration"], binary=binary)
+    shifts = None
     if binary:
-        conf_step, *_, words = io.read_conf_binary(conf_path, s.N)
+        conf_step, *_, words, shifts = io.read_conf_binary(conf_path, s.N, with_shift=True)
         if reload_path and not ckpt_reload:
             opts["seed"] = words[0] | (words[1] << 16) | (words[2] << 32)
     else:
@@ -259,6 +302,11 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
     last_path = path(inp.get("lastconf_file", "last_conf.dat"))
     ckpt_every = _num(inp, "checkpoint_every", int, 0)
     ckpt_path = path(inp["checkpoint_file"]) if "checkpoint_file" in inp else None
+    back_in_box = _bool(inp.get("back_in_box", "false"))
+    ext = None
+    if _bool(inp.get("external_forces", "false")):
+        ext = io.read_external_forces(path(inp["external_forces_file"]), s,
+                                      as_json=_bool(inp.get("external_forces_as_JSON", "false")))
     reload = None
     if ckpt_reload:
         with open(reload_path, "rb") as f:
@@ -270,16 +318,33 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
     with HIPBatch(s, device=device, **opts) as b:
         b.set_sort_interval(_num(inp, "CUDA_sort_every", int, 0))
         b.set_fix_diffusion(0)   # re-boxing is done here, at the reference's steps
+        if shifts is not None and shifts.any():
+            b.set_images(shifts)
+        if ext is not None:
+            # the forces of the run's first step read the step counter `start`
+            b.set_external_forces(ext, step0=start)
         if reload is not None:
             b.restore(reload)
 
         def snapshot(step):
+            """(U, K, state with stored positions, state as printed)."""
             E = b.energies()[0]
             U, K = E[0] / s.N, (E[2] + E[3]) / s.N
             st = b.state(0)
             t = s.copy()
             t.pos, t.a1, t.a3, t.vel, t.L = st["pos"], st["a1"], st["a3"], st["vel"], st["L"]
-            return U, K, t
+            img = b.images(0)
+            shown = t.copy()
+            if back_in_box:
+                # each strand by the whole boxes that bring its absolute
+                # centre of mass into the box
+                absolute = t.pos + img * s.box
+                for k in np.unique(s.strand):
+                    m = s.strand == k
+                    shown.pos[m] = t.pos[m] - np.floor(absolute[m].mean(axis=0) / s.box) * s.box
+            else:
+                shown.pos = t.pos + img * s.box
+            return U, K, t, shown, img
 
         energy_f = open(energy_path, "w")
         traj_f = open(traj_path, "w") if traj_path else None
@@ -308,13 +373,13 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
                     b.fix_diffusion()
                 state = None
                 if c and traj_f:
-                    U, K, state = snapshot(t)
-                    traj_f.write(_conf_text(state, t, U, K))
+                    U, K, _, shown, _ = snapshot(t)
+                    traj_f.write(_conf_text(shown, t, U, K))
                     traj_f.flush()
                 if fix and t < end:
                     b.fix_diffusion()
                 if e:
-                    U, K, state = snapshot(t)
+                    U, K, *_ = snapshot(t)
                     line = energy_line(t, dt, U, K)
                     energy_f.write(line + "\n")
                     energy_f.flush()
@@ -331,10 +396,10 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
             if cur < start + steps:
                 b.run(start + steps - cur)
                 cur = start + steps
-            U, K, state = snapshot(cur)
+            U, K, stored, state, img = snapshot(cur)
             io.write_conf(last_path, state, step=cur, energies=(U + K, U, K))
             if "lastconf_file_bin" in inp:
-                io.write_conf_binary(path(inp["lastconf_file_bin"]), state, step=cur, energies=(U + K, U, K))
+                io.write_conf_binary(path(inp["lastconf_file_bin"]), stored, step=cur, energies=(U + K, U, K), shift=img)
         finally:
             energy_f.close()
             if traj_f:

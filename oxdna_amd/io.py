@@ -209,18 +209,23 @@ _BIN_HEAD = np.dtype([("step", "<i8"), ("seed", "<u2", 3), ("box", "<f8", 3), ("
 _BIN_PART = np.dtype([("pos", "<f8", 3), ("shift", "<i4", 3), ("a", "<f8", 9), ("vel", "<f8", 3), ("L", "<f8", 3)])
 
 
-def write_conf_binary(path: str, s: System, step: int = 0, energies=(0.0, 0.0, 0.0), seed=(0, 0, 0)) -> None:
+def write_conf_binary(path: str, s: System, step: int = 0, energies=(0.0, 0.0, 0.0), seed=(0, 0, 0), shift=None) -> None:
     """Writes the reference's binary configuration.  `energies` = (total,
     potential, kinetic) per nucleotide; `seed` = the three drand48 state
     words the reference stores (its generator's state; this library draws
-    from Philox, so they carry nothing here).  Positions are unwrapped, so
-    the image shifts are 0."""
+    from Philox, so they carry nothing here).  Positions are the stored
+    ones (moved by whole boxes when strands are re-boxed) and `shift` the
+    (N, 3) periodic image counts, as the reference writes p->pos and its
+    _pos_shift (src/Observables/Configurations/BinaryConfiguration.cpp:
+    55-70); None writes zeros."""
     vel = s.vel if s.vel is not None else np.zeros((s.N, 3))
     L = s.L if s.L is not None else np.zeros((s.N, 3))
     head = np.zeros(1, _BIN_HEAD)
     head["step"], head["seed"], head["box"], head["E"] = step, seed, s.box, energies
     rec = np.zeros(s.N, _BIN_PART)
     rec["pos"] = s.pos
+    if shift is not None:
+        rec["shift"] = shift
     rec["a"] = np.concatenate([s.a1, np.cross(s.a3, s.a1), s.a3], axis=1)
     rec["vel"] = vel
     rec["L"] = L
@@ -230,10 +235,11 @@ def write_conf_binary(path: str, s: System, step: int = 0, energies=(0.0, 0.0, 0
         f.write(b"\n")
 
 
-def read_conf_binary(path: str, N: int):
+def read_conf_binary(path: str, N: int, with_shift: bool = False):
     """Returns (step, box, pos, a1, a2, a3, vel, L, seed) of the reference's
-    binary configuration; the axes are taken as stored (the reference does
-    not re-orthonormalise them here)."""
+    binary configuration (and the (N, 3) image shifts last with
+    with_shift); the axes are taken as stored (the reference does not
+    re-orthonormalise them here)."""
     with open(path, "rb") as f:
         raw = f.read()
     need = _BIN_HEAD.itemsize + N * _BIN_PART.itemsize
@@ -242,8 +248,9 @@ def read_conf_binary(path: str, N: int):
     head = np.frombuffer(raw, _BIN_HEAD, 1)[0]
     rec = np.frombuffer(raw, _BIN_PART, N, _BIN_HEAD.itemsize)
     a = rec["a"]
-    return (int(head["step"]), head["box"].copy(), rec["pos"].copy(), a[:, 0:3].copy(), a[:, 3:6].copy(), a[:, 6:9].copy(),
-            rec["vel"].copy(), rec["L"].copy(), tuple(int(x) for x in head["seed"]))
+    out = (int(head["step"]), head["box"].copy(), rec["pos"].copy(), a[:, 0:3].copy(), a[:, 3:6].copy(), a[:, 6:9].copy(),
+           rec["vel"].copy(), rec["L"].copy(), tuple(int(x) for x in head["seed"]))
+    return out + (rec["shift"].astype(np.int32),) if with_shift else out
 
 
 def load_system(topology: str, conf: str, T: float = 0.1, salt: float = 0.5, name: str = "", binary: bool = False) -> System:
@@ -254,3 +261,125 @@ def load_system(topology: str, conf: str, T: float = 0.1, salt: float = 0.5, nam
         step, box, pos, a1, a2, a3, vel, L = read_conf(conf, len(btype))
     return System(pos=pos, a1=a1, a3=a3, btype=btype, n3=n3, n5=n5, strand=strand, box=box, T=T, salt=salt,
                   name=name, vel=vel, L=L)
+
+
+EXT_TYPES = ("string", "trap", "mutual_trap", "constant_trap", "repulsion_plane", "sphere")
+
+
+def _vec3(v) -> tuple:
+    if isinstance(v, (list, tuple)):
+        return tuple(float(x) for x in v)
+    parts = str(v).replace(" ", "").split(",")
+    if len(parts) != 3:
+        raise ValueError(f"could not parse the vector {v!r}")
+    return tuple(float(x) for x in parts)
+
+
+def particles_from_string(text, s: System) -> list:
+    """Particle indices of an external force's `particle` key, as the
+    reference's Utils::get_particles_from_string (src/Utilities/Utils.cpp:
+    409-500): comma-separated indices, `last`, `all` or -1 (every
+    particle), and ranges a-b that walk from a along the 5' neighbours (n5)
+    until b, both included."""
+    out = []
+    for tok in str(text).replace(" ", "").split(","):
+        if "-" in tok and not tok.startswith("-"):
+            a, b = tok.split("-", 1)
+            p = [s.N - 1 if x == "last" else int(x) for x in (a, b)]
+            for x in p:
+                if not 0 <= x < s.N:
+                    raise ValueError(f"particle {x} out of range")
+            j = p[0]
+            while True:
+                out.append(j)
+                if j == p[1]:
+                    break
+                j = int(s.n5[j])
+                if j < 0 or j == p[0]:
+                    raise ValueError(f"could not get from particle {p[0]} to particle {p[1]}")
+        elif tok == "last":
+            out.append(s.N - 1)
+        elif tok in ("all", "-1"):
+            out.extend(range(s.N))
+        else:
+            j = int(tok)
+            if not 0 <= j < s.N:
+                raise ValueError(f"particle {j} out of range")
+            out.append(j)
+    return out
+
+
+def _force_blocks(text: str) -> list:
+    """key = value blocks in braces of an external_forces_file (the
+    reference's plain format, src/Forces/ForceFactory.cpp:140-200: '#'
+    comments, one force per {...})."""
+    clean = "\n".join(line.split("#", 1)[0] for line in text.splitlines())
+    blocks, depth, cur = [], 0, []
+    for ch in clean:
+        if ch == "{":
+            depth += 1
+            if depth > 1:
+                raise ValueError("external forces file: nested braces")
+            cur = []
+        elif ch == "}":
+            depth -= 1
+            if depth < 0:
+                raise ValueError("external forces file: unbalanced braces")
+            body = "".join(cur).strip()
+            if not body:
+                raise ValueError("external forces file: nothing between braces")
+            blocks.append(parse_input(body))
+        elif depth == 1:
+            cur.append(ch)
+    return blocks
+
+
+def read_external_forces(path: str, s: System, as_json: bool = False) -> list:
+    """The forces of an external_forces_file, expanded to one record per
+    particle in the file's order: dicts with the reference's keys (type,
+    particle, F0, rate, stiff, stiff_rate, r0, ref_particle, PBC, dir,
+    dir_as_centre, pos0, position, end_position, v, r_ext, center).  Types:
+    string, trap, mutual_trap, constant_trap, repulsion_plane, sphere; any
+    other type raises."""
+    with open(path) as f:
+        text = f.read()
+    if as_json:
+        import json
+        blocks = [{k: (v if isinstance(v, (list, tuple)) else str(v)) for k, v in b.items()} for b in json.loads(text)]
+    else:
+        blocks = _force_blocks(text)
+    out = []
+    for b in blocks:
+        t = b.get("type")
+        if t not in EXT_TYPES:
+            raise ValueError(f"external force type {t!r} is not supported (supported: {', '.join(EXT_TYPES)})")
+        rec = {"type": t}
+        for key in ("F0", "rate", "stiff", "stiff_rate", "r0", "position", "end_position", "v", "r_ext"):
+            if key in b:
+                rec[key] = float(b[key])
+        for key in ("dir", "pos0"):
+            if key in b:
+                rec[key] = _vec3(b[key])
+        if "center" in b:
+            rec["dir"] = _vec3(b["center"])
+        for key in ("PBC", "dir_as_centre"):
+            if key in b:
+                rec[key] = str(b[key]).strip().lower() in ("1", "true", "yes")
+        required = {"string": ("F0", "rate", "dir"), "trap": ("stiff", "rate", "dir", "pos0"),
+                    "mutual_trap": ("stiff", "r0"), "constant_trap": ("stiff", "r0"),
+                    "repulsion_plane": ("stiff", "position", "dir"), "sphere": ("stiff", "r0")}[t]
+        for key in required:
+            if key not in rec:
+                raise ValueError(f"external force {t}: missing key {key}")
+        if t in ("mutual_trap", "constant_trap"):
+            rec["ref_particle"] = int(b["ref_particle"])
+            pids = [int(b["particle"])]
+            if pids[0] == -1:
+                raise ValueError(f"{t} cannot act on every particle")
+        else:
+            pids = particles_from_string(b["particle"], s)
+        for pid in pids:
+            r = dict(rec)
+            r["particle"] = pid
+            out.append(r)
+    return out

@@ -74,7 +74,93 @@ OBSERVABLES = """data_output_1 = {
 """
 
 
+# external forces of every supported type, acting from step 500 on (the
+# configuration's step counter is kept), on a duplex whose second strand
+# starts one box length away in x, so that the run re-boxes it (fix_diffusion
+# every 100 steps) and prints absolute positions afterwards
+EXT_FORCES = """{
+type = string
+particle = 0-3
+F0 = 0.3
+rate = 2e-4
+dir = 1, 0.5, 0
+}
+{
+type = trap
+particle = 10
+stiff = 1.0
+rate = 1e-4
+dir = 0, 0, 1
+pos0 = 10, 10, 10
+}
+{
+type = mutual_trap
+particle = 5
+ref_particle = 34
+stiff = 1.5
+r0 = 1.2
+PBC = 1
+rate = 1e-5
+stiff_rate = 1e-4
+}
+{
+type = constant_trap
+particle = 7
+ref_particle = 30
+stiff = 0.2
+r0 = 3
+}
+{
+type = repulsion_plane
+particle = -1
+stiff = 2
+dir = 0, 0, -1
+position = 13
+v = -1e-3
+end_position = 12.6
+}
+{
+type = sphere
+particle = all
+stiff = 1.5
+r0 = 3.5
+rate = 1e-4
+center = 10, 10, 10
+}
+"""
+
+EXT_INPUT = dict(INPUT, restart_step_counter="0", external_forces="1", external_forces_file="ext.txt",
+                 fix_diffusion="1", fix_diffusion_every="100", print_conf_interval="100")
+
+
+def write_case(out, s, inp, files=(), observables=""):
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        io.write_topology(os.path.join(d, "t.top"), s)
+        io.write_conf(os.path.join(d, "c.dat"), s, step=int(inp.get("_step", 0)))
+        for name, text in files:
+            with open(os.path.join(d, name), "w") as f:
+                f.write(text)
+        with open(os.path.join(d, "input"), "w") as f:
+            for k, v in inp.items():
+                if not k.startswith("_"):
+                    f.write(f"{k} = {v}\n")
+            f.write(observables)
+        r = subprocess.run([REF_BIN, "input"], cwd=d, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference failed: {r.stdout[-2000:]} {r.stderr[-2000:]}")
+        os.makedirs(out, exist_ok=True)
+        pairs = [("input", "input"), ("t.top", "t.top"), ("c.dat", "c.dat"), ("energy.dat", "energy_ref.dat"),
+                 ("trajectory.dat", "trajectory_ref.dat"), ("last_conf.dat", "last_conf_ref.dat"),
+                 ("last_conf.bin", "last_conf_ref.bin")] + [(n, n) for n, _ in files]
+        if observables:
+            pairs.append(("obs.dat", "obs_ref.dat"))
+        for src_, dst in pairs:
+            shutil.copy(os.path.join(d, src_), os.path.join(out, dst))
+    print("wrote", out)
+
+
 def main():
+    ext_case()
     s = systems.duplex_20bp(3)
     rng = np.random.default_rng(11)
     s.vel = rng.normal(scale=np.sqrt(s.T), size=(s.N, 3))
@@ -96,6 +182,16 @@ def main():
                          ("last_conf.bin", "last_conf_ref.bin"), ("obs.dat", "obs_ref.dat")):
             shutil.copy(os.path.join(d, src), os.path.join(OUT, dst))
     print("wrote", OUT)
+
+
+def ext_case():
+    s = systems.duplex_20bp(3)
+    rng = np.random.default_rng(12)
+    s.vel = rng.normal(scale=np.sqrt(s.T), size=(s.N, 3))
+    s.vel -= s.vel.mean(axis=0)
+    s.L = rng.normal(scale=np.sqrt(s.T), size=(s.N, 3))
+    s.pos[s.strand == 1] += np.array([s.box[0], 0.0, 0.0])
+    write_case(os.path.join(HERE, "driver_ext"), s, dict(EXT_INPUT, _step=500), files=[("ext.txt", EXT_FORCES)])
 
 
 if __name__ == "__main__":

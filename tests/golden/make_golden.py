@@ -68,7 +68,12 @@ def read_seq_file(path: str) -> dict:
     return dict(seq_stck=stck, seq_stck_fact_eps=kv["STCK_FACT_EPS"], seq_hydr_AT=at, seq_hydr_GC=gc)
 
 
-def run_reference(s: systems.System, T_str: str, salt: float, extra: dict | None = None, interaction="DNA2_nomesh"):
+def run_reference(s: systems.System, T_str: str, salt: float, extra: dict | None = None, interaction="DNA2_nomesh",
+                  step: int = 0, files: dict | None = None):
+    """Step-0 observables of the reference binary on s.  step: the step
+    counter written in the configuration (kept with restart_step_counter =
+    false); files: extra input files (name -> text) written next to the
+    input file, such as an external_forces_file."""
     extra = extra or {}
     s = s.copy()
     # zero velocities, so that the pressure observable holds the pair virial
@@ -80,7 +85,10 @@ def run_reference(s: systems.System, T_str: str, salt: float, extra: dict | None
     s.L[:, 0] = 1e-6
     with tempfile.TemporaryDirectory(dir="/tmp") as d:
         io.write_topology(os.path.join(d, "t.top"), s)
-        io.write_conf(os.path.join(d, "c.dat"), s)
+        io.write_conf(os.path.join(d, "c.dat"), s, step=step)
+        for fname, text in (files or {}).items():
+            with open(os.path.join(d, fname), "w") as f:
+                f.write(text)
         if extra.get("use_average_seq") == "false":
             shutil.copy(SEQ_FILE, os.path.join(d, "seq.txt"))
             extra = dict(extra, seq_dep_file="seq.txt")
@@ -89,7 +97,7 @@ def run_reference(s: systems.System, T_str: str, salt: float, extra: dict | None
             "T": T_str, "salt_concentration": repr(salt), "dt": "0.003", "verlet_skin": "0.05",
             "thermostat": "no", "refresh_vel": "0", "topology": "t.top", "conf_file": "c.dat",
             "trajectory_file": "traj.dat", "energy_file": "energy.dat", "print_energy_every": "1000",
-            "print_conf_interval": "100000", "no_stdout_energy": "1", "restart_step_counter": "1",
+            "print_conf_interval": "100000", "no_stdout_energy": "1", "restart_step_counter": "0" if step else "1",
             "box_type": "orthogonal", "time_scale": "linear", "plugin_search_path": PLUGIN_DIR, "log_file": "log.dat",
         }
         inp.update({k: str(v) for k, v in extra.items()})

@@ -42,11 +42,45 @@ def test_driver_refuses_what_it_does_not_simulate():
     never run without the missing part."""
     import pytest
     base = io.read_input(os.path.join(D, "input"))
-    for key, val in (("external_forces", "1"), ("sim_type", "MC"), ("interaction_type", "DNA"),
-                     ("use_average_seq", "false")):
+    for key, val in (("sim_type", "MC"), ("interaction_type", "DNA")):
         with pytest.raises(ValueError, match=key):
             driver.batch_options(dict(base, **{key: val}))
+    # sequence dependence without its parameter file
+    with pytest.raises(ValueError, match="seq_dep_file"):
+        driver.batch_options(dict(base, use_average_seq="false"))
     assert driver.batch_options(dict(base, external_forces="false"))["dt"] == 0.003
+
+
+def test_driver_reads_seq_dep_file(tmp_path):
+    """use_average_seq = false reads the file's well depths in FP32, as the
+    reference does; they are the ones the fixture seqdep_md_25C pins."""
+    from _common import load_golden
+    p = tmp_path / "seq.txt"
+    keys = ["STCK_FACT_EPS = 0.18"]
+    names = "AGCT"
+    _, want, _ = load_golden("seqdep_md_25C")
+    for i in range(4):
+        for j in range(4):
+            keys.append(f"STCK_{names[i]}_{names[j]} = {float(want['seq_stck'][i][j])!r}")
+    keys += [f"HYDR_T_A = {float(want['seq_hydr_AT'])!r}", f"HYDR_C_G = {float(want['seq_hydr_GC'])!r}"]
+    p.write_text("\n".join(keys) + "\n")
+    base = io.read_input(os.path.join(D, "input"))
+    got = driver.batch_options(dict(base, use_average_seq="false", seq_dep_file=str(p)))
+    assert np.array_equal(np.array(got["seq_stck"]), want["seq_stck"])
+    assert got["seq_stck_fact_eps"] == want["seq_stck_fact_eps"]
+    assert got["seq_hydr_AT"] == want["seq_hydr_AT"] and got["seq_hydr_GC"] == want["seq_hydr_GC"]
+
+
+def test_seed_defaults_like_the_reference():
+    """A given seed is kept; without one (or with 0) each run draws a new
+    one from the clock and /dev/urandom (src/Managers/SimManager.cpp:
+    78-90), so replicas launched from one input differ."""
+    base = io.read_input(os.path.join(D, "input"))
+    assert driver.batch_options(dict(base, seed="12345"))["seed"] == 12345
+    no_seed = {k: v for k, v in base.items() if k != "seed"}
+    seeds = {driver.batch_options(no_seed)["seed"] for _ in range(8)}
+    assert len(seeds) > 1 and 0 not in seeds
+    assert driver.batch_options(dict(base, seed="0"))["seed"] != 0
 
 
 def test_output_schedule_matches_reference_files():

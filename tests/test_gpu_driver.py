@@ -226,3 +226,36 @@ def test_driver_checkpoint_reload(hip, tmp_path):
     ea = (tmp_path / "e_a.dat").read_text().splitlines()
     eb = (tmp_path / "e_b1.dat").read_text().splitlines() + (tmp_path / "e_b2.dat").read_text().splitlines()[1:]
     assert ea == eb
+
+
+@pytest.mark.parametrize("precision,e_tol,x_tol", [("double", 2e-6, 1e-10), ("mixed", 5e-6, 1e-5)])
+def test_driver_external_forces_and_images(hip, tmp_path, precision, e_tol, x_tol):
+    """tests/golden/driver_ext: 300 NVE steps from step 500 (the step
+    counter is kept, so rates act) under every supported external force
+    type, with the second strand starting one box length away, so that the
+    run re-boxes it (fix_diffusion every 100 steps).  The energy file, the
+    trajectory and the last configuration, printed with absolute positions
+    (stored position + image count x box, as the reference prints them),
+    follow the reference binary's; the binary last configuration carries
+    the same positions and image counts."""
+    from oxdna_amd import driver, io
+    from test_driver import _confs
+    d = os.path.join(GOLDEN, "driver_ext")
+    for f in ("input", "t.top", "c.dat", "ext.txt"):
+        shutil.copy(os.path.join(d, f), tmp_path / f)
+    res = driver.run(str(tmp_path / "input"), [f"backend_precision = {precision}"])
+    assert res["step"] == 800
+    mine = np.loadtxt(tmp_path / "energy.dat", ndmin=2)
+    ref = np.loadtxt(os.path.join(d, "energy_ref.dat"), ndmin=2)
+    assert mine.shape == ref.shape
+    de = np.abs(mine - ref).max()
+    traj, traj_ref = _confs(tmp_path / "trajectory.dat"), _confs(os.path.join(d, "trajectory_ref.dat"))
+    last, last_ref = _confs(tmp_path / "last_conf.dat"), _confs(os.path.join(d, "last_conf_ref.dat"))
+    assert [t for t, _ in traj] == [t for t, _ in traj_ref]
+    dx = max(np.abs(a[:, :9] - b[:, :9]).max() for (_, a), (_, b) in zip(traj + last, traj_ref + last_ref))
+    print(f"{precision}: energy file {de:.1e}, positions/axes {dx:.1e}")
+    assert de <= e_tol and dx <= x_tol
+    got = io.read_conf_binary(str(tmp_path / "last_conf.bin"), 40, with_shift=True)
+    want = io.read_conf_binary(os.path.join(d, "last_conf_ref.bin"), 40, with_shift=True)
+    assert np.array_equal(got[-1], want[-1]) and want[-1].any()
+    assert np.abs(got[2] - want[2]).max() <= x_tol
