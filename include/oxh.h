@@ -114,11 +114,43 @@ oxh_status oxh_batch_destroy(oxh_batch *b);
  * (src/CUDA/Lists/CUDABaseList.h:55, CUDASimpleVerletList.cu:231-289). */
 oxh_status oxh_batch_update_lists(oxh_batch *b, int force);
 
-/* Forces and torques of the current state into the device buffers (F lab
- * frame, T body frame).  Replaces CUDABaseInteraction::compute_forces
- * (src/CUDA/Interactions/CUDABaseInteraction.h:60) with
- * CUDADNAInteraction's kernel (src/CUDA/Interactions/CUDA_DNA.cuh:832-904). */
+/* Forces and torques of the batch's current state into the batch's own
+ * buffers (F lab frame, T body frame; written, not accumulated, with the
+ * batch's external forces added; read them with oxh_batch_get_forces).
+ * The kernels are CUDADNAInteraction's (src/CUDA/Interactions/
+ * CUDA_DNA.cuh:832-904); oxh_interaction_compute_forces below is the
+ * accumulating surface of CUDABaseInteraction::compute_forces. */
 oxh_status oxh_batch_compute_forces(oxh_batch *b);
+
+/* The GPU interaction surface with caller-owned device buffers, as
+ * CUDABaseInteraction::compute_forces(lists, d_poss, d_orientations,
+ * d_forces, d_torques, d_bonds, d_box) (src/CUDA/Interactions/
+ * CUDABaseInteraction.h:60, called by MD_CUDABackend::_forces_second_step,
+ * src/CUDA/Backends/MD_CUDABackend.cu:518-520, after _set_external_forces
+ * (:528) has filled F and T): the pair forces of the state in d_pos / d_quat are
+ * ADDED to what d_forces / d_torques already hold (external forces, or
+ * zeros).  Batches only (not slot pools).
+ *   d_pos     4 doubles per particle (x, y, z, unused), absolute positions;
+ *   d_quat    4 doubles per particle, the unit quaternion (x, y, z, w) of
+ *             the rotation whose matrix has the columns a1, a2 = a3 x a1,
+ *             a3 (the reference's p->orientation);
+ *   d_forces  4 numbers per particle: force (lab frame) and, in .w, the
+ *             particle's share of the potential energy;
+ *   d_torques 4 numbers per particle: torque (body frame, as the
+ *             reference's p->torque) and, in .w, its share of the
+ *             hydrogen-bonding energy.
+ * The numbers are floats when the batch was created with
+ * backend_precision = mixed and doubles with double (the reference's
+ * c_number4).  Particle k of system s sits at index first_s + k, first_s =
+ * N_0 + ... + N_{s-1}.  Topology, box and parameters are the batch's.  The
+ * state becomes the batch's; the lists are rebuilt for the systems in which
+ * a particle moved more than the Verlet skin since their last build (the
+ * reference's CUDABaseList::update when _d_are_lists_old,
+ * src/CUDA/Backends/MD_CUDABackend.cu:580-590).  stream: a hipStream_t, or
+ * NULL for the batch's own; the additions are queued on it (the call
+ * returns after the list check, before they finish). */
+oxh_status oxh_interaction_compute_forces(oxh_batch *b, const double *d_pos, const double *d_quat, void *d_forces, void *d_torques,
+		void *stream);
 
 /* Advances every system by n_steps MD steps: first half kick + drift +
  * rotation, skin-triggered list rebuild, forces, second half kick,

@@ -24,6 +24,40 @@ def _c(a, dt):
     return np.ascontiguousarray(a, dtype=dt)
 
 
+def quat_from_frame(a1, a3) -> np.ndarray:
+    """(N, 4) unit quaternions (x, y, z, w) of the rotations whose matrices
+    have the columns a1, a2 = a3 x a1, a3 (the reference's p->orientation),
+    the layout oxh_interaction_compute_forces reads.  Shepperd's method:
+    the largest of 1 + trace and 1 + 2 M_kk - trace picks the component
+    computed from a square root."""
+    a1 = np.asarray(a1, np.float64)
+    a3 = np.asarray(a3, np.float64)
+    a2 = np.cross(a3, a1)
+    M = np.stack([a1, a2, a3], axis=2)   # M[:, row, col], columns a1, a2, a3
+    tr = M[:, 0, 0] + M[:, 1, 1] + M[:, 2, 2]
+    cand = np.stack([tr, M[:, 0, 0], M[:, 1, 1], M[:, 2, 2]], axis=1)
+    pick = np.argmax(cand, axis=1)
+    q = np.zeros((len(a1), 4))
+    for k in range(4):
+        sel = pick == k
+        m = M[sel]
+        if k == 0:
+            s = 2.0 * np.sqrt(1.0 + tr[sel])
+            q[sel] = np.stack([(m[:, 2, 1] - m[:, 1, 2]) / s, (m[:, 0, 2] - m[:, 2, 0]) / s,
+                               (m[:, 1, 0] - m[:, 0, 1]) / s, 0.25 * s], axis=1)
+        else:
+            i = k - 1
+            j, l = (i + 1) % 3, (i + 2) % 3
+            s = 2.0 * np.sqrt(1.0 + m[:, i, i] - m[:, j, j] - m[:, l, l])
+            comp = np.zeros((len(m), 4))
+            comp[:, i] = 0.25 * s
+            comp[:, j] = (m[:, j, i] + m[:, i, j]) / s
+            comp[:, l] = (m[:, l, i] + m[:, i, l]) / s
+            comp[:, 3] = (m[:, l, j] - m[:, j, l]) / s
+            q[sel] = comp
+    return q / np.linalg.norm(q, axis=1, keepdims=True)
+
+
 def make_desc(s: System, keep: list, **opts) -> SystemDesc:
     """oxh_system_desc of System s; option names are the reference's
     input keys (verlet_skin, dt, thermostat, diff_coeff, newtonian_steps,
@@ -211,6 +245,17 @@ class HIPBatch:
         out = np.zeros(self.systems[sys].N, np.int32)
         check(lib().oxh_batch_get_order(self.h, sys, ptr(out)), "oxh_batch_get_order")
         return out
+
+    def interaction_forces(self, d_pos: int, d_quat: int, d_forces: int, d_torques: int, stream: int = 0):
+        """Adds the pair forces and torques of the state in the caller's
+        device buffers to the caller's device buffers
+        (oxh_interaction_compute_forces): d_pos and d_quat hold 4 doubles
+        per particle (see quat_from_frame), d_forces and d_torques 4 floats
+        (mixed) or 4 doubles (double) per particle, in upload order.  All
+        arguments are device addresses (ints); stream is a hipStream_t
+        handle or 0 for the batch's own."""
+        check(lib().oxh_interaction_compute_forces(self.h, d_pos, d_quat, d_forces, d_torques, stream or None),
+              "oxh_interaction_compute_forces")
 
     def forces(self, sys: int = 0):
         n = self.systems[sys].N

@@ -15,6 +15,7 @@
 #include "../../include/oxh.h"
 #include "oxh_device.h"
 #include "oxh_internal.h"
+#include "oxh_step.h"
 
 extern "C" int oxh_internal_hilbert_probe(int n, const void *d_xyzb, void *d_out, hipStream_t st);
 
@@ -1305,6 +1306,107 @@ oxh_status oxh_batch_compute_forces(oxh_batch *B) {
 	HIPCHK(hipSetDevice(B->b.device));
 	enqueue_forces(B);
 	return sync_and_check(B);
+}
+
+}  // extern "C"
+
+namespace {
+
+// The caller's state (upload order: particle k of system s at index
+// first_s + k) into the batch's slots, with the FP32 mirror and the skin
+// check against the positions of each system's last list build.
+__global__ void k_load_caller_state(int N, const double4 *__restrict__ upos, const double4 *__restrict__ uq, const int *__restrict__ orig,
+		const int2 *__restrict__ pinfo, const int2 *__restrict__ bonds, const oxh_sys *__restrict__ systems,
+		const double4 *__restrict__ rlist, double4 *__restrict__ pos, double4 *__restrict__ q64, float4 *__restrict__ q32,
+		float4 *__restrict__ bpos, double4 *__restrict__ back64, oxh_sys_state *__restrict__ st, int *__restrict__ any) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= N) return;
+	const int s = pinfo[i].y;
+	const oxh_sys &S = systems[s];
+	if(oxh::padding(i, S)) return;
+	const int u = S.first + orig[i];
+	double4 r = upos[u];
+	r.w = 0.0;
+	const double4 q = uq[u];
+	pos[i] = r;
+	q64[i] = q;
+	oxh::write_mirror(i, r, q, S, bonds[i], q32, bpos, back64);
+	const double4 rl = rlist[i];
+	const double dx = r.x - rl.x, dy = r.y - rl.y, dz = r.z - rl.z;
+	if(dx * dx + dy * dy + dz * dz > S.skin2) {
+		st[s].need_rebuild = 1;
+		*any = 1;
+	}
+}
+
+// caller's F / T (upload order) += the batch's forces and torques, .w
+// included (energy share, HB energy share)
+template<class FT>
+__global__ void k_add_to_caller(int N, const FT *__restrict__ F, const FT *__restrict__ T, const int *__restrict__ orig,
+		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, FT *__restrict__ uF, FT *__restrict__ uT) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= N) return;
+	const oxh_sys &S = systems[pinfo[i].y];
+	if(oxh::padding(i, S)) return;
+	const int u = S.first + orig[i];
+	FT a = uF[u];
+	const FT f = F[i];
+	a.x += f.x;
+	a.y += f.y;
+	a.z += f.z;
+	a.w += f.w;
+	uF[u] = a;
+	FT b = uT[u];
+	const FT t = T[i];
+	b.x += t.x;
+	b.y += t.y;
+	b.z += t.z;
+	b.w += t.w;
+	uT[u] = b;
+}
+
+}  // namespace
+
+extern "C" {
+
+oxh_status oxh_interaction_compute_forces(oxh_batch *B, const double *d_pos, const double *d_quat, void *d_forces, void *d_torques,
+		void *stream) {
+	if(!B || !d_pos || !d_quat || !d_forces || !d_torques) return fail(OXH_ERR_ARG, "oxh_interaction_compute_forces: bad arguments");
+	if(B->pool) return fail(OXH_ERR_ARG, "oxh_interaction_compute_forces: slot pools take their states through oxh_pool_admit");
+	oxh::Batch &b = B->b;
+	HIPCHK(hipSetDevice(b.device));
+	hipStream_t st = stream ? (hipStream_t) stream : b.stream;
+	// work queued on the batch's own stream comes first
+	if(st != b.stream) HIPCHK(hipStreamSynchronize(b.stream));
+	const int bs = 256, g = (b.N + bs - 1) / bs;
+	k_load_caller_state<<<g, bs, 0, st>>>(b.N, (const double4 *) d_pos, (const double4 *) d_quat, b.orig, b.pinfo, b.bonds, b.d_sys,
+			b.rlist, b.pos, b.q64, b.q32, b.bpos, b.back64, b.d_state, b.any_rebuild);
+	std::vector<oxh_sys_state> sst(b.nsys);
+	for(int attempt = 0;; attempt++) {
+		if(attempt == 12) return fail(OXH_ERR_SYSTEM, "neighbour list keeps overflowing");
+		// lists of the systems whose particles moved more than the skin
+		// (every system after a capacity change), then the forces
+		oxh::launch_cells(st, b, b.pos);
+		oxh::launch_build_list(st, b, 0, b.pos);
+		if(B->persist) oxh::launch_persist_forces(st, b, false, B->persist_block);
+		else oxh::launch_forces(st, b, false);
+		k_clear_need<<<(b.nsys + 63) / 64, 64, 0, st>>>(b.nsys, b.d_state, 0, b.any_rebuild);
+		HIPCHK(hipMemcpyAsync(sst.data(), b.d_state, sizeof(oxh_sys_state) * b.nsys, hipMemcpyDeviceToHost, st));
+		HIPCHK(hipStreamSynchronize(st));
+		HIPCHK(hipGetLastError());
+		bool overflow = false;
+		for(auto &x : sst) overflow |= (x.error & oxh::ERR_OVERFLOW) != 0;
+		if(!overflow) break;
+		TRY(grow_lists(B, sst));
+		HIPCHK(hipStreamSynchronize(b.stream));
+		k_clear_need<<<(b.nsys + 63) / 64, 64, 0, st>>>(b.nsys, b.d_state, 1, b.any_rebuild);
+	}
+	if(b.fp64)
+		k_add_to_caller<double4><<<g, bs, 0, st>>>(b.N, b.F64, b.T64, b.orig, b.pinfo, b.d_sys, (double4 *) d_forces, (double4 *) d_torques);
+	else
+		k_add_to_caller<float4><<<g, bs, 0, st>>>(b.N, b.F, b.T, b.orig, b.pinfo, b.d_sys, (float4 *) d_forces, (float4 *) d_torques);
+	HIPCHK(hipGetLastError());
+	return OXH_OK;
 }
 
 oxh_status oxh_batch_run(oxh_batch *B, long long n_steps) {

@@ -155,3 +155,31 @@ def test_served_pool_plan():
     # nothing above MID_CAP: no large pools
     pools3, classes3 = serve.plan_pools(8 << 30, 4000)
     assert len(pools3) == 2 and classes3 == [serve.SMALL_CAP, serve.MID_CAP]
+
+
+def test_quat_from_frame_reproduces_the_axes():
+    """The quaternion layout oxh_interaction_compute_forces reads: rotating
+    the unit vectors by q gives back a1, a2 = a3 x a1 and a3 (the kernels'
+    own axis formulas), for frames that exercise every branch of the
+    conversion (trace and each diagonal element largest)."""
+    from oxdna_amd import backend
+    rng = np.random.default_rng(3)
+    # random rotations plus the four branch cases: identity and 180-degree turns
+    q0 = rng.normal(size=(200, 4))
+    q0 /= np.linalg.norm(q0, axis=1, keepdims=True)
+    special = np.array([[0, 0, 0, 1], [1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0.]])
+    q0 = np.vstack([q0, special])
+
+    def axes(q):
+        x, y, z, w = q.T
+        a1 = np.stack([x * x - y * y - z * z + w * w, 2 * (x * y + z * w), 2 * (x * z - y * w)], axis=1)
+        a3 = np.stack([2 * (x * z + y * w), 2 * (y * z - x * w), -x * x - y * y + z * z + w * w], axis=1)
+        return a1, a3
+
+    a1, a3 = axes(q0)
+    q = backend.quat_from_frame(a1, a3)
+    b1, b3 = axes(q)
+    assert np.abs(np.linalg.norm(q, axis=1) - 1).max() < 1e-14
+    assert np.abs(b1 - a1).max() < 1e-14 and np.abs(b3 - a3).max() < 1e-14
+    # q and -q are the same rotation
+    assert np.all(np.minimum(np.abs(q - q0).max(axis=1), np.abs(q + q0).max(axis=1)) < 1e-14)

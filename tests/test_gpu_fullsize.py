@@ -1,0 +1,93 @@
+"""Precision at full size, on the two plain workloads, against the FP64
+oracle (the reference's DNA2_nomesh formulation, pinned on the reference
+binary's fixtures):
+
+  - 24,576 nt: 512 random-sequence 24-bp duplexes, density 1e-3, 25 C,
+    0.5 M, Bussi (newtonian_steps 10, bussi_tau 100), dt 0.003, skin 0.05,
+    Hilbert re-sorts every 500 steps.  1e5 MD steps; every 1,000 steps the
+    forces and torques of the current state are compared with the oracle's
+    for the same state (FP32 pair terms: max error <= 1e-5 of the largest
+    force and <= 1e-5 per particle, floor 0.1 x RMS), and the potential
+    energy within 1e-6 per nucleotide;
+  - 196,608 nt: 1,024 random-sequence 96-bp duplexes on a hexagonal
+    lattice of spacing 2.6, 30 C, 0.2 M (Debye-Hueckel range 4.6, ~380
+    neighbours per nucleotide), Langevin, 200 steps from the ideal helices;
+    the forces of the reached state against the oracle once in the mixed
+    form (bars as above) and once all-FP64 (1e-9).
+  - Both: after a rebuild at the final state, the Verlet pair set equals
+    the FP64 predicate's bit for bit.
+
+The errors are printed (pytest -s shows them).
+"""
+import numpy as np
+import pytest
+
+from _common import vec_rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from oxdna_amd import _lib, backend
+    assert _lib.device_count() >= 1, "no GPU visible"
+    return backend
+
+
+def _check_state(po, P, s, b, tol, label):
+    st = b.state()
+    t = s.copy()
+    t.pos, t.a1, t.a3 = st["pos"], st["a1"], st["a3"]
+    F, T, Ep = b.forces()
+    ref = po.forces(P, t)
+    fb, fp = vec_rel_err(F, ref["F"])
+    tb, tp = vec_rel_err(T, ref["T"])
+    dU = abs(Ep.sum() - ref["U"]) / s.N
+    print(f"{label}: F {fb:.2e} (per particle {fp:.2e}), T {tb:.2e} ({tp:.2e}), |dU|/N {dU:.1e}", flush=True)
+    assert fb <= tol and fp <= tol and tb <= tol and tp <= tol, label
+    assert dU <= 1e-6, label
+    return t, max(fb, fp, tb, tp)
+
+
+def _pairs_exact(po, P, b, t, skin=0.05):
+    b.update_lists(True)
+    got = b.pairs()
+    rv = P.rcut + 2 * skin
+    want = po.verlet_pairs(t.pos, t.n3, t.n5, t.box, rv * rv, max_pairs=max(1024, len(got) + 1024))
+    print(f"pair set: {len(want)} pairs, mean {2 * len(want) / t.N:.1f} neighbours", flush=True)
+    assert got.shape == want.shape and np.array_equal(got, want)
+
+
+def test_24576nt_forces_every_1000_steps_over_1e5(hip):
+    from oracle import pyoracle as po
+    from oxdna_amd import systems
+    s = systems.duplex_gas(1, n_duplex=512, bp=24, density=1e-3, tc=25.0)
+    P = po.Params(T=s.T, salt=s.salt)
+    worst = 0.0
+    with hip.HIPBatch(s, thermostat="bussi", newtonian_steps=10, bussi_tau=100, dt=0.003, verlet_skin=0.05,
+                      refresh_vel=True, seed=1) as b:
+        b.set_sort_interval(500)
+        t, e = _check_state(po, P, s, b, 1e-5, "step 0")
+        for k in range(1, 101):
+            b.run(1000)
+            t, e = _check_state(po, P, s, b, 1e-5, f"step {1000 * k}")
+            worst = max(worst, e)
+        nreb, err = b.info()
+        assert err == 0
+        print(f"worst relative error over 101 checks {worst:.2e}; {nreb} list builds", flush=True)
+        _pairs_exact(po, P, b, t)
+
+
+@pytest.mark.parametrize("precision,tol", [("mixed", 1e-5), ("double", 1e-9)])
+def test_196608nt_forces_mixed_and_fp64(hip, precision, tol):
+    from oracle import pyoracle as po
+    from oxdna_amd import systems
+    s = systems.duplex_bundle(1, nx=32, ny=32, bp=96, spacing=2.6, lz=40.0, tc=30.0, salt=0.2)
+    assert s.N == 196608
+    P = po.Params(T=s.T, salt=s.salt)
+    with hip.HIPBatch(s, thermostat="langevin", diff_coeff=2.5, dt=0.003, verlet_skin=0.05, refresh_vel=True, seed=1,
+                      backend_precision=precision) as b:
+        b.run(200)
+        t, _ = _check_state(po, P, s, b, tol, f"196,608 nt {precision}, step 200")
+        if precision == "mixed":
+            _pairs_exact(po, P, b, t)
