@@ -58,7 +58,13 @@ def test_external_forces_vs_reference(hip, tmp_path, name, precision):
     print(f"{name} {precision}: F {big:.2e}/{pp:.2e}")
     assert big <= tol and pp <= tol
     assert vec_rel_err(T, z["Tbody"])[0] <= tol
-    assert np.array_equal(T, T0)
+    # no torque from the external forces: the interaction's torques (a small
+    # mixed-form batch leaves the workgroup engine when forces are set, and
+    # the grid engine's lanes per nucleotide sum in another order)
+    if precision == "double":
+        assert np.array_equal(T, T0)
+    else:
+        assert vec_rel_err(T, T0)[0] <= 1e-6
     scale = np.abs(z["E8"]).sum()
     assert np.all(np.abs(e8 - z["E8"]) <= tol * scale)
     # without the external forces the forces are the interaction's alone

@@ -6,8 +6,8 @@ state are ADDED to what the caller's F and T hold, as after
 MD_CUDABackend::_set_external_forces (src/CUDA/Backends/
 MD_CUDABackend.cu:518-533).
 
-Checked through the C ABI with torch device buffers as the caller's
-memory: the additions equal pre-set values + the batch's own forces bit for
+Checked through the C ABI with device buffers from the HIP runtime
+(tests/_hipmem.py) as the caller's memory: the additions equal pre-set values + the batch's own forces bit for
 bit (FP32 additions in the mixed form, FP64 in the double form), the
 forces match the FP64 oracle at the force tests' bars, the energy shares
 in .w sum to the potential energy, a state moved by more than the Verlet
@@ -18,6 +18,7 @@ import numpy as np
 import pytest
 
 from _common import load_golden, oracle_params, vec_rel_err
+from _hipmem import DeviceArray, Stream
 
 pytestmark = pytest.mark.gpu
 
@@ -29,38 +30,33 @@ def hip():
     return backend
 
 
-def _dev(torch, a, dtype):
-    return torch.from_numpy(np.ascontiguousarray(a)).to(device="cuda:0", dtype=dtype).contiguous()
-
-
-def _state4(torch, hip, s):
+def _state4(hip, s):
     pos4 = np.zeros((s.N, 4))
     pos4[:, :3] = s.pos
-    return _dev(torch, pos4, torch.float64), _dev(torch, hip.quat_from_frame(s.a1, s.a3), torch.float64)
+    return DeviceArray(pos4), DeviceArray(hip.quat_from_frame(s.a1, s.a3))
 
 
 @pytest.mark.parametrize("precision", ["mixed", "double"])
 @pytest.mark.parametrize("name", ["hairpin_md_50C", "dense_md_excl_30C", "bundle_md_30C"])
 def test_accumulates_onto_preset_forces(hip, name, precision):
-    import torch
     from oracle import pyoracle as po
     s, opts, ref = load_golden(name)
-    ftype = torch.float32 if precision == "mixed" else torch.float64
+    ftype = np.float32 if precision == "mixed" else np.float64
     tol = 1e-5 if precision == "mixed" else 1e-9
     rng = np.random.default_rng(5)
     ext_F = rng.normal(size=(s.N, 4))
     ext_T = rng.normal(size=(s.N, 4))
     with hip.HIPBatch(s, backend_precision=precision, **opts) as b:
         F0, T0, E0 = b.forces()
-        d_pos, d_quat = _state4(torch, hip, s)
-        d_F = _dev(torch, ext_F, ftype)
-        d_T = _dev(torch, ext_T, ftype)
-        b.interaction_forces(d_pos.data_ptr(), d_quat.data_ptr(), d_F.data_ptr(), d_T.data_ptr())
+        d_pos, d_quat = _state4(hip, s)
+        d_F = DeviceArray(ext_F.astype(ftype))
+        d_T = DeviceArray(ext_T.astype(ftype))
+        b.interaction_forces(d_pos.ptr, d_quat.ptr, d_F.ptr, d_T.ptr)
         b.sync()
         F1, T1, E1 = b.forces()
         U = b.energies()[0, 0]
-    got_F = d_F.cpu().numpy()
-    got_T = d_T.cpu().numpy()
+    got_F = d_F.numpy()
+    got_T = d_T.numpy()
     npt = np.float32 if precision == "mixed" else np.float64
     # the batch's forces of the same state, added in the working precision
     want_F = ext_F.astype(npt) + np.concatenate([F1, E1[:, None]], axis=1).astype(npt)
@@ -85,12 +81,11 @@ def test_moved_state_rebuilds_lists_on_callers_stream(hip):
     """Moving every particle by more than the skin between two calls
     rebuilds the lists: the forces of the moved state match the oracle.
     Work goes on a caller-created stream (default priority)."""
-    import torch
     from oracle import pyoracle as po
     s, opts, ref = load_golden("dense_md_excl_30C")
     P = oracle_params(s, opts)
     rng = np.random.default_rng(11)
-    stream = torch.cuda.Stream(device="cuda:0")
+    stream = Stream()
     with hip.HIPBatch(s, **opts) as b:
         reb0, _ = b.info()
         t = s.copy()
@@ -98,20 +93,19 @@ def test_moved_state_rebuilds_lists_on_callers_stream(hip):
             # a rigid shift of 0.3 (more than the skin) plus thermal-size noise
             t.pos = s.pos + 0.3 * (it + 1) + 0.02 * rng.normal(size=s.pos.shape)
             want = po.forces(P, t)
-            with torch.cuda.stream(stream):
-                d_pos, d_quat = _state4(torch, hip, t)
-                d_F = torch.zeros((s.N, 4), dtype=torch.float32, device="cuda:0")
-                d_T = torch.zeros((s.N, 4), dtype=torch.float32, device="cuda:0")
-            b.interaction_forces(d_pos.data_ptr(), d_quat.data_ptr(), d_F.data_ptr(), d_T.data_ptr(),
-                                 stream.cuda_stream)
+            d_pos, d_quat = _state4(hip, t)
+            d_F = DeviceArray.zeros((s.N, 4), np.float32)
+            d_T = DeviceArray.zeros((s.N, 4), np.float32)
+            b.interaction_forces(d_pos.ptr, d_quat.ptr, d_F.ptr, d_T.ptr, stream.handle)
             stream.synchronize()
-            F = d_F.cpu().numpy()[:, :3].astype(np.float64)
-            T = d_T.cpu().numpy()[:, :3].astype(np.float64)
+            F = d_F.numpy()[:, :3].astype(np.float64)
+            T = d_T.numpy()[:, :3].astype(np.float64)
             big_f, pp_f = vec_rel_err(F, want["F"])
             big_t, pp_t = vec_rel_err(T, want["T"])
             print(f"move {it}: F {big_f:.2e}/{pp_f:.2e} T {big_t:.2e}/{pp_t:.2e}")
             assert big_f <= 1e-5 and big_t <= 1e-5 and pp_f <= 1e-5 and pp_t <= 1e-5
         reb1, _ = b.info()
+    stream.close()
     assert reb1 - reb0 == 3
 
 
