@@ -152,6 +152,30 @@ oxh_status oxh_batch_compute_forces(oxh_batch *b);
 oxh_status oxh_interaction_compute_forces(oxh_batch *b, const double *d_pos, const double *d_quat, void *d_forces, void *d_torques,
 		void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the same additions as oxh_interaction_compute_forces, for ONE
+ * system described by desc (topology from its btype / n3 / n5 host arrays,
+ * parameters and backend_precision from its keys; its pos, a1, a3, vel and
+ * L are not read), with the state in d_pos / d_quat and the results added
+ * onto d_forces / d_torques (layouts as above, particle k at index k).
+ * Nothing is allocated: the Verlet list (built afresh on every call, all
+ * pairs), the FP32 mirror, F, T and the parameters live in a fixed device
+ * array of the library, OXH_NOALLOC_SCRATCH_BYTES on each GPU it runs on,
+ * held until the process ends.  Limits: at most OXH_NOALLOC_MAX_N
+ * nucleotides (larger N is rejected with OXH_ERR_ARG before any GPU work)
+ * and OXH_NOALLOC_MAX_NEIGH Verlet neighbours per nucleotide (more returns
+ * OXH_ERR_SYSTEM and adds nothing).  The call waits for the list and force
+ * kernels, then queues the additions on `stream` (NULL: the null stream)
+ * and returns.  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (other threads, other streams, a call issued before
+ * the previous call's additions have run) race; serialise them. */
+#define OXH_NOALLOC_MAX_N 4096
+#define OXH_NOALLOC_MAX_NEIGH 256
+#define OXH_NOALLOC_SCRATCH_BYTES 5425408
+oxh_status oxh_interaction_compute_forces_noalloc(int device, const oxh_system_desc *desc, const double *d_pos, const double *d_quat,
+		void *d_forces, void *d_torques, void *stream);
+
 /* Advances every system by n_steps MD steps: first half kick + drift +
  * rotation, skin-triggered list rebuild, forces, second half kick,
  * thermostat.  Replaces the loop of MD_CUDABackend::sim_step
@@ -224,7 +248,7 @@ oxh_status oxh_batch_get_order(oxh_batch *b, int sys, int *order);
 
 /* ---- external forces (the reference's external_forces = true with an
  * external_forces_file; ForceFactory::add_force, src/Forces/
- * ForceFactory.cpp:66-110, the CPU classes in src/Forces/*.cpp and the
+ * ForceFactory.cpp:66-110, the CPU force classes under src/Forces/ and the
  * CUDA_trap family set by MD_CUDABackend::_set_external_forces,
  * src/CUDA/Backends/MD_CUDABackend.cu:528-533, CUDA_MD.cuh:97-554).
  * Types, by the reference's `type` key: */

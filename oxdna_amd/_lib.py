@@ -25,7 +25,7 @@ EXPORTS = [
     "oxh_pool_create", "oxh_pool_admit", "oxh_pool_release", "oxh_pool_status", "oxh_copy_bench", "oxh_batch_stress",
     "oxh_batch_fix_diffusion", "oxh_batch_set_fix_diffusion", "oxh_batch_checkpoint_bytes", "oxh_batch_save",
     "oxh_batch_restore", "oxh_batch_get_order", "oxh_batch_set_external_forces", "oxh_batch_get_images",
-    "oxh_batch_set_images", "oxh_interaction_compute_forces",
+    "oxh_batch_set_images", "oxh_interaction_compute_forces", "oxh_interaction_compute_forces_noalloc",
 ]
 
 # external force types (include/oxh.h OXH_EXT_*), by the reference's `type` key
@@ -77,6 +77,7 @@ def lib():
         L.oxh_batch_update_lists.argtypes = [V, I]
         L.oxh_batch_compute_forces.argtypes = [V]
         L.oxh_interaction_compute_forces.argtypes = [V, P, P, P, P, P]
+        L.oxh_interaction_compute_forces_noalloc.argtypes = [I, C.POINTER(SystemDesc), P, P, P, P, P]
         L.oxh_batch_run.argtypes = [V, LL]
         L.oxh_batch_hilbert_sort.argtypes = [V]
         L.oxh_batch_sync.argtypes = [V]

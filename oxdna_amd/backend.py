@@ -387,3 +387,18 @@ def philox(ctr: np.ndarray, key: np.ndarray, device: int = 0) -> np.ndarray:
     out = np.zeros_like(ctr)
     check(lib().oxh_philox(device, len(ctr), ptr(ctr), ptr(key), ptr(out)), "oxh_philox")
     return out
+
+
+def interaction_forces_noalloc(system: System, d_pos: int, d_quat: int, d_forces: int, d_torques: int, stream: int = 0,
+                               device: int = 0, **opts):
+    """oxh_interaction_compute_forces_noalloc: adds the pair forces and
+    torques of one system's state (caller device buffers, layouts as in
+    HIPBatch.interaction_forces) to the caller's device buffers, with no
+    device allocation (the library's fixed scratch array; at most
+    OXH_NOALLOC_MAX_N = 4096 nucleotides).  Topology and parameters come
+    from `system` and the options, as for HIPBatch; stream 0 is the null
+    stream."""
+    keep: list = []
+    d = make_desc(system, keep, **opts)
+    check(lib().oxh_interaction_compute_forces_noalloc(device, C.byref(d), d_pos, d_quat, d_forces, d_torques, stream or None),
+          "oxh_interaction_compute_forces_noalloc")

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -450,9 +451,11 @@ oxh_status rebuild_from_rlist(oxh_batch *B, int only = -1) {
 
 // Per-system parameters of desc d (validation, T-dependent model constants,
 // box, Verlet radii, integrator and thermostat); first / n / cells are the
-// caller's.  Returns r_verlet in rv.
-oxh_status sys_setup(oxh_batch *B, const oxh_system_desc &d, int s, oxh_sys &S, double &rv) {
-	if(d.N <= 0 || !d.pos || !d.a1 || !d.a3 || !d.btype || !d.n3 || !d.n5)
+// caller's.  Returns r_verlet in rv.  state_arrays: the desc must carry the
+// state (pos, a1, a3) as well as the topology.  B (may be NULL) records
+// the Bussi step alignment of a batch.
+oxh_status sys_setup(oxh_batch *B, const oxh_system_desc &d, int s, oxh_sys &S, double &rv, bool state_arrays = true) {
+	if(d.N <= 0 || !d.btype || !d.n3 || !d.n5 || (state_arrays && (!d.pos || !d.a1 || !d.a3)))
 		return fail(OXH_ERR_ARG, "system %d: missing arrays or N <= 0", s);
 	for(int k = 0; k < 3; k++)
 		if(!(d.box[k] > 0)) return fail(OXH_ERR_ARG, "system %d: box side %d must be > 0", s, k);
@@ -529,9 +532,11 @@ oxh_status sys_setup(oxh_batch *B, const oxh_system_desc &d, int s, oxh_sys &S, 
 	else if(d.thermostat == OXH_THERMOSTAT_BUSSI) {
 		if(d.bussi_tau <= 0) return fail(OXH_ERR_ARG, "system %d: bussi_tau must be > 0", s);
 		S.bussi_c = std::exp(-S.newtonian_steps / (double) d.bussi_tau);
-		B->any_bussi = true;
-		if(B->bussi_ns == 0) B->bussi_ns = S.newtonian_steps;
-		else if(B->bussi_ns != S.newtonian_steps) B->bussi_aligned = false;
+		if(B) {
+			B->any_bussi = true;
+			if(B->bussi_ns == 0) B->bussi_ns = S.newtonian_steps;
+			else if(B->bussi_ns != S.newtonian_steps) B->bussi_aligned = false;
+		}
 	}
 	else if(d.thermostat != OXH_THERMOSTAT_NO) return fail(OXH_ERR_ARG, "system %d: unknown thermostat %d", s, d.thermostat);
 	return OXH_OK;
@@ -2317,5 +2322,174 @@ extern "C" oxh_status oxh_copy_bench(int device, long long bytes, int reps, doub
 	(void) hipEventDestroy(e1);
 	(void) hipFree(a);
 	(void) hipFree(b);
+	return OXH_OK;
+}
+
+// ------------------------------------------------- forces without allocation
+// oxh_interaction_compute_forces_noalloc: one system's pair forces with every
+// buffer beyond the caller's inputs and outputs in `scratch`, a fixed device
+// array of this library (one copy per GPU, held until the process ends).
+// Its regions, each 256-byte aligned: the system's parameters and state
+// word, the rebuild flag, the topology staging, the FP64 state and list
+// positions, the FP32 mirror, F / T in both precisions, bonds, particle
+// info, original indices and the Verlet list (OXH_NOALLOC_MAX_NEIGH
+// entries per particle).  Host code takes its address per device from
+// hipGetSymbolAddress and never uses the array's name as a device address.
+namespace {
+
+constexpr int NA_N = OXH_NOALLOC_MAX_N;
+constexpr int NA_NEIGH = OXH_NOALLOC_MAX_NEIGH;
+
+constexpr size_t na_up(size_t x) { return (x + 255) & ~(size_t) 255; }
+
+struct NaLayout {
+	static constexpr size_t sys = 0;
+	static constexpr size_t state = na_up(sys + sizeof(oxh_sys));
+	static constexpr size_t any = na_up(state + sizeof(oxh_sys_state));
+	static constexpr size_t topo = na_up(any + sizeof(int));
+	static constexpr size_t pos = na_up(topo + NA_N * sizeof(int4));
+	static constexpr size_t q64 = pos + NA_N * sizeof(double4);
+	static constexpr size_t rlist = q64 + NA_N * sizeof(double4);
+	static constexpr size_t back64 = rlist + NA_N * sizeof(double4);
+	static constexpr size_t F64 = back64 + NA_N * sizeof(double4);
+	static constexpr size_t T64 = F64 + NA_N * sizeof(double4);
+	static constexpr size_t q32 = T64 + NA_N * sizeof(double4);
+	static constexpr size_t bpos = q32 + NA_N * sizeof(float4);
+	static constexpr size_t F = bpos + NA_N * sizeof(float4);
+	static constexpr size_t T = F + NA_N * sizeof(float4);
+	static constexpr size_t bonds = T + NA_N * sizeof(float4);
+	static constexpr size_t pinfo = bonds + NA_N * sizeof(int2);
+	static constexpr size_t nnbr = pinfo + NA_N * sizeof(int2);
+	static constexpr size_t orig = nnbr + NA_N * sizeof(int2);
+	static constexpr size_t nbr = na_up(orig + NA_N * sizeof(int));
+	static constexpr size_t end = na_up(nbr + (size_t) NA_N * NA_NEIGH * sizeof(int));
+};
+static_assert(NaLayout::end == OXH_NOALLOC_SCRATCH_BYTES, "scratch size in include/oxh.h");
+
+static __device__ __align__(256) unsigned char scratch[NaLayout::end];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_na_mutex;
+unsigned char *g_na_ptr[NA_MAX_DEVICES] = {};
+// host staging of the topology (n3, n5, btype) and of the state word read back
+int4 g_na_topo[NA_N];
+oxh_sys_state g_na_state;
+
+// the device address of `scratch` on `device` (current), looked up once
+oxh_status na_scratch(int device, unsigned char **out) {
+	if(!g_na_ptr[device]) {
+		void *p = nullptr;
+		hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if(e != hipSuccess || !p) return fail(OXH_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", device, hipGetErrorString(e));
+		g_na_ptr[device] = (unsigned char *) p;
+	}
+	*out = g_na_ptr[device];
+	return OXH_OK;
+}
+
+// parameters, state word and rebuild flag; the caller's state and the
+// staged topology into the scratch arrays, with the FP32 mirror
+__global__ void k_na_load(int N, const oxh_sys S, const int4 *__restrict__ topo, const double4 *__restrict__ upos,
+		const double4 *__restrict__ uq, oxh_sys *__restrict__ d_sys, oxh_sys_state *__restrict__ st, int *__restrict__ any,
+		double4 *__restrict__ pos, double4 *__restrict__ q64, float4 *__restrict__ q32, float4 *__restrict__ bpos,
+		double4 *__restrict__ back64, int2 *__restrict__ bonds, int2 *__restrict__ pinfo, int *__restrict__ orig) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i == 0) {
+		*d_sys = S;
+		st->K_t = st->K_r = 0.0;
+		st->need_rebuild = 1;
+		st->error = 0;
+		st->n_rebuilds = 0;
+		*any = 1;
+	}
+	if(i >= N) return;
+	const int4 t = topo[i];
+	const int2 bd = make_int2(t.x, t.y);
+	bonds[i] = bd;
+	pinfo[i] = make_int2(t.z, 0);
+	orig[i] = i;
+	double4 r = upos[i];
+	r.w = 0.0;
+	const double4 q = uq[i];
+	pos[i] = r;
+	q64[i] = q;
+	oxh::write_mirror(i, r, q, S, bd, q32, bpos, back64);
+}
+
+}  // namespace
+
+extern "C" oxh_status oxh_interaction_compute_forces_noalloc(int device, const oxh_system_desc *desc, const double *d_pos,
+		const double *d_quat, void *d_forces, void *d_torques, void *stream) {
+	if(!desc || !d_pos || !d_quat || !d_forces || !d_torques)
+		return fail(OXH_ERR_ARG, "oxh_interaction_compute_forces_noalloc: bad arguments");
+	if(device < 0 || device >= NA_MAX_DEVICES) return fail(OXH_ERR_ARG, "oxh_interaction_compute_forces_noalloc: bad device %d", device);
+	const int N = desc->N;
+	if(N > NA_N)
+		return fail(OXH_ERR_ARG, "oxh_interaction_compute_forces_noalloc: %d nucleotides, at most %d (OXH_NOALLOC_MAX_N)", N, NA_N);
+	if(desc->backend_precision != 0 && desc->backend_precision != 1)
+		return fail(OXH_ERR_ARG, "oxh_interaction_compute_forces_noalloc: backend_precision must be 0 or 1");
+	oxh_sys S;
+	double rv = 0;
+	TRY(sys_setup(nullptr, *desc, 0, S, rv, false));
+	S.first = 0;
+	S.n = N;
+	S.use_cells = 0;   // all-pairs list build
+	S.cell_first = S.ncell = S.stencil = 0;
+	S.has_ext = 0;
+	std::lock_guard<std::mutex> lock(g_na_mutex);
+	HIPCHK(hipSetDevice(device));
+	unsigned char *base = nullptr;
+	TRY(na_scratch(device, &base));
+	hipStream_t st = (hipStream_t) stream;
+	for(int i = 0; i < N; i++) g_na_topo[i] = make_int4(desc->n3[i], desc->n5[i], desc->btype[i], 0);
+	HIPCHK(hipMemcpyAsync(base + NaLayout::topo, g_na_topo, (size_t) N * sizeof(int4), hipMemcpyHostToDevice, st));
+
+	oxh::Batch b;
+	b.device = device;
+	b.stream = st;
+	b.N = N;
+	b.nsys = 1;
+	b.d_sys = (oxh_sys *) (base + NaLayout::sys);
+	b.d_state = (oxh_sys_state *) (base + NaLayout::state);
+	b.any_rebuild = (int *) (base + NaLayout::any);
+	b.pos = (double4 *) (base + NaLayout::pos);
+	b.q64 = (double4 *) (base + NaLayout::q64);
+	b.rlist = (double4 *) (base + NaLayout::rlist);
+	b.back64 = (double4 *) (base + NaLayout::back64);
+	b.F64 = (double4 *) (base + NaLayout::F64);
+	b.T64 = (double4 *) (base + NaLayout::T64);
+	b.q32 = (float4 *) (base + NaLayout::q32);
+	b.bpos = (float4 *) (base + NaLayout::bpos);
+	b.F = (float4 *) (base + NaLayout::F);
+	b.T = (float4 *) (base + NaLayout::T);
+	b.bonds = (int2 *) (base + NaLayout::bonds);
+	b.pinfo = (int2 *) (base + NaLayout::pinfo);
+	b.nnbr = (int2 *) (base + NaLayout::nnbr);
+	b.orig = (int *) (base + NaLayout::orig);
+	b.nbr = (int *) (base + NaLayout::nbr);
+	b.nstride = N;
+	b.max_neigh = NA_NEIGH;
+	b.fp64 = desc->backend_precision == 1;
+	b.lanes = 8;
+	b.far_lanes = 0;
+	b.force_block = 256;
+	b.pack = false;
+
+	const int bs = 256, g = (N + bs - 1) / bs;
+	k_na_load<<<g, bs, 0, st>>>(N, S, (const int4 *) (base + NaLayout::topo), (const double4 *) d_pos, (const double4 *) d_quat,
+			b.d_sys, b.d_state, b.any_rebuild, b.pos, b.q64, b.q32, b.bpos, b.back64, b.bonds, b.pinfo, b.orig);
+	oxh::launch_build_list(st, b, 1, b.pos);
+	oxh::launch_forces(st, b, false);
+	HIPCHK(hipMemcpyAsync(&g_na_state, b.d_state, sizeof(oxh_sys_state), hipMemcpyDeviceToHost, st));
+	HIPCHK(hipStreamSynchronize(st));
+	HIPCHK(hipGetLastError());
+	if(g_na_state.error & oxh::ERR_LIST)
+		return fail(OXH_ERR_SYSTEM, "oxh_interaction_compute_forces_noalloc: a particle has more than %d neighbours (OXH_NOALLOC_MAX_NEIGH)",
+				NA_NEIGH);
+	if(b.fp64)
+		k_add_to_caller<double4><<<g, bs, 0, st>>>(N, b.F64, b.T64, b.orig, b.pinfo, b.d_sys, (double4 *) d_forces, (double4 *) d_torques);
+	else
+		k_add_to_caller<float4><<<g, bs, 0, st>>>(N, b.F, b.T, b.orig, b.pinfo, b.d_sys, (float4 *) d_forces, (float4 *) d_torques);
+	HIPCHK(hipGetLastError());
 	return OXH_OK;
 }
