@@ -231,6 +231,17 @@ __global__ void k_clear_need(int nsys, oxh_sys_state *st, int set_value, int *an
 	if(s == 0) *any = set_value;
 }
 
+// counts a list build for every system whose rebuild flag is set, then
+// clears the flags
+__global__ void k_count_rebuilds(int nsys, oxh_sys_state *st, int *any) {
+	int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if(s < nsys && st[s].need_rebuild) {
+		st[s].need_rebuild = 0;
+		st[s].n_rebuilds += 1;
+	}
+	if(s == 0) *any = 0;
+}
+
 oxh_status sync_and_check(oxh_batch *B) {
 	HIPCHK(hipStreamSynchronize(B->b.stream));
 	HIPCHK(hipGetLastError());
@@ -1238,6 +1249,7 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 	oxh::launch_prepare_mirror(st, b);
 	oxh::launch_slot_of(st, b);
 	// the workgroup engine takes systems of at most one particle per thread
+	if(const char *x = getenv("OXH_XCD")) b.xcd = atoi(x) != 0;
 	if(const char *p = getenv("OXH_PERSIST_BLOCK")) B->persist_block = atoi(p) == 256 ? 256 : 512;
 	B->persist = true;
 	for(int s = 0; s < n_sys; s++) {
@@ -1247,6 +1259,14 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 	}
 	if(const char *p = getenv("OXH_PERSIST")) B->persist &= atoi(p) != 0;
 	B->persist &= !b.fp64;   // the workgroup engine computes in the mixed form
+	// systems (slots) small enough are held in LDS by the workgroup engine
+	if(B->persist) {
+		int nmax = 0;
+		for(int s = 0; s < n_sys; s++) nmax = std::max(nmax, B->pool ? B->cap[s] : b.h_sys[s].n);
+		if(nmax <= oxh::PERSIST_LDS_MAX_N) b.persist_lds_n = nmax;
+		if(const char *p = getenv("OXH_PERSIST_LDS"))
+			if(atoi(p) == 0) b.persist_lds_n = 0;
+	}
 	// half-list form of the step's close pairs: grid-engine batches in the
 	// mixed form, timed against the full list at upload (OXH_EDGE=0 / 1
 	// forbids / forces it)
@@ -1395,13 +1415,16 @@ oxh_status oxh_interaction_compute_forces(oxh_batch *B, const double *d_pos, con
 		oxh::launch_build_list(st, b, 0, b.pos);
 		if(B->persist) oxh::launch_persist_forces(st, b, false, B->persist_block);
 		else oxh::launch_forces(st, b, false);
-		k_clear_need<<<(b.nsys + 63) / 64, 64, 0, st>>>(b.nsys, b.d_state, 0, b.any_rebuild);
 		HIPCHK(hipMemcpyAsync(sst.data(), b.d_state, sizeof(oxh_sys_state) * b.nsys, hipMemcpyDeviceToHost, st));
 		HIPCHK(hipStreamSynchronize(st));
 		HIPCHK(hipGetLastError());
 		bool overflow = false;
 		for(auto &x : sst) overflow |= (x.error & oxh::ERR_OVERFLOW) != 0;
-		if(!overflow) break;
+		if(!overflow) {
+			// the systems rebuilt count one list build each, as in a step
+			k_count_rebuilds<<<(b.nsys + 63) / 64, 64, 0, st>>>(b.nsys, b.d_state, b.any_rebuild);
+			break;
+		}
 		TRY(grow_lists(B, sst));
 		HIPCHK(hipStreamSynchronize(b.stream));
 		k_clear_need<<<(b.nsys + 63) / 64, 64, 0, st>>>(b.nsys, b.d_state, 1, b.any_rebuild);

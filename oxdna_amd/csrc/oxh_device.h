@@ -168,6 +168,21 @@ __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, double &n0, d
 	n1 = r * s;
 }
 
+// Workgroups of a launch go to the chip's 8 XCDs in turn (workgroup b runs
+// on XCD b % 8), and each XCD has its own L2.  A launch whose workgroups
+// gather their neighbours' data takes its grid as xcd_grid(n) and its
+// workgroup index as xcd_block(n): XCD x then runs the contiguous logical
+// blocks [x ceil(n / 8), (x + 1) ceil(n / 8)), so particles that are close
+// in memory order (strand by strand, or Hilbert-sorted) and in space are
+// gathered through one L2.  -1: a padding workgroup of the rounded-up grid.
+constexpr unsigned N_XCD = 8;
+__host__ __device__ __forceinline__ unsigned xcd_grid(unsigned n) { return (n + N_XCD - 1) / N_XCD * N_XCD; }
+__device__ __forceinline__ int xcd_block(unsigned n) {
+	const unsigned per = (n + N_XCD - 1) / N_XCD;
+	const unsigned lb = (blockIdx.x % N_XCD) * per + blockIdx.x / N_XCD;
+	return lb < n ? (int) lb : -1;
+}
+
 // particle i is padding of a slot: beyond its system's current size
 __device__ __forceinline__ bool padding(int i, const oxh_sys &S) { return i - S.first >= S.n; }
 

@@ -36,8 +36,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 		const float4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
 		const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh, const oxh_sys *__restrict__ systems,
 		oxh_sys_state *__restrict__ sstate, float4 *__restrict__ Fout, float4 *__restrict__ Tout, double *__restrict__ esplit,
-		int *__restrict__ hbcount, int k, const int *__restrict__ limit, int *__restrict__ any, double *__restrict__ stress) {
-	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+		int *__restrict__ hbcount, int k, const int *__restrict__ limit, int *__restrict__ any, double *__restrict__ stress, int xcd) {
+	const int blk = xcd ? xcd_block(((unsigned) N * K + blockDim.x - 1) / blockDim.x) : (int) blockIdx.x;
+	if(blk < 0) return;   // whole padding workgroup
+	const int t = blk * blockDim.x + threadIdx.x;
 	const int i = t / K;
 	const int l = t % K;
 	if(k >= 0 && t == 0) *any = 0;   // inside a step: this step's list builds are done
@@ -245,8 +247,10 @@ template<int K, bool SPLIT, bool SMALL>
 __global__ void __launch_bounds__(256) k_forces_far(int N, const double4 *__restrict__ pos, const float4 *__restrict__ q32,
 		const float4 *__restrict__ bpos, const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh,
 		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, float4 *__restrict__ Fio, float4 *__restrict__ Tio,
-		double *__restrict__ esplit, int k_step, const int *__restrict__ limit, double *__restrict__ stress) {
-	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+		double *__restrict__ esplit, int k_step, const int *__restrict__ limit, double *__restrict__ stress, int xcd) {
+	const int blk = xcd ? xcd_block(((unsigned) N * K + blockDim.x - 1) / blockDim.x) : (int) blockIdx.x;
+	if(blk < 0) return;
+	const int t = blk * blockDim.x + threadIdx.x;
 	const int i = t / K;
 	const int l = t % K;
 	if(i >= N) return;
@@ -329,8 +333,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
 		const float4 *__restrict__ q32, const float4 *__restrict__ bpos, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
 		const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh, const oxh_sys *__restrict__ systems,
 		oxh_sys_state *__restrict__ sstate, double4 *__restrict__ Fout, double4 *__restrict__ Tout, double *__restrict__ esplit,
-		int *__restrict__ hbcount, int k, const int *__restrict__ limit, int *__restrict__ any, double *__restrict__ stress) {
-	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+		int *__restrict__ hbcount, int k, const int *__restrict__ limit, int *__restrict__ any, double *__restrict__ stress, int xcd) {
+	const int blk = xcd ? xcd_block(((unsigned) N * K + blockDim.x - 1) / blockDim.x) : (int) blockIdx.x;
+	if(blk < 0) return;
+	const int t = blk * blockDim.x + threadIdx.x;
 	const int i = t / K;
 	const int l = t % K;
 	if(k >= 0 && t == 0) *any = 0;
@@ -355,8 +361,10 @@ template<int K, bool SPLIT>
 __global__ void __launch_bounds__(256) k_forces_far_dp(int N, const double4 *__restrict__ pos, const double4 *__restrict__ q64,
 		const double4 *__restrict__ back64, const float4 *__restrict__ bpos, const int *__restrict__ nbr, const int2 *__restrict__ nnbr, int max_neigh,
 		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, double4 *__restrict__ Fio, double4 *__restrict__ Tio,
-		double *__restrict__ esplit, int k_step, const int *__restrict__ limit, double *__restrict__ stress) {
-	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+		double *__restrict__ esplit, int k_step, const int *__restrict__ limit, double *__restrict__ stress, int xcd) {
+	const int blk = xcd ? xcd_block(((unsigned) N * K + blockDim.x - 1) / blockDim.x) : (int) blockIdx.x;
+	if(blk < 0) return;
+	const int t = blk * blockDim.x + threadIdx.x;
 	const int i = t / K;
 	const int l = t % K;
 	if(i >= N) return;
@@ -411,28 +419,30 @@ template<int K, bool FAR>
 static void launch_dp_close(hipStream_t st, const Batch &b, bool split, int k) {
 	const int block = 64;
 	size_t threads = (size_t) b.N * K;
-	dim3 grid((unsigned) ((threads + block - 1) / block));
+	const unsigned nblk = (unsigned) ((threads + block - 1) / block);
+	dim3 grid(b.xcd ? xcd_grid(nblk) : nblk);
 	const int *lim = k >= 0 ? b.limit : nullptr;
 	if(split)
 		k_forces_dp<K, true, FAR><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-				b.d_sys, b.d_state, b.F64, b.T64, b.esplit, b.hbcount, k, lim, b.any_rebuild, b.stress);
+				b.d_sys, b.d_state, b.F64, b.T64, b.esplit, b.hbcount, k, lim, b.any_rebuild, b.stress, b.xcd);
 	else
 		k_forces_dp<K, false, FAR><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr, b.max_neigh,
-				b.d_sys, b.d_state, b.F64, b.T64, nullptr, b.want_hbcount ? b.hbcount : nullptr, k, lim, b.any_rebuild, nullptr);
+				b.d_sys, b.d_state, b.F64, b.T64, nullptr, b.want_hbcount ? b.hbcount : nullptr, k, lim, b.any_rebuild, nullptr, b.xcd);
 }
 
 template<int K>
 static void launch_far_dp(hipStream_t st, const Batch &b, bool split, int k) {
 	const int block = 256;
 	size_t threads = (size_t) b.N * K;
-	dim3 grid((unsigned) ((threads + block - 1) / block));
+	const unsigned nblk = (unsigned) ((threads + block - 1) / block);
+	dim3 grid(b.xcd ? xcd_grid(nblk) : nblk);
 	const int *lim = k >= 0 ? b.limit : nullptr;
 	if(split)
 		k_forces_far_dp<K, true><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.back64, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F64,
-				b.T64, b.esplit, k, lim, b.stress);
+				b.T64, b.esplit, k, lim, b.stress, b.xcd);
 	else
 		k_forces_far_dp<K, false><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.back64, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F64,
-				b.T64, nullptr, k, lim, nullptr);
+				b.T64, nullptr, k, lim, nullptr, b.xcd);
 }
 
 // all-FP64 forces: close (+ bonded) kernel with b.lanes, far pairs inline
@@ -455,23 +465,24 @@ static void launch_dp(hipStream_t st, const Batch &b, bool split, int k) {
 template<int K, bool PACK>
 static void launch_kp(hipStream_t st, const Batch &b, bool split, int block, bool far_inline, int k) {
 	size_t threads = (size_t) b.N * K;
-	dim3 grid((unsigned) ((threads + block - 1) / block));
+	const unsigned nblk = (unsigned) ((threads + block - 1) / block);
+	dim3 grid(b.xcd ? xcd_grid(nblk) : nblk);
 	int *hbc = b.want_hbcount ? b.hbcount : nullptr;
 	if(split) {
 		if(far_inline)
 			k_forces<K, true, true, PACK><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr,
-					b.max_neigh, b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, b.stress);
+					b.max_neigh, b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, b.stress, b.xcd);
 		else
 			k_forces<K, true, false, PACK><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr,
-					b.max_neigh, b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, b.stress);
+					b.max_neigh, b.d_sys, b.d_state, b.F, b.T, b.esplit, b.hbcount, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, b.stress, b.xcd);
 	}
 	else {
 		if(far_inline)
 			k_forces<K, false, true, PACK><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr,
-					b.max_neigh, b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, nullptr);
+					b.max_neigh, b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, nullptr, b.xcd);
 		else
 			k_forces<K, false, false, PACK><<<grid, block, 0, st>>>(b.N, b.pos, b.q64, b.q32, b.bpos, b.bonds, b.pinfo, b.nbr, b.nnbr,
-					b.max_neigh, b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, nullptr);
+					b.max_neigh, b.d_sys, b.d_state, b.F, b.T, nullptr, hbc, k, k >= 0 ? b.limit : nullptr, b.any_rebuild, nullptr, b.xcd);
 	}
 }
 
@@ -485,13 +496,14 @@ static void launch_k(hipStream_t st, const Batch &b, bool split, int block, bool
 template<int K, bool SMALL>
 static void launch_far_k(hipStream_t st, const Batch &b, bool split, int block, int k) {
 	size_t threads = (size_t) b.N * K;
-	dim3 grid((unsigned) ((threads + block - 1) / block));
+	const unsigned nblk = (unsigned) ((threads + block - 1) / block);
+	dim3 grid(b.xcd ? xcd_grid(nblk) : nblk);
 	if(split)
 		k_forces_far<K, true, SMALL><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F,
-				b.T, b.esplit, k, k >= 0 ? b.limit : nullptr, b.stress);
+				b.T, b.esplit, k, k >= 0 ? b.limit : nullptr, b.stress, b.xcd);
 	else
 		k_forces_far<K, false, SMALL><<<grid, block, 0, st>>>(b.N, b.pos, b.q32, b.bpos, b.nbr, b.nnbr, b.max_neigh, b.pinfo, b.d_sys, b.F,
-				b.T, nullptr, k, k >= 0 ? b.limit : nullptr, nullptr);
+				b.T, nullptr, k, k >= 0 ? b.limit : nullptr, nullptr, b.xcd);
 }
 
 // slot pools take any system later, so they keep the small-box path

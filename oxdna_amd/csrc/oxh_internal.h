@@ -108,6 +108,12 @@ struct Batch {
 	size_t scratch_bytes = 0;
 
 	int force_block = 256;
+	// force kernels: workgroups mapped to XCDs in contiguous ranges
+	// (xcd_block; OXH_XCD=0 takes the hardware order)
+	int xcd = 1;
+	// workgroup engine with each system resident in LDS: the largest
+	// system's size (0 = the engine reads and writes HBM every step)
+	int persist_lds_n = 0;
 	bool pack = false;   // close pairs' HB/CRST/CXST through LDS work lists (k_forces PACK)
 	int any_cells = 0;
 	int thermostat_mask = 0;     // bit t set if some system uses thermostat t
@@ -140,6 +146,12 @@ __host__ __device__ inline int persist_lanes(int n, int block) {
 }
 
 void launch_persist(hipStream_t st, const Batch &b, int nsteps, int block);
+// dynamic LDS of the LDS-resident workgroup engine for systems of at most
+// nmax nucleotides
+size_t persist_lds_bytes(int nmax);
+// largest system the LDS-resident engine takes: its dynamic LDS plus the
+// engine's static LDS within 64 KiB per workgroup
+constexpr int PERSIST_LDS_MAX_N = 80;
 void launch_persist_forces(hipStream_t st, const Batch &b, bool split, int block);
 
 // k = index of the step inside the current chunk (checked against limit);
