@@ -9,9 +9,9 @@ rebuild, forces, second half kick, thermostat.
 Workloads:
   headline (default)  the plain 196,608-nt system below in the mixed form
            (FP32 pair terms, FP64 state): `value`, `roofline` and
-           `cpu_baseline`; the same system all-FP64 (`fp64`) and the served
-           stream against the plain path one job at a time (`served`) in
-           the same line
+           `cpu_baseline`; the same system all-FP64 (`fp64`), the batched
+           hairpins (`batched`) and the served stream against the plain
+           path one job at a time (`served`) in the same line
   plain2   1,024 random-sequence 96-bp duplexes (196,608 nt), hexagonal
            lattice of axis spacing 2.6, box 83.2 x 72.05 x 40, T = 30 C,
            0.2 M, Langevin (diff_coeff 2.5), skin 0.05, dt 0.003
@@ -261,7 +261,8 @@ def cpu_baselines(s, ref_keys, steps_lo: int, steps_hi: int, variants=("DNA2", "
             "binary": "oracle/_ref/oxDNA_release (reference CPU sources, -O3 -ffast-math -march=x86-64-v3)"}
 
 
-def served_measure(jobs_n: int, step_scale: float, rate_mult: float, budget_gb: int, seed: int) -> dict:
+def served_measure(jobs_n: int, step_scale: float, rate_mult: float, budget_gb: int, seed: int, small_cap: int = 256,
+                   quantum_small: int = 1024, more_rates=(8.0,)) -> dict:
     """Served stream: jobs of log-uniform size 40-20,000 nt (single
     strands, duplexes, hairpins, bundles), T 20-60 C, Langevin, with step
     counts log-uniform over [1e4, 1e6] x step_scale.  First the plain path
@@ -303,7 +304,7 @@ def served_measure(jobs_n: int, step_scale: float, rate_mult: float, budget_gb: 
     log(f"plain one at a time: {t_plain:.2f} s ({t_setup:.2f} s upload/setup), {plain:.3e} nt-steps/s; "
         f"stepping only {work / t_run:.3e}")
 
-    srv = serve.Server(budget_bytes=budget_gb << 30)
+    srv = serve.Server(budget_bytes=budget_gb << 30, small_cap=small_cap, quantum_small=quantum_small)
     hbm = srv.hbm_bytes()
     rate = rate_mult * len(jobs) / t_plain
     arr = serve.poisson_arrivals(len(jobs), rate, seed=seed)
@@ -315,6 +316,17 @@ def served_measure(jobs_n: int, step_scale: float, rate_mult: float, budget_gb: 
     for j in sorted(jobs, key=lambda j: -j.finished)[:5]:
         log(f"  job {j.job_id}: N {j.system.N} steps {j.steps} arrived {j.arrival:.2f} s admitted {j.admitted:.2f} s "
             f"finished {j.finished:.2f} s ({1e3 * (j.finished - j.admitted) / j.steps:.4f} ms/step in its pool)")
+    others = {}
+    for m in more_rates:
+        r2 = m * len(jobs) / t_plain
+        for j, a in zip(jobs, serve.poisson_arrivals(len(jobs), r2, seed=seed)):
+            j.arrival = float(a)
+            j.admitted = j.finished = None
+        o = srv.run(jobs)
+        others[f"{m:g}x"] = {"value": o["nt_steps_per_s"], "served_over_plain": o["nt_steps_per_s"] / plain,
+                             "p50_latency_s": o["p50"], "p99_latency_s": o["p99"], "makespan_s": o["makespan"]}
+        log(f"served at {r2:.2f} jobs/s ({m}x plain): makespan {o['makespan']:.2f} s, {o['nt_steps_per_s']:.3e} nt-steps/s "
+            f"({o['nt_steps_per_s'] / plain:.2f}x), p50 {o['p50']:.3f} s p99 {o['p99']:.3f} s")
     for j in jobs:
         j.arrival = 0.0
         j.admitted = j.finished = None
@@ -332,7 +344,8 @@ def served_measure(jobs_n: int, step_scale: float, rate_mult: float, budget_gb: 
         "plain_one_at_a_time": {"value": plain, "seconds": t_plain, "setup_seconds": t_setup,
                                 "stepping_only_value": work / t_run},
         "served_over_plain": res["nt_steps_per_s"] / plain, "capacity_over_plain": cap["nt_steps_per_s"] / plain,
-        "hbm_bytes_resident": hbm,
+        "other_offered_loads": others,
+        "hbm_bytes_resident": hbm, "workgroup_pool_slot_nt": small_cap, "workgroup_pool_quantum_steps": quantum_small,
     }
 
 
@@ -373,6 +386,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="headline: skip the all-FP64 run")
     ap.add_argument("--no-served", action="store_true", help="headline: skip the served stream")
+    ap.add_argument("--no-batched", action="store_true", help="headline: skip the batched hairpins")
     ap.add_argument("--cpu-steps", type=int, default=4,
                     help="CPU baseline: steps timed per variant (the 196,608-nt system takes seconds per step)")
     ap.add_argument("--sort-every", type=int, default=500, help="Hilbert re-sort interval in steps (0 = never)")
@@ -382,6 +396,8 @@ def main():
     ap.add_argument("--step-scale", type=float, default=0.01, help="served: factor on the jobs' step counts")
     ap.add_argument("--rate-mult", type=float, default=4.0, help="served: arrival rate / plain path's job rate")
     ap.add_argument("--budget-gb", type=int, default=8, help="served: HBM budget of the resident pools")
+    ap.add_argument("--small-cap", type=int, default=256, help="served: slot size of the workgroup pool (nucleotides)")
+    ap.add_argument("--quantum-small", type=int, default=1024, help="served: steps between the workgroup pool's admissions")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (systems are independent and a "
@@ -392,7 +408,8 @@ def main():
     if _lib.device_count() < 1:
         raise SystemExit("no GPU visible")
     if args.workload == "served":
-        sv = served_measure(args.jobs, args.step_scale, args.rate_mult, args.budget_gb, args.seed)
+        sv = served_measure(args.jobs, args.step_scale, args.rate_mult, args.budget_gb, args.seed, args.small_cap,
+                            args.quantum_small)
         out = {"metric": "nucleotide-steps/s", "value": sv["value"], "unit": "nucleotide-steps/s", "n_gpus": 1,
                "steps": int(sv["work_nt_steps"]), "warmup": 0, "ms_per_step": None, "higher_is_better": True,
                "scaling": "weak", "vs_baseline": None, "dtype": "f32 forces / f64 state", "data": "synthetic",
@@ -418,8 +435,17 @@ def main():
                              "steps": args.steps, "warmup": args.warmup, "dtype": "f64 forces / f64 state",
                              "workload": r64["desc"] + " (backend_precision = double)",
                              "roofline": roofline(r64, copy_gbps, profile_counters(name, "double", r64["stats"]))}
+        if not args.no_batched:
+            # the workgroup engine runs up to 256 steps per launch: time whole launches
+            nb = max(args.steps, 2560)
+            rb = run_plain("batched", "mixed", nb, 256, args.sort_every, args.seed)
+            extra["batched"] = {"value": rb["value"], "unit": "nucleotide-steps/s", "ms_per_step": rb["ms_per_step"],
+                                "steps": nb, "warmup": 256, "dtype": "f32 forces / f64 state", "workload": rb["desc"],
+                                "n_systems": rb["n_systems"], "n_nucleotides": rb["n_total"],
+                                "engine": rb["stats"]["engine"]}
         if not args.no_served:
-            extra["served"] = served_measure(args.jobs, args.step_scale, args.rate_mult, args.budget_gb, args.seed)
+            extra["served"] = served_measure(args.jobs, args.step_scale, args.rate_mult, args.budget_gb, args.seed,
+                                             args.small_cap, args.quantum_small)
     cpu = None
     if not args.no_cpu and r["ref_keys"] is not None and r["system"] is not None:
         try:

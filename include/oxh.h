@@ -304,6 +304,53 @@ oxh_status oxh_batch_set_images(oxh_batch *b, int sys, const int *img3);
  * each particle's share of the potential energy (Epart, N doubles). */
 oxh_status oxh_batch_get_forces(oxh_batch *b, int sys, double *F, double *T, double *Epart);
 
+/* ---- FFS order parameters.  Per-pair quantities of system `sys` at its
+ * current state, FP64, for n pairs (2 ints each: particle indices in the
+ * system's upload order), one value per pair into out.  They are what the
+ * reference's order parameters are made of (src/Utilities/
+ * OrderParameters.cpp, src/Observables/OrderParameterValues.cpp:55-66) and
+ * what its CUDA FFS backend precomputes per pair (hb_op_precalc,
+ * near_hb_op_precalc, dist_op_precalc, src/CUDA/Interactions/
+ * CUDA_DNA.cuh:909-1069):
+ *   OXH_PAIR_HB_ENERGY        hydrogen-bonding energy (0 for bonded
+ *                             neighbours; a "bond" parameter counts pairs
+ *                             below -0.1);
+ *   OXH_PAIR_NEAR_HB          1 when the pair is complementary, its base-base
+ *                             distance is in the HB range and at least 6 of
+ *                             the 7 HB factors are active, else 0;
+ *   OXH_PAIR_BASE_DISTANCE    |base(q) - base(p)| ("mindistance");
+ *   OXH_PAIR_CENTRE_DISTANCE  |q - p| ("mindistance" with use_COM).
+ * HB quantities take the nearest image; distances take the plain
+ * difference for two particles of one strand and the nearest image
+ * otherwise, as the reference's MinDistanceParameter. */
+#define OXH_PAIR_HB_ENERGY 0
+#define OXH_PAIR_NEAR_HB 1
+#define OXH_PAIR_BASE_DISTANCE 2
+#define OXH_PAIR_CENTRE_DISTANCE 3
+oxh_status oxh_batch_pair_quantities(oxh_batch *b, int sys, int kind, int n, const int *pairs, double *out);
+
+/* ---- barostat.  One Monte Carlo volume move of system `sys`, the
+ * reference's MC barostat of MD runs (use_barostat, P, delta_L,
+ * barostat_isotropic, barostat_molecular: src/Backends/MDBackend.cpp:43-60;
+ * the moves of src/Backends/MCMoves/VolumeMove.cpp:49-122 and
+ * MoleculeVolumeMove.cpp, applied by MD_CPUBackend, src/Backends/
+ * MD_CPUBackend.cpp:200-205; the CUDA backend's MD_CUDABackend::
+ * _apply_barostat, src/CUDA/Backends/MD_CUDABackend.cu:451-516).  Each box
+ * side grows by delta_L (u - 1/2) (u = u4[0] for all three when
+ * isotropic, else u4[0..2]); positions scale with the box (molecular:
+ * every strand moves with its centre of mass); the lists are rebuilt and
+ * the move is accepted when exp(-(dU + P dV - n T ln(V'/V)) / T) > u4[3],
+ * with n = N (atomic) or the number of strands + 1 (molecular), the
+ * CPU backend's counts.  A rejected move restores the box and positions.
+ * u4 are uniform numbers in [0, 1) from the caller's generator (the
+ * reference draws drand48).  accepted, dE (may be NULL): the decision and
+ * dU.  Batches only; a system with external forces is refused. */
+oxh_status oxh_batch_barostat_move(oxh_batch *b, int sys, double P, double delta_L, int isotropic, int molecular, const double *u4,
+		int *accepted, double *dE);
+
+/* Box sides of system `sys` (3 doubles). */
+oxh_status oxh_batch_box(oxh_batch *b, int sys, double *box3);
+
 /* Per-system FP64 sums: U (potential), U_hb, K_t, K_r (4 doubles per
  * system, n_sys entries).  Replaces the reference's CUDA energy print
  * (src/CUDA/Backends/MD_CUDABackend.cu:603-606, thrust transform_reduce). */

@@ -26,7 +26,11 @@ EXPORTS = [
     "oxh_batch_fix_diffusion", "oxh_batch_set_fix_diffusion", "oxh_batch_checkpoint_bytes", "oxh_batch_save",
     "oxh_batch_restore", "oxh_batch_get_order", "oxh_batch_set_external_forces", "oxh_batch_get_images",
     "oxh_batch_set_images", "oxh_interaction_compute_forces", "oxh_interaction_compute_forces_noalloc",
+    "oxh_batch_pair_quantities", "oxh_batch_barostat_move", "oxh_batch_box",
 ]
+
+# per-pair quantities of FFS order parameters (include/oxh.h OXH_PAIR_*)
+OXH_PAIR = {"hb_energy": 0, "near_hb": 1, "base_distance": 2, "centre_distance": 3}
 
 # external force types (include/oxh.h OXH_EXT_*), by the reference's `type` key
 OXH_EXT = {"string": 1, "trap": 2, "mutual_trap": 3, "constant_trap": 4, "repulsion_plane": 5, "sphere": 6}
@@ -89,6 +93,9 @@ def lib():
         L.oxh_batch_get_images.argtypes = [V, I, P]
         L.oxh_batch_set_images.argtypes = [V, I, P]
         L.oxh_batch_energies.argtypes = [V, P]
+        L.oxh_batch_pair_quantities.argtypes = [V, I, I, I, P, P]
+        L.oxh_batch_barostat_move.argtypes = [V, I, D, D, I, I, P, P, P]
+        L.oxh_batch_box.argtypes = [V, I, P]
         L.oxh_batch_energy_split.argtypes = [V, I, P]
         L.oxh_batch_hb_count.argtypes = [V, P]
         L.oxh_batch_get_pairs.argtypes = [V, I, P, LL, P]

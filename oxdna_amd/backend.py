@@ -271,6 +271,33 @@ class HIPBatch:
         check(lib().oxh_batch_energies(self.h, ptr(out)), "oxh_batch_energies")
         return out
 
+    def pair_quantities(self, kind: str, pairs, sys: int = 0) -> np.ndarray:
+        """Per-pair FFS order-parameter quantities of system `sys`
+        (oxh_batch_pair_quantities): kind 'hb_energy', 'near_hb',
+        'base_distance' or 'centre_distance'; pairs (n, 2) particle indices
+        in the system's upload order."""
+        pr = _c(np.asarray(pairs, dtype=np.int32).reshape(-1, 2), np.int32)
+        out = np.zeros(len(pr))
+        check(lib().oxh_batch_pair_quantities(self.h, sys, _lib.OXH_PAIR[kind], len(pr), ptr(pr), ptr(out)),
+              "oxh_batch_pair_quantities")
+        return out
+
+    def barostat_move(self, P: float, delta_L: float, u4, isotropic: bool = True, molecular: bool = False,
+                      sys: int = 0):
+        """One Monte Carlo volume move (oxh_batch_barostat_move) with the
+        uniform numbers u4 (4 values in [0, 1)); returns (accepted, dU)."""
+        u = _c(np.asarray(u4, dtype=np.float64).reshape(4), np.float64)
+        acc = C.c_int(0)
+        dE = C.c_double(0.0)
+        check(lib().oxh_batch_barostat_move(self.h, sys, float(P), float(delta_L), int(bool(isotropic)), int(bool(molecular)),
+                                            ptr(u), C.byref(acc), C.byref(dE)), "oxh_batch_barostat_move")
+        return bool(acc.value), dE.value
+
+    def box(self, sys: int = 0) -> np.ndarray:
+        out = np.zeros(3)
+        check(lib().oxh_batch_box(self.h, sys, ptr(out)), "oxh_batch_box")
+        return out
+
     def energy_split(self, sys: int = 0) -> np.ndarray:
         out = np.zeros(8)
         check(lib().oxh_batch_energy_split(self.h, sys, ptr(out)), "oxh_batch_energy_split")

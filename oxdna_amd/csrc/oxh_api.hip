@@ -1928,6 +1928,95 @@ oxh_status oxh_batch_get_forces(oxh_batch *B, int sys, double *F, double *T, dou
 	return OXH_OK;
 }
 
+oxh_status oxh_batch_pair_quantities(oxh_batch *B, int sys, int kind, int n, const int *pairs, double *out) {
+	TRY(check_sys(B, sys));
+	if(kind < OXH_PAIR_HB_ENERGY || kind > OXH_PAIR_CENTRE_DISTANCE) return fail(OXH_ERR_ARG, "unknown pair quantity %d", kind);
+	if(n < 0 || (n > 0 && (!pairs || !out))) return fail(OXH_ERR_ARG, "oxh_batch_pair_quantities: bad arguments");
+	if(n == 0) return OXH_OK;
+	oxh::Batch &b = B->b;
+	const int ns = B->n[sys];
+	const std::vector<int> &sid = B->strand_of[sys];
+	std::vector<int4> h(n);
+	for(int k = 0; k < n; k++) {
+		const int p = pairs[2 * k], q = pairs[2 * k + 1];
+		if(p < 0 || q < 0 || p >= ns || q >= ns) return fail(OXH_ERR_ARG, "pair %d: particle index out of range", k);
+		h[k] = make_int4(p, q, sid[p] == sid[q] ? 1 : 0, 0);
+	}
+	HIPCHK(hipSetDevice(b.device));
+	int4 *d_pairs = nullptr;
+	double *d_out = nullptr;
+	HIPCHK(hipMalloc((void **) &d_pairs, (size_t) n * sizeof(int4)));
+	if(hipMalloc((void **) &d_out, (size_t) n * sizeof(double)) != hipSuccess) {
+		(void) hipFree(d_pairs);
+		return fail(OXH_ERR_ALLOC, "oxh_batch_pair_quantities: allocation failed");
+	}
+	hipError_t e = hipMemcpyAsync(d_pairs, h.data(), (size_t) n * sizeof(int4), hipMemcpyHostToDevice, b.stream);
+	if(e == hipSuccess) {
+		oxh::launch_pair_quantities(b.stream, b, sys, kind, n, d_pairs, d_out);
+		e = hipGetLastError();
+	}
+	if(e == hipSuccess) e = hipMemcpyAsync(out, d_out, (size_t) n * sizeof(double), hipMemcpyDeviceToHost, b.stream);
+	if(e == hipSuccess) e = hipStreamSynchronize(b.stream);
+	(void) hipFree(d_pairs);
+	(void) hipFree(d_out);
+	if(e != hipSuccess) return fail(OXH_ERR_HIP, "oxh_batch_pair_quantities: %s", hipGetErrorString(e));
+	return OXH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// potential energy of system sys at the current state (the batch's force
+// evaluation, then its FP64 reduction)
+oxh_status system_energy(oxh_batch *B, int sys, double &U) {
+	enqueue_forces(B);
+	oxh::launch_reduce(B->b.stream, B->b);
+	std::vector<double> r((size_t) B->b.nsys * oxh::RED_W);
+	HIPCHK(hipMemcpyAsync(r.data(), B->b.red_out, r.size() * sizeof(double), hipMemcpyDeviceToHost, B->b.stream));
+	TRY(sync_and_check(B));
+	U = r[(size_t) sys * oxh::RED_W];
+	return OXH_OK;
+}
+
+// the box of system sys set to L, with everything that follows from it:
+// inverse sides, FP32 copies, the FP32 rounding bound of the list
+// predicate, the small-box flag and the cells (the cell count per side only
+// goes down, so the system keeps its cell range; cells stay at least as
+// wide as its stencil needs)
+oxh_status set_system_box(oxh_batch *B, int sys, const double L[3]) {
+	oxh_sys &S = B->b.h_sys[sys];
+	const double rv = std::sqrt(S.rv2);
+	for(int k = 0; k < 3; k++) {
+		if(!(L[k] > 0)) return fail(OXH_ERR_ARG, "system %d: box side %d must be > 0", sys, k);
+		S.box[k] = L[k];
+		S.inv_box[k] = 1.0 / L[k];
+		S.boxf[k] = (float) L[k];
+		S.inv_boxf[k] = (float) (1.0 / L[k]);
+	}
+	const double lmax = std::max(L[0], std::max(L[1], L[2]));
+	const double dr = 4.0 * lmax * std::ldexp(1.0, -24);
+	S.eps2 = (float) (4.0 * (2.0 * rv * dr * std::sqrt(3.0) + 3.0 * dr * dr + 8.0 * rv * rv * std::ldexp(1.0, -24)));
+	S.small_box = 0;
+	for(int k = 0; k < 3; k++)
+		if(L[k] < 2.0 * (rv + 2.0 * 0.4814)) S.small_box = 1;
+	if(S.use_cells) {
+		for(int k = 0; k < 3; k++) {
+			const int fit = (int) std::floor(L[k] * S.stencil / rv);
+			S.nc[k] = std::min(S.nc[k], fit);
+			if(S.nc[k] < 2 * S.stencil + 1) return fail(OXH_ERR_SYSTEM, "system %d: box side %d too short for its cells", sys, k);
+			S.cell_w[k] = (float) (L[k] / S.nc[k]);
+		}
+		S.ncell = S.nc[0] * S.nc[1] * S.nc[2];
+	}
+	HIPCHK(hipMemcpyAsync(B->b.d_sys + sys, &S, sizeof(oxh_sys), hipMemcpyHostToDevice, B->b.stream));
+	return sync_and_check(B);
+}
+
+}  // namespace
+
+extern "C" {
+
 oxh_status oxh_batch_energies(oxh_batch *B, double *out4) {
 	if(!B || !out4) return fail(OXH_ERR_ARG, "bad arguments");
 	HIPCHK(hipSetDevice(B->b.device));
@@ -2077,6 +2166,103 @@ oxh_status oxh_batch_set_state(oxh_batch *B, int sys, const double *pos, const d
 	TRY(sync_and_check(B));
 	TRY(oxh_batch_update_lists(B, 1));
 	return oxh_batch_compute_forces(B);
+}
+
+oxh_status oxh_batch_barostat_move(oxh_batch *B, int sys, double P, double delta_L, int isotropic, int molecular, const double *u4,
+		int *accepted, double *dE) {
+	TRY(check_sys(B, sys));
+	if(B->pool) return fail(OXH_ERR_ARG, "oxh_batch_barostat_move: slot pools keep their boxes");
+	if(!u4) return fail(OXH_ERR_ARG, "oxh_batch_barostat_move: null random numbers");
+	oxh::Batch &b = B->b;
+	if(b.h_sys[sys].has_ext)
+		return fail(OXH_ERR_ARG, "oxh_batch_barostat_move: system %d has external forces, whose potential the move would need", sys);
+	const int f = B->first[sys], n = B->n[sys];
+	const oxh_sys S0 = b.h_sys[sys];
+	const double T = S0.T;
+	double U0 = 0, U1 = 0;
+	TRY(system_energy(B, sys, U0));
+	// the proposal: every side moved by delta_L (u - 1/2), the same u for
+	// all three when isotropic
+	const double L0[3] = {S0.box[0], S0.box[1], S0.box[2]};
+	double L1[3];
+	for(int k = 0; k < 3; k++) L1[k] = L0[k] + delta_L * (u4[isotropic ? 0 : k] - 0.5);
+	std::vector<double4> p0;
+	std::vector<int> o;
+	TRY(download(B, p0, b.pos, f, n));
+	TRY(download(B, o, b.orig, f, n));
+	TRY(sync_and_check(B));
+	std::vector<double4> p1 = p0;
+	if(molecular) {
+		// every strand moved with its centre of mass (the mean of its
+		// particles' positions), which scales with the box
+		const std::vector<int> &sid = B->strand_of[sys];
+		const int ns = B->n_strands[sys];
+		std::vector<double> com(3 * (size_t) ns, 0.0);
+		std::vector<int> cnt(ns, 0);
+		for(int k = 0; k < n; k++) {
+			const int m = sid[o[k]];
+			com[3 * m] += p0[k].x;
+			com[3 * m + 1] += p0[k].y;
+			com[3 * m + 2] += p0[k].z;
+			cnt[m]++;
+		}
+		for(int k = 0; k < n; k++) {
+			const int m = sid[o[k]];
+			p1[k].x += com[3 * m] / cnt[m] * (L1[0] / L0[0] - 1.0);
+			p1[k].y += com[3 * m + 1] / cnt[m] * (L1[1] / L0[1] - 1.0);
+			p1[k].z += com[3 * m + 2] / cnt[m] * (L1[2] / L0[2] - 1.0);
+		}
+	}
+	else
+		for(int k = 0; k < n; k++) {
+			p1[k].x *= L1[0] / L0[0];
+			p1[k].y *= L1[1] / L0[1];
+			p1[k].z *= L1[2] / L0[2];
+		}
+	if(b.graph_exec) {
+		(void) hipGraphExecDestroy(b.graph_exec);
+		b.graph_exec = nullptr;
+	}
+	auto apply = [&](const double *L, const std::vector<double4> &pos, const oxh_sys *restore) -> oxh_status {
+		if(restore) {
+			b.h_sys[sys] = *restore;
+			HIPCHK(hipMemcpyAsync(b.d_sys + sys, restore, sizeof(oxh_sys), hipMemcpyHostToDevice, b.stream));
+		}
+		else TRY(set_system_box(B, sys, L));
+		HIPCHK(hipMemcpyAsync(b.pos + f, pos.data(), (size_t) n * sizeof(double4), hipMemcpyHostToDevice, b.stream));
+		oxh::launch_prepare_mirror(b.stream, b);
+		TRY(sync_and_check(B));
+		return oxh_batch_update_lists(B, 1);
+	};
+	oxh_status st = apply(L1, p1, nullptr);
+	if(st == OXH_OK) st = system_energy(B, sys, U1);
+	if(st != OXH_OK) {
+		// a box too small for its cells, or a failed evaluation: back to the
+		// state before the move
+		const std::string err = g_err;
+		TRY(apply(L0, p0, &S0));
+		TRY(system_energy(B, sys, U0));
+		g_err = err;
+		return st;
+	}
+	const double V0 = L0[0] * L0[1] * L0[2], V1 = L1[0] * L1[1] * L1[2];
+	const double n_obj = molecular ? (double) B->n_strands[sys] + 1.0 : (double) n;
+	const double x = -((U1 - U0) + P * (V1 - V0) - n_obj * T * std::log(V1 / V0)) / T;
+	const bool acc = std::exp(x) > u4[3];
+	if(dE) *dE = U1 - U0;
+	if(accepted) *accepted = acc ? 1 : 0;
+	if(!acc) {
+		TRY(apply(L0, p0, &S0));
+		TRY(system_energy(B, sys, U0));
+	}
+	return OXH_OK;
+}
+
+oxh_status oxh_batch_box(oxh_batch *B, int sys, double *box3) {
+	TRY(check_sys(B, sys));
+	if(!box3) return fail(OXH_ERR_ARG, "null box buffer");
+	for(int k = 0; k < 3; k++) box3[k] = B->b.h_sys[sys].box[k];
+	return OXH_OK;
 }
 
 oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {  // 11 doubles

@@ -177,5 +177,8 @@ int launch_hilbert_sort(hipStream_t st, Batch &b);
 void launch_ext_forces(hipStream_t st, const Batch &b);
 // slot_of from orig (after an upload or a sort)
 void launch_slot_of(hipStream_t st, const Batch &b);
+// per-pair FFS order-parameter quantities (OXH_PAIR_*) of system sys:
+// pairs (p, q, same strand, unused) in the system's upload order
+void launch_pair_quantities(hipStream_t st, const Batch &b, int sys, int kind, int n, const int4 *d_pairs, double *d_out);
 
 }  // namespace oxh

@@ -184,13 +184,18 @@ class DataOutput:
     with split = true, each term preceded by a space;
     src/Observables/PotentialEnergy.cpp:21-55), kinetic_energy
     (src/Observables/KineticEnergy.cpp:37-50) and hb_list with only_count
-    (pairs with HB energy below -0.1; src/Observables/HBList.cpp:141-164).
+    (pairs with HB energy below -0.1; src/Observables/HBList.cpp:141-164)
+    and order_parameters (src/Observables/OrderParameterValues.cpp:22-80:
+    each bond parameter's count of pairs with HB energy below -0.1, then
+    each mindistance parameter's distance, every value followed by a space,
+    in the stream's default format; the file from order_parameters_file or
+    op_file in the column block, else in the input file).
     Numbers print with "%10.<precision>lf", or "%.<precision>g" with
     general_format (src/Observables/BaseObservable.cpp:32-43)."""
 
-    TYPES = ("step", "potential_energy", "kinetic_energy", "hb_list")
+    TYPES = ("step", "potential_energy", "kinetic_energy", "hb_list", "order_parameters")
 
-    def __init__(self, text: str, dt: float, path):
+    def __init__(self, text: str, dt: float, path, inp: dict | None = None, N: int = 0):
         d = _block(text)
         if "name" not in d:
             raise ValueError("data_output block without a name")
@@ -212,8 +217,14 @@ class DataOutput:
                 raise ValueError("hb_list is supported with only_count = true")
             prec = int(float(c.get("precision", "6")))
             fmt = f"{{:.{prec}g}}" if _bool(c.get("general_format", "false")) else f"{{:10.{prec}f}}"
-            self.cols.append(dict(type=t, fmt=fmt, split=_bool(c.get("split", "false")),
-                                  md_units=c.get("units", "steps") == "MD"))
+            col = dict(type=t, fmt=fmt, split=_bool(c.get("split", "false")), md_units=c.get("units", "steps") == "MD")
+            if t == "order_parameters":
+                src = next((x for x in (c, inp or {}) for k in ("order_parameters_file", "op_file") if k in x), None)
+                if src is None:
+                    raise KeyError("order_parameters needs order_parameters_file or op_file")
+                fname = src.get("order_parameters_file", src.get("op_file"))
+                col["params"] = io.read_order_parameters(path(fname), N)
+            self.cols.append(col)
             k += 1
         self.f = open(self.name, "w")
 
@@ -239,12 +250,66 @@ class DataOutput:
             elif t == "kinetic_energy":
                 E = b.energies()[0]
                 out.append(c["fmt"].format((E[2] + E[3]) / N))
-            else:
+            elif t == "hb_list":
                 out.append(str(int(b.hb_count()[0])))
+            else:
+                hb, dist = order_parameter_values(b, c["params"])
+                out.append("".join(f"{v} " for v in hb) + "".join(f"{v:g} " for v in dist))
         return " ".join(out)
 
     def close(self):
         self.f.close()
+
+
+def order_parameter_values(b, params: list, sys: int = 0):
+    """Values of the order parameters `params` (io.read_order_parameters)
+    at the batch's current state, from the GPU's per-pair quantities: the
+    count of each bond parameter's pairs with hydrogen-bonding energy below
+    -0.1 (OrderParameterValues::get_output_string,
+    src/Observables/OrderParameterValues.cpp:55-66) and each mindistance
+    parameter's smallest base-base (or, with use_COM, centre-centre)
+    distance (MinDistanceParameter::calculate_state,
+    src/Utilities/OrderParameters.cpp:73-98).  Returns (counts, distances)
+    in the file's order of each kind."""
+    hb, dist = [], []
+    for p in params:
+        pairs = np.array(p["pairs"], dtype=np.int32).reshape(-1, 2)
+        if p["type"] == "bond":
+            e = b.pair_quantities("hb_energy", pairs, sys) if len(pairs) else np.zeros(0)
+            hb.append(int(np.count_nonzero(e < io.HB_CUTOFF)))
+        else:
+            d = b.pair_quantities("centre_distance" if p["use_COM"] else "base_distance", pairs, sys) if len(pairs) else None
+            dist.append(float(d.min()) if d is not None else -1.0)
+    return hb, dist
+
+
+def barostat_plan(inp: dict, seed: int, start: int, end: int) -> dict:
+    """The MC barostat of an MD input (use_barostat, P, delta_L,
+    barostat_probability, barostat_isotropic, barostat_molecular;
+    src/Backends/MDBackend.cpp:43-60): the reference tries a volume move in
+    each step with probability barostat_probability
+    (MDBackend::_is_barostat_active, :121-126); here the steps of the tries
+    in (start, end] are drawn up front as geometric gaps, and each try takes
+    four uniform numbers, from a generator seeded by the run's seed."""
+    out = dict(steps=set(), attempts=0, accepted=0)
+    if not _bool(inp.get("use_barostat", "false")):
+        return out
+    for key in ("P", "delta_L", "barostat_probability"):
+        if key not in inp:
+            raise KeyError(f"use_barostat needs {key}")
+    p = float(inp["barostat_probability"])
+    if not 0.0 < p <= 1.0:
+        raise ValueError("barostat_probability must be in (0, 1]")
+    rng = np.random.default_rng([int(seed) & 0xFFFFFFFFFFFFFFFF, 0xBA80])
+    t = start
+    while True:
+        t += int(rng.geometric(p))
+        if t > end:
+            break
+        out["steps"].add(t)
+    out.update(P=float(inp["P"]), delta_L=float(inp["delta_L"]), rng=rng,
+               isotropic=_bool(inp.get("barostat_isotropic", "true")), molecular=_bool(inp.get("barostat_molecular", "false")))
+    return out
 
 
 def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
@@ -333,17 +398,18 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
             st = b.state(0)
             t = s.copy()
             t.pos, t.a1, t.a3, t.vel, t.L = st["pos"], st["a1"], st["a3"], st["vel"], st["L"]
+            t.box = b.box(0)   # the barostat changes it
             img = b.images(0)
             shown = t.copy()
             if back_in_box:
                 # each strand by the whole boxes that bring its absolute
                 # centre of mass into the box
-                absolute = t.pos + img * s.box
+                absolute = t.pos + img * t.box
                 for k in np.unique(s.strand):
                     m = s.strand == k
-                    shown.pos[m] = t.pos[m] - np.floor(absolute[m].mean(axis=0) / s.box) * s.box
+                    shown.pos[m] = t.pos[m] - np.floor(absolute[m].mean(axis=0) / t.box) * t.box
             else:
-                shown.pos = t.pos + img * s.box
+                shown.pos = t.pos + img * t.box
             return U, K, t, shown, img
 
         energy_f = open(energy_path, "w")
@@ -355,15 +421,22 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
             ckpts = ([t for t in range(start + ckpt_every, start + steps + 1, ckpt_every)]
                      if ckpt_every > 0 and ckpt_path else [])
             blocks = [k for k in inp if k.startswith("data_output_") and k.rsplit("_", 1)[1].isdigit()]
-            outputs = [DataOutput(inp[k], dt, path) for k in sorted(blocks, key=lambda x: int(x.rsplit("_", 1)[1]))]
+            outputs = [DataOutput(inp[k], dt, path, inp, s.N) for k in sorted(blocks, key=lambda x: int(x.rsplit("_", 1)[1]))]
             out_steps = [set(o.steps(start, start + steps)) for o in outputs]
-            events = sorted(set([t for t, _, _ in plan] + fixes + ckpts + [t for o in out_steps for t in o]))
+            baro = barostat_plan(inp, opts["seed"], start, start + steps)
+            events = sorted(set([t for t, _, _ in plan] + fixes + ckpts + [t for o in out_steps for t in o]
+                                + list(baro["steps"])))
             todo = {t: (e, c) for t, e, c in plan}
             end = start + steps
             for t in events:
                 if t > cur:
                     b.run(t - cur)
                     cur = t
+                if t in baro["steps"]:
+                    acc, _ = b.barostat_move(baro["P"], baro["delta_L"], baro["rng"].random(4), baro["isotropic"],
+                                             baro["molecular"])
+                    baro["attempts"] += 1
+                    baro["accepted"] += int(acc)
                 e, c = todo.get(t, (False, False))
                 fix = t in fixes
                 # inside the loop the reference prints a due configuration,
@@ -406,7 +479,8 @@ def run(input_path: str, overrides=(), workdir: str | None = None) -> dict:
                 traj_f.close()
             for o in locals().get("outputs", []):
                 o.close()
-    return dict(energy_lines=lines, final=state, step=cur, U=U, K=K)
+    return dict(energy_lines=lines, final=state, step=cur, U=U, K=K,
+                barostat=dict(attempts=baro["attempts"], accepted=baro["accepted"]) if baro["attempts"] else None)
 
 
 def _conf_text(s, step, U, K) -> str:

@@ -309,29 +309,36 @@ def particles_from_string(text, s: System) -> list:
     return out
 
 
-def _force_blocks(text: str) -> list:
-    """key = value blocks in braces of an external_forces_file (the
-    reference's plain format, src/Forces/ForceFactory.cpp:140-200: '#'
-    comments, one force per {...})."""
+def _blocks(text: str, what: str) -> list:
+    """key = value blocks in braces, as the reference's external_forces_file
+    and order_parameters_file hold them (src/Forces/ForceFactory.cpp:
+    140-200, src/Utilities/OrderParameters.cpp:301-340: '#' comments, one
+    entry per {...})."""
     clean = "\n".join(line.split("#", 1)[0] for line in text.splitlines())
     blocks, depth, cur = [], 0, []
     for ch in clean:
         if ch == "{":
             depth += 1
             if depth > 1:
-                raise ValueError("external forces file: nested braces")
+                raise ValueError(f"{what}: nested braces")
             cur = []
         elif ch == "}":
             depth -= 1
             if depth < 0:
-                raise ValueError("external forces file: unbalanced braces")
+                raise ValueError(f"{what}: unbalanced braces")
             body = "".join(cur).strip()
             if not body:
-                raise ValueError("external forces file: nothing between braces")
+                raise ValueError(f"{what}: nothing between braces")
             blocks.append(parse_input(body))
         elif depth == 1:
             cur.append(ch)
+    if depth != 0:
+        raise ValueError(f"{what}: unbalanced braces")
     return blocks
+
+
+def _force_blocks(text: str) -> list:
+    return _blocks(text, "external forces file")
 
 
 def read_external_forces(path: str, s: System, as_json: bool = False) -> list:
@@ -382,4 +389,59 @@ def read_external_forces(path: str, s: System, as_json: bool = False) -> list:
             r = dict(rec)
             r["particle"] = pid
             out.append(r)
+    return out
+
+
+# ------------------------------------------------------- order parameters
+HB_CUTOFF = -0.1   # src/defs.h:62
+
+
+def read_order_parameters(path: str, N: int) -> list:
+    """The parameters of an order_parameters_file (the reference's op_file,
+    parsed as OrderParameters::init_from_file does, src/Utilities/
+    OrderParameters.cpp:301-598): one {...} block per parameter with
+    `order_parameter` = bond or mindistance, a `name` and keys pair1,
+    pair2, ... of the form "a, b" (one pair) or "a1-a2, b1-b2" (a1 with b2,
+    a1 + 1 with b2 - 1, ..., the two ranges of equal length).  Pairs are
+    kept as (min, max); a bond parameter keeps each pair once.  Optional:
+    cutoff (bond; the reference uses it in FFS runs only), interfaces and
+    use_COM (mindistance: centre-centre instead of base-base distances).
+    Returns dicts with type, name, pairs, cutoff, use_COM, interfaces.  The
+    twist parameter is not supported and raises."""
+    with open(path) as f:
+        text = f.read()
+    out = []
+    for d in _blocks(text, "order parameters file"):
+        t = d.get("order_parameter")
+        if t is None:
+            raise KeyError("order parameter without order_parameter")
+        if t not in ("bond", "mindistance"):
+            raise ValueError(f"order parameter type {t!r} is not supported (supported: bond, mindistance)")
+        if "name" not in d:
+            raise KeyError(f"{t} order parameter without a name")
+        pairs = []
+        for key in (k for k in d if k.startswith("pair")):
+            vals = [v.strip() for v in d[key].split(",")]
+            if len(vals) != 2:
+                raise ValueError(f"order parameters file: invalid value {d[key]!r} for key {key}")
+            ranged = ["-" in v for v in vals]
+            if ranged[0] != ranged[1]:
+                raise ValueError(f"order parameters file: invalid value {d[key]!r} for key {key}")
+            if ranged[0]:
+                (s1, e1), (s2, e2) = ([int(x) for x in v.split("-")] for v in vals)
+                if s1 - e1 != s2 - e2 or s1 > e1 or s2 > e2:
+                    raise ValueError(f"order parameters file: invalid list {d[key]!r}")
+                new = [(m, e2 - c) for c, m in enumerate(range(s1, e1 + 1))]
+            else:
+                new = [(int(vals[0]), int(vals[1]))]
+            for a, b in new:
+                if not (0 <= a < N and 0 <= b < N):
+                    raise ValueError(f"order parameters file: particle index out of range in {d[key]!r}")
+                pairs.append((min(a, b), max(a, b)))
+        if t == "bond":
+            pairs = sorted(set(pairs))
+        interfaces = [float(x) for x in d["interfaces"].split(",")] if "interfaces" in d else []
+        use_com = d.get("use_COM", "false").strip().lower() in ("1", "true", "yes")
+        out.append(dict(type=t, name=d["name"], pairs=pairs, cutoff=float(d.get("cutoff", HB_CUTOFF)),
+                        use_COM=use_com, interfaces=interfaces))
     return out

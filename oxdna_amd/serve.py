@@ -37,9 +37,12 @@ from .backend import HIPPool
 from .systems import System
 
 # jobs up to SMALL_CAP nucleotides run in the workgroup pool (one
-# 512-thread workgroup each); MID_CAP bounds the shared grid pool's slots;
-# larger jobs get single-slot pools
-SMALL_CAP = 512
+# 512-thread workgroup each, at least 2 lanes per nucleotide: above 256
+# nucleotides a workgroup has one lane per nucleotide, whose pair loop is a
+# long chain of dependent gathers, and such jobs step faster in the grid
+# pool); MID_CAP bounds the shared grid pool's slots; larger jobs get
+# single-slot pools
+SMALL_CAP = 256
 MID_CAP = 8192
 
 
@@ -69,21 +72,21 @@ def footprint_bytes(n_nucleotides: int, max_neigh: int = 64) -> int:
 
 
 def plan_pools(budget_bytes: int, largest: int, small_slots: int = 256, mid_slots: tuple = (16, 4),
-               large_pools: int = 2):
+               large_pools: int = 2, small_cap: int = SMALL_CAP):
     """Pools as lists of slot capacities, one list per pool: the workgroup
     pool (small_slots x SMALL_CAP), the mid-size grid pool (mid_slots[0]
     slots of MID_CAP / 4 and mid_slots[1] of MID_CAP) and large_pools
     single-slot pools of `largest`; fewer mid-size slots if the budget
     is short.  Returns (pools, size class of each pool)."""
-    small = [SMALL_CAP] * small_slots
+    small = [small_cap] * small_slots
     n1, n2 = mid_slots
     mid = [MID_CAP // 4] * n1 + [MID_CAP] * n2
     large = [[max(largest, MID_CAP)] for _ in range(large_pools)] if largest > MID_CAP else []
-    fixed = footprint_bytes(SMALL_CAP * small_slots) + sum(footprint_bytes(p[0], 128) for p in large)
+    fixed = footprint_bytes(small_cap * small_slots) + sum(footprint_bytes(p[0], 128) for p in large)
     while len(mid) > 1 and fixed + footprint_bytes(sum(mid), 128) > budget_bytes:
         mid.pop(0)
     pools = [small, mid] + large
-    classes = [SMALL_CAP, MID_CAP] + [p[0] for p in large]
+    classes = [small_cap, MID_CAP] + [p[0] for p in large]
     return pools, classes
 
 
@@ -91,8 +94,8 @@ class Server:
     """Resident pools and the request loop."""
 
     def __init__(self, budget_bytes: int = 8 << 30, largest: int = 20000, small_slots: int = 256, device: int = 0,
-                 quantum_small: int = 1024, quantum_grid: int = 256, quantum_large: int = 1024):
-        pools, classes = plan_pools(budget_bytes, largest, small_slots)
+                 quantum_small: int = 1024, quantum_grid: int = 256, quantum_large: int = 1024, small_cap: int = SMALL_CAP):
+        pools, classes = plan_pools(budget_bytes, largest, small_slots, small_cap=small_cap)
         self.pools = [HIPPool(caps, device=device) for caps in pools]
         self.classes = classes
         self.quantum = [quantum_small, quantum_grid] + [quantum_large] * (len(pools) - 2)

@@ -150,7 +150,7 @@ def test_served_pool_plan():
     total = sum(serve.footprint_bytes(sum(p), 128) for p in pools)
     assert total <= 8 << 30
     # a tight budget drops mid-size slots but keeps one
-    pools2, _ = serve.plan_pools(serve.footprint_bytes(512 * 256) + 3 * serve.footprint_bytes(20000, 128), 20000)
+    pools2, _ = serve.plan_pools(serve.footprint_bytes(serve.SMALL_CAP * 256) + 3 * serve.footprint_bytes(20000, 128), 20000)
     assert 1 <= len(pools2[1]) < len(pools[1])
     # nothing above MID_CAP: no large pools
     pools3, classes3 = serve.plan_pools(8 << 30, 4000)
