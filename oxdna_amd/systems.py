@@ -372,21 +372,22 @@ def served_job(job_id: int, seed: int = 0, n_min: int = 40, n_max: int = 20000):
 
 
 def _chunks_single(b: _Builder, rng, n, L):
-    """Single strands of at most 60 nt scattered on a lattice."""
+    """Single strands of at most 60 nt scattered on a lattice, one strand
+    per lattice cell; the n nucleotides are spread as evenly as possible
+    (the first n mod n_str strands take one more), so that no strand is
+    longer than its cell."""
     n_str = max(1, math.ceil(n / 60))
-    per = n // n_str
+    per, extra = divmod(n, n_str)
     side = math.ceil(n_str ** (1.0 / 3.0))
     a = L / side
     k = 0
-    rem = n
     for ix in range(side):
         for iy in range(side):
             for iz in range(side):
                 if k == n_str:
                     return
-                m = per if k < n_str - 1 else rem
+                m = per + (1 if k < extra else 0)
                 add_single_strand(b, rng.integers(0, 4, m), (np.array([ix, iy, iz]) + 0.5) * a, random_rotation(rng))
-                rem -= m
                 k += 1
 
 
