@@ -258,27 +258,74 @@ oxh_status oxh_batch_get_order(oxh_batch *b, int sys, int *order);
 #define OXH_EXT_CONSTANT_TRAP 4     /* "constant_trap": ConstantTrap */
 #define OXH_EXT_REPULSION_PLANE 5   /* "repulsion_plane": RepulsionPlane */
 #define OXH_EXT_SPHERE 6            /* "sphere": RepulsiveSphere */
+#define OXH_EXT_LOWDIM_TRAP 7       /* "lowdim_trap": LowdimMovingTrap */
+#define OXH_EXT_ATTRACTION_PLANE 8  /* "attraction_plane": AttractionPlane */
+#define OXH_EXT_PLANE_MOVING 9      /* "repulsion_plane_moving": RepulsionPlaneMoving */
+#define OXH_EXT_SPHERE_SMOOTH 10    /* "sphere_smooth": RepulsiveSphereSmooth */
+#define OXH_EXT_SPHERE_MOVING 11    /* "repulsive_sphere_moving": RepulsiveSphereMoving */
+#define OXH_EXT_KEPLER_POINSOT 12   /* "repulsive_kepler_poinsot": RepulsiveKeplerPoinsot */
+#define OXH_EXT_LJ_WALL 13          /* "LJ_wall": LJWall */
+#define OXH_EXT_TWIST 14            /* "twist": ConstantRateTorque */
+#define OXH_EXT_CENTRAL 15          /* "generic_central_force": GenericCentralForce (force_type = gravity) */
+#define OXH_EXT_LJ_CONE 16          /* "LJ_cone": LJCone */
+#define OXH_EXT_ELLIPSOID 17        /* "ellipsoid": RepulsiveEllipsoid */
+#define OXH_EXT_COM 18              /* "com": COMForce */
+#define OXH_EXT_YUKAWA_SPHERE 19    /* "yukawa_sphere": YukawaSphere */
 
-/* One force acting on one particle, with the reference's keys:
- *   string:          F0, rate, dir (normalised here unless dir_as_centre,
- *                    then dir is the centre)
- *   trap:            stiff, rate, dir (normalised here), pos0
+/* One force acting on one particle, with the reference's keys (vectors
+ * called "normalised" are normalised here, as the reference's init does):
+ *   string:          F0, rate, dir (normalised unless dir_as_centre, then
+ *                    dir is the centre)
+ *   trap:            stiff, rate, dir (normalised), pos0
  *   mutual_trap:     stiff, stiff_rate, r0, rate, ref_particle, PBC
  *   constant_trap:   stiff, r0, ref_particle, PBC
  *   repulsion_plane: stiff, position, end_position, v, dir (normalised)
- *   sphere:          stiff, r0, rate, r_ext, center (in dir)
+ *   sphere:          stiff, r0, rate, r_ext, center
+ *   lowdim_trap:     stiff, rate, pos0, dir (normalised), visibility
+ *   attraction_plane: stiff, position, dir (normalised)
+ *   repulsion_plane_moving: stiff, dir (normalised), the reference
+ *                    particles ref_particle..ref_last (contiguous, as the
+ *                    reference requires)
+ *   sphere_smooth:   stiff, r0, r_ext, smooth, alpha, center
+ *   repulsive_sphere_moving: stiff, r0, rate, r_ext, origin (in pos0),
+ *                    target, steps
+ *   repulsive_kepler_poinsot: stiff, rate, apex, base, base_radius, kappa,
+ *                    center
+ *   LJ_wall:         stiff, position, sigma, n, cutoff, dir (normalised)
+ *   twist:           stiff, rate, base (in F0), axis (normalised), pos0,
+ *                    center, mask
+ *   generic_central_force: F0, inner_cut_off, outer_cut_off, center
+ *   LJ_cone:         stiff, sigma, alpha, n, cutoff, dir (normalised),
+ *                    pos0
+ *   ellipsoid:       stiff, r_1, r_2, center
+ *   com:             stiff, r0, rate, com_list (n_com), ref_list (n_ref):
+ *                    every particle of com_list carries the record
+ *   yukawa_sphere:   radius, epsilon (WCA_epsilon), sigma (WCA_sigma), n
+ *                    (WCA_n), debye_length, debye_A, cutoff, center
  * A force the reference applies to a list of particles (particle = "0-9,
- * 12" or -1 for all) is one record per particle, in the file's order. */
+ * 12" or -1 for all) is one record per particle, in the file's order.
+ * Keys are taken as the reference's init leaves them, defaults and key
+ * aliases included (e.g. only_repulsive sets cutoff = 2^(1/n)). */
 typedef struct oxh_ext_force {
 	int type;
 	int particle;        /* index in the system, upload order */
-	int ref_particle;    /* mutual_trap, constant_trap */
+	int ref_particle;    /* mutual_trap, constant_trap, repulsion_plane_moving */
 	int pbc;             /* mutual_trap, constant_trap: 1 = nearest image */
 	int dir_as_centre;   /* string */
 	double F0, rate, stiff, stiff_rate, r0;
 	double position, end_position, v, r_ext;
 	double dir[3];
 	double pos0[3];
+	int ref_last;        /* repulsion_plane_moving */
+	int n;               /* LJ_wall, LJ_cone: exponent; yukawa_sphere: WCA_n */
+	int visibility[3];   /* lowdim_trap */
+	int n_com, n_ref;    /* com */
+	const int *com_list; /* com: n_com indices in the system */
+	const int *ref_list; /* com: n_ref indices in the system */
+	long long steps;     /* repulsive_sphere_moving */
+	double sigma, cutoff, alpha, smooth, epsilon, debye_length, debye_A, radius;
+	double apex, base, base_radius, kappa, inner_cut_off, outer_cut_off;
+	double center[3], axis[3], mask[3], target[3], r_1[3], r_2[3];
 } oxh_ext_force;
 
 /* Replaces the external forces of system `sys` (n = 0 removes them).
@@ -384,7 +431,8 @@ oxh_status oxh_batch_hb_count(oxh_batch *b, int *out);
 oxh_status oxh_batch_get_pairs(oxh_batch *b, int sys, int *pairs, long long max, long long *n);
 
 /* List statistics of system `sys`: rebuild count and the last error bits
- * (1 broken FENE bond, 2 neighbour overflow, 4 cell overflow). */
+ * (1 broken FENE bond, 2 neighbour overflow, 4 cell overflow, 16 a
+ * particle inside a yukawa_sphere's wall, where the reference throws). */
 oxh_status oxh_batch_system_info(oxh_batch *b, int sys, long long *n_rebuilds, int *error);
 
 /* Replaces the state (positions, orientations, velocities) of system `sys`

@@ -33,7 +33,10 @@ EXPORTS = [
 OXH_PAIR = {"hb_energy": 0, "near_hb": 1, "base_distance": 2, "centre_distance": 3}
 
 # external force types (include/oxh.h OXH_EXT_*), by the reference's `type` key
-OXH_EXT = {"string": 1, "trap": 2, "mutual_trap": 3, "constant_trap": 4, "repulsion_plane": 5, "sphere": 6}
+OXH_EXT = {"string": 1, "trap": 2, "mutual_trap": 3, "constant_trap": 4, "repulsion_plane": 5, "sphere": 6,
+           "lowdim_trap": 7, "attraction_plane": 8, "repulsion_plane_moving": 9, "sphere_smooth": 10,
+           "repulsive_sphere_moving": 11, "repulsive_kepler_poinsot": 12, "LJ_wall": 13, "twist": 14,
+           "generic_central_force": 15, "LJ_cone": 16, "ellipsoid": 17, "com": 18, "yukawa_sphere": 19}
 
 
 class SystemDesc(C.Structure):
@@ -60,6 +63,14 @@ class ExtForce(C.Structure):
         ("F0", C.c_double), ("rate", C.c_double), ("stiff", C.c_double), ("stiff_rate", C.c_double),
         ("r0", C.c_double), ("position", C.c_double), ("end_position", C.c_double), ("v", C.c_double),
         ("r_ext", C.c_double), ("dir", C.c_double * 3), ("pos0", C.c_double * 3),
+        ("ref_last", C.c_int), ("n", C.c_int), ("visibility", C.c_int * 3), ("n_com", C.c_int), ("n_ref", C.c_int),
+        ("com_list", C.c_void_p), ("ref_list", C.c_void_p), ("steps", C.c_longlong),
+        ("sigma", C.c_double), ("cutoff", C.c_double), ("alpha", C.c_double), ("smooth", C.c_double),
+        ("epsilon", C.c_double), ("debye_length", C.c_double), ("debye_A", C.c_double), ("radius", C.c_double),
+        ("apex", C.c_double), ("base", C.c_double), ("base_radius", C.c_double), ("kappa", C.c_double),
+        ("inner_cut_off", C.c_double), ("outer_cut_off", C.c_double),
+        ("center", C.c_double * 3), ("axis", C.c_double * 3), ("mask", C.c_double * 3), ("target", C.c_double * 3),
+        ("r_1", C.c_double * 3), ("r_2", C.c_double * 3),
     ]
 
 

@@ -206,26 +206,41 @@ class HIPBatch:
         return out
 
     def set_external_forces(self, forces, sys: int = 0, step0: int = 0):
-        """External forces of system `sys`: a list of dicts with the
-        reference's keys (type, particle, F0, rate, stiff, dir, ...; see
-        oxdna_amd.io.read_external_forces), one record per particle, in the
-        file's order.  step0 is the reference's step counter at the next
-        force evaluation."""
+        """External forces of system `sys`: a list of dicts whose keys are
+        the fields of the C record (type, particle, F0, rate, stiff, dir,
+        center, ...; see oxdna_amd.io.read_external_forces), one record per
+        particle, in the file's order.  step0 is the reference's step
+        counter at the next force evaluation."""
         arr = (_lib.ExtForce * max(1, len(forces)))()
+        keep = []   # com lists stay alive through the call
+        scalars = (("F0", 0.0), ("rate", 0.0), ("stiff", 0.0), ("stiff_rate", 0.0), ("r0", 0.0), ("position", 0.0),
+                   ("end_position", 1e6), ("v", 0.0), ("r_ext", 1e10), ("sigma", 1.0), ("cutoff", 1e6), ("alpha", 0.0),
+                   ("smooth", 0.0), ("epsilon", 1.0), ("debye_length", 0.0), ("debye_A", 0.0), ("radius", 0.0),
+                   ("apex", 0.0), ("base", 0.0), ("base_radius", 0.0), ("kappa", 0.0), ("inner_cut_off", 0.0),
+                   ("outer_cut_off", 0.0))
         for k, f in enumerate(forces):
             e = arr[k]
             e.type = _lib.OXH_EXT[f["type"]]
             e.particle = int(f["particle"])
             e.ref_particle = int(f.get("ref_particle", -1))
+            e.ref_last = int(f.get("ref_last", -1))
             e.pbc = int(bool(f.get("PBC", False)))
             e.dir_as_centre = int(bool(f.get("dir_as_centre", False)))
-            for key, default in (("F0", 0.0), ("rate", 0.0), ("stiff", 0.0), ("stiff_rate", 0.0), ("r0", 0.0),
-                                 ("position", 0.0), ("end_position", 1e6), ("v", 0.0), ("r_ext", 1e10)):
+            e.n = int(f.get("n", 6))
+            e.steps = int(f.get("steps", 0))
+            for key, default in scalars:
                 setattr(e, key, float(f.get(key, default)))
-            d = f.get("dir", f.get("center", (0.0, 0.0, 0.0)))
-            for c in range(3):
-                e.dir[c] = float(d[c])
-                e.pos0[c] = float(f.get("pos0", (0.0, 0.0, 0.0))[c])
+            for key in ("dir", "pos0", "center", "axis", "mask", "target", "r_1", "r_2"):
+                v = f.get(key, (0.0, 0.0, 0.0))
+                for c in range(3):
+                    getattr(e, key)[c] = float(v[c])
+            for c, v in enumerate(f.get("visibility", (0, 0, 0))):
+                e.visibility[c] = int(v)
+            if f["type"] == "com":
+                com, ref = _c(f["com_list"], np.int32), _c(f["ref_list"], np.int32)
+                keep += [com, ref]
+                e.n_com, e.n_ref = len(com), len(ref)
+                e.com_list, e.ref_list = ptr(com), ptr(ref)
         check(lib().oxh_batch_set_external_forces(self.h, sys, len(forces), C.byref(arr), int(step0)),
               "oxh_batch_set_external_forces")
 

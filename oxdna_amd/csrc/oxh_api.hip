@@ -61,6 +61,10 @@ struct oxh_batch {
 	// external forces of each system: (particle, record) in the caller's
 	// order
 	std::vector<std::vector<std::pair<int, oxh_ext_rec>>> ext_of;
+	// centre-of-mass groups of each system's "com" forces: (com list,
+	// reference list), indices in the system; a com record's ref is its
+	// group's position in this list
+	std::vector<std::vector<std::pair<std::vector<int>, std::vector<int>>>> grp_of;
 };
 
 namespace {
@@ -1015,7 +1019,8 @@ oxh_status oxh_batch_destroy(oxh_batch *B) {
 	void *ptrs[] = {b.pos, b.vel, b.Lm, b.q64, b.rlist, b.q32, b.bpos, b.F, b.T, b.bonds, b.pinfo, b.orig, b.pstep, b.nbr, b.nnbr,
 			b.cell_count, b.cell_items, b.cell_owner, b.pos32, b.d_sys, b.d_state, b.red_blocks, b.red_sys_blocks, b.red_partial, b.red_out, b.red_done,
 			b.esplit, b.stress, b.hbcount, b.keys, b.keys_alt, b.perm, b.perm_alt, b.inv_perm, b.radix_hist, b.scratch, B->bk_pos, B->bk_vel,
-			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state, B->bk_rlist, b.limit, b.any_rebuild, b.F64, b.T64, B->bk_F64, B->bk_T64, b.eown, b.ein, b.ecount, b.E1, b.E2, b.E3, b.B1, b.B2, b.B3, b.cell_raw, b.cell_of, b.back64, b.img, b.slot_of, b.ext_off, b.ext_rec};
+			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state, B->bk_rlist, b.limit, b.any_rebuild, b.F64, b.T64, B->bk_F64, B->bk_T64, b.eown, b.ein, b.ecount, b.E1, b.E2, b.E3, b.B1, b.B2, b.B3, b.cell_raw, b.cell_of, b.back64, b.img, b.slot_of, b.ext_off, b.ext_rec,
+			b.ext_grp, b.ext_grp_sys, b.ext_idx, b.ext_com};
 	for(void *p : ptrs)
 		if(p) (void) hipFree(p);
 	if(b.stream) (void) hipStreamDestroy(b.stream);
@@ -1489,10 +1494,11 @@ oxh_status oxh_batch_run(oxh_batch *B, long long n_steps) {
 		std::vector<oxh_sys_state> st;
 		TRY(read_states(B, st));
 		bool overflow = false;
-		int bad = -1;
+		int bad = -1, bad_ext = -1;
 		for(int s = 0; s < b.nsys; s++) {
 			if(st[s].error & oxh::ERR_OVERFLOW) overflow = true;
 			if((st[s].error & 1) && bad < 0) bad = s;
+			if((st[s].error & oxh::ERR_EXT) && bad_ext < 0) bad_ext = s;
 		}
 		if(overflow) {
 			// replay the chunk from its start with larger buffers and the
@@ -1520,6 +1526,7 @@ oxh_status oxh_batch_run(oxh_batch *B, long long n_steps) {
 			continue;
 		}
 		if(bad >= 0) return fail(OXH_ERR_SYSTEM, "system %d: a bonded pair exceeded the FENE range", bad);
+		if(bad_ext >= 0) return fail(OXH_ERR_SYSTEM, "system %d: a particle is beyond the radius of a yukawa_sphere force", bad_ext);
 	}
 	return OXH_OK;
 }
@@ -1743,37 +1750,47 @@ oxh_status oxh_batch_set_external_forces(oxh_batch *B, int sys, int n, const oxh
 	oxh::Batch &b = B->b;
 	const int N = B->n[sys];
 	std::vector<std::pair<int, oxh_ext_rec>> recs;
+	std::vector<std::pair<std::vector<int>, std::vector<int>>> groups;
 	for(int k = 0; k < n; k++) {
 		const oxh_ext_force &f = forces[k];
 		if(f.particle < 0 || f.particle >= N) return fail(OXH_ERR_ARG, "external force %d: no particle %d in system %d", k, f.particle, sys);
 		oxh_ext_rec r;
 		std::memset(&r, 0, sizeof(r));
 		r.type = f.type;
-		auto unit = [&](double *out) -> bool {
-			const double m = std::sqrt(f.dir[0] * f.dir[0] + f.dir[1] * f.dir[1] + f.dir[2] * f.dir[2]);
+		// LR_vector::normalize: times the reciprocal of the module
+		// (src/Utilities/LR_vector.h:86-89,131-136)
+		auto unit = [](const double *v, double *out) -> bool {
+			const double m = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
 			if(!(m > 0)) return false;
-			for(int c = 0; c < 3; c++) out[c] = f.dir[c] / m;
+			const double inv = 1. / m;
+			for(int c = 0; c < 3; c++) out[c] = v[c] * inv;
 			return true;
 		};
+		auto copy3 = [](const double *v, double *out) {
+			for(int c = 0; c < 3; c++) out[c] = v[c];
+		};
+		auto in_system = [&](int i) { return i >= 0 && i < N; };
 		switch(f.type) {
 		case OXH_EXT_STRING:
 			r.a = f.F0;
 			r.b = f.rate;
 			if(f.dir_as_centre) {
 				r.flags = 2;
-				for(int c = 0; c < 3; c++) r.v1[c] = f.dir[c];
+				copy3(f.dir, r.v1);
 			}
-			else if(!unit(r.v1)) return fail(OXH_ERR_ARG, "external force %d: zero dir", k);
+			else if(!unit(f.dir, r.v1)) return fail(OXH_ERR_ARG, "external force %d: zero dir", k);
 			break;
 		case OXH_EXT_TRAP:
+		case OXH_EXT_LOWDIM_TRAP:
 			r.a = f.stiff;
 			r.b = f.rate;
-			if(!unit(r.v1)) return fail(OXH_ERR_ARG, "external force %d: zero dir", k);
-			for(int c = 0; c < 3; c++) r.v2[c] = f.pos0[c];
+			if(!unit(f.dir, r.v1)) return fail(OXH_ERR_ARG, "external force %d: zero dir", k);
+			copy3(f.pos0, r.v2);
+			if(f.type == OXH_EXT_LOWDIM_TRAP) r.flags = (f.visibility[0] ? 1 : 0) | (f.visibility[1] ? 2 : 0) | (f.visibility[2] ? 4 : 0);
 			break;
 		case OXH_EXT_MUTUAL_TRAP:
 		case OXH_EXT_CONSTANT_TRAP:
-			if(f.ref_particle < 0 || f.ref_particle >= N)
+			if(!in_system(f.ref_particle))
 				return fail(OXH_ERR_ARG, "external force %d: no reference particle %d in system %d", k, f.ref_particle, sys);
 			r.ref = f.ref_particle;
 			r.flags = f.pbc ? 1 : 0;
@@ -1787,14 +1804,129 @@ oxh_status oxh_batch_set_external_forces(oxh_batch *B, int sys, int n, const oxh
 			r.b = f.position;
 			r.c = f.end_position;
 			r.d = f.v;
-			if(!unit(r.v1)) return fail(OXH_ERR_ARG, "external force %d: zero dir", k);
+			if(!unit(f.dir, r.v1)) return fail(OXH_ERR_ARG, "external force %d: zero dir", k);
+			break;
+		case OXH_EXT_ATTRACTION_PLANE:
+			r.a = f.stiff;
+			r.b = f.position;
+			if(!unit(f.dir, r.v1)) return fail(OXH_ERR_ARG, "external force %d: zero dir", k);
+			break;
+		case OXH_EXT_PLANE_MOVING:
+			if(!in_system(f.ref_particle) || !in_system(f.ref_last) || f.ref_last < f.ref_particle)
+				return fail(OXH_ERR_ARG, "external force %d: reference particles %d..%d are not a range of system %d", k, f.ref_particle,
+						f.ref_last, sys);
+			r.a = f.stiff;
+			r.ref = f.ref_particle;
+			r.ref2 = f.ref_last;
+			if(!unit(f.dir, r.v1)) return fail(OXH_ERR_ARG, "external force %d: zero dir", k);
 			break;
 		case OXH_EXT_SPHERE:
 			r.a = f.stiff;
 			r.b = f.rate;
 			r.c = f.r0;
 			r.d = f.r_ext;
-			for(int c = 0; c < 3; c++) r.v1[c] = f.dir[c];
+			copy3(f.center, r.v1);
+			break;
+		case OXH_EXT_SPHERE_SMOOTH:
+			r.a = f.stiff;
+			r.c = f.r0;
+			r.d = f.r_ext;
+			r.e = f.smooth;
+			r.f = f.alpha;
+			copy3(f.center, r.v1);
+			break;
+		case OXH_EXT_SPHERE_MOVING:
+			r.a = f.stiff;
+			r.b = f.rate;
+			r.c = f.r0;
+			r.d = f.r_ext;
+			r.steps = f.steps;
+			copy3(f.pos0, r.v1);
+			copy3(f.target, r.v2);
+			break;
+		case OXH_EXT_KEPLER_POINSOT:
+			if(!(f.apex > f.base)) return fail(OXH_ERR_ARG, "external force %d: repulsive_kepler_poinsot needs apex > base", k);
+			if(!(f.base_radius > 0)) return fail(OXH_ERR_ARG, "external force %d: repulsive_kepler_poinsot needs base_radius > 0", k);
+			if(!(f.kappa > 0)) return fail(OXH_ERR_ARG, "external force %d: repulsive_kepler_poinsot needs kappa > 0", k);
+			r.a = f.stiff;
+			r.b = f.rate;
+			r.c = f.apex;
+			r.d = f.base;
+			r.e = f.base_radius;
+			r.f = f.kappa;
+			copy3(f.center, r.v1);
+			break;
+		case OXH_EXT_LJ_WALL:
+		case OXH_EXT_LJ_CONE:
+			if(f.n <= 0 || f.n % 2 != 0) return fail(OXH_ERR_ARG, "external force %d: n (%d) must be a positive even integer", k, f.n);
+			if(!(f.sigma > 0)) return fail(OXH_ERR_ARG, "external force %d: sigma must be > 0", k);
+			r.a = f.stiff;
+			r.c = f.sigma;
+			r.d = f.cutoff;
+			r.n = f.n;
+			if(!unit(f.dir, r.v1)) return fail(OXH_ERR_ARG, "external force %d: zero dir", k);
+			if(f.type == OXH_EXT_LJ_WALL) r.b = f.position;
+			else {
+				copy3(f.pos0, r.v2);
+				r.e = std::sin(f.alpha);
+				r.f = std::cos(f.alpha);
+				r.g = std::tan(f.alpha);
+			}
+			break;
+		case OXH_EXT_TWIST:
+			r.a = f.stiff;
+			r.b = f.rate;
+			r.c = f.F0;
+			if(!unit(f.axis, r.v1)) return fail(OXH_ERR_ARG, "external force %d: zero axis", k);
+			copy3(f.pos0, r.v2);
+			copy3(f.center, r.v3);
+			copy3(f.mask, r.v4);
+			break;
+		case OXH_EXT_CENTRAL:
+			r.a = f.F0;
+			r.c = f.inner_cut_off;
+			r.d = f.outer_cut_off;
+			copy3(f.center, r.v1);
+			break;
+		case OXH_EXT_ELLIPSOID:
+			r.a = f.stiff;
+			copy3(f.center, r.v1);
+			copy3(f.r_1, r.v2);
+			copy3(f.r_2, r.v3);
+			break;
+		case OXH_EXT_COM: {
+			if(f.n_com <= 0 || f.n_ref <= 0 || !f.com_list || !f.ref_list)
+				return fail(OXH_ERR_ARG, "external force %d: com needs non-empty com_list and ref_list", k);
+			// the reference keeps both lists as sets of particles
+			std::vector<int> com(f.com_list, f.com_list + f.n_com), ref(f.ref_list, f.ref_list + f.n_ref);
+			for(auto *v : {&com, &ref}) {
+				std::sort(v->begin(), v->end());
+				v->erase(std::unique(v->begin(), v->end()), v->end());
+				for(int i : *v)
+					if(!in_system(i)) return fail(OXH_ERR_ARG, "external force %d: no particle %d in system %d", k, i, sys);
+			}
+			int g = 0;
+			while(g < (int) groups.size() && !(groups[g].first == com && groups[g].second == ref)) g++;
+			if(g == (int) groups.size()) groups.push_back({com, ref});
+			r.a = f.stiff;
+			r.b = f.rate;
+			r.c = f.r0;
+			r.ref = g;
+			r.n = (int) com.size();
+			break;
+		}
+		case OXH_EXT_YUKAWA_SPHERE:
+			if(f.n <= 0) return fail(OXH_ERR_ARG, "external force %d: WCA_n must be > 0", k);
+			if(!(f.debye_length > 0)) return fail(OXH_ERR_ARG, "external force %d: debye_length must be > 0", k);
+			r.a = f.radius;
+			r.b = f.epsilon;
+			r.c = f.sigma;
+			r.d = f.debye_length;
+			r.e = f.debye_A;
+			r.f = f.cutoff;
+			r.g = f.sigma * std::pow(2., 1. / f.n);
+			r.n = f.n;
+			copy3(f.center, r.v1);
 			break;
 		default: return fail(OXH_ERR_ARG, "external force %d: unknown type %d", k, f.type);
 		}
@@ -1802,29 +1934,61 @@ oxh_status oxh_batch_set_external_forces(oxh_batch *B, int sys, int n, const oxh
 	}
 	HIPCHK(hipStreamSynchronize(b.stream));
 	if(B->ext_of.empty()) B->ext_of.resize(b.nsys);
+	if(B->grp_of.empty()) B->grp_of.resize(b.nsys);
 	std::stable_sort(recs.begin(), recs.end(), [](const std::pair<int, oxh_ext_rec> &x, const std::pair<int, oxh_ext_rec> &y) {
 		return x.first < y.first;
 	});
 	B->ext_of[sys] = recs;
-	// CSR over every system's particles: offsets per (first + particle)
-	std::vector<int> off(b.N + 1, 0);
+	B->grp_of[sys] = groups;
+	// CSR over every system's particles: offsets per (first + particle);
+	// com groups numbered over the batch, systems in order
+	std::vector<int> off(b.N + 1, 0), gbase(b.nsys + 1, 0), idx, gsys;
+	std::vector<int4> grp;
 	std::vector<oxh_ext_rec> all;
-	for(int s = 0; s < b.nsys; s++)
+	for(int s = 0; s < b.nsys; s++) {
+		gbase[s + 1] = gbase[s] + (int) B->grp_of[s].size();
 		for(auto &x : B->ext_of[s]) off[B->first[s] + x.first + 1]++;
+		for(auto &gr : B->grp_of[s]) {
+			const int c0 = (int) idx.size();
+			idx.insert(idx.end(), gr.first.begin(), gr.first.end());
+			const int r0 = (int) idx.size();
+			idx.insert(idx.end(), gr.second.begin(), gr.second.end());
+			grp.push_back(make_int4(c0, (int) gr.first.size(), r0, (int) gr.second.size()));
+			gsys.push_back(s);
+		}
+	}
 	for(int i = 0; i < b.N; i++) off[i + 1] += off[i];
 	all.reserve(off[b.N]);
 	for(int s = 0; s < b.nsys; s++)
-		for(auto &x : B->ext_of[s]) all.push_back(x.second);   // systems in order, particles sorted inside each
-	if(b.ext_off) HIPCHK(hipFree(b.ext_off));
-	if(b.ext_rec) HIPCHK(hipFree(b.ext_rec));
+		for(auto &x : B->ext_of[s]) {   // systems in order, particles sorted inside each
+			all.push_back(x.second);
+			if(x.second.type == OXH_EXT_COM) all.back().ref += gbase[s];
+		}
+	void *old[] = {b.ext_off, b.ext_rec, b.ext_grp, b.ext_grp_sys, b.ext_idx, b.ext_com};
+	for(void *p : old)
+		if(p) HIPCHK(hipFree(p));
 	b.ext_off = nullptr;
 	b.ext_rec = nullptr;
+	b.ext_grp = nullptr;
+	b.ext_grp_sys = nullptr;
+	b.ext_idx = nullptr;
+	b.ext_com = nullptr;
 	b.n_ext = (int) all.size();
+	b.n_grp = (int) grp.size();
 	if(b.n_ext > 0) {
 		TRY(dalloc(B, &b.ext_off, b.N + 1));
 		TRY(dalloc(B, &b.ext_rec, all.size()));
 		HIPCHK(hipMemcpyAsync(b.ext_off, off.data(), (b.N + 1) * sizeof(int), hipMemcpyHostToDevice, b.stream));
 		HIPCHK(hipMemcpyAsync(b.ext_rec, all.data(), all.size() * sizeof(oxh_ext_rec), hipMemcpyHostToDevice, b.stream));
+	}
+	if(b.n_grp > 0) {
+		TRY(dalloc(B, &b.ext_grp, grp.size()));
+		TRY(dalloc(B, &b.ext_grp_sys, gsys.size()));
+		TRY(dalloc(B, &b.ext_idx, idx.size()));
+		TRY(dalloc(B, &b.ext_com, 6 * grp.size()));
+		HIPCHK(hipMemcpyAsync(b.ext_grp, grp.data(), grp.size() * sizeof(int4), hipMemcpyHostToDevice, b.stream));
+		HIPCHK(hipMemcpyAsync(b.ext_grp_sys, gsys.data(), gsys.size() * sizeof(int), hipMemcpyHostToDevice, b.stream));
+		HIPCHK(hipMemcpyAsync(b.ext_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, b.stream));
 	}
 	oxh_sys &S = b.h_sys[sys];
 	S.has_ext = recs.empty() ? 0 : 1;
@@ -1844,8 +2008,12 @@ oxh_status oxh_batch_set_external_forces(oxh_batch *B, int sys, int n, const oxh
 		(void) hipGraphExecDestroy(b.graph_exec);
 		b.graph_exec = nullptr;
 	}
+	k_clear_err<<<(b.nsys + 63) / 64, 64, 0, b.stream>>>(b.nsys, b.d_state, oxh::ERR_EXT);
 	enqueue_forces(B);
-	return sync_and_check(B);
+	std::vector<oxh_sys_state> st;
+	TRY(read_states(B, st));
+	if(st[sys].error & oxh::ERR_EXT) return fail(OXH_ERR_SYSTEM, "system %d: a particle is beyond the radius of a yukawa_sphere force", sys);
+	return OXH_OK;
 }
 
 oxh_status oxh_batch_get_images(oxh_batch *B, int sys, int *img3) {

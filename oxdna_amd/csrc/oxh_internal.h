@@ -99,6 +99,15 @@ struct Batch {
 	int n_ext = 0;
 	int *ext_off = nullptr;
 	oxh_ext_rec *ext_rec = nullptr;
+	// centre-of-mass groups of "com" forces: ext_grp[g] = (first, count)
+	// of the group's com list and (first, count) of its reference list in
+	// ext_idx (indices in the system, upload order); ext_com[g] holds the
+	// two centres (6 doubles), written each force evaluation
+	int n_grp = 0;
+	int4 *ext_grp = nullptr;
+	int *ext_grp_sys = nullptr;
+	int *ext_idx = nullptr;
+	double *ext_com = nullptr;
 
 	// Hilbert sort scratch
 	unsigned int *keys = nullptr, *keys_alt = nullptr;
@@ -127,6 +136,7 @@ struct Batch {
 
 // oxh_sys_state.error bits
 constexpr int ERR_BOND = 1, ERR_LIST = 2, ERR_CELLS = 4, ERR_EDGE = 8;
+constexpr int ERR_EXT = 16;   // a particle where an external force is undefined (the reference throws)
 constexpr int ERR_OVERFLOW = ERR_LIST | ERR_CELLS | ERR_EDGE;   // fixed by growing a buffer and replaying
 
 constexpr int RED_CHUNK = 2048;

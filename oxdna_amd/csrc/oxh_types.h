@@ -81,9 +81,11 @@ typedef struct oxh_sys {
 // ref is the partner's index inside the system (upload order); flags & 1:
 // nearest image between the two particles (PBC).
 typedef struct oxh_ext_rec {
-	int type, flags, ref, pad_;
-	double a, b, c, d;
-	double v1[3], v2[3];
+	int type, flags, ref, ref2;   // ref2: the last of a range of reference particles
+	int n, pad_;
+	long long steps;
+	double a, b, c, d, e, f, g;
+	double v1[3], v2[3], v3[3], v4[3];
 } oxh_ext_rec;
 
 // per-system device-side state written by the kernels

@@ -263,7 +263,10 @@ def load_system(topology: str, conf: str, T: float = 0.1, salt: float = 0.5, nam
                   name=name, vel=vel, L=L)
 
 
-EXT_TYPES = ("string", "trap", "mutual_trap", "constant_trap", "repulsion_plane", "sphere")
+EXT_TYPES = ("string", "trap", "mutual_trap", "constant_trap", "repulsion_plane", "sphere", "lowdim_trap",
+             "attraction_plane", "repulsion_plane_moving", "sphere_smooth", "repulsive_sphere_moving",
+             "repulsive_kepler_poinsot", "LJ_wall", "twist", "generic_central_force", "LJ_cone", "ellipsoid", "com",
+             "yukawa_sphere")
 
 
 def _vec3(v) -> tuple:
@@ -341,12 +344,142 @@ def _force_blocks(text: str) -> list:
     return _blocks(text, "external forces file")
 
 
+def _flag(v) -> bool:
+    return str(v).strip().lower() in ("1", "true", "yes")
+
+
+def _ext_record(t: str, b: dict, s: System):
+    """One block of an external_forces_file as the reference's force class
+    reads it in its init (src/Forces/<Class>.cpp), defaults and key
+    handling included: the record's keys are the fields of the C record
+    (include/oxh.h oxh_ext_force).  Returns (record, particle indices)."""
+    def need(*keys):
+        for k in keys:
+            if k not in b:
+                raise ValueError(f"external force {t}: missing key {k}")
+
+    def num(k, default=None):
+        if k in b:
+            return float(b[k])
+        if default is None:
+            raise ValueError(f"external force {t}: missing key {k}")
+        return default
+
+    def vec(k, default=None):
+        if k in b:
+            return _vec3(b[k])
+        if default is None:
+            raise ValueError(f"external force {t}: missing key {k}")
+        return default
+
+    rec = {"type": t}
+    if t == "string":                      # ConstantRateForce.cpp:20-42
+        rec.update(F0=num("F0"), rate=num("rate"), dir=vec("dir"), dir_as_centre=_flag(b.get("dir_as_centre", 0)))
+    elif t == "trap":                      # MovingTrap.cpp:21-53
+        rec.update(stiff=num("stiff"), rate=num("rate"), dir=vec("dir"), pos0=vec("pos0"))
+    elif t in ("mutual_trap", "constant_trap"):   # MutualTrap.cpp:24-60, ConstantTrap.cpp:22-47
+        rec.update(stiff=num("stiff"), r0=num("r0"), PBC=_flag(b.get("PBC", 0)), ref_particle=int(num("ref_particle")))
+        if t == "mutual_trap":
+            rec.update(rate=num("rate", 0.0), stiff_rate=num("stiff_rate", 0.0))
+        need("particle")
+        pid = int(b["particle"])
+        if pid == -1:
+            raise ValueError(f"{t} cannot act on every particle")
+        return rec, [pid]
+    elif t == "repulsion_plane":           # RepulsionPlane.cpp:22-54
+        rec.update(stiff=num("stiff"), position=num("position"), dir=vec("dir"), v=num("v", 0.0),
+                   end_position=num("end_position", 1e6))
+    elif t == "sphere":                    # RepulsiveSphere.cpp:22-48
+        rec.update(stiff=num("stiff"), r0=num("r0"), rate=num("rate", 0.0), r_ext=num("r_ext", 1e10),
+                   center=vec("center", (0.0, 0.0, 0.0)))
+    elif t == "lowdim_trap":               # LowdimMovingTrap.cpp:21-63
+        need("visibility")
+        vis = [int(x) for x in str(b["visibility"]).replace(" ", "").split(",")]
+        if len(vis) != 3:
+            raise ValueError(f"external force {t}: could not parse visibility {b['visibility']!r}")
+        rec.update(stiff=num("stiff"), rate=num("rate"), pos0=vec("pos0"), dir=vec("dir"), visibility=tuple(vis))
+    elif t == "attraction_plane":          # AttractionPlane.cpp:20-41
+        rec.update(stiff=num("stiff"), position=num("position"), dir=vec("dir"))
+    elif t == "repulsion_plane_moving":    # RepulsionPlaneMoving.cpp:17-46
+        need("ref_particle")
+        refs = sorted(particles_from_string(b["ref_particle"], s))
+        if refs[-1] - refs[0] + 1 != len(refs):
+            raise ValueError("repulsion_plane_moving needs a contiguous list of ref_particle indices")
+        rec.update(stiff=num("stiff"), dir=vec("dir"), ref_particle=refs[0], ref_last=refs[-1])
+    elif t == "sphere_smooth":             # RepulsiveSphereSmooth.cpp:22-48
+        # the reference reads r_ext into smooth and alpha as well
+        # (RepulsiveSphereSmooth.cpp:30-32), so r_ext is required
+        r_ext = num("r_ext")
+        rec.update(stiff=num("stiff"), r0=num("r0"), r_ext=r_ext, smooth=r_ext, alpha=r_ext,
+                   center=vec("center", (0.0, 0.0, 0.0)))
+    elif t == "repulsive_sphere_moving":   # RepulsiveSphereMoving.cpp:24-85
+        center = vec("center", (0.0, 0.0, 0.0))
+        steps = int(num("steps", num("move_steps", 0.0)))
+        rec.update(stiff=num("stiff"), r0=num("r0"), rate=num("rate", 0.0), r_ext=num("r_ext", 1e10),
+                   pos0=vec("origin", center), target=vec("target", (0.0, 0.0, 0.0)), steps=steps)
+    elif t == "repulsive_kepler_poinsot":  # RepulsiveKeplerPoinsot.cpp:34-103
+        rec.update(stiff=num("stiff"), rate=num("rate", 0.0), apex=num("apex", 1.2), base=num("base", 0.7),
+                   base_radius=num("base_radius", 0.45), kappa=num("kappa", 25.0), center=vec("center", (0.0, 0.0, 0.0)))
+        if rec["apex"] <= rec["base"]:
+            raise ValueError("repulsive_kepler_poinsot: require apex > base")
+        if rec["base_radius"] <= 0:
+            raise ValueError("repulsive_kepler_poinsot: require base_radius > 0")
+        if rec["kappa"] <= 0:
+            rec["kappa"] = 25.0
+    elif t in ("LJ_wall", "LJ_cone"):      # LJWall.cpp:26-57, LJCone.cpp:26-62
+        n = int(num("n", 6.0))
+        if n % 2 != 0:
+            raise ValueError(f"{t}: n ({n}) should be an even integer")
+        cutoff = 2.0 ** (1.0 / n) if _flag(b.get("only_repulsive", 0)) else 1e6
+        rec.update(stiff=num("stiff", 1.0), sigma=num("sigma", 1.0), n=n, cutoff=cutoff, dir=vec("dir"))
+        if t == "LJ_wall":
+            rec["position"] = num("position")
+        else:
+            rec.update(alpha=num("alpha"), pos0=vec("pos0"))
+    elif t == "twist":                     # ConstantRateTorque.cpp:17-60
+        rec.update(stiff=num("stiff"), rate=num("rate"), F0=num("base"), axis=vec("axis"), pos0=vec("pos0"),
+                   center=vec("center"), mask=vec("mask", (0.0, 0.0, 0.0)))
+    elif t == "generic_central_force":     # GenericCentralForce.cpp:130-172
+        need("force_type")
+        ft = str(b["force_type"]).strip()
+        if ft == "interpolated":
+            raise ValueError("generic_central_force: force_type = interpolated (a potential_file table) is not supported")
+        if ft != "gravity":
+            raise ValueError(f"GenericCentralForce: unsupported type {ft!r}")
+        rec.update(F0=num("F0"), inner_cut_off=num("inner_cut_off", 0.0), outer_cut_off=num("outer_cut_off", 0.0),
+                   center=vec("center"))
+    elif t == "ellipsoid":                 # RepulsiveEllipsoid.cpp:22-49
+        rec.update(stiff=num("stiff"), r_2=vec("r_2"), r_1=vec("r_1", (1e-6, 1e-6, 1e-6)),
+                   center=vec("center", (0.0, 0.0, 0.0)))
+    elif t == "com":                       # COMForce.cpp:22-42
+        need("com_list", "ref_list")
+        com = particles_from_string(b["com_list"], s)
+        ref = particles_from_string(b["ref_list"], s)
+        rec.update(stiff=num("stiff"), r0=num("r0"), rate=num("rate", 0.0), com_list=tuple(com), ref_list=tuple(ref))
+        return rec, com
+    elif t == "yukawa_sphere":             # YukawaSphere.cpp:22-54
+        # the reference reads WCA_n into the WCA sigma and keeps n = 6
+        # (YukawaSphere.cpp:27-29)
+        sigma = num("WCA_sigma", 1.0)
+        if "WCA_n" in b:
+            sigma = float(b["WCA_n"])
+        debye_length = num("debye_length")
+        rec.update(radius=num("radius"), epsilon=num("WCA_epsilon", 1.0), sigma=sigma, n=6, debye_length=debye_length,
+                   debye_A=num("debye_A"), cutoff=num("cutoff", 4.0 * debye_length), center=vec("center", (0.0, 0.0, 0.0)))
+    else:
+        raise ValueError(f"external force type {t!r} is not supported (supported: {', '.join(EXT_TYPES)})")
+    need("particle")
+    return rec, particles_from_string(b["particle"], s)
+
+
 def read_external_forces(path: str, s: System, as_json: bool = False) -> list:
     """The forces of an external_forces_file, expanded to one record per
-    particle in the file's order: dicts with the reference's keys (type,
-    particle, F0, rate, stiff, stiff_rate, r0, ref_particle, PBC, dir,
-    dir_as_centre, pos0, position, end_position, v, r_ext, center).  Types:
-    string, trap, mutual_trap, constant_trap, repulsion_plane, sphere; any
+    particle in the file's order: dicts whose keys are the fields of the C
+    record (type, particle, F0, rate, stiff, dir, center, ...), filled as
+    the reference's force classes read their keys.  Types: those the
+    reference's CUDA backend accepts (MD_CUDABackend::
+    _apply_external_forces_changes, src/CUDA/Backends/MD_CUDABackend.cu:
+    108-220) apart from the metadynamics ones, plus constant_trap; any
     other type raises."""
     with open(path) as f:
         text = f.read()
@@ -360,31 +493,7 @@ def read_external_forces(path: str, s: System, as_json: bool = False) -> list:
         t = b.get("type")
         if t not in EXT_TYPES:
             raise ValueError(f"external force type {t!r} is not supported (supported: {', '.join(EXT_TYPES)})")
-        rec = {"type": t}
-        for key in ("F0", "rate", "stiff", "stiff_rate", "r0", "position", "end_position", "v", "r_ext"):
-            if key in b:
-                rec[key] = float(b[key])
-        for key in ("dir", "pos0"):
-            if key in b:
-                rec[key] = _vec3(b[key])
-        if "center" in b:
-            rec["dir"] = _vec3(b["center"])
-        for key in ("PBC", "dir_as_centre"):
-            if key in b:
-                rec[key] = str(b[key]).strip().lower() in ("1", "true", "yes")
-        required = {"string": ("F0", "rate", "dir"), "trap": ("stiff", "rate", "dir", "pos0"),
-                    "mutual_trap": ("stiff", "r0"), "constant_trap": ("stiff", "r0"),
-                    "repulsion_plane": ("stiff", "position", "dir"), "sphere": ("stiff", "r0")}[t]
-        for key in required:
-            if key not in rec:
-                raise ValueError(f"external force {t}: missing key {key}")
-        if t in ("mutual_trap", "constant_trap"):
-            rec["ref_particle"] = int(b["ref_particle"])
-            pids = [int(b["particle"])]
-            if pids[0] == -1:
-                raise ValueError(f"{t} cannot act on every particle")
-        else:
-            pids = particles_from_string(b["particle"], s)
+        rec, pids = _ext_record(t, b, s)
         for pid in pids:
             r = dict(rec)
             r["particle"] = pid

@@ -1,7 +1,7 @@
 """Writes the external-force fixtures of tests/golden/external/ with the
 reference binary (run in the build container, after `make -f oracle/ref.mk`):
 
-    python tests/golden/make_external_golden.py
+    python tests/golden/make_external_golden.py [case names]
 
 Each case is a system, an external_forces_file in the reference's format and
 the step counter the configuration carries (kept with restart_step_counter
@@ -125,6 +125,164 @@ pos0 = 0.5, 0.5, 0.5
 """
 
 
+# one file per further type, each on the 20-bp duplex (centre near
+# (9.6, 9.7, 10), z from 6.1 to 13.9) at step 1000, where the rates act;
+# the parameters put a good share of the particles in each force's range
+FORCES_TYPES = {
+    "lowdim_trap": """{
+type = lowdim_trap
+particle = 0-9
+stiff = 1.2
+rate = 1e-4
+pos0 = 9.5, 9.5, 10
+dir = 0, 1, 1
+visibility = 1, 0, 1
+}
+""",
+    "attraction_plane": """{
+type = attraction_plane
+particle = -1
+stiff = 0.7
+position = -10.0
+dir = 0, 0.2, 1
+}
+""",
+    "repulsion_plane_moving": """{
+type = repulsion_plane_moving
+particle = 20-39
+ref_particle = 3-7
+stiff = 0.5
+dir = 0, 0, 1
+}
+""",
+    "sphere_smooth": """{
+type = sphere_smooth
+particle = all
+stiff = 1.5
+r0 = 1.0
+r_ext = 3.0
+center = 9.6, 9.7, 10.0
+}
+""",
+    "repulsive_sphere_moving": """{
+type = repulsive_sphere_moving
+particle = all
+stiff = 1.0
+r0 = 0.5
+rate = 1e-4
+origin = 11.9, 9.7, 7
+target = 11.9, 9.7, 13
+steps = 2000
+}
+""",
+    "repulsive_kepler_poinsot": """{
+type = repulsive_kepler_poinsot
+particle = all
+stiff = 0.003
+rate = 1e-4
+apex = 3.0
+base = 1.0
+base_radius = 0.3
+kappa = 10
+center = 9.6, 9.7, 10.0
+}
+{
+type = repulsive_kepler_poinsot
+particle = all
+stiff = 3e-5
+apex = 3.0
+base_radius = 0.35
+center = 9.6, 9.7, 8.0
+}
+""",
+    "LJ_wall": """{
+type = LJ_wall
+particle = all
+stiff = 0.5
+position = -5.0
+sigma = 1.0
+n = 6
+dir = 0, 0, 1
+}
+{
+type = LJ_wall
+particle = 0-19
+position = 15.0
+only_repulsive = true
+dir = 0, 0, -1
+}
+""",
+    "LJ_cone": """{
+type = LJ_cone
+particle = all
+stiff = 0.2
+alpha = 0.6
+n = 6
+dir = 0, 0, 1
+pos0 = 9.6, 9.7, 2.0
+}
+""",
+    "twist": """{
+type = twist
+particle = 5-8
+stiff = 1.0
+rate = 1e-3
+base = 0.3
+axis = 0, 0, 1
+center = 9.6, 9.7, 10
+pos0 = 11, 9.7, 10
+mask = 1, 1, 0
+}
+""",
+    "generic_central_force": """{
+type = generic_central_force
+particle = all
+force_type = gravity
+F0 = 0.25
+inner_cut_off = 1.0
+outer_cut_off = 3.5
+center = 9.6, 9.7, 11.0
+}
+""",
+    "ellipsoid": """{
+type = ellipsoid
+particle = all
+stiff = 0.6
+r_2 = 2.0, 2.0, 3.0
+center = 9.6, 9.7, 10.0
+}
+""",
+    "com": """{
+type = com
+com_list = 0-9
+ref_list = 30-39
+stiff = 1.0
+r0 = 1.5
+rate = 1e-4
+}
+{
+type = com
+com_list = 12, 14
+ref_list = 25-27
+stiff = 0.5
+r0 = 0.5
+}
+""",
+    "yukawa_sphere": """{
+type = yukawa_sphere
+particle = all
+radius = 7.0
+center = 9.6,9.7,10.0
+debye_length = 1.0
+debye_A = 2.0
+WCA_epsilon = 0.5
+WCA_sigma = 2.0
+WCA_n = 3
+}
+""",
+}
+
+
 def hairpin_at_corner():
     s = systems.hairpin_set(3, n=4)[1]
     s.pos = s.pos - s.pos.mean(axis=0) + np.array([0.5, 0.5, 0.5])
@@ -136,11 +294,16 @@ def cases():
     yield "ext_duplex_step0", d, "25C", 0.5, FORCES_A, 0
     yield "ext_duplex_step1000", d, "25C", 0.5, FORCES_A, 1000
     yield "ext_hairpin_pbc", hairpin_at_corner(), "40C", 0.5, FORCES_B, 250
+    for t, text in FORCES_TYPES.items():
+        yield f"ext_{t}_step1000", d, "25C", 0.5, text, 1000
 
 
-def main():
+def main(only=None):
+    """Writes every case, or those named in `only`."""
     os.makedirs(OUT, exist_ok=True)
     for name, s, T_str, salt, text, step in cases():
+        if only and name not in only:
+            continue
         E, F, T, stress, hb = mg.run_reference(s, T_str, salt, {"external_forces": "1", "external_forces_file": "ext.txt"},
                                                step=step, files={"ext.txt": text})
         E0, F0, _, _, _ = mg.run_reference(s, T_str, salt, step=step)
@@ -152,4 +315,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

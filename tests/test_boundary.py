@@ -34,22 +34,24 @@ def test_library_exports_every_declared_symbol():
     assert set(syms) == set(_lib.EXPORTS)
 
 
-def test_system_desc_layout_matches_c(tmp_path):
-    from oxdna_amd._lib import SystemDesc
-    fields = [f[0] for f in SystemDesc._fields_]
+@pytest.mark.parametrize("py_name, c_name", [("SystemDesc", "oxh_system_desc"), ("ExtForce", "oxh_ext_force")])
+def test_struct_layouts_match_c(tmp_path, py_name, c_name):
+    from oxdna_amd import _lib
+    cls = getattr(_lib, py_name)
+    fields = [f[0] for f in cls._fields_]
     src = tmp_path / "layout.c"
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{ROOT}/include/oxh.h"', "int main(void) {",
-             '  printf("%zu\\n", sizeof(oxh_system_desc));']
+             f'  printf("%zu\\n", sizeof({c_name}));']
     for f in fields:
-        lines.append(f'  printf("%zu\\n", offsetof(oxh_system_desc, {f}));')
+        lines.append(f'  printf("%zu\\n", offsetof({c_name}, {f}));')
     lines.append("  return 0; }")
     src.write_text("\n".join(lines))
     exe = tmp_path / "layout"
     subprocess.run(["gcc", str(src), "-o", str(exe)], check=True)
     out = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
-    assert out[0] == C.sizeof(SystemDesc)
+    assert out[0] == C.sizeof(cls)
     for f, off in zip(fields, out[1:]):
-        assert getattr(SystemDesc, f).offset == off, f
+        assert getattr(cls, f).offset == off, f
 
 
 def test_topology_formats_round_trip(tmp_path):

@@ -1,15 +1,20 @@
 """External forces on the GPU (oxh_batch_set_external_forces) against the
 reference binary: forces and torques of a state under every supported
 force type (string, trap, mutual_trap, constant_trap, repulsion_plane,
-sphere), with particle lists and ranges, rates at a nonzero step counter,
-nearest images and a moving plane's clamp (fixtures of
-tests/golden/make_external_golden.py: ForceDump of the reference's CPU
-backend, which adds the external forces to the interaction's).
+sphere, lowdim_trap, attraction_plane, repulsion_plane_moving,
+sphere_smooth, repulsive_sphere_moving, repulsive_kepler_poinsot, LJ_wall,
+LJ_cone, twist, generic_central_force, ellipsoid, com, yukawa_sphere), with
+particle lists and ranges, rates at a nonzero step counter, nearest images
+and a moving plane's clamp (fixtures of tests/golden/make_external_golden.py:
+ForceDump of the reference's CPU backend, which adds the external forces to
+the interaction's).
 
 Bars: the force tests' FP32 bar (1e-5 of the largest force, per particle
-1e-5) in the mixed form, 1e-9 in the all-FP64 form.  External forces carry
-no torque and no potential energy in the energy split, so torques and the
-split equal those of the interaction alone.
+1e-5) in the mixed form, 1e-9 in the all-FP64 form; in the all-FP64 form
+the external part alone (forces with minus forces without) within 1e-8 of
+its largest value.  External forces carry no torque and no potential
+energy in the energy split, so torques and the split equal those of the
+interaction alone.
 """
 import glob
 import os
@@ -69,6 +74,11 @@ def test_external_forces_vs_reference(hip, tmp_path, name, precision):
     assert np.all(np.abs(e8 - z["E8"]) <= tol * scale)
     # without the external forces the forces are the interaction's alone
     assert vec_rel_err(F0, z["F_pairs"])[0] <= tol
+    if precision == "double":
+        ext, ext_ref = F - F0, z["F"] - z["F_pairs"]
+        err = np.abs(ext - ext_ref).max() / np.abs(ext_ref).max()
+        print(f"{name}: external part {err:.2e}")
+        assert err <= 1e-8
 
 
 def test_external_forces_follow_steps_and_sorts(hip, tmp_path):
