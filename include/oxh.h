@@ -427,7 +427,10 @@ oxh_status oxh_batch_hb_count(oxh_batch *b, int *out);
 /* The current Verlet pair set of system `sys` as unordered pairs
  * (i < j, original indices), written to pairs (2 ints each) up to max
  * pairs; *n receives the total.  Exposes what the reference keeps in
- * CUDABaseList::d_matrix_neighs (src/CUDA/Lists/CUDABaseList.h:25-62). */
+ * CUDABaseList::d_matrix_neighs (src/CUDA/Lists/CUDABaseList.h:25-62):
+ * the same set, except that a system whose far pairs are kept by their
+ * backbone sites (oxh_batch_stats out[11]) leaves out the far pairs whose
+ * Debye-Hueckel term is zero until the next rebuild. */
 oxh_status oxh_batch_get_pairs(oxh_batch *b, int sys, int *pairs, long long max, long long *n);
 
 /* List statistics of system `sys`: rebuild count and the last error bits
@@ -453,14 +456,20 @@ oxh_status oxh_batch_set_state(oxh_batch *b, int sys, const double *pos, const d
  * reference's use_edge), out[9] number of systems, out[10] 1 when the
  * close kernel takes the packed form (HB, cross and coaxial stacking of a
  * workgroup's close pairs evaluated from work lists in LDS, one pair per
- * thread; same results bit for bit), else 0.
+ * thread; same results bit for bit), else 0, out[11] number of systems
+ * whose far pairs are kept by their backbone sites (far pairs then enter
+ * the list only while their backbone sites are within the Debye-Hueckel
+ * cut-off plus twice the verlet_skin, and a backbone site that moves more
+ * than the skin triggers a rebuild too: the same forces from a shorter
+ * list; OXH_FAR_BACK=0 keeps every far pair within r_verlet, as the
+ * reference's list does).
  * Shapes are picked by timing at upload (the all-FP64 mode times the lanes
  * only, in blocks of 64 with the far pairs inline; pools keep 8 lanes);
  * OXH_FORCE_LANES / OXH_FAR_LANES fix them, OXH_PERSIST=0 disables the
  * workgroup engine, OXH_EDGE=0 / 1 forbids / forces the half-list form,
  * OXH_PACK=0 / 1 forbids / forces the packed form (never used with one
  * lane per nucleotide). */
-oxh_status oxh_batch_stats(oxh_batch *b, double *out11);
+oxh_status oxh_batch_stats(oxh_batch *b, double *out12);
 
 /* Average duration in ms of n back-to-back launches of the force kernel on
  * the batch's stream, timed with HIP events (the reference times its steps

@@ -359,12 +359,12 @@ class HIPBatch:
         return ms.value
 
     def stats(self):
-        out = np.zeros(11)
+        out = np.zeros(12)
         check(lib().oxh_batch_stats(self.h, ptr(out)), "oxh_batch_stats")
         return dict(N=int(out[0]), max_neigh=int(out[1]), mean_neigh=out[2], device_bytes=int(out[3]),
                     lanes=int(out[4]), block=int(out[5]), far_lanes=int(out[6]), far_block=int(out[7]),
                     engine={0: "grid", 1: "workgroup", 2: "grid-edge"}[int(out[8])], n_systems=int(out[9]),
-                    pack=bool(out[10]))
+                    pack=bool(out[10]), far_back=int(out[11]))
 
 
 class HIPPool(HIPBatch):

@@ -58,6 +58,7 @@ struct oxh_batch {
 	long long fix_every = 0, since_fix = 0;
 	std::vector<long long> target, done_steps;
 	double4 *bk_rlist = nullptr;
+	float4 *bk_blist = nullptr;
 	// external forces of each system: (particle, record) in the caller's
 	// order
 	std::vector<std::vector<std::pair<int, oxh_ext_rec>>> ext_of;
@@ -412,6 +413,7 @@ oxh_status backup(oxh_batch *B) {
 	HIPCHK(hipMemcpyAsync(B->bk_pstep, b.pstep, N * sizeof(long long), hipMemcpyDeviceToDevice, b.stream));
 	HIPCHK(hipMemcpyAsync(B->bk_state, b.d_state, b.nsys * sizeof(oxh_sys_state), hipMemcpyDeviceToDevice, b.stream));
 	HIPCHK(hipMemcpyAsync(B->bk_rlist, b.rlist, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
+	if(b.blist) HIPCHK(hipMemcpyAsync(B->bk_blist, b.blist, N * sizeof(float4), hipMemcpyDeviceToDevice, b.stream));
 	return OXH_OK;
 }
 
@@ -431,6 +433,7 @@ oxh_status restore(oxh_batch *B) {
 	HIPCHK(hipMemcpyAsync(b.pstep, B->bk_pstep, N * sizeof(long long), hipMemcpyDeviceToDevice, b.stream));
 	HIPCHK(hipMemcpyAsync(b.d_state, B->bk_state, b.nsys * sizeof(oxh_sys_state), hipMemcpyDeviceToDevice, b.stream));
 	HIPCHK(hipMemcpyAsync(b.rlist, B->bk_rlist, N * sizeof(double4), hipMemcpyDeviceToDevice, b.stream));
+	if(b.blist) HIPCHK(hipMemcpyAsync(b.blist, B->bk_blist, N * sizeof(float4), hipMemcpyDeviceToDevice, b.stream));
 	oxh::launch_prepare_mirror(b.stream, b);
 	return sync_and_check(B);
 }
@@ -467,6 +470,25 @@ oxh_status rebuild_from_rlist(oxh_batch *B, int only = -1) {
 // Per-system parameters of desc d (validation, T-dependent model constants,
 // box, Verlet radii, integrator and thermostat); first / n / cells are the
 // caller's.  Returns r_verlet in rv.  state_arrays: the desc must carry the
+// Bounds of far pairs kept by their backbone sites (used where far_back is
+// set): a backbone site may move by `back_skin` (the Verlet skin, or
+// OXH_BACK_SKIN) from its place at the build before a rebuild; a far pair
+// is kept while its sites are closer than the Debye-Hueckel cut-off plus
+// twice that, plus a margin far above the FP32 error of wrapped sites in
+// this box
+void far_back_bounds(oxh_sys &S, double skin) {
+	double back_skin = skin;
+	if(const char *e = getenv("OXH_BACK_SKIN")) back_skin = atof(e);
+	const double lmax = std::max(S.box[0], std::max(S.box[1], S.box[2]));
+	const double margin = 1e-3 + std::ldexp(lmax, -18);
+	const double r = S.dh_rc_d + 2.0 * back_skin + margin;
+	S.rbf2 = (float) (r * r);
+	S.sback2f = (float) (back_skin * back_skin);
+	// sites of a close pair are within R_CLOSE + 2 skin + 2 backbone arms
+	const double rg = std::max(r, (double) oxh::R_CLOSE + 2.0 * skin + 2.0 * 0.4814 + margin);
+	S.rgate2f = (float) (rg * rg);
+}
+
 // state (pos, a1, a3) as well as the topology.  B (may be NULL) records
 // the Bussi step alignment of a batch.
 oxh_status sys_setup(oxh_batch *B, const oxh_system_desc &d, int s, oxh_sys &S, double &rv, bool state_arrays = true) {
@@ -505,6 +527,7 @@ oxh_status sys_setup(oxh_batch *B, const oxh_system_desc &d, int s, oxh_sys &S, 
 	S.small_box = 0;
 	for(int k = 0; k < 3; k++)
 		if(d.box[k] < 2.0 * (rv + 2.0 * 0.4814)) S.small_box = 1;
+	far_back_bounds(S, d.verlet_skin);
 	// integrator and thermostat (src/Backends/Thermostats/*.cpp)
 	S.dt = d.dt;
 	S.T = d.T;
@@ -1020,7 +1043,7 @@ oxh_status oxh_batch_destroy(oxh_batch *B) {
 			b.cell_count, b.cell_items, b.cell_owner, b.pos32, b.d_sys, b.d_state, b.red_blocks, b.red_sys_blocks, b.red_partial, b.red_out, b.red_done,
 			b.esplit, b.stress, b.hbcount, b.keys, b.keys_alt, b.perm, b.perm_alt, b.inv_perm, b.radix_hist, b.scratch, B->bk_pos, B->bk_vel,
 			B->bk_Lm, B->bk_q64, B->bk_F, B->bk_T, B->bk_pstep, B->bk_state, B->bk_rlist, b.limit, b.any_rebuild, b.F64, b.T64, B->bk_F64, B->bk_T64, b.eown, b.ein, b.ecount, b.E1, b.E2, b.E3, b.B1, b.B2, b.B3, b.cell_raw, b.cell_of, b.back64, b.img, b.slot_of, b.ext_off, b.ext_rec,
-			b.ext_grp, b.ext_grp_sys, b.ext_idx, b.ext_com};
+			b.ext_grp, b.ext_grp_sys, b.ext_idx, b.ext_com, b.blist, B->bk_blist};
 	for(void *p : ptrs)
 		if(p) (void) hipFree(p);
 	if(b.stream) (void) hipStreamDestroy(b.stream);
@@ -1279,6 +1302,24 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 		const char *e = getenv("OXH_EDGE");
 		if(!B->pool && !B->persist && !b.fp64 && !(e && atoi(e) == 0)) TRYB(alloc_edges(B, std::min(b.max_neigh, 64)));
 	}
+	// far pairs kept by their backbone sites: grid-engine batches (not
+	// pools) and systems with cells in boxes wide enough that backbone
+	// sites take the centres' nearest image (OXH_FAR_BACK=0 keeps every far
+	// pair within r_verlet)
+	{
+		const char *e = getenv("OXH_FAR_BACK");
+		bool any = false;
+		for(int s = 0; s < n_sys; s++) {
+			oxh_sys &S = b.h_sys[s];
+			S.far_back = (!B->pool && !B->persist && S.use_cells && !S.small_box && !(e && atoi(e) == 0)) ? 1 : 0;
+			any |= S.far_back != 0;
+		}
+		if(any) {
+			TRYB(dalloc(B, &b.blist, N));
+			TRYB(dalloc(B, &B->bk_blist, N));
+			if(!up(b.d_sys, b.h_sys.data(), n_sys * sizeof(oxh_sys))) return bail(fail(OXH_ERR_HIP, "system upload failed"));
+		}
+	}
 	TRYB(dalloc(B, &B->bk_rlist, N));
 	TRYB(dalloc(B, &b.any_rebuild, 1));
 	if(hipMemsetAsync(b.any_rebuild, 0, sizeof(int), st) != hipSuccess) return bail(fail(OXH_ERR_HIP, "memset failed"));
@@ -1348,7 +1389,8 @@ namespace {
 __global__ void k_load_caller_state(int N, const double4 *__restrict__ upos, const double4 *__restrict__ uq, const int *__restrict__ orig,
 		const int2 *__restrict__ pinfo, const int2 *__restrict__ bonds, const oxh_sys *__restrict__ systems,
 		const double4 *__restrict__ rlist, double4 *__restrict__ pos, double4 *__restrict__ q64, float4 *__restrict__ q32,
-		float4 *__restrict__ bpos, double4 *__restrict__ back64, oxh_sys_state *__restrict__ st, int *__restrict__ any) {
+		float4 *__restrict__ bpos, double4 *__restrict__ back64, oxh_sys_state *__restrict__ st, int *__restrict__ any,
+		const float4 *__restrict__ blist) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= N) return;
 	const int s = pinfo[i].y;
@@ -1360,10 +1402,10 @@ __global__ void k_load_caller_state(int N, const double4 *__restrict__ upos, con
 	const double4 q = uq[u];
 	pos[i] = r;
 	q64[i] = q;
-	oxh::write_mirror(i, r, q, S, bonds[i], q32, bpos, back64);
+	const float4 bp = oxh::write_mirror(i, r, q, S, bonds[i], q32, bpos, back64);
 	const double4 rl = rlist[i];
 	const double dx = r.x - rl.x, dy = r.y - rl.y, dz = r.z - rl.z;
-	if(dx * dx + dy * dy + dz * dz > S.skin2) {
+	if(dx * dx + dy * dy + dz * dz > S.skin2 || (S.far_back && blist && oxh::back_moved(bp, blist[i], S))) {
 		st[s].need_rebuild = 1;
 		*any = 1;
 	}
@@ -1410,7 +1452,7 @@ oxh_status oxh_interaction_compute_forces(oxh_batch *B, const double *d_pos, con
 	if(st != b.stream) HIPCHK(hipStreamSynchronize(b.stream));
 	const int bs = 256, g = (b.N + bs - 1) / bs;
 	k_load_caller_state<<<g, bs, 0, st>>>(b.N, (const double4 *) d_pos, (const double4 *) d_quat, b.orig, b.pinfo, b.bonds, b.d_sys,
-			b.rlist, b.pos, b.q64, b.q32, b.bpos, b.back64, b.d_state, b.any_rebuild);
+			b.rlist, b.pos, b.q64, b.q32, b.bpos, b.back64, b.d_state, b.any_rebuild, b.blist);
 	std::vector<oxh_sys_state> sst(b.nsys);
 	for(int attempt = 0;; attempt++) {
 		if(attempt == 12) return fail(OXH_ERR_SYSTEM, "neighbour list keeps overflowing");
@@ -2168,6 +2210,10 @@ oxh_status set_system_box(oxh_batch *B, int sys, const double L[3]) {
 	S.small_box = 0;
 	for(int k = 0; k < 3; k++)
 		if(L[k] < 2.0 * (rv + 2.0 * 0.4814)) S.small_box = 1;
+	// the lists are rebuilt after a box change, so the far-pair filter can
+	// change with it: off in a small box, its margin follows the box
+	if(S.small_box) S.far_back = 0;
+	far_back_bounds(S, std::sqrt(S.skin2));
 	if(S.use_cells) {
 		for(int k = 0; k < 3; k++) {
 			const int fit = (int) std::floor(L[k] * S.stencil / rv);
@@ -2433,7 +2479,7 @@ oxh_status oxh_batch_box(oxh_batch *B, int sys, double *box3) {
 	return OXH_OK;
 }
 
-oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {  // 11 doubles
+oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {  // 12 doubles
 	if(!B || !out6) return fail(OXH_ERR_ARG, "bad arguments");
 	HIPCHK(hipSetDevice(B->b.device));
 	std::vector<int2> nn;
@@ -2457,6 +2503,9 @@ oxh_status oxh_batch_stats(oxh_batch *B, double *out6) {  // 11 doubles
 	out6[8] = B->persist ? 1.0 : (B->b.edge ? 2.0 : 0.0);
 	out6[9] = B->b.nsys;
 	out6[10] = B->b.pack ? 1.0 : 0.0;
+	int far_back = 0;
+	for(const oxh_sys &S : B->b.h_sys) far_back += S.far_back;
+	out6[11] = far_back;
 	return OXH_OK;
 }
 
@@ -2506,6 +2555,7 @@ std::vector<std::pair<void *, size_t>> ckpt_layout(oxh_batch *B) {
 		a.push_back({b.F64, N * sizeof(double4)});
 		a.push_back({b.T64, N * sizeof(double4)});
 	}
+	if(b.blist) a.push_back({b.blist, N * sizeof(float4)});
 	return a;
 }
 }  // namespace

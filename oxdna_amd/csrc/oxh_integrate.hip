@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(256) k_first_step(int N, double4 *__restrict__
 		double4 *__restrict__ q64, float4 *__restrict__ q32, float4 *__restrict__ bpos, double4 *__restrict__ back64,
 		const FT *__restrict__ F, const FT *__restrict__ T, const double4 *__restrict__ rlist, const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo,
 		long long *__restrict__ pstep, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st, int ncell,
-		int *__restrict__ cell_count, int k, const int *__restrict__ limit, int *__restrict__ any) {
+		int *__restrict__ cell_count, int k, const int *__restrict__ limit, int *__restrict__ any, const float4 *__restrict__ blist) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	// cell counters start every step at zero; they are only filled (after
 	// this kernel) in steps that rebuild the lists
@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(256) k_first_step(int N, double4 *__restrict__
 	const int s = pinfo[i].y;
 	const oxh_sys &S = systems[s];
 	if(idle(i, S, s, k, limit)) return;
-	if(first_step_particle(i, S, pos, vel, Lm, q64, q32, bpos, back64, F, T, rlist, bonds, pstep)) {
+	if(first_step_particle(i, S, pos, vel, Lm, q64, q32, bpos, back64, F, T, rlist, bonds, pstep, blist)) {
 		st[s].need_rebuild = 1;
 		*any = 1;
 	}
@@ -60,10 +60,10 @@ void launch_first_step(hipStream_t st, const Batch &b, int k) {
 	int n = std::max(b.N, b.ncell_total);
 	if(b.fp64)
 		k_first_step<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.back64, b.F64, b.T64, b.rlist, b.bonds, b.pinfo,
-				b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild);
+				b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild, b.blist);
 	else
 		k_first_step<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.back64, b.F, b.T, b.rlist, b.bonds, b.pinfo,
-				b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild);
+				b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild, b.blist);
 }
 
 // second half kick + Langevin / Brownian thermostats; the first particle
@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(256) k_second_first(int N, double4 *__restrict
 		double4 *__restrict__ back64, const FT *__restrict__ F, const FT *__restrict__ T, const double4 *__restrict__ rlist,
 		const int2 *__restrict__ bonds, const int2 *__restrict__ pinfo, const int *__restrict__ orig, long long *__restrict__ pstep, const oxh_sys *__restrict__ systems,
 		oxh_sys_state *__restrict__ st, int ncell, int *__restrict__ cell_count, int k, const int *__restrict__ limit,
-		int *__restrict__ any) {
+		int *__restrict__ any, const float4 *__restrict__ blist) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i < ncell) cell_count[i] = 0;
 	if(i >= N) return;
@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(256) k_second_first(int N, double4 *__restrict
 	if(padding(i, S)) return;
 	if(!(limit && k >= limit[s])) second_step_particle(i, S, vel, Lm, F, T, orig, pstep);
 	if(!(limit && k + 1 >= limit[s])) {
-		if(first_step_particle(i, S, pos, vel, Lm, q64, q32, bpos, back64, F, T, rlist, bonds, pstep)) {
+		if(first_step_particle(i, S, pos, vel, Lm, q64, q32, bpos, back64, F, T, rlist, bonds, pstep, blist)) {
 			st[s].need_rebuild = 1;
 			*any = 1;
 		}
@@ -120,10 +120,10 @@ void launch_second_first(hipStream_t st, const Batch &b, int k) {
 	int n = std::max(b.N, b.ncell_total);
 	if(b.fp64)
 		k_second_first<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.back64, b.F64, b.T64, b.rlist, b.bonds,
-				b.pinfo, b.orig, b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild);
+				b.pinfo, b.orig, b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild, b.blist);
 	else
 		k_second_first<<<(n + 255) / 256, 256, 0, st>>>(b.N, b.pos, b.vel, b.Lm, b.q64, b.q32, b.bpos, b.back64, b.F, b.T, b.rlist, b.bonds, b.pinfo,
-				b.orig, b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild);
+				b.orig, b.pstep, b.d_sys, b.d_state, b.ncell_total, b.cell_count, k, b.limit, b.any_rebuild, b.blist);
 }
 
 // -------------------------------------------------------- FP64 reductions

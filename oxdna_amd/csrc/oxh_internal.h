@@ -96,6 +96,9 @@ struct Batch {
 
 	// external forces: CSR over (system first + original index), N + 1
 	// offsets, records in the order the caller gave them per particle
+	// backbone sites at the last list build (systems with far_back), the
+	// bound of their moves before a rebuild
+	float4 *blist = nullptr;
 	int n_ext = 0;
 	int *ext_off = nullptr;
 	oxh_ext_rec *ext_rec = nullptr;

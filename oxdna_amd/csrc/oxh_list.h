@@ -27,6 +27,23 @@ __device__ __forceinline__ double sqr_dist(double4 a, double4 b, const oxh_sys &
 
 constexpr int BUILD_LANES = 32;
 
+// Far pairs kept by their backbone sites (S.far_back): a far pair enters
+// the list only while the backbone sites recorded at the build (blist:
+// wrapped FP32 sites) are within sqrt(rbf2).  Far pairs interact through
+// Debye-Hueckel alone, between backbone sites and within its cut-off; rbf2
+// adds twice the largest site move allowed before a rebuild, so a pair
+// left out stays beyond the cut-off, with a zero term, until the next
+// build.  The sites' distance is taken first: beyond sqrt(rgate2f) (which
+// no close pair's sites reach) a candidate is no pair, and only the rest
+// have their centres compared.
+__device__ __forceinline__ float back_dist2(float4 a, float4 b, const oxh_sys &S) {
+	float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+	dx -= S.boxf[0] * rintf(dx * S.inv_boxf[0]);
+	dy -= S.boxf[1] * rintf(dy * S.inv_boxf[1]);
+	dz -= S.boxf[2] * rintf(dz * S.inv_boxf[2]);
+	return dx * dx + dy * dy + dz * dz;
+}
+
 // The pair predicate of the lists: candidate j of particle i (FP64 centres
 // p, pj; FP32 wrapped centres p32, q for the prefilter when from_cells) is a
 // close pair (closer than sqrt(rclose2)) or a far pair (closer than r_verlet).
@@ -83,12 +100,14 @@ template<int K>
 __device__ __forceinline__ void build_row(int i, int l, int s, const double4 *__restrict__ pos, const int2 *__restrict__ bonds,
 		const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st, const int *__restrict__ count,
 		const int *__restrict__ items, int cap, int *__restrict__ nbr, int2 *__restrict__ nnbr, int max_neigh,
-		double4 *__restrict__ rlist, const float4 *__restrict__ pos32) {
+		double4 *__restrict__ rlist, const float4 *__restrict__ pos32, const float4 *__restrict__ blist) {
 	const oxh_sys &S = systems[s];
 	const double4 p = pos[i];
 	const int2 bd = bonds[i];
+	const bool back = S.far_back && blist;
 	int *row = nbr + (size_t) i * max_neigh;
 	const float4 p32 = S.use_cells ? pos32[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+	const float4 bi = back ? blist[i] : make_float4(0.f, 0.f, 0.f, 0.f);
 	const int g0 = (threadIdx.x & 63) & ~(K - 1);          // first lane of this group in the wave
 	const uint64_t gmask = (K == 64) ? ~0ull : (((1ull << K) - 1ull) << g0);
 	const uint64_t below = ((1ull << (threadIdx.x & 63)) - 1ull) & gmask;
@@ -97,8 +116,13 @@ __device__ __forceinline__ void build_row(int i, int l, int s, const double4 *__
 	// FP32 prefilter on wrapped centres
 	auto test = [&](int j, bool from_cells) {
 		bool close = false, far = false;
-		if(j >= 0 && j != i && j != bd.x && j != bd.y)
-			classify_pair(S, p, p32, from_cells ? pos32[j] : make_float4(0.f, 0.f, 0.f, 0.f), pos, j, from_cells, close, far);
+		if(j >= 0 && j != i && j != bd.x && j != bd.y) {
+			const float db2 = back ? back_dist2(bi, blist[j], S) : 0.f;
+			if(!back || db2 < S.rgate2f) {
+				classify_pair(S, p, p32, from_cells ? pos32[j] : make_float4(0.f, 0.f, 0.f, 0.f), pos, j, from_cells, close, far);
+				if(far && back && !(db2 < S.rbf2)) far = false;
+			}
+		}
 		const uint64_t bc = __ballot(close) & gmask, bf = __ballot(far) & gmask;
 		if(close) {
 			int k = nc + __popcll(bc & below);

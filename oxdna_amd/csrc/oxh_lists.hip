@@ -35,7 +35,7 @@ __global__ void k_cells_clear(int ncell, const int *__restrict__ owner, const ox
 // particle's cell in cell_of, its wrapped FP32 centre in pos32.
 __global__ void k_cells_fill(int N, const double4 *__restrict__ pos, const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems,
 		oxh_sys_state *__restrict__ st, int *__restrict__ count, int *__restrict__ raw, int *__restrict__ cell_of, int cap,
-		float4 *__restrict__ pos32, const int *__restrict__ any) {
+		float4 *__restrict__ pos32, const int *__restrict__ any, const float4 *__restrict__ bpos, float4 *__restrict__ blist) {
 	if(!*any) return;
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= N) return;
@@ -51,6 +51,9 @@ __global__ void k_cells_fill(int N, const double4 *__restrict__ pos, const int2 
 	pos32[i] = make_float4((float) (p.x - S.box[0] * floor(p.x * S.inv_box[0])), (float) (p.y - S.box[1] * floor(p.y * S.inv_box[1])),
 			(float) (p.z - S.box[2] * floor(p.z * S.inv_box[2])), 0.f);
 	cell_of[i] = c;
+	// a build from the current state records the backbone sites it
+	// filters far pairs with (a rebuild from rlist keeps those of its build)
+	if(blist && S.far_back) blist[i] = bpos[i];
 	int slot = atomicAdd(&count[c], 1);
 	if(slot < cap) raw[(size_t) c * cap + slot] = i;
 	else atomicOr(&st[s].error, ERR_CELLS);
@@ -87,7 +90,7 @@ void launch_cells_fill_sort(hipStream_t st, const Batch &b) {
 	if(!b.any_cells) return;
 	int bs = 256;
 	k_cells_fill<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, b.pos, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_raw, b.cell_of,
-			b.cell_cap, b.pos32, b.any_rebuild);
+			b.cell_cap, b.pos32, b.any_rebuild, b.bpos, b.blist);
 	k_cells_rank<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_raw, b.cell_of, b.cell_items,
 			b.cell_cap, b.any_rebuild);
 }
@@ -97,7 +100,7 @@ void launch_cells(hipStream_t st, const Batch &b, const double4 *src) {
 	int bs = 256;
 	k_cells_clear<<<(b.ncell_total + bs - 1) / bs, bs, 0, st>>>(b.ncell_total, b.cell_owner, b.d_state, b.cell_count);
 	k_cells_fill<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, src, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_raw, b.cell_of,
-			b.cell_cap, b.pos32, b.any_rebuild);
+			b.cell_cap, b.pos32, b.any_rebuild, b.bpos, src == b.pos ? b.blist : nullptr);
 	k_cells_rank<<<(b.N + bs - 1) / bs, bs, 0, st>>>(b.N, b.pinfo, b.d_sys, b.d_state, b.cell_count, b.cell_raw, b.cell_of, b.cell_items,
 			b.cell_cap, b.any_rebuild);
 }
@@ -105,7 +108,8 @@ void launch_cells(hipStream_t st, const Batch &b, const double4 *src) {
 __global__ void __launch_bounds__(256) k_build_list(int N, const double4 *__restrict__ pos, const int2 *__restrict__ bonds,
 		const int2 *__restrict__ pinfo, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st,
 		const int *__restrict__ count, const int *__restrict__ items, int cap, int *__restrict__ nbr, int2 *__restrict__ nnbr,
-		int max_neigh, double4 *__restrict__ rlist, int force_all, const float4 *__restrict__ pos32, const int *__restrict__ any) {
+		int max_neigh, double4 *__restrict__ rlist, int force_all, const float4 *__restrict__ pos32, const int *__restrict__ any,
+		const float4 *__restrict__ blist) {
 	constexpr int K = BUILD_LANES;
 	if(!force_all && !*any) return;
 	// grid-stride over the K-lane groups: a step without a rebuild only
@@ -118,7 +122,7 @@ __global__ void __launch_bounds__(256) k_build_list(int N, const double4 *__rest
 		const int s = pinfo[i].y;
 		if(!force_all && !st[s].need_rebuild) continue;
 		if(padding(i, systems[s])) continue;
-		build_row<K>(i, l, s, pos, bonds, systems, st, count, items, cap, nbr, nnbr, max_neigh, rlist, pos32);
+		build_row<K>(i, l, s, pos, bonds, systems, st, count, items, cap, nbr, nnbr, max_neigh, rlist, pos32, blist);
 	}
 }
 
@@ -201,11 +205,17 @@ __global__ void __launch_bounds__(256) k_edge_rows(int N, const int2 *__restrict
 // back to build_row.
 constexpr int CELL_CMAX = 2048;
 
+// BACK: some system keeps far pairs by backbone sites; for such systems
+// the candidates' recorded backbone sites are staged instead of their
+// centres, and a candidate's centre is read only when its site passes.
+template<bool BACK>
 __global__ void __launch_bounds__(256) k_build_cells(int ncell_total, const double4 *__restrict__ pos, const int2 *__restrict__ bonds,
 		const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ st, const int *__restrict__ cell_owner,
 		const int *__restrict__ count, const int *__restrict__ items, int cap, int *__restrict__ nbr, int2 *__restrict__ nnbr,
-		int max_neigh, double4 *__restrict__ rlist, int force_all, const float4 *__restrict__ pos32, const int *__restrict__ any) {
-	__shared__ int cid[CELL_CMAX];
+		int max_neigh, double4 *__restrict__ rlist, int force_all, const float4 *__restrict__ pos32, const int *__restrict__ any,
+		const float4 *__restrict__ blist) {
+	// candidates: wrapped FP32 centre (or backbone site) with the
+	// particle index in w
 	__shared__ float4 cpos[CELL_CMAX];
 	__shared__ int ccell[128], cbase[129];
 	__shared__ int nkept;
@@ -263,7 +273,7 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell_total, const doub
 		if(M > CELL_CMAX) {
 			for(int h = tid / BUILD_LANES; h < nh; h += blockDim.x / BUILD_LANES)
 				build_row<BUILD_LANES>(items[(size_t) c * cap + h], tid % BUILD_LANES, s, pos, bonds, systems, st, count, items, cap, nbr,
-						nnbr, max_neigh, rlist, pos32);
+						nnbr, max_neigh, rlist, pos32, blist);
 			__syncthreads();
 			continue;
 		}
@@ -275,8 +285,9 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell_total, const doub
 				else hi = mid;
 			}
 			const int j = items[(size_t) ccell[lo] * cap + (m - cbase[lo])];
-			cid[m] = j;
-			cpos[m] = pos32[j];
+			float4 q = (BACK && S.far_back) ? blist[j] : pos32[j];
+			q.w = __int_as_float(j);
+			cpos[m] = q;
 		}
 		__syncthreads();
 		for(int h = w; h < nh; h += blockDim.x / 64) {
@@ -284,6 +295,8 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell_total, const doub
 			const double4 p = pos[i];
 			const float4 p32 = pos32[i];
 			const int2 bd = bonds[i];
+			const bool back = BACK && S.far_back;
+			const float4 bi = back ? blist[i] : make_float4(0.f, 0.f, 0.f, 0.f);
 			int *row = nbr + (size_t) i * max_neigh;
 			int ncl = 0, nfa = 0;
 			for(int m0 = 0; m0 < M; m0 += 64) {
@@ -291,8 +304,18 @@ __global__ void __launch_bounds__(256) k_build_cells(int ncell_total, const doub
 				bool close = false, far = false;
 				int j = -1;
 				if(m < M) {
-					j = cid[m];
-					if(j != i && j != bd.x && j != bd.y) classify_pair(S, p, p32, cpos[m], pos, j, true, close, far);
+					const float4 q = cpos[m];
+					j = __float_as_int(q.w);
+					if(j != i && j != bd.x && j != bd.y) {
+						if(!back) classify_pair(S, p, p32, q, pos, j, true, close, far);
+						else {
+							const float db2 = back_dist2(bi, q, S);
+							if(db2 < S.rgate2f) {
+								classify_pair(S, p, p32, pos32[j], pos, j, true, close, far);
+								if(far && !(db2 < S.rbf2)) far = false;
+							}
+						}
+					}
 				}
 				const uint64_t bc = __ballot(close), bf = __ballot(far);
 				if(close) {
@@ -325,13 +348,18 @@ void launch_build_list(hipStream_t st, const Batch &b, int force_all, const doub
 	int bs = 256;
 	size_t threads = (size_t) b.N * BUILD_LANES;
 	const unsigned grid = (unsigned) std::min<size_t>((threads + bs - 1) / bs, 2048);
-	if(b.cell_build)
-		k_build_cells<<<(unsigned) std::min(b.ncell_total, 4096), 256, 0, st>>>(b.ncell_total, src, b.bonds, b.d_sys, b.d_state,
+	if(b.cell_build && b.blist)
+		k_build_cells<true><<<(unsigned) std::min(b.ncell_total, 4096), 256, 0, st>>>(b.ncell_total, src, b.bonds, b.d_sys, b.d_state,
 				b.cell_owner, b.cell_count, b.cell_items, b.cell_cap, b.nbr, b.nnbr, b.max_neigh, src == b.rlist ? nullptr : b.rlist,
-				force_all, b.pos32, b.any_rebuild);
+				force_all, b.pos32, b.any_rebuild, b.blist);
+	else if(b.cell_build)
+		k_build_cells<false><<<(unsigned) std::min(b.ncell_total, 4096), 256, 0, st>>>(b.ncell_total, src, b.bonds, b.d_sys, b.d_state,
+				b.cell_owner, b.cell_count, b.cell_items, b.cell_cap, b.nbr, b.nnbr, b.max_neigh, src == b.rlist ? nullptr : b.rlist,
+				force_all, b.pos32, b.any_rebuild, nullptr);
 	else
 		k_build_list<<<grid, bs, 0, st>>>(b.N, src, b.bonds, b.pinfo, b.d_sys, b.d_state, b.cell_count,
-				b.cell_items, b.cell_cap, b.nbr, b.nnbr, b.max_neigh, src == b.rlist ? nullptr : b.rlist, force_all, b.pos32, b.any_rebuild);
+				b.cell_items, b.cell_cap, b.nbr, b.nnbr, b.max_neigh, src == b.rlist ? nullptr : b.rlist, force_all, b.pos32, b.any_rebuild,
+				b.blist);
 	if(b.ecap > 0)
 		k_edge_rows<<<grid, bs, 0, st>>>(b.N, b.pinfo, b.d_sys, b.d_state, b.nbr, b.nnbr, b.max_neigh, b.ecap, b.eown, b.ein, b.ecount,
 				force_all, b.any_rebuild);

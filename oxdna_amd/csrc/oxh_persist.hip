@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(BLOCK) k_persist(PersistArgs a) {
 	__syncthreads();
 	for(int it = 0; it < nsteps; it++) {
 		// 1. first half kick, drift, rotation, FP32 mirror, skin check
-		if(tid < n && first_step_particle(first + tid, S, a.pos, a.vel, a.Lm, a.q64, a.q32, a.bpos, nullptr, a.F, a.T, a.rlist, a.bonds, a.pstep))
+		if(tid < n && first_step_particle(first + tid, S, a.pos, a.vel, a.Lm, a.q64, a.q32, a.bpos, nullptr, a.F, a.T, a.rlist, a.bonds, a.pstep, nullptr))
 			need = 1;
 		__syncthreads();
 		// 2. Verlet rebuild (all pairs of the system; such systems use no cells)
@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(BLOCK) k_persist(PersistArgs a) {
 		if(rebuild) {
 			for(int u = tid; u < n * BUILD_LANES; u += BLOCK)
 				build_row<BUILD_LANES>(first + u / BUILD_LANES, u % BUILD_LANES, s, a.pos, a.bonds, a.systems, a.state, a.cell_count,
-						a.cell_items, a.cell_cap, a.nbr, a.nnbr, a.max_neigh, a.rlist, a.pos32);
+						a.cell_items, a.cell_cap, a.nbr, a.nnbr, a.max_neigh, a.rlist, a.pos32, nullptr);
 			__syncthreads();
 		}
 		// 3. forces and torques, K lanes per nucleotide, far pairs inline
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(BLOCK) k_persist_lds(PersistArgs a, int nmax) 
 	__syncthreads();
 	for(int it = 0; it < nsteps; it++) {
 		// 1. first half kick, drift, rotation, FP32 mirror, skin check
-		if(tid < n && first_step_particle(tid, S, L.pos, L.vel, L.Lm, L.q64, L.q32, L.bpos, nullptr, L.F, L.T, L.rlist, L.bonds, L.pstep))
+		if(tid < n && first_step_particle(tid, S, L.pos, L.vel, L.Lm, L.q64, L.q32, L.bpos, nullptr, L.F, L.T, L.rlist, L.bonds, L.pstep, nullptr))
 			need = 1;
 		__syncthreads();
 		// 2. Verlet rebuild (all pairs of the system)
@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(BLOCK) k_persist_lds(PersistArgs a, int nmax) 
 		if(rebuild) {
 			for(int u = tid; u < n * BUILD_LANES; u += BLOCK)
 				build_row<BUILD_LANES>(u / BUILD_LANES, u % BUILD_LANES, 0, L.pos, L.bonds, &S, st, a.cell_count, a.cell_items,
-						a.cell_cap, L.nbr, L.nnbr, stride, L.rlist, a.pos32);
+						a.cell_cap, L.nbr, L.nnbr, stride, L.rlist, a.pos32, nullptr);
 			__syncthreads();
 		}
 		// 3. forces and torques, K lanes per nucleotide, far pairs inline

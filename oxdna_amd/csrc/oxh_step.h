@@ -32,7 +32,7 @@ __device__ __forceinline__ double4 qmul(double4 a, double4 b) {
 // dh_half_charged_ends, src/Interactions/DNA2Interaction.cpp:162-165)
 // back64 (all-FP64 mode, else nullptr): the FP64 backbone offset
 // POS_MM_BACK1 a1 + POS_MM_BACK2 a2 of the particle, for the FP64 far pairs
-__device__ __forceinline__ void write_mirror(int i, double4 r, double4 q, const oxh_sys &S, int2 bd, float4 *q32, float4 *bpos,
+__device__ __forceinline__ float4 write_mirror(int i, double4 r, double4 q, const oxh_sys &S, int2 bd, float4 *q32, float4 *bpos,
 		double4 *back64) {
 	q32[i] = make_float4((float) q.x, (float) q.y, (float) q.z, (float) q.w);
 	double sqx = q.x * q.x, sqy = q.y * q.y, sqz = q.z * q.z, sqw = q.w * q.w;
@@ -45,18 +45,33 @@ __device__ __forceinline__ void write_mirror(int i, double4 r, double4 q, const 
 	bx -= S.box[0] * floor(bx * S.inv_box[0]);
 	by -= S.box[1] * floor(by * S.inv_box[1]);
 	bz -= S.box[2] * floor(bz * S.inv_box[2]);
-	bpos[i] = make_float4((float) bx, (float) by, (float) bz, cf);
+	const float4 bp = make_float4((float) bx, (float) by, (float) bz, cf);
+	bpos[i] = bp;
 	if(back64) back64[i] = make_double4(b1 * a1x + b2 * a2x, b1 * a1y + b2 * a2y, b1 * a1z + b2 * a2z, 0.0);
+	return bp;
+}
+
+// With far pairs kept by their backbone sites: true when the wrapped FP32
+// backbone site b has moved more than sqrt(sback2f) from blist, its place
+// at the last list build (nearest image, so wrapping does not count)
+__device__ __forceinline__ bool back_moved(float4 b, float4 bl, const oxh_sys &S) {
+	float dx = b.x - bl.x, dy = b.y - bl.y, dz = b.z - bl.z;
+	dx -= S.boxf[0] * rintf(dx * S.inv_boxf[0]);
+	dy -= S.boxf[1] * rintf(dy * S.inv_boxf[1]);
+	dz -= S.boxf[2] * rintf(dz * S.inv_boxf[2]);
+	return dx * dx + dy * dy + dz * dz > S.sback2f;
 }
 
 // first half kick, drift and rotation of particle i, FP32 mirror, step
 // counter; returns true when the particle moved more than the skin since
-// the last list build (src/CUDA/Backends/CUDA_mixed.cuh:23-69)
+// the last list build, or its backbone site more than its bound when far
+// pairs are kept by backbone sites (blist: the sites at the build; may be
+// null) (src/CUDA/Backends/CUDA_mixed.cuh:23-69)
 template<class FT>
 __device__ __forceinline__ bool first_step_particle(int i, const oxh_sys &S, double4 *__restrict__ pos, double4 *__restrict__ vel,
 		double4 *__restrict__ Lm, double4 *__restrict__ q64, float4 *__restrict__ q32, float4 *__restrict__ bpos,
 		double4 *__restrict__ back64, const FT *__restrict__ F, const FT *__restrict__ T, const double4 *__restrict__ rlist,
-		const int2 *__restrict__ bonds, long long *__restrict__ pstep) {
+		const int2 *__restrict__ bonds, long long *__restrict__ pstep, const float4 *__restrict__ blist) {
 	const double hdt = 0.5 * S.dt;
 
 	const FT f = F[i];
@@ -91,11 +106,12 @@ __device__ __forceinline__ bool first_step_particle(int i, const oxh_sys &S, dou
 		q.w *= qn;
 		q64[i] = q;
 	}
-	write_mirror(i, r, q, S, bonds[i], q32, bpos, back64);
+	const float4 bp = write_mirror(i, r, q, S, bonds[i], q32, bpos, back64);
 	pstep[i] += 1;
 
 	double4 rl = rlist[i];
 	double dx = r.x - rl.x, dy = r.y - rl.y, dz = r.z - rl.z;
+	if(S.far_back && blist && back_moved(bp, blist[i], S)) return true;
 	return dx * dx + dy * dy + dz * dz > S.skin2;
 }
 

@@ -37,6 +37,16 @@ typedef struct oxh_sys {
 	// FP32 squared distances within eps2 of rv2 or rclose2 are re-decided in
 	// FP64 (bound on the FP32 rounding error of wrapped coordinates)
 	float rv2f, rclose2f, eps2;
+	// far pairs kept by their backbone sites (far_back = 1): a far pair
+	// enters the list only while its backbone sites at the build are
+	// closer than sqrt(rbf2), the Debye-Hueckel cut-off plus twice the
+	// largest backbone-site move allowed before a rebuild plus a margin
+	// above the FP32 error; a site that moves more than sqrt(sback2f) from
+	// its place at the build triggers a rebuild.  Candidates whose sites
+	// are farther apart than sqrt(rgate2f) (at least sqrt(rbf2), and beyond
+	// the sites' reach of any close pair) are no pair at all.
+	int far_back;
+	float rbf2, sback2f, rgate2f;
 	// model constants (FP32) that depend on T / salt / options
 	float f1_eps[2][5][5];
 	float f1_shift[2][5][5];
@@ -69,15 +79,39 @@ typedef struct oxh_sys {
 // One external force acting on one particle (the reference's
 // external_forces_file entries, src/Forces/*.cpp, expanded to one record
 // per particle and stored sorted by particle, in the file's order).
-// Parameters by type (OXH_EXT_* in include/oxh.h):
+// Parameters by type (OXH_EXT_* in include/oxh.h; unit = normalised):
 //   string:          a = F0, b = rate, v1 = dir (unit, or the centre when
 //                    flags & 2: dir_as_centre)
 //   trap:            a = stiff, b = rate, v1 = dir (unit), v2 = pos0
+//   lowdim_trap:     as trap; flags bits 0-2 = visibility of x, y, z
 //   mutual_trap:     a = stiff, b = stiff_rate, c = r0, d = rate, ref
 //   constant_trap:   a = stiff, c = r0, ref
 //   repulsion_plane: a = stiff, b = position, c = end_position, d = v,
 //                    v1 = dir (unit)
+//   attraction_plane: a = stiff, b = position, v1 = dir (unit)
+//   repulsion_plane_moving: a = stiff, v1 = dir (unit), reference
+//                    particles ref..ref2
 //   sphere:          a = stiff, b = rate, c = r0, d = r_ext, v1 = center
+//   sphere_smooth:   a = stiff, c = r0, d = r_ext, e = smooth, f = alpha,
+//                    v1 = center
+//   repulsive_sphere_moving: a = stiff, b = rate, c = r0, d = r_ext,
+//                    v1 = origin, v2 = target, steps
+//   repulsive_kepler_poinsot: a = stiff, b = rate, c = apex, d = base,
+//                    e = base_radius, f = kappa, v1 = center
+//   LJ_wall:         a = stiff, b = position, c = sigma, d = cutoff, n,
+//                    v1 = dir (unit)
+//   LJ_cone:         a = stiff, c = sigma, d = cutoff, e/f/g = sin/cos/tan
+//                    of alpha, n, v1 = dir (unit), v2 = pos0
+//   twist:           a = stiff, b = rate, c = base, v1 = axis (unit),
+//                    v2 = pos0, v3 = center, v4 = mask
+//   generic_central_force: a = F0, c = inner cut-off, d = outer cut-off,
+//                    v1 = center
+//   ellipsoid:       a = stiff, v1 = center, v2 = r_1, v3 = r_2
+//   com:             a = stiff, b = rate, c = r0, ref = centre group,
+//                    n = particles in the com list
+//   yukawa_sphere:   a = radius, b = WCA epsilon, c = WCA sigma, d = Debye
+//                    length, e = Debye amplitude, f = cutoff, g = WCA
+//                    cutoff, n = WCA n, v1 = center
 // ref is the partner's index inside the system (upload order); flags & 1:
 // nearest image between the two particles (PBC).
 typedef struct oxh_ext_rec {

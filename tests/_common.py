@@ -54,3 +54,45 @@ def vec_rel_err(got, ref):
     big = m.max() if m.size else 1.0
     floor = 0.1 * np.sqrt(np.mean(m ** 2)) if m.size else 1.0
     return float(d.max() / big), float(np.max(d / np.maximum(m, floor)))
+
+
+R_CLOSE = 1.676   # close-pair bound of the lists (oxdna_amd/csrc/oxh_device.h)
+
+
+def check_pair_set(got, t, P, skin=0.05, far_back=False, po=None):
+    """A GPU pair set (unordered, i < j) against the FP64 Verlet predicate
+    of the oracle (every pair with centres closer than r_cut + 2 skin).
+    Without far_back the two are equal.  With far_back the list keeps the
+    close pairs (centres within R_CLOSE + 2 skin) and those far pairs whose
+    backbone sites are within the Debye-Hueckel cut-off plus 2 skin: pairs
+    nearer than that bound must be present, pairs farther than it plus the
+    filter's margin (1e-3 + 2^-18 of the longest box side, far above the
+    FP32 error of wrapped sites) plus 1e-4 must be absent.
+    Returns (predicate pairs, pairs kept)."""
+    if po is None:
+        from oracle import pyoracle as po
+    rv = P.rcut + 2 * skin
+    want = po.verlet_pairs(t.pos, t.n3, t.n5, t.box, rv * rv, max_pairs=max(1024, len(got) + 1024))
+    if not far_back:
+        assert got.shape == want.shape and np.array_equal(got, want)
+        return len(want), len(got)
+    a2 = np.cross(t.a3, t.a1)
+    back = t.pos - 0.34 * t.a1 + 0.3408 * a2
+
+    def dist(x, p):
+        d = x[p[:, 1]] - x[p[:, 0]]
+        d -= t.box * np.rint(d / t.box)
+        return np.sqrt((d * d).sum(axis=1))
+
+    wset = {tuple(x) for x in want}
+    gset = {tuple(x) for x in got}
+    assert gset <= wset, "pairs outside the Verlet predicate"
+    dc, db = dist(t.pos, want), dist(back, want)
+    bound = P.dh()["RC"] + 2 * skin
+    must = (dc < R_CLOSE + 2 * skin - 1e-9) | (db < bound)
+    margin = 1e-3 + np.ldexp(float(np.max(t.box)), -18) + 1e-4
+    never = (dc >= R_CLOSE + 2 * skin + 1e-9) & (db >= bound + margin)
+    keep = np.array([tuple(x) in gset for x in want], bool)
+    assert np.all(keep[must]), f"{int((~keep[must]).sum())} pairs in range left out"
+    assert not np.any(keep[never]), f"{int(keep[never].sum())} far pairs out of range kept"
+    return len(want), len(got)

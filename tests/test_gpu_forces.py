@@ -12,7 +12,7 @@ equals the reference's hb_list count.
 import numpy as np
 import pytest
 
-from _common import golden_names, load_golden, oracle_params, vec_rel_err
+from _common import check_pair_set, golden_names, load_golden, oracle_params, vec_rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -69,11 +69,10 @@ def test_pair_set_exact(hip, name):
     from oracle import pyoracle as po
     s, opts, _ = load_golden(name)
     P = oracle_params(s, opts)
-    rv = P.rcut + 2 * 0.05
     with hip.HIPBatch(s, verlet_skin=0.05, **opts) as b:
         got = b.pairs()
-    want = po.verlet_pairs(s.pos, s.n3, s.n5, s.box, rv * rv)
-    assert got.shape == want.shape and np.array_equal(got, want)
+        far_back = b.stats()["far_back"] > 0
+    check_pair_set(got, s, P, 0.05, far_back, po)
 
 
 @pytest.mark.parametrize("kind", ["bundle", "gas"])
@@ -91,12 +90,69 @@ def test_pair_set_exact_cells(hip, kind):
         s = systems.duplex_gas(4, n_duplex=64, bp=12, density=2e-3)
     assert s.N > 1024
     P = po.Params(T=s.T, salt=s.salt)
-    rv = P.rcut + 2 * 0.05
     with hip.HIPBatch(s, verlet_skin=0.05, thermostat="no", salt_concentration=s.salt) as b:
         got = b.pairs()
-    want = po.verlet_pairs(s.pos, s.n3, s.n5, s.box, rv * rv)
-    print(f"{kind}: N={s.N}, {len(want)} pairs")
-    assert got.shape == want.shape and np.array_equal(got, want)
+        far_back = b.stats()["far_back"] > 0
+    n_want, n_got = check_pair_set(got, s, P, 0.05, far_back, po)
+    print(f"{kind}: N={s.N}, {n_want} pairs, {n_got} kept (far pairs by backbone sites: {far_back})")
+
+
+@pytest.mark.parametrize("kind", ["bundle", "gas"])
+def test_pair_set_exact_cells_full(hip, monkeypatch, kind):
+    """OXH_FAR_BACK=0: every far pair within r_verlet, the reference's
+    list, equal to the brute-force FP64 predicate's."""
+    from oracle import pyoracle as po
+    from oxdna_amd import systems
+    monkeypatch.setenv("OXH_FAR_BACK", "0")
+    if kind == "bundle":
+        s = systems.duplex_bundle(2, nx=6, ny=6, bp=32, spacing=2.6, lz=14.0, tc=30.0, salt=0.2)
+    else:
+        s = systems.duplex_gas(4, n_duplex=64, bp=12, density=2e-3)
+    P = po.Params(T=s.T, salt=s.salt)
+    with hip.HIPBatch(s, verlet_skin=0.05, thermostat="no", salt_concentration=s.salt) as b:
+        got = b.pairs()
+        assert b.stats()["far_back"] == 0
+    check_pair_set(got, s, P, 0.05, False, po)
+
+
+@pytest.mark.parametrize("salt", [0.2, 1.0])
+def test_far_pairs_by_backbone_sites_over_a_run(hip, monkeypatch, salt):
+    """Far pairs kept by backbone sites through a Langevin run with many
+    rebuilds on a dense bundle (half-size cells), at a salt where the
+    Debye-Hueckel reach exceeds that of close pairs (0.2 M) and at one where
+    close pairs reach farther (1 M: their sites' distance must not drop
+    them): after 3,000 steps the
+    pair set follows the filter's rule for the current state (a rebuild
+    is forced first), the forces of that state equal the FP64 oracle's at
+    the bars of evolved states (1e-5 of the largest, 2e-5 per particle, as
+    in the full-size runs), and equal those of a batch with every far pair
+    (OXH_FAR_BACK=0) on the same state to FP32 rounding."""
+    from oracle import pyoracle as po
+    from oxdna_amd import systems
+    s = systems.duplex_bundle(3, nx=6, ny=6, bp=32, spacing=2.6, lz=14.0, tc=30.0, salt=salt)
+    P = po.Params(T=s.T, salt=s.salt)
+    with hip.HIPBatch(s, verlet_skin=0.05, thermostat="langevin", diff_coeff=2.5, dt=0.003, seed=5,
+                      salt_concentration=s.salt) as b:
+        assert b.stats()["far_back"] == 1
+        b.run(3000)
+        n_reb = b.info()[0]
+        st = b.state()
+        t = s.copy()
+        t.pos, t.a1, t.a3 = st["pos"], st["a1"], st["a3"]
+        F, T, _ = b.forces()
+        b.update_lists(True)
+        n_want, n_got = check_pair_set(b.pairs(), t, P, 0.05, True, po)
+    ref = po.forces(P, t)
+    fb, fp = vec_rel_err(F, ref["F"])
+    tb, tp = vec_rel_err(T, ref["T"])
+    print(f"{n_reb} rebuilds in 3000 steps; {n_got} of {n_want} pairs kept; F {fb:.2e}/{fp:.2e} T {tb:.2e}/{tp:.2e}")
+    assert n_reb >= 20 and n_got < (0.8 if salt < 0.5 else 1.0) * n_want
+    assert max(fb, tb) <= 1e-5 and max(fp, tp) <= 2e-5
+    monkeypatch.setenv("OXH_FAR_BACK", "0")
+    with hip.HIPBatch(t, verlet_skin=0.05, thermostat="no", salt_concentration=s.salt) as c:
+        assert c.stats()["far_back"] == 0
+        F0, T0, _ = c.forces()
+    assert vec_rel_err(F, F0)[0] <= 2e-6 and vec_rel_err(T, T0)[0] <= 2e-6
 
 
 def test_deterministic(hip):
