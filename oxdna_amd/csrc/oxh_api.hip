@@ -1302,23 +1302,24 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 		const char *e = getenv("OXH_EDGE");
 		if(!B->pool && !B->persist && !b.fp64 && !(e && atoi(e) == 0)) TRYB(alloc_edges(B, std::min(b.max_neigh, 64)));
 	}
-	// far pairs kept by their backbone sites: grid-engine batches (not
-	// pools) and systems with cells in boxes wide enough that backbone
-	// sites take the centres' nearest image (OXH_FAR_BACK=0 keeps every far
-	// pair within r_verlet)
+	// far pairs kept by their backbone sites: grid-engine batches and
+	// pools, systems with cells in boxes wide enough that backbone sites
+	// take the centres' nearest image (a pool's slots decide at admission;
+	// OXH_FAR_BACK=0 keeps every far pair within r_verlet)
 	{
 		const char *e = getenv("OXH_FAR_BACK");
+		const bool allowed = !B->persist && !(e && atoi(e) == 0);
 		bool any = false;
 		for(int s = 0; s < n_sys; s++) {
 			oxh_sys &S = b.h_sys[s];
-			S.far_back = (!B->pool && !B->persist && S.use_cells && !S.small_box && !(e && atoi(e) == 0)) ? 1 : 0;
+			S.far_back = (allowed && !B->pool && S.use_cells && !S.small_box) ? 1 : 0;
 			any |= S.far_back != 0;
 		}
-		if(any) {
+		if(any || (allowed && B->pool && ncell > 0)) {
 			TRYB(dalloc(B, &b.blist, N));
 			TRYB(dalloc(B, &B->bk_blist, N));
-			if(!up(b.d_sys, b.h_sys.data(), n_sys * sizeof(oxh_sys))) return bail(fail(OXH_ERR_HIP, "system upload failed"));
 		}
+		if(any && !up(b.d_sys, b.h_sys.data(), n_sys * sizeof(oxh_sys))) return bail(fail(OXH_ERR_HIP, "system upload failed"));
 	}
 	TRYB(dalloc(B, &B->bk_rlist, N));
 	TRYB(dalloc(B, &b.any_rebuild, 1));
@@ -1603,6 +1604,7 @@ oxh_status oxh_pool_admit(oxh_batch *B, int slot, const oxh_system_desc *desc, l
 	double est = 16;
 	sys_cells(d, S, rv, b.h_sys[slot].cell_first, B->cell_budget[slot], max_occ, est);
 	if(S.use_cells && B->persist) return fail(OXH_ERR_ARG, "slot %d: the workgroup engine takes no cell lists", slot);
+	S.far_back = (b.blist && S.use_cells && !S.small_box) ? 1 : 0;
 	S.first = f;
 	S.n = d.N;
 	// host staging and upload of the slot's particles
@@ -1636,6 +1638,9 @@ oxh_status oxh_pool_admit(oxh_batch *B, int slot, const oxh_system_desc *desc, l
 	if(d.refresh_vel) oxh::launch_refresh_vel(st, b, f, d.N);
 	oxh::launch_prepare_mirror(st, b);
 	oxh::launch_slot_of(st, b);
+	// the slot's first list is built from rlist: its backbone sites are the
+	// recorded ones
+	if(S.far_back) HIPCHK(hipMemcpyAsync(b.blist + f, b.bpos + f, d.N * sizeof(float4), hipMemcpyDeviceToDevice, st));
 	// reduction blocks follow the slots' sizes
 	std::vector<int3> rb;
 	std::vector<int2> rsb;
