@@ -38,6 +38,11 @@ typedef struct oxh_batch oxh_batch;
 #define OXH_THERMOSTAT_BUSSI 2
 #define OXH_THERMOSTAT_BROWNIAN 3   /* "brownian" / "john" */
 
+/* interaction_type (src/Interactions/InteractionFactory.cpp:51-89) */
+#define OXH_INTERACTION_DNA2 0      /* "DNA2" / "DNA2_nomesh": oxDNA2 (DNA2Interaction) */
+#define OXH_INTERACTION_RNA2 1      /* "RNA2": oxRNA2, oxRNA with Debye-Hueckel (RNA2Interaction) */
+#define OXH_INTERACTION_RNA 2       /* "RNA": oxRNA (RNAInteraction) */
+
 /* One system: state, topology and the input-file keys that set up the
  * interaction, the list and the integrator.  Arrays are host arrays in the
  * system's own particle order; they are copied at upload.
@@ -90,6 +95,14 @@ typedef struct oxh_system_desc {
 	 * state; default), 1 "double" (every term in FP64).  All systems of a
 	 * batch use the same; slot pools take mixed only. */
 	int backend_precision;
+	/* OXH_INTERACTION_*: the model of the pair terms.  All systems of a
+	 * batch use the same.  The oxRNA models take the average-sequence
+	 * parameters (no seq_stck), salt / dh_* / dh_half_charged_ends /
+	 * max_backbone_force as RNA2Interaction::get_settings reads them
+	 * (src/Interactions/RNAInteraction2.cpp:33-72; dh_lambda default
+	 * 0.3667258, dh_strength 0.0858), and run on the grid engine
+	 * (not in slot pools or the workgroup engine). */
+	int interaction;
 } oxh_system_desc;
 
 /* Fills a descriptor with the reference's defaults (no arrays set). */

@@ -14,6 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboxh.so")
 
 OXH_THERMOSTAT = {"no": 0, "langevin": 1, "bussi": 2, "brownian": 3, "john": 3}
+# interaction_type -> OXH_INTERACTION_*
+OXH_INTERACTION = {"DNA2": 0, "DNA2_nomesh": 0, "RNA2": 1, "RNA": 2}
 
 # every entry point declared in include/oxh.h
 EXPORTS = [
@@ -53,6 +55,7 @@ class SystemDesc(C.Structure):
         ("verlet_skin", C.c_double), ("dt", C.c_double), ("thermostat", C.c_int), ("diff_coeff", C.c_double),
         ("gamma_trans", C.c_double), ("pt", C.c_double), ("newtonian_steps", C.c_int), ("bussi_tau", C.c_int),
         ("seed", C.c_ulonglong), ("refresh_vel", C.c_int), ("backend_precision", C.c_int),
+        ("interaction", C.c_int),
     ]
 
 

@@ -100,6 +100,10 @@ def make_desc(s: System, keep: list, **opts) -> SystemDesc:
     if prec not in ("mixed", "double"):
         raise ValueError(f"backend_precision {prec!r}: use 'mixed' or 'double'")
     d.backend_precision = 1 if prec == "double" else 0
+    it = opts.get("interaction_type", "DNA2")
+    if it not in _lib.OXH_INTERACTION:
+        raise ValueError(f"interaction_type {it!r}: use one of {sorted(_lib.OXH_INTERACTION)}")
+    d.interaction = _lib.OXH_INTERACTION[it]
     if "seq_stck" in opts:
         st = _c(opts["seq_stck"], np.float64).ravel()
         keep.append(st)

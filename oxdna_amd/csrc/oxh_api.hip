@@ -16,6 +16,7 @@
 #include "../../include/oxh.h"
 #include "oxh_device.h"
 #include "oxh_internal.h"
+#include "oxh_rna.h"
 #include "oxh_step.h"
 
 extern "C" int oxh_internal_hilbert_probe(int n, const void *d_xyzb, void *d_out, hipStream_t st);
@@ -110,7 +111,13 @@ inline double SQRd(double x) { return x * x; }
 // (src/Interactions/DNAInteraction.cpp:298-377), DNA2Interaction::
 // get_settings / init (src/Interactions/DNA2Interaction.cpp:62-155).
 // The reference's float-literal constants go through FP64 library calls.
+void rna_model_constants(const oxh_system_desc &d, oxh_sys &S, double &rcut);
+
 void model_constants(const oxh_system_desc &d, oxh_sys &S, double &rcut) {
+	if(d.interaction == OXH_INTERACTION_RNA2 || d.interaction == OXH_INTERACTION_RNA) {
+		rna_model_constants(d, S, rcut);
+		return;
+	}
 	const double T = d.T;
 	const float STCK_RC = 0.9f, STCK_R0 = 0.4f, STCK_A = 6.f, HYDR_RC = 0.75f, HYDR_R0 = 0.4f, HYDR_A = 8.f;
 	const double stck_shift = SQRd(1 - std::exp((double) (-(STCK_RC - STCK_R0) * STCK_A)));
@@ -176,6 +183,68 @@ void model_constants(const oxh_system_desc &d, oxh_sys &S, double &rcut) {
 	const double arm = std::sqrt(SQRd((double) -0.34f) + SQRd((double) 0.3408f));
 	rcut = std::max(2 * arm + (double) 0.711879214356f, 2 * (double) 0.4f + (double) 0.783775f);
 	rcut = std::max(rcut, 2.0 * arm + RC);
+}
+
+// The oxRNA constants of one system into the same per-system fields:
+// RNAInteraction::init (src/Interactions/RNAInteraction.cpp:110-118,
+// 275-291: stacking and HB well depths, r_cut), RNAInteraction::
+// get_settings (:91-103: max_backbone_force) and, for oxRNA2, RNA2Interaction::
+// get_settings / init (src/Interactions/RNAInteraction2.cpp:33-102:
+// Debye-Hueckel with V = 0).  Plain oxRNA has no Debye-Hueckel term
+// (dh_rc = 0).
+void rna_model_constants(const oxh_system_desc &d, oxh_sys &S, double &rcut) {
+	namespace R = oxh::rna;
+	const double T = d.T;
+	const double stck_shift = SQRd(1 - std::exp((double) (-(R::F1_RC[1] - R::F1_R0[1]) * R::F1_A[1])));
+	const double hydr_shift = SQRd(1 - std::exp((double) (-(R::F1_RC[0] - R::F1_R0[0]) * R::F1_A[0])));
+	for(int i = 0; i < 5; i++)
+		for(int j = 0; j < 5; j++) {
+			S.f1_eps_d[1][i][j] = (double) R::STCK_BASE_EPS + (double) R::STCK_FACT_EPS * T;
+			S.f1_eps_d[0][i][j] = (double) R::HYDR_EPS;
+			S.f1_shift_d[1][i][j] = S.f1_eps_d[1][i][j] * stck_shift;
+			S.f1_shift_d[0][i][j] = S.f1_eps_d[0][i][j] * hydr_shift;
+			for(int t = 0; t < 2; t++) {
+				S.f1_eps[t][i][j] = (float) S.f1_eps_d[t][i][j];
+				S.f1_shift[t][i][j] = (float) S.f1_shift_d[t][i][j];
+			}
+		}
+	const double arm = std::sqrt(SQRd((double) R::POS_BACK_A1) + SQRd((double) R::POS_BACK_A2) + SQRd((double) R::POS_BACK_A3));
+	rcut = std::max(2 * arm + (double) R::EXCL_RC1, 2 * std::fabs((double) R::POS_BASE) + std::max((double) R::HYDR_RCHIGH, (double) R::F2_RCHIGH[0]));
+	S.dh_half = d.dh_half_charged_ends ? 1 : 0;
+	if(d.interaction == OXH_INTERACTION_RNA2) {
+		const float salt_f = (float) d.salt;
+		const double lf = d.dh_lambda >= 0 ? d.dh_lambda : 0.3667258;
+		const double q = d.dh_strength >= 0 ? d.dh_strength : 0.0858;
+		const double l = lf * std::sqrt(T / 0.1f) / std::sqrt((double) salt_f);
+		const double x = d.dh_rhigh > 0 ? d.dh_rhigh : 3.0 * l;
+		S.dh_prefactor_d = q;
+		S.dh_minus_kappa_d = -1.0 / l;
+		S.dh_rhigh_d = x;
+		S.dh_b_d = -(std::exp(-x / l) * q * q * (x + l) * (x + l)) / (4. * x * x * x * l * l * (-q));
+		S.dh_rc_d = x * (q * x + 3. * q * l) / (q * (x + l));
+		rcut = std::max(rcut, 2. * arm + S.dh_rc_d);
+	}
+	S.fene_r0_d = (double) R::FENE_R0;
+	S.use_mbf = 0;
+	if(d.max_backbone_force >= 0) {
+		const double FE = (double) R::FENE_EPS, FD2 = (double) R::FENE_DELTA2;
+		const double fmax = d.max_backbone_force;
+		S.use_mbf = 1;
+		S.mbf_fmax_d = fmax;
+		S.mbf_xmax_d = (-FE + std::sqrt(FE * FE + 4.f * fmax * fmax * FD2)) / (2.f * fmax);
+		S.mbf_finf_d = d.max_backbone_force_far >= 0 ? d.max_backbone_force_far : (double) 0.04f;
+	}
+	S.hb_multi_d = 1.0;
+	S.dh_prefactor = (float) S.dh_prefactor_d;
+	S.dh_minus_kappa = (float) S.dh_minus_kappa_d;
+	S.dh_rhigh = (float) S.dh_rhigh_d;
+	S.dh_rc = (float) S.dh_rc_d;
+	S.dh_b = (float) S.dh_b_d;
+	S.hb_multi = 1.f;
+	S.fene_r0 = (float) S.fene_r0_d;
+	S.mbf_fmax = (float) S.mbf_fmax_d;
+	S.mbf_xmax = (float) S.mbf_xmax_d;
+	S.mbf_finf = (float) S.mbf_finf_d;
 }
 
 // orientation (a1, a3) -> unit quaternion (x, y, z, w); M has columns a1, a2, a3
@@ -498,6 +567,10 @@ oxh_status sys_setup(oxh_batch *B, const oxh_system_desc &d, int s, oxh_sys &S, 
 		if(!(d.box[k] > 0)) return fail(OXH_ERR_ARG, "system %d: box side %d must be > 0", s, k);
 	if(!(d.T > 0)) return fail(OXH_ERR_ARG, "system %d: T must be > 0", s);
 	if(!(d.salt > 0)) return fail(OXH_ERR_ARG, "system %d: salt concentration must be > 0", s);
+	if(d.interaction < OXH_INTERACTION_DNA2 || d.interaction > OXH_INTERACTION_RNA)
+		return fail(OXH_ERR_ARG, "system %d: unknown interaction %d", s, d.interaction);
+	if(d.interaction != OXH_INTERACTION_DNA2 && d.seq_stck)
+		return fail(OXH_ERR_ARG, "system %d: the oxRNA models take the average-sequence parameters only", s);
 	for(int i = 0; i < d.N; i++) {
 		if(d.n3[i] >= d.N || d.n5[i] >= d.N || d.n3[i] < -1 || d.n5[i] < -1)
 			return fail(OXH_ERR_ARG, "system %d: wrong topology for particle %d", s, i);
@@ -867,6 +940,26 @@ oxh_status autotune_forces(oxh_batch *B) {
 	const int lanes[] = {1, 2, 4, 8, 16, 32};
 	const int blocks[] = {64, 128, 256};
 	float best = 1e30f;
+	if(b.model) {
+		// oxRNA: one FP64 kernel over the whole row, blocks of 64
+		b.far_lanes = 0;
+		b.pack = false;
+		int bl = 4;
+		for(int L : {1, 2, 4, 8, 16}) {
+			b.lanes = L;
+			float ms;
+			TRY(timeit(ms));
+			if(ms < best) {
+				best = ms;
+				bl = L;
+			}
+		}
+		(void) hipEventDestroy(e0);
+		(void) hipEventDestroy(e1);
+		b.lanes = bl;
+		enqueue_forces(B);
+		return sync_and_check(B);
+	}
 	if(b.fp64) {
 		// all-FP64 mode: close kernel in blocks of 64 with far pairs inline,
 		// then far pairs in their own kernel (blocks of 256)
@@ -1084,7 +1177,11 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 				return bail(fail(OXH_ERR_ARG, "system %d: backend_precision must be 0 (mixed) or 1 (double)", s));
 			if((descs[s].backend_precision == 1) != b.fp64)
 				return bail(fail(OXH_ERR_ARG, "all systems of a batch need the same backend_precision"));
+			if(descs[s].interaction != descs[0].interaction)
+				return bail(fail(OXH_ERR_ARG, "all systems of a batch need the same interaction"));
 		}
+		b.model = descs[0].interaction == OXH_INTERACTION_DNA2 ? 0 : 1;
+		if(b.model && B->pool) return bail(fail(OXH_ERR_ARG, "slot pools run the oxDNA2 model only"));
 	}
 #define TRYB(x)                               \
 	do {                                      \
@@ -1287,6 +1384,7 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 	}
 	if(const char *p = getenv("OXH_PERSIST")) B->persist &= atoi(p) != 0;
 	B->persist &= !b.fp64;   // the workgroup engine computes in the mixed form
+	B->persist &= !b.model;  // and the oxDNA2 terms
 	// systems (slots) small enough are held in LDS by the workgroup engine
 	if(B->persist) {
 		int nmax = 0;
@@ -1300,7 +1398,7 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 	// forbids / forces it)
 	{
 		const char *e = getenv("OXH_EDGE");
-		if(!B->pool && !B->persist && !b.fp64 && !(e && atoi(e) == 0)) TRYB(alloc_edges(B, std::min(b.max_neigh, 64)));
+		if(!B->pool && !B->persist && !b.fp64 && !b.model && !(e && atoi(e) == 0)) TRYB(alloc_edges(B, std::min(b.max_neigh, 64)));
 	}
 	// far pairs kept by their backbone sites: grid-engine batches and
 	// pools, systems with cells in boxes wide enough that backbone sites
@@ -1308,7 +1406,7 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 	// OXH_FAR_BACK=0 keeps every far pair within r_verlet)
 	{
 		const char *e = getenv("OXH_FAR_BACK");
-		const bool allowed = !B->persist && !(e && atoi(e) == 0);
+		const bool allowed = !B->persist && !b.model && !(e && atoi(e) == 0);
 		bool any = false;
 		for(int s = 0; s < n_sys; s++) {
 			oxh_sys &S = b.h_sys[s];
@@ -1594,6 +1692,7 @@ oxh_status oxh_pool_admit(oxh_batch *B, int slot, const oxh_system_desc *desc, l
 	const oxh_system_desc &d = *desc;
 	if(d.N <= 0 || d.N > B->cap[slot]) return fail(OXH_ERR_ARG, "slot %d holds at most %d nucleotides (job has %d)", slot, B->cap[slot], d.N);
 	if(d.backend_precision != 0) return fail(OXH_ERR_ARG, "slot pools run the mixed precision form only");
+	if(d.interaction != OXH_INTERACTION_DNA2) return fail(OXH_ERR_ARG, "slot pools run the oxDNA2 model only");
 	oxh::Batch &b = B->b;
 	oxh_sys S;
 	double rv = 0;
@@ -2149,6 +2248,8 @@ oxh_status oxh_batch_pair_quantities(oxh_batch *B, int sys, int kind, int n, con
 	if(n < 0 || (n > 0 && (!pairs || !out))) return fail(OXH_ERR_ARG, "oxh_batch_pair_quantities: bad arguments");
 	if(n == 0) return OXH_OK;
 	oxh::Batch &b = B->b;
+	if(b.model && kind <= OXH_PAIR_BASE_DISTANCE)
+		return fail(OXH_ERR_ARG, "oxh_batch_pair_quantities: only the centre distance is available for the oxRNA models");
 	const int ns = B->n[sys];
 	const std::vector<int> &sid = B->strand_of[sys];
 	std::vector<int4> h(n);
@@ -2902,6 +3003,7 @@ extern "C" oxh_status oxh_interaction_compute_forces_noalloc(int device, const o
 	b.nstride = N;
 	b.max_neigh = NA_NEIGH;
 	b.fp64 = desc->backend_precision == 1;
+	b.model = desc->interaction == OXH_INTERACTION_DNA2 ? 0 : 1;
 	b.lanes = 8;
 	b.far_lanes = 0;
 	b.force_block = 256;

@@ -560,6 +560,10 @@ static void launch_edge(hipStream_t st, const Batch &b, bool far_inline, int k) 
 namespace {
 void launch_pair_forces(hipStream_t st, const Batch &b, bool split, int k) {
 	if(b.N == 0) return;
+	if(b.model) {
+		launch_rna_forces(st, b, split, k);
+		return;
+	}
 	if(b.fp64) {
 		switch(b.lanes) {
 		case 1: launch_dp<1>(st, b, split, k); break;

@@ -127,6 +127,7 @@ struct Batch {
 	// system's size (0 = the engine reads and writes HBM every step)
 	int persist_lds_n = 0;
 	bool pack = false;   // close pairs' HB/CRST/CXST through LDS work lists (k_forces PACK)
+	int model = 0;       // pair terms: 0 oxDNA2, 1 oxRNA (k_forces_rna; Debye-Hueckel per system)
 	int any_cells = 0;
 	int thermostat_mask = 0;     // bit t set if some system uses thermostat t
 	int min_newtonian = 1;
@@ -190,6 +191,9 @@ int launch_hilbert_sort(hipStream_t st, Batch &b);
 void launch_ext_forces(hipStream_t st, const Batch &b);
 // slot_of from orig (after an upload or a sort)
 void launch_slot_of(hipStream_t st, const Batch &b);
+// oxRNA / oxRNA2 pair forces (written, not accumulated) into F / T, or
+// F64 / T64 in the all-FP64 form; b.lanes threads per nucleotide
+void launch_rna_forces(hipStream_t st, const Batch &b, bool split, int k);
 // per-pair FFS order-parameter quantities (OXH_PAIR_*) of system sys:
 // pairs (p, q, same strand, unused) in the system's upload order
 void launch_pair_quantities(hipStream_t st, const Batch &b, int sys, int kind, int n, const int4 *d_pairs, double *d_out);

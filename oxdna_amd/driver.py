@@ -1,5 +1,5 @@
-"""Host driver: runs an oxDNA input file (sim_type = MD, interaction DNA2)
-on the HIP batch, writing the reference's output files.
+"""Host driver: runs an oxDNA input file (sim_type = MD, interaction DNA2,
+RNA2 or RNA) on the HIP batch, writing the reference's output files.
 
     python -m oxdna_amd.driver input [key=value ...]
 
@@ -47,7 +47,9 @@ being built from topology and conf_file first (here conf_file only shapes
 the batch).  reload_from also takes the reference's own reload file, a
 binary configuration, with the reference's rules: no conf_file, no seed,
 restart_step_counter false.  sim_type must be MD and interaction_type
-DNA2 or DNA2_nomesh (the GPU interaction is the analytic DNA2_nomesh form).
+DNA2 or DNA2_nomesh (the GPU interaction is the analytic DNA2_nomesh form),
+RNA2 or RNA (oxRNA with the analytic hydrogen-bonding f4 of the reference's
+GPU kernel; average sequence, no mismatch_repulsion).
 """
 from __future__ import annotations
 
@@ -112,15 +114,22 @@ def batch_options(inp: dict, path=lambda p: p) -> dict:
     if st != "MD":
         raise ValueError(f"sim_type = {st}: this driver runs molecular dynamics only")
     it = inp.get("interaction_type", "DNA")
-    if it not in ("DNA2", "DNA2_nomesh"):
-        raise ValueError(f"interaction_type = {it}: this driver runs the oxDNA2 interaction (DNA2)")
+    if it not in ("DNA2", "DNA2_nomesh", "RNA2", "RNA"):
+        raise ValueError(f"interaction_type = {it}: this driver runs the oxDNA2 (DNA2) and oxRNA (RNA, RNA2) interactions")
+    rna = it in ("RNA2", "RNA")
+    if rna and _bool(inp.get("mismatch_repulsion", "false")):
+        raise ValueError("mismatch_repulsion = true is not available")
+    if rna and "seq_dep_file_RNA" in inp and not _bool(inp.get("use_average_seq", "true")):
+        raise ValueError("the oxRNA models run with the average-sequence parameters (use_average_seq = true)")
     th = inp.get("thermostat", "no")
     if th == "john":
         th = "brownian"
     opts = dict(
+        interaction_type=it,
         T=io.temperature(inp["T"]),
         dt=_num(inp, "dt"),
-        salt_concentration=_num(inp, "salt_concentration"),   # required by DNA2, as in the reference
+        # required by DNA2; RNA2 defaults to 1 M (src/Interactions/RNAInteraction2.cpp:36-38)
+        salt_concentration=_num(inp, "salt_concentration", float, 1.0) if rna else _num(inp, "salt_concentration"),
         verlet_skin=_num(inp, "verlet_skin", float, 0.05),
         thermostat=th,
         refresh_vel=_bool(inp.get("refresh_vel", "false")),
@@ -134,6 +143,8 @@ def batch_options(inp: dict, path=lambda p: p) -> dict:
         if key in inp:
             opts[key] = _num(inp, key, cast)
     if not _bool(inp.get("use_average_seq", "true")):
+        if rna:
+            raise ValueError("the oxRNA models run with the average-sequence parameters (use_average_seq = true)")
         if "seq_dep_file" not in inp:
             raise ValueError("use_average_seq = false needs seq_dep_file")
         opts.update(seq_dep_options(path(inp["seq_dep_file"])))

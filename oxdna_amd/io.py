@@ -20,7 +20,8 @@ import numpy as np
 
 from .systems import System
 
-_BASES = {"A": 0, "G": 1, "C": 2, "T": 3, "a": 0, "g": 1, "c": 2, "t": 3}
+# src/Utilities/Utils.cpp:20-37 (U is read as T, N_T = 3)
+_BASES = {"A": 0, "G": 1, "C": 2, "T": 3, "U": 3, "a": 0, "g": 1, "c": 2, "t": 3, "u": 3}
 _NAMES = "AGCT"
 
 

@@ -175,6 +175,63 @@ def add_duplex(b: _Builder, seq, centre, R=None, perp=(1.0, 0.0, 0.0)):
     b.add_strand(pos[bp:], a1[bp:], a3[bp:], seq2)
 
 
+# oxRNA A-form duplex.  Helical step (twist, rise) and the pose of base
+# pair 0 in the helix frame (axis z, strand 1 running 5' -> 3' along +z):
+# centre, a1 and a3 of the strand-1 and strand-2 nucleotides, averaged over
+# the base pairs of a relaxed 8-bp oxRNA duplex (the reference's
+# examples/RNA_DUPLEX_MELT/init.conf) after undoing each pair's step.
+RNA_TWIST = 0.6000194497435797
+RNA_RISE = 0.3614667206242939
+RNA_BP = (((0.789424, -0.101093, -0.0306), (-0.639166, 0.732138, -0.235459), (0.103122, -0.221809, -0.969622)),
+          ((0.050729, 0.795566, -0.313699), (0.645737, -0.73524, 0.206025), (-0.225051, 0.074571, 0.971489)))
+
+
+def rna_duplex_geometry(bp: int):
+    """Centres, a1 and a3 of an oxRNA duplex of bp base pairs along z,
+    centred at the origin, in the builder's strand order (3' end first):
+    rows [0, bp) strand 1, [bp, 2 bp) strand 2; base pair k of strand 1
+    (5' -> 3') pairs with row bp + k."""
+    out = {1: [], 2: []}
+    z0 = (bp - 1) * RNA_RISE / 2.0
+    for k in range(bp):
+        R = rotation_matrix((0.0, 0.0, 1.0), k * RNA_TWIST)
+        for strand in (1, 2):
+            c, a1, a3 = (np.array(v, dtype=np.float64) for v in RNA_BP[strand - 1])
+            a1 = a1 / np.linalg.norm(a1)
+            a3 = a3 - (a3 @ a1) * a1
+            a3 = a3 / np.linalg.norm(a3)
+            out[strand].append((R @ c + np.array([0.0, 0.0, k * RNA_RISE - z0]), R @ a1, R @ a3))
+    rows = list(reversed(out[1])) + out[2]
+    return (np.array([r[0] for r in rows]), np.array([r[1] for r in rows]), np.array([r[2] for r in rows]))
+
+
+def add_rna_duplex(b: _Builder, seq, centre, R=None):
+    """Adds an oxRNA duplex whose strand 1 reads `seq` 5' -> 3' (A=0, G=1,
+    C=2, U=3), centred at `centre`, axis along R @ z."""
+    bp = len(seq)
+    pos, a1, a3 = rna_duplex_geometry(bp)
+    if R is not None:
+        pos, a1, a3 = pos @ R.T, a1 @ R.T, a3 @ R.T
+    pos = pos + np.asarray(centre, dtype=np.float64)
+    b.add_strand(pos[:bp], a1[:bp], a3[:bp], list(reversed(list(seq))))
+    b.add_strand(pos[bp:], a1[bp:], a3[bp:], [3 - x for x in seq])
+
+
+def rna_duplex_bundle(seed: int = 0, nx: int = 16, ny: int = 16, bp: int = 48, spacing: float = 2.6,
+                      lz: float = 20.0, tc: float = 37.0, salt: float = 1.0) -> System:
+    """nx*ny random-sequence oxRNA duplexes, axes along z on a hexagonal
+    lattice with axis spacing `spacing`, periodic orthogonal box."""
+    rng = np.random.default_rng(seed)
+    dy = spacing * math.sqrt(3.0) / 2.0
+    Lx, Ly = nx * spacing, ny * dy
+    b = _Builder()
+    for j in range(ny):
+        for i in range(nx):
+            c = np.array([(i + 0.5 * (j % 2) + 0.25) * spacing, (j + 0.5) * dy, lz / 2.0])
+            add_rna_duplex(b, rng.integers(0, 4, bp), c)
+    return b.build([Lx, Ly, lz], "rna_bundle", T=celsius(tc), salt=salt)
+
+
 def _loop_arc(B1, B2, up, n_loop, bond=FENE_R0):
     """Backbone-site positions of n_loop nucleotides on a circular arc from
     B1 to B2, bulging along `up`, consecutive sites `bond` apart."""
