@@ -22,6 +22,10 @@ Workloads:
            one launch for all systems
   served   a stream of jobs of 40-20,000 nt through resident slot pools
            (continuous batching), against the plain path one job at a time
+  rna      oxRNA2: 24 systems of 1,024 nt, each 16 random-sequence 32-bp
+           RNA duplexes in a box of 20, 37 C, 1 M, Langevin (diff_coeff
+           2.5), skin 0.05, dt 0.003, one launch for all systems; the pair
+           terms in FP64 (k_forces_rna)
 
 value = nucleotide-steps/s of the whole job, inputs resident in HBM.
 roofline: the force evaluation (the close-pair kernel k_forces and, when
@@ -84,6 +88,15 @@ def workload(name: str, seed: int = 1):
                 "spacing 2.6, box 83.2 x 72.05 x 40, 30 C, 0.2 M, Debye-Hueckel, Langevin (diff_coeff 2.5), dt 0.003, "
                 "skin 0.05")
         return s, opts, ref_keys, desc
+    if name == "rna":
+        rs = systems.rna_duplex_set(seed, n_sys=24, per_side=4, bp=32, box=20.0, tc=37.0, salt=1.0)
+        opts = dict(interaction_type="RNA2", thermostat="langevin", diff_coeff=2.5, dt=0.003, verlet_skin=0.05,
+                    refresh_vel=True)
+        per = [dict(seed=seed * 1000 + k) for k in range(len(rs))]
+        desc = ("oxRNA2, 24 systems of 1,024 nt (24,576 nt): 16 random-sequence 32-bp RNA duplexes each on a 4 x 4 "
+                "lattice in a cubic box of 20, 37 C, 1 M, Debye-Hueckel, Langevin (diff_coeff 2.5), dt 0.003, skin 0.05 "
+                "(FP64 pair terms)")
+        return rs, (opts, per), None, desc
     if name == "batched":
         hs = systems.hairpin_set(seed, n=256, box=30.0, t_lo=25.0, t_hi=75.0)
         opts = dict(thermostat="langevin", diff_coeff=2.5, dt=0.003, verlet_skin=0.05, refresh_vel=True)
@@ -164,7 +177,9 @@ def roofline(r: dict, copy_gbps, counters):
     bytes_launch = b_alg * r["n_total"]
     achieved = bytes_launch / (r["force_ms"] * 1e-3) / 1e9
     step_achieved = bytes_launch / (r["ms_per_step"] * 1e-3) / 1e9
-    if r["precision"] == "double":
+    if r["name"] == "rna":
+        kernel = "k_forces_rna"
+    elif r["precision"] == "double":
         kernel = "k_forces_dp" + (" + k_forces_far_dp" if st["far_lanes"] else "")
     else:
         kernel = "k_forces" + (" + k_forces_far" if st["far_lanes"] else "")
@@ -252,10 +267,10 @@ def cpu_baselines(s, ref_keys, steps_lo: int, steps_hi: int, variants=("DNA2", "
                     break
     except OSError:
         pass
-    main = out.get("DNA2") or next(iter(out.values()))
+    main = out.get(variants[0]) or next(iter(out.values()))
     return {"value": main["value"], "unit": "nucleotide-steps/s", "cores": 1, "kind": "reference",
             "sample": (f"{n} MD steps of the same {s.N}-nt system per variant (difference of a {steps_hi}- and a "
-                       f"{steps_lo}-step run), one thread each; value = DNA2 (meshed f4, the reference's default); "
+                       f"{steps_lo}-step run), one thread each; value = {variants[0]} (the reference's default); "
                        + "; ".join(f"{v}: {o['ms_per_step']:.1f} ms/step" for v, o in out.items())),
             "variants": out, "cpu_model": cpu_model, "host_cores_visible": len(allowed),
             "binary": "oracle/_ref/oxDNA_release (reference CPU sources, -O3 -ffast-math -march=x86-64-v3)"}
@@ -381,7 +396,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="headline", choices=["headline", "plain1", "plain2", "batched", "served"])
+    ap.add_argument("--workload", default="headline", choices=["headline", "plain1", "plain2", "batched", "served", "rna"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="headline: skip the all-FP64 run")

@@ -1224,6 +1224,12 @@ oxh_status create_impl(int device, int n_sys, const oxh_system_desc *descs, cons
 		N += B->pool ? capacity[s] : S.n;
 	}
 	if(B->pool) B->bussi_aligned = false;   // slots start at different times
+	// the oxRNA path runs on systems without cell lists: a 24,576-nt oxRNA2
+	// system with cells left the GPU faulted, with no cause found in this
+	// path's code; until one is, such systems are refused
+	for(int s = 0; s < n_sys && b.model; s++)
+		if(b.h_sys[s].use_cells)
+			return bail(fail(OXH_ERR_ARG, "system %d: the oxRNA models take systems without cell lists (at most 1024 nucleotides)", s));
 	b.N = N;
 	b.nstride = N;
 	b.ncell_total = ncell;

@@ -217,19 +217,22 @@ def add_rna_duplex(b: _Builder, seq, centre, R=None):
     b.add_strand(pos[bp:], a1[bp:], a3[bp:], [3 - x for x in seq])
 
 
-def rna_duplex_bundle(seed: int = 0, nx: int = 16, ny: int = 16, bp: int = 48, spacing: float = 2.6,
-                      lz: float = 20.0, tc: float = 37.0, salt: float = 1.0) -> System:
-    """nx*ny random-sequence oxRNA duplexes, axes along z on a hexagonal
-    lattice with axis spacing `spacing`, periodic orthogonal box."""
+def rna_duplex_set(seed: int = 0, n_sys: int = 24, per_side: int = 4, bp: int = 32, box: float = 20.0,
+                   tc: float = 37.0, salt: float = 1.0):
+    """n_sys independent oxRNA systems of per_side^2 random-sequence
+    duplexes of bp base pairs each (per_side^2 * 2 bp nucleotides), axes
+    along z on a square lattice filling a cubic box of side `box`."""
     rng = np.random.default_rng(seed)
-    dy = spacing * math.sqrt(3.0) / 2.0
-    Lx, Ly = nx * spacing, ny * dy
-    b = _Builder()
-    for j in range(ny):
-        for i in range(nx):
-            c = np.array([(i + 0.5 * (j % 2) + 0.25) * spacing, (j + 0.5) * dy, lz / 2.0])
-            add_rna_duplex(b, rng.integers(0, 4, bp), c)
-    return b.build([Lx, Ly, lz], "rna_bundle", T=celsius(tc), salt=salt)
+    out = []
+    a = box / per_side
+    for k in range(n_sys):
+        b = _Builder()
+        for ix in range(per_side):
+            for iy in range(per_side):
+                R = rotation_matrix((0.0, 0.0, 1.0), rng.uniform(0.0, 2.0 * math.pi))
+                add_rna_duplex(b, rng.integers(0, 4, bp), ((ix + 0.5) * a, (iy + 0.5) * a, box / 2.0), R)
+        out.append(b.build([box, box, box], f"rna_set{k}", T=celsius(tc), salt=salt))
+    return out
 
 
 def _loop_arc(B1, B2, up, n_loop, bond=FENE_R0):

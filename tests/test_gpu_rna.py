@@ -12,8 +12,9 @@ tests/golden/make_rna_golden.py).
     the reference CPU backend's cubic-mesh interpolation of the HB f4
     factors (RNAInteraction.cpp:312-324), which the GPU evaluates
     analytically as the reference's GPU kernel does (CUDA_RNA.cuh:743-761):
-    bars of 2e-3 of the HB energy and 2e-2 of the largest force, several
-    times the measured departure.  The HB forces and torques themselves
+    bars of 5e-5 of the HB energy and 2e-3 of the largest force and torque,
+    about ten times the largest departure measured on MI355X (5.4e-6 and
+    3.3e-4).  The HB forces and torques themselves
     are pinned to the HB energy by central differences of the GPU's own
     energy (translations and rotations of single nucleotides);
   - the reference's published split of its oxRNA force-field test
@@ -89,10 +90,11 @@ def test_rna_fp64_vs_reference(hip, name):
         # the reference prints the pressure observable with 8 significant digits
         assert np.all(np.abs(W - want) <= 2e-8 * np.abs(want).max() + 1e-12)
     else:
-        assert ehb <= 2e-3 * abs(ref["E8"][4])
-        assert ef <= 2e-2 and et <= 2e-2
+        assert ehb <= 5e-5 * abs(ref["E8"][4])
+        assert ef <= 2e-3 and et <= 2e-3
         assert abs(hb - ref["hb_count"]) <= max(1, ref["hb_count"] // 50)
-        assert np.all(np.abs(W - want) <= 2e-2 * np.abs(want).max() + 1e-12)
+        print(f"  stress {np.abs(W - want).max() / np.abs(want).max():.2e}")
+        assert np.all(np.abs(W - want) <= 5e-3 * np.abs(want).max() + 1e-12)
 
 
 @pytest.mark.parametrize("name", ["rna_contacts_nohb", "rna_nicked24_nohb", "rna_tangle12_nohb"])
@@ -126,7 +128,7 @@ def test_rna_published_split(hip):
     print("published", ref["published"], "got", np.round(got, 6))
     other = [0, 1, 2, 3, 5, 6, 7]
     assert np.all(np.abs(got[other] - ref["published"][other]) <= 6e-7)
-    assert abs(got[4] - ref["published"][4]) <= 2e-3 * abs(ref["published"][4])
+    assert abs(got[4] - ref["published"][4]) <= 5e-5 * abs(ref["published"][4]) + 6e-7
 
 
 def _quat_rotate(a1, a3, axis, ang):
@@ -191,3 +193,32 @@ def test_rna_nve_conserves_energy(hip):
     print(f"E/N {(U0 + K0) / s.N:.6f} -> {(U1 + K1) / s.N:.6f}, drift {drift:.2e}")
     assert err == 0
     assert drift < 1e-4
+
+
+def test_rna_refuses_cell_lists(hip):
+    """oxRNA systems large enough for cell lists are refused at creation
+    (the oxRNA path runs on systems of at most 1,024 nucleotides)."""
+    from oxdna_amd import systems
+    rng = np.random.default_rng(4)
+    b0 = systems._Builder()
+    for k in range(40):
+        systems.add_rna_duplex(b0, rng.integers(0, 4, 16), ((k % 8) * 3.0 + 1.5, (k // 8) * 3.0 + 1.5, 10.0))
+    s = b0.build([24.0, 24.0, 20.0], "rna1280", T=systems.celsius(37.0), salt=1.0)
+    assert s.N == 1280
+    with pytest.raises(Exception, match="without cell lists"):
+        hip.HIPBatch(s, interaction_type="RNA2")
+
+
+def test_rna_batch_of_systems(hip):
+    """A batch of independent oxRNA2 systems (one launch for all) gives
+    each system the forces it has alone (to FP64 rounding: the launch
+    shapes, and so the order of the sums, may differ)."""
+    from oxdna_amd import systems
+    rs = systems.rna_duplex_set(3, n_sys=3, per_side=2, bp=12, box=12.0)
+    with hip.HIPBatch(rs, backend_precision="double", interaction_type="RNA2") as b:
+        together = [b.forces(k) for k in range(3)]
+    for k, s in enumerate(rs):
+        with hip.HIPBatch(s, backend_precision="double", interaction_type="RNA2") as c:
+            F, T, E = c.forces()
+        assert vec_rel_err(together[k][0], F)[0] <= 1e-12 and vec_rel_err(together[k][1], T)[0] <= 1e-12
+
