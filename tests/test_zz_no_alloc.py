@@ -69,6 +69,41 @@ def test_noalloc_matches_batch_entry_point(precision, monkeypatch):
         assert big_f <= 10 * tol and big_t <= 10 * tol   # rounding of the additions onto O(1) values
 
 
+def test_noalloc_rna_matches_batch_entry_point(monkeypatch):
+    """The same for oxRNA2 systems (interaction in the descriptor, FP64
+    terms): bit for bit against the batch entry point, and against the
+    reference binary's fixtures of HB-free states at the FP64 bar."""
+    from oxdna_amd import _lib, backend
+    from test_gpu_rna import load_rna
+    assert _lib.device_count() >= 1, "no GPU visible"
+    for k, v in PINNED.items():
+        monkeypatch.setenv(k, v)
+    rng = np.random.default_rng(23)
+    cases = []
+    for name in ["rna_contacts_nohb", "rna_ff16_nohb"]:   # 432 then 16 nucleotides
+        s, opts, ref = load_rna(name)
+        preF, preT = rng.normal(size=(s.N, 4)), rng.normal(size=(s.N, 4))
+        d_pos, d_quat = _state(backend, s)
+        dF, dT = DeviceArray(preF), DeviceArray(preT)
+        with backend.HIPBatch(s, backend_precision="double", **opts) as b:
+            b.interaction_forces(d_pos.ptr, d_quat.ptr, dF.ptr, dT.ptr)
+            b.sync()
+        cases.append((name, s, opts, ref, preF, preT, d_pos, d_quat, dF.numpy(), dT.numpy()))
+    outs = []
+    for name, s, opts, ref, preF, preT, d_pos, d_quat, _, _ in cases:
+        dF, dT = DeviceArray(preF), DeviceArray(preT)
+        backend.interaction_forces_noalloc(s, d_pos.ptr, d_quat.ptr, dF.ptr, dT.ptr, backend_precision="double", **opts)
+        outs.append((dF, dT))
+    synchronize()
+    for (name, s, opts, ref, preF, preT, _, _, wantF, wantT), (dF, dT) in zip(cases, outs):
+        gotF, gotT = dF.numpy(), dT.numpy()
+        assert np.array_equal(gotF, wantF) and np.array_equal(gotT, wantT), name
+        big_f, _ = vec_rel_err(gotF[:, :3] - preF[:, :3], ref["F"])
+        big_t, _ = vec_rel_err(gotT[:, :3] - preT[:, :3], ref["T"])
+        print(f"{name}: F {big_f:.2e} T {big_t:.2e}")
+        assert big_f <= 1e-8 and big_t <= 1e-8
+
+
 def test_noalloc_rejects_oversize_system():
     from oxdna_amd import _lib, backend, systems
     assert _lib.device_count() >= 1, "no GPU visible"
