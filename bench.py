@@ -10,8 +10,10 @@ Workloads:
   headline (default)  the plain 196,608-nt system below in the mixed form
            (FP32 pair terms, FP64 state): `value`, `roofline` and
            `cpu_baseline`; the same system all-FP64 (`fp64`), the batched
-           hairpins (`batched`) and the served stream against the plain
-           path one job at a time (`served`) in the same line
+           hairpins (`batched`), the batched oxRNA2 systems (`rna`, with
+           the reference CPU binary on one of them) and the served stream
+           against the plain path one job at a time (`served`) in the same
+           line
   plain2   1,024 random-sequence 96-bp duplexes (196,608 nt), hexagonal
            lattice of axis spacing 2.6, box 83.2 x 72.05 x 40, T = 30 C,
            0.2 M, Langevin (diff_coeff 2.5), skin 0.05, dt 0.003
@@ -197,6 +199,24 @@ def roofline(r: dict, copy_gbps, counters):
         out["valu_busy"] = counters.get("valu_busy")
         out["counters_source"] = counters["path"] + ("" if counters["same_shape"] else " (other launch shape: not used)")
     return out
+
+
+def rna_cpu_baseline(seed: int):
+    """The reference CPU binary (RNA2, one thread) on one of the rna
+    workload's 24 systems for 100 steps; value = the whole workload's
+    nucleotide-steps/s at that rate (24 systems one after another)."""
+    rs, opts, _, _ = workload("rna", seed)
+    ref = dict(interaction_type="RNA2", T="37C", salt_concentration="1.0", thermostat="langevin", diff_coeff="2.5",
+               box_type="cubic")
+    c = cpu_baselines(rs[0], ref, 10, 110, variants=("RNA2",))
+    if c is None:
+        return None
+    ms = c["variants"]["RNA2"]["ms_per_step"] * len(rs)
+    n = sum(x.N for x in rs)
+    return {"value": n / (ms * 1e-3), "unit": "nucleotide-steps/s", "cores": 1, "kind": "reference",
+            "sample": (f"100 MD steps of one 1,024-nt system of the workload (difference of a 110- and a 10-step run), "
+                       f"one thread, RNA2: {c['variants']['RNA2']['ms_per_step']:.2f} ms/step, x {len(rs)} systems"),
+            "cpu_model": c["cpu_model"]}
 
 
 def cpu_baselines(s, ref_keys, steps_lo: int, steps_hi: int, variants=("DNA2", "DNA2_nomesh")):
@@ -402,6 +422,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="headline: skip the all-FP64 run")
     ap.add_argument("--no-served", action="store_true", help="headline: skip the served stream")
     ap.add_argument("--no-batched", action="store_true", help="headline: skip the batched hairpins")
+    ap.add_argument("--no-rna", action="store_true", help="headline: skip the batched oxRNA2 systems")
     ap.add_argument("--cpu-steps", type=int, default=4,
                     help="CPU baseline: steps timed per variant (the 196,608-nt system takes seconds per step)")
     ap.add_argument("--sort-every", type=int, default=500, help="Hilbert re-sort interval in steps (0 = never)")
@@ -458,6 +479,16 @@ def main():
                                 "steps": nb, "warmup": 256, "dtype": "f32 forces / f64 state", "workload": rb["desc"],
                                 "n_systems": rb["n_systems"], "n_nucleotides": rb["n_total"],
                                 "engine": rb["stats"]["engine"]}
+        if not args.no_rna:
+            try:
+                rr = run_plain("rna", "mixed", args.steps, args.warmup, args.sort_every, args.seed)
+                extra["rna"] = {"value": rr["value"], "unit": "nucleotide-steps/s", "ms_per_step": rr["ms_per_step"],
+                                "steps": args.steps, "warmup": args.warmup, "dtype": "f64 forces / f64 state",
+                                "workload": rr["desc"], "n_systems": rr["n_systems"], "n_nucleotides": rr["n_total"],
+                                "roofline": roofline(rr, copy_gbps, profile_counters("rna", "mixed", rr["stats"])),
+                                "cpu_baseline": None if args.no_cpu else rna_cpu_baseline(args.seed)}
+            except Exception as e:  # the headline stands on its own
+                log(f"rna workload failed: {e}")
         if not args.no_served:
             extra["served"] = served_measure(args.jobs, args.step_scale, args.rate_mult, args.budget_gb, args.seed,
                                              args.small_cap, args.quantum_small)
