@@ -1,5 +1,5 @@
-# Builds the HIPDNA2 interaction plugin (oxdna_amd/plugin/HIPDNA2.so) for the
-# reference oxDNA program: compiled against the reference's headers where
+# Builds the HIPDNA2 and HIPRNA2 interaction plugins (oxdna_amd/plugin/HIPDNA2.so,
+# HIPRNA2.so) for the reference oxDNA program: compiled against the reference's headers where
 # they lie (REF/src, read only) and linked to liboxh.so and the HIP runtime.
 #
 #   make -f oxdna_amd/plugin/plugin.mk          # needs oxdna_amd/liboxh.so
@@ -16,13 +16,17 @@ CXX      ?= g++
 CXXFLAGS := -std=c++14 -O2 -fPIC -DNOCUDA -DJSON_ENABLED -D_FORCE_INLINES -D__HIP_PLATFORM_AMD__ \
             -I$(SRC) -I$(SRC)/extern -Iinclude -I$(ROCM)/include -w
 
-all: $(PKG)/plugin/HIPDNA2.so
+all: $(PKG)/plugin/HIPDNA2.so $(PKG)/plugin/HIPRNA2.so
 
 $(PKG)/plugin/HIPDNA2.so: $(PKG)/plugin/HIPDNA2.cpp include/oxh.h $(PKG)/liboxh.so
 	$(CXX) $(CXXFLAGS) -shared $< -o $@ -L$(PKG) -loxh -L$(ROCM)/lib -lamdhip64 \
 		-Wl,-rpath,'$$ORIGIN/..' -Wl,-rpath,$(ROCM)/lib
 
+$(PKG)/plugin/HIPRNA2.so: $(PKG)/plugin/HIPRNA2.cpp include/oxh.h $(PKG)/liboxh.so
+	$(CXX) $(CXXFLAGS) -shared $< -o $@ -L$(PKG) -loxh -L$(ROCM)/lib -lamdhip64 \
+		-Wl,-rpath,'$$ORIGIN/..' -Wl,-rpath,$(ROCM)/lib
+
 clean:
-	rm -f $(PKG)/plugin/HIPDNA2.so
+	rm -f $(PKG)/plugin/HIPDNA2.so $(PKG)/plugin/HIPRNA2.so
 
 .PHONY: all clean

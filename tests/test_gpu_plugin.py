@@ -67,3 +67,32 @@ def test_reference_binary_runs_on_gpu_plugin(ready, tmp_path, case, precision, e
     print(f"{case} {precision}: energy file {de:.1e}, positions/axes {dx:.1e}, velocities/L {dv:.1e}")
     assert de <= e_tol
     assert dx <= x_tol and dv <= 20 * x_tol
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_reference_binary_runs_rna_test_on_gpu_plugin(ready, tmp_path, precision):
+    """The reference's own oxRNA2 test (test/RNA/FORCE_FIELD/AVG_SEQ: input
+    file, topology, configuration; tests/golden/rna/ff16_input) run by the
+    reference binary with `interaction_type = HIPRNA2`: the potential
+    energy split it writes (its split_energy.dat, computed on the GPU)
+    matches its published reference.dat (6 decimals; the HB column within
+    the CPU mesh's departure from the analytic f4 the GPU evaluates)."""
+    if not os.path.exists(os.path.join(PLUGIN_DIR, "HIPRNA2.so")):
+        pytest.skip("HIPRNA2.so was not built (built where the reference sources are)")
+    src = os.path.join(GOLDEN, "rna", "ff16_input")
+    (tmp_path / "run").mkdir()
+    for f in ("init.top", "init.dat"):
+        shutil.copy(os.path.join(src, f), tmp_path / f)
+    text = open(os.path.join(src, "run", "input")).read().replace("interaction_type = RNA2", "interaction_type = HIPRNA2")
+    text += f"\nplugin_search_path = {PLUGIN_DIR}\nbackend_precision = {precision}\n"
+    (tmp_path / "run" / "input").write_text(text)
+    print(f"running the reference binary with HIPRNA2 ({precision})", flush=True)
+    r = subprocess.run([REF_BIN, "input"], cwd=tmp_path / "run", capture_output=True, text=True, timeout=300)
+    log = (tmp_path / "run" / "log.dat").read_text() if (tmp_path / "run" / "log.dat").exists() else ""
+    assert r.returncode == 0, (r.stdout + r.stderr + log)[-3000:]
+    got = np.loadtxt(tmp_path / "run" / "split_energy.dat", ndmin=2)[0]
+    want = np.loadtxt(os.path.join(src, "run", "reference.dat"))
+    print(precision, "got", got, "want", want)
+    other = [0, 1, 2, 3, 5, 6, 7]
+    assert np.all(np.abs(got[other] - want[other]) <= 1.01e-6)
+    assert abs(got[4] - want[4]) <= 5e-5 * abs(want[4]) + 1.01e-6
