@@ -12,9 +12,12 @@
 // One kernel evaluates a nucleotide's bonded neighbours and its whole
 // Verlet row (close pairs: every term; far pairs, centres beyond the
 // close bound: Debye-Hueckel only) with K lanes per nucleotide, as the
-// oxDNA2 kernels do, and writes (not accumulates) F and T; mixed-precision
-// batches take the FP64 results rounded into their FP32 force buffers.
+// oxDNA2 kernels do, and writes (not accumulates) F and T.  The mixed form
+// takes the angles of the angular factors (acos, sin, f4) in FP32, every
+// other quantity in FP64, and rounds the results into its FP32 force
+// buffers; the all-FP64 form takes everything in FP64.
 #include <cfloat>
+#include <type_traits>
 
 #include "oxh_internal.h"
 #include "oxh_device.h"
@@ -28,7 +31,10 @@ namespace rna {
 // every expression below promotes them the same way.  The f4 / f5 type is
 // a template argument, so that each table entry folds into a constant.
 
-__device__ __forceinline__ double lracos(double x) { return x > 1. ? 0. : (x < -1. ? (double) PIf : acos(x)); }
+// the angle of a cosine, clamped as the reference's LRACOS (src/defs.h),
+// in the angle type A: double in the all-FP64 form, float in the mixed one
+template<typename A>
+__device__ __forceinline__ A lracos(double x) { return x > 1. ? A(0) : (x < -1. ? (A) PIf : acos((A) x)); }
 
 template<int TY>
 __device__ __forceinline__ double f1(const oxh_sys &S, double r, int n3, int n5) {
@@ -81,32 +87,32 @@ __device__ __forceinline__ double f2D(double r) {
 	return val;
 }
 
-template<int TY>
-__device__ __forceinline__ double f4(double t) {
-	double val = 0.;
-	t -= (double) F4_T0[TY];
+template<int TY, typename A>
+__device__ __forceinline__ A f4(A t) {
+	A val = 0;
+	t -= (A) F4_T0[TY];
 	if(t < 0) t *= -1;
-	if(t < (double) F4_TC[TY]) {
-		if(t > (double) F4_TS[TY]) val = (double) F4_B[TY] * SQR((double) F4_TC[TY] - t);
-		else val = 1. - (double) F4_A[TY] * SQR(t);
+	if(t < (A) F4_TC[TY]) {
+		if(t > (A) F4_TS[TY]) val = (A) F4_B[TY] * SQR((A) F4_TC[TY] - t);
+		else val = A(1) - (A) F4_A[TY] * SQR(t);
 	}
 	return val;
 }
 
 // (df4/dtheta) / sin(theta), with the reference's guard for a small sine
-template<int TY>
-__device__ __forceinline__ double f4Dsin(double t) {
-	double val = 0., m = 1.;
-	double tt0 = t - (double) F4_T0[TY];
+template<int TY, typename A>
+__device__ __forceinline__ A f4Dsin(A t) {
+	A val = 0, m = 1;
+	A tt0 = t - (A) F4_T0[TY];
 	if(tt0 < 0) {
 		tt0 *= -1;
-		m = -1.;
+		m = -1;
 	}
-	if(tt0 < (double) F4_TC[TY]) {
-		const double sint = sin(t);
-		if(tt0 > (double) F4_TS[TY]) val = m * 2 * (double) F4_B[TY] * (tt0 - (double) F4_TC[TY]) / sint;
-		else if(SQR(sint) > 1e-8) val = -m * 2 * (double) F4_A[TY] * tt0 / sint;
-		else val = -m * 2 * (double) F4_A[TY];
+	if(tt0 < (A) F4_TC[TY]) {
+		const A sint = sin(t);
+		if(tt0 > (A) F4_TS[TY]) val = m * 2 * (A) F4_B[TY] * (tt0 - (A) F4_TC[TY]) / sint;
+		else if(SQR(sint) > A(1e-8)) val = -m * 2 * (A) F4_A[TY] * tt0 / sint;
+		else val = -m * 2 * (A) F4_A[TY];
 	}
 	return val;
 }
@@ -200,6 +206,7 @@ __device__ __forceinline__ dv3 min_image(double4 a, double4 b, const oxh_sys &S)
 // FENE, bonded excluded volume and stacking of the bonded pair p -> n3 = q,
 // r = q - p.  Force on q into fq (p takes -fq), lab torques on p and q,
 // energies into e[0..2]; err bit 0 when the FENE bond is beyond its range.
+template<typename A>
 __device__ void bonded(const oxh_sys &S, const Nuc &p, const Nuc &q, const dv3 &r, dv3 &fq, dv3 &tp, dv3 &tq, double e[3], int &err) {
 	const dv3 pback = s_back(p), qback = s_back(q);
 	// backbone
@@ -253,29 +260,29 @@ __device__ void bonded(const oxh_sys &S, const Nuc &p, const Nuc &q, const dv3 &
 	const dv3 rbdir = rback * (1. / rbackmod);
 	const double cost5 = dot(p.a3, rstackdir);
 	const double cost6 = -dot(q.a3, rstackdir);
-	const double t5 = lracos(cost5), t6 = lracos(cost6);
+	const A t5 = lracos<A>(cost5), t6 = lracos<A>(cost6);
 	const dv3 pb3 = s_bbv3(p), qb5 = s_bbv5(q);
 	const double costB1 = -dot(rbdir, pb3);
 	const double costB2 = -dot(rbdir, qb5);
 	const double cosphi1 = dot(p.a2, rback) / rbackmod;
 	const double cosphi2 = dot(q.a2, rback) / rbackmod;
 	const double ff1 = f1<1>(S, rstackmod, q.type, p.type);
-	const double pt5 = (double) PIf - t5;
-	const double f4t5 = f4<STCK_T5>(pt5);
-	const double f4t6 = f4<STCK_T6>(t6);
-	const double tB1 = lracos(costB1), tB2 = lracos(costB2);
-	const double f4tB1 = f4<STCK_TB1>(tB1);
-	const double f4tB2 = f4<STCK_TB2>(tB2);
+	const A pt5 = (A) PIf - t5;
+	const double f4t5 = f4<STCK_T5, A>(pt5);
+	const double f4t6 = f4<STCK_T6, A>(t6);
+	const A tB1 = lracos<A>(costB1), tB2 = lracos<A>(costB2);
+	const double f4tB1 = f4<STCK_TB1, A>(tB1);
+	const double f4tB2 = f4<STCK_TB2, A>(tB2);
 	const double f5phi1 = f5<STCK_PHI1>(cosphi1);
 	const double f5phi2 = f5<STCK_PHI2>(cosphi2);
 	const double es = ff1 * f4t5 * f4t6 * f5phi1 * f5phi2 * f4tB1 * f4tB2;
 	e[2] = es;
 	if(es != 0.) {
 		const double ff1D = f1D<1>(S, rstackmod, q.type, p.type);
-		const double f4t5Dsin = f4Dsin<STCK_T5>(pt5);
-		const double f4t6Dsin = f4Dsin<STCK_T6>(t6);
-		const double f4tB1Dsin = f4Dsin<STCK_TB1>(tB1);
-		const double f4tB2Dsin = f4Dsin<STCK_TB2>(tB2);
+		const double f4t5Dsin = f4Dsin<STCK_T5, A>(pt5);
+		const double f4t6Dsin = f4Dsin<STCK_T6, A>(t6);
+		const double f4tB1Dsin = f4Dsin<STCK_TB1, A>(tB1);
+		const double f4tB2Dsin = f4Dsin<STCK_TB2, A>(tB2);
 		const double f5phi1D = f5D<STCK_PHI1>(cosphi1);
 		const double f5phi2D = f5D<STCK_PHI2>(cosphi2);
 
@@ -309,6 +316,7 @@ __device__ void bonded(const oxh_sys &S, const Nuc &p, const Nuc &q, const dv3 &
 // Excluded volume, hydrogen bonding, cross stacking and coaxial stacking of
 // the non-bonded ordered pair (p, q), r = q - p.  Adds the force on q to fq
 // and the lab torque on p to tp; energies into e[0..3].
+template<typename A>
 __device__ void nonbonded(const oxh_sys &S, const Nuc &p, const Nuc &q, const dv3 &r, dv3 &fq, dv3 &tp, double e[4]) {
 	const dv3 pback = s_back(p), qback = s_back(q), pbase = s_base(p), qbase = s_base(q);
 	// excluded volume of four site pairs
@@ -339,23 +347,23 @@ __device__ void nonbonded(const oxh_sys &S, const Nuc &p, const Nuc &q, const dv
 			const dv3 rhdir = rbb * (1. / rhmod);
 			const double cost1 = -dot(a1, b1), cost2 = -dot(b1, rhdir), cost3 = dot(a1, rhdir);
 			const double cost4 = dot(a3, b3), cost7 = -dot(b3, rhdir), cost8 = dot(a3, rhdir);
-			const double t1 = lracos(cost1), t2 = lracos(cost2), t3 = lracos(cost3);
-			const double t4 = lracos(cost4), t7 = lracos(cost7), t8 = lracos(cost8);
+			const A t1 = lracos<A>(cost1), t2 = lracos<A>(cost2), t3 = lracos<A>(cost3);
+			const A t4 = lracos<A>(cost4), t7 = lracos<A>(cost7), t8 = lracos<A>(cost8);
 			const double ff1 = f1<0>(S, rhmod, q.type, p.type);
-			const double f4t1 = f4<HYDR_T1>(t1), f4t2 = f4<HYDR_T2>(t2), f4t3 = f4<HYDR_T3>(t3);
-			const double f4t4 = f4<HYDR_T4>(t4), f4t7 = f4<HYDR_T7>(t7), f4t8 = f4<HYDR_T8>(t8);
+			const double f4t1 = f4<HYDR_T1, A>(t1), f4t2 = f4<HYDR_T2, A>(t2), f4t3 = f4<HYDR_T3, A>(t3);
+			const double f4t4 = f4<HYDR_T4, A>(t4), f4t7 = f4<HYDR_T7, A>(t7), f4t8 = f4<HYDR_T8, A>(t8);
 			const double eh = ff1 * f4t1 * f4t2 * f4t3 * f4t4 * f4t7 * f4t8;
 			e[1] = eh;
 			if(eh != 0.) {
 				const double ff1D = f1D<0>(S, rhmod, q.type, p.type);
 				// d f4(acos c) / dc = -f4Dsin, with the signs the reference's
 				// mesh derivatives carry
-				const double f4t1Dsin = -f4Dsin<HYDR_T1>(t1);
-				const double f4t2Dsin = -f4Dsin<HYDR_T2>(t2);
-				const double f4t3Dsin = f4Dsin<HYDR_T3>(t3);
-				const double f4t4Dsin = f4Dsin<HYDR_T4>(t4);
-				const double f4t7Dsin = -f4Dsin<HYDR_T7>(t7);
-				const double f4t8Dsin = f4Dsin<HYDR_T8>(t8);
+				const double f4t1Dsin = -f4Dsin<HYDR_T1, A>(t1);
+				const double f4t2Dsin = -f4Dsin<HYDR_T2, A>(t2);
+				const double f4t3Dsin = f4Dsin<HYDR_T3, A>(t3);
+				const double f4t4Dsin = f4Dsin<HYDR_T4, A>(t4);
+				const double f4t7Dsin = -f4Dsin<HYDR_T7, A>(t7);
+				const double f4t8Dsin = f4Dsin<HYDR_T8, A>(t8);
 				dv3 force = rhdir * (-ff1D * f4t1 * f4t2 * f4t3 * f4t4 * f4t7 * f4t8);
 				dv3 tl = cross(a3, b3) * (-(-ff1 * f4t1 * f4t2 * f4t3 * f4t4Dsin * f4t7 * f4t8));
 				tl -= cross(a1, b1) * (-ff1 * f4t1Dsin * f4t2 * f4t3 * f4t4 * f4t7 * f4t8);
@@ -378,21 +386,21 @@ __device__ void nonbonded(const oxh_sys &S, const Nuc &p, const Nuc &q, const dv
 		if((double) F2_RCLOW[0] < rcmod && rcmod < (double) F2_RCHIGH[0]) {
 			const dv3 rcdir = rbb * (1. / rcmod);
 			const double cost2 = -dot(b1, rcdir), cost3 = dot(a1, rcdir), cost7 = -dot(b3, rcdir), cost8 = dot(a3, rcdir);
-			const double t1 = lracos(-dot(a1, b1)), t2 = lracos(cost2), t3 = lracos(cost3);
-			const double t7 = lracos(cost7), t8 = lracos(cost8);
+			const A t1 = lracos<A>(-dot(a1, b1)), t2 = lracos<A>(cost2), t3 = lracos<A>(cost3);
+			const A t7 = lracos<A>(cost7), t8 = lracos<A>(cost8);
 			const double ff2 = f2<0>(rcmod);
-			const double f4t1 = f4<CRST_T1>(t1), f4t2 = f4<CRST_T2>(t2), f4t3 = f4<CRST_T3>(t3);
-			const double f4t7 = f4<CRST_T7>(t7) + f4<CRST_T7>((double) PIf - t7);
-			const double f4t8 = f4<CRST_T8>(t8) + f4<CRST_T8>((double) PIf - t8);
+			const double f4t1 = f4<CRST_T1, A>(t1), f4t2 = f4<CRST_T2, A>(t2), f4t3 = f4<CRST_T3, A>(t3);
+			const double f4t7 = f4<CRST_T7, A>(t7) + f4<CRST_T7, A>((A) PIf - t7);
+			const double f4t8 = f4<CRST_T8, A>(t8) + f4<CRST_T8, A>((A) PIf - t8);
 			const double ec = ff2 * f4t1 * f4t2 * f4t3 * f4t7 * f4t8;
 			e[2] = ec;
 			if(ec != 0.) {
 				const double ff2D = f2D<0>(rcmod);
-				const double f4t1Dsin = -f4Dsin<CRST_T1>(t1);
-				const double f4t2Dsin = -f4Dsin<CRST_T2>(t2);
-				const double f4t3Dsin = f4Dsin<CRST_T3>(t3);
-				const double f4t7Dsin = -f4Dsin<CRST_T7>(t7) + f4Dsin<CRST_T7>((double) PIf - t7);
-				const double f4t8Dsin = f4Dsin<CRST_T8>(t8) - f4Dsin<CRST_T8>((double) PIf - t8);
+				const double f4t1Dsin = -f4Dsin<CRST_T1, A>(t1);
+				const double f4t2Dsin = -f4Dsin<CRST_T2, A>(t2);
+				const double f4t3Dsin = f4Dsin<CRST_T3, A>(t3);
+				const double f4t7Dsin = -f4Dsin<CRST_T7, A>(t7) + f4Dsin<CRST_T7, A>((A) PIf - t7);
+				const double f4t8Dsin = f4Dsin<CRST_T8, A>(t8) - f4Dsin<CRST_T8, A>((A) PIf - t8);
 				// the reference's theta4 factor is 1 here, its derivative 0
 				dv3 force = rcdir * (-(ff2D * f4t1 * f4t2 * f4t3 * f4t7 * f4t8));
 				dv3 tl = cross(a1, b1) * (-(-ff2 * f4t1Dsin * f4t2 * f4t3 * f4t7 * f4t8));
@@ -418,27 +426,27 @@ __device__ void nonbonded(const oxh_sys &S, const Nuc &p, const Nuc &q, const dv
 		if((double) F2_RCLOW[1] < rsmod && rsmod < (double) F2_RCHIGH[1]) {
 			const dv3 rsdir = rstack * (1. / rsmod);
 			const double cost5 = dot(a3, rsdir), cost6 = -dot(b3, rsdir);
-			const double t1 = lracos(-dot(a1, b1)), t4 = lracos(dot(a3, b3)), t5 = lracos(cost5), t6 = lracos(cost6);
+			const A t1 = lracos<A>(-dot(a1, b1)), t4 = lracos<A>(dot(a3, b3)), t5 = lracos<A>(cost5), t6 = lracos<A>(cost6);
 			const double rbmod = sqrt(dot(rback, rback));
 			const dv3 rbdir = rback * (1. / rbmod);
 			const double cosphi3 = dot(rsdir, cross(rbdir, a1));
 			const double cosphi4 = dot(rsdir, cross(rbdir, b1));
 			const double ff2 = f2<1>(rsmod);
-			const double t1b = (double) (2 * PIf) - t1;
-			const double f4t1 = f4<CXST_T1>(t1) + f4<CXST_T1>(t1b);
-			const double f4t4 = f4<CXST_T4>(t4);
-			const double f4t5 = f4<CXST_T5>(t5) + f4<CXST_T5>((double) PIf - t5);
-			const double f4t6 = f4<CXST_T6>(t6) + f4<CXST_T6>((double) PIf - t6);
+			const A t1b = (A) (2 * PIf) - t1;
+			const double f4t1 = f4<CXST_T1, A>(t1) + f4<CXST_T1, A>(t1b);
+			const double f4t4 = f4<CXST_T4, A>(t4);
+			const double f4t5 = f4<CXST_T5, A>(t5) + f4<CXST_T5, A>((A) PIf - t5);
+			const double f4t6 = f4<CXST_T6, A>(t6) + f4<CXST_T6, A>((A) PIf - t6);
 			const double f5c3 = f5<CXST_PHI3>(cosphi3);
 			const double f5c4 = f5<CXST_PHI4>(cosphi4);
 			const double ecx = ff2 * f4t1 * f4t4 * f4t5 * f4t6 * f5c3 * f5c4;
 			e[3] = ecx;
 			if(ecx != 0.) {
 				const double ff2D = f2D<1>(rsmod);
-				const double f4t1Dsin = -f4Dsin<CXST_T1>(t1) + f4Dsin<CXST_T1>(t1b);
-				const double f4t4Dsin = f4Dsin<CXST_T4>(t4);
-				const double f4t5Dsin = f4Dsin<CXST_T5>(t5) - f4Dsin<CXST_T5>((double) PIf - t5);
-				const double f4t6Dsin = -f4Dsin<CXST_T6>(t6) + f4Dsin<CXST_T6>((double) PIf - t6);
+				const double f4t1Dsin = -f4Dsin<CXST_T1, A>(t1) + f4Dsin<CXST_T1, A>(t1b);
+				const double f4t4Dsin = f4Dsin<CXST_T4, A>(t4);
+				const double f4t5Dsin = f4Dsin<CXST_T5, A>(t5) - f4Dsin<CXST_T5, A>((A) PIf - t5);
+				const double f4t6Dsin = -f4Dsin<CXST_T6, A>(t6) + f4Dsin<CXST_T6, A>((A) PIf - t6);
 				const double f5Dc3 = f5D<CXST_PHI3>(cosphi3);
 				const double f5Dc4 = f5D<CXST_PHI4>(cosphi4);
 				dv3 force = rsdir * (-ff2D * f4t1 * f4t4 * f4t5 * f4t6 * f5c3 * f5c4);
@@ -535,6 +543,8 @@ __global__ void __launch_bounds__(64) k_forces_rna(int N, const double4 *__restr
 		const int2 *__restrict__ nnbr, int max_neigh, const oxh_sys *__restrict__ systems, oxh_sys_state *__restrict__ sstate,
 		OUT4 *__restrict__ Fout, OUT4 *__restrict__ Tout, double *__restrict__ esplit, int *__restrict__ hbcount, int kstep,
 		const int *__restrict__ limit, int *__restrict__ any, double *__restrict__ stress, int xcd) {
+	// angles (acos, sin and the f4 factors) in FP32 in the mixed form
+	using A = typename std::conditional<std::is_same<OUT4, float4>::value, float, double>::type;
 	const int blk = xcd ? xcd_block(((unsigned) N * K + blockDim.x - 1) / blockDim.x) : (int) blockIdx.x;
 	if(blk < 0) return;
 	const int t = blk * blockDim.x + threadIdx.x;
@@ -570,7 +580,7 @@ __global__ void __launch_bounds__(64) k_forces_rna(int N, const double4 *__restr
 		const dv3 r = n3side ? min_image(P, Pj, S) : min_image(Pj, P, S);
 		dv3 fq = {0, 0, 0}, tp = {0, 0, 0}, tq = {0, 0, 0};
 		double e3[3];
-		bonded(S, n3side ? np : nj, n3side ? nj : np, r, fq, tp, tq, e3, err);
+		bonded<A>(S, n3side ? np : nj, n3side ? nj : np, r, fq, tp, tq, e3, err);
 		if(n3side) {
 			F -= fq;
 			T += tp;
@@ -600,7 +610,7 @@ __global__ void __launch_bounds__(64) k_forces_rna(int N, const double4 *__restr
 		double e = 0.;
 		if(k < nn.x) {
 			double e4[4];
-			nonbonded(S, np, nj, r, fq, tp, e4);
+			nonbonded<A>(S, np, nj, r, fq, tp, e4);
 			e = e4[0] + e4[1] + e4[2] + e4[3];
 			Ehb += 0.5 * e4[1];
 			if(hbcount && e4[1] < -0.1) nhb++;

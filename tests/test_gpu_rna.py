@@ -5,8 +5,8 @@ tests/golden/make_rna_golden.py).
   - states without hydrogen bonds (the "_nohb" twins, the collapsed single
     strands): forces, torques, the energy split and the stress tensor equal
     the reference binary's to FP64 rounding (1e-9 relative), in the
-    all-FP64 form; the mixed form rounds the same FP64 terms into FP32
-    outputs (1e-6);
+    all-FP64 form; the mixed form (angular factors in FP32, FP32 outputs)
+    at the oxDNA2 mixed form's bar (1e-5 of the largest force);
   - states with hydrogen bonds: every energy term but HB equals the
     reference's to FP64 rounding; the HB energy and the forces differ by
     the reference CPU backend's cubic-mesh interpolation of the HB f4
@@ -99,15 +99,17 @@ def test_rna_fp64_vs_reference(hip, name):
 
 @pytest.mark.parametrize("name", ["rna_contacts_nohb", "rna_nicked24_nohb", "rna_tangle12_nohb"])
 def test_rna_mixed_vs_reference(hip, name):
-    """The mixed form: the same FP64 terms rounded into FP32 outputs."""
+    """The mixed form: the angular factors (acos, sin, f4) in FP32, every
+    other quantity in FP64, the results rounded into FP32 outputs."""
     s, opts, ref = load_rna(name)
     F, T, Ep, e8, W, hb, _ = _eval(hip, s, opts, precision="mixed")
     ef = vec_rel_err(F, ref["F"])[0]
     et = vec_rel_err(T, ref["T"])[0]
-    print(f"{name}: mixed F {ef:.2e} T {et:.2e}")
-    assert ef <= 1e-6 and et <= 1e-6
     scale = np.abs(ref["E8"]).sum() + 1e-6 * s.N
-    assert np.all(np.abs(e8 - ref["E8"]) <= 1e-10 * scale)
+    de = np.abs(e8 - ref["E8"]).max() / scale
+    print(f"{name}: mixed F {ef:.2e} T {et:.2e} split {de:.2e}")
+    assert ef <= 1e-5 and et <= 1e-5
+    assert de <= 1e-6
 
 
 @pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16])
