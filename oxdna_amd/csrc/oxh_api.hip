@@ -2254,8 +2254,8 @@ oxh_status oxh_batch_pair_quantities(oxh_batch *B, int sys, int kind, int n, con
 	if(n < 0 || (n > 0 && (!pairs || !out))) return fail(OXH_ERR_ARG, "oxh_batch_pair_quantities: bad arguments");
 	if(n == 0) return OXH_OK;
 	oxh::Batch &b = B->b;
-	if(b.model && kind <= OXH_PAIR_BASE_DISTANCE)
-		return fail(OXH_ERR_ARG, "oxh_batch_pair_quantities: only the centre distance is available for the oxRNA models");
+	if(b.model && kind == OXH_PAIR_NEAR_HB)
+		return fail(OXH_ERR_ARG, "oxh_batch_pair_quantities: the near-HB criterion is not available for the oxRNA models");
 	const int ns = B->n[sys];
 	const std::vector<int> &sid = B->strand_of[sys];
 	std::vector<int4> h(n);

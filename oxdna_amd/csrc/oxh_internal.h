@@ -194,6 +194,9 @@ void launch_slot_of(hipStream_t st, const Batch &b);
 // oxRNA / oxRNA2 pair forces (written, not accumulated) into F / T, or
 // F64 / T64 in the all-FP64 form; b.lanes threads per nucleotide
 void launch_rna_forces(hipStream_t st, const Batch &b, bool split, int k);
+// the FFS per-pair quantities of oxRNA systems (OXH_PAIR_HB_ENERGY,
+// _BASE_DISTANCE, _CENTRE_DISTANCE)
+void launch_rna_pair_quantities(hipStream_t st, const Batch &b, int sys, int kind, int n, const int4 *d_pairs, double *d_out);
 // per-pair FFS order-parameter quantities (OXH_PAIR_*) of system sys:
 // pairs (p, q, same strand, unused) in the system's upload order
 void launch_pair_quantities(hipStream_t st, const Batch &b, int sys, int kind, int n, const int4 *d_pairs, double *d_out);

@@ -89,6 +89,10 @@ __global__ void k_pair_quantities(int n, int kind, int s, const int4 *__restrict
 
 void launch_pair_quantities(hipStream_t st, const Batch &b, int sys, int kind, int n, const int4 *d_pairs, double *d_out) {
 	if(n <= 0) return;
+	if(b.model) {
+		launch_rna_pair_quantities(st, b, sys, kind, n, d_pairs, d_out);
+		return;
+	}
 	k_pair_quantities<<<(n + 255) / 256, 256, 0, st>>>(n, kind, sys, d_pairs, b.pos, b.q64, b.pinfo, b.bonds, b.slot_of, b.d_sys, d_out);
 }
 

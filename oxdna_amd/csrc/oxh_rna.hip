@@ -19,6 +19,7 @@
 #include <cfloat>
 #include <type_traits>
 
+#include "../../include/oxh.h"
 #include "oxh_internal.h"
 #include "oxh_device.h"
 #include "oxh_rna.h"
@@ -688,7 +689,48 @@ void launch_lanes(hipStream_t st, const Batch &b, OUT4 *F, OUT4 *T, bool split, 
 	}
 }
 
+// FFS order-parameter quantities of listed oxRNA pairs (p, q, same
+// strand) of system s, in the system's upload order: the hydrogen-bonding
+// energy (zero for bonded neighbours), or the base-base / centre-centre
+// distance (the plain difference for pairs of one strand, the nearest image
+// otherwise), as src/Utilities/OrderParameters.cpp:73-98 takes them
+__global__ void k_rna_pair_quantities(int n, int kind, int s, const int4 *__restrict__ pairs, const double4 *__restrict__ pos,
+		const double4 *__restrict__ q64, const int2 *__restrict__ pinfo, const int2 *__restrict__ bonds, const int *__restrict__ slot_of,
+		const oxh_sys *__restrict__ systems, double *__restrict__ out) {
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if(k >= n) return;
+	const oxh_sys &S = systems[s];
+	const int4 pr = pairs[k];
+	const int p = slot_of[S.first + pr.x], q = slot_of[S.first + pr.y];
+	const double4 P = pos[p], Q = pos[q];
+	const bool plain = kind >= OXH_PAIR_BASE_DISTANCE && pr.z != 0;
+	const dv3 r = plain ? dv3{Q.x - P.x, Q.y - P.y, Q.z - P.z} : min_image(P, Q, S);
+	if(kind == OXH_PAIR_CENTRE_DISTANCE) {
+		out[k] = sqrt(dot(r, r));
+		return;
+	}
+	const Nuc np = nuc(q64[p], pinfo[p].x), nq = nuc(q64[q], pinfo[q].x);
+	if(kind == OXH_PAIR_BASE_DISTANCE) {
+		const dv3 rb = sep(r, s_base(nq), s_base(np));
+		out[k] = sqrt(dot(rb, rb));
+		return;
+	}
+	const int2 bd = bonds[p];
+	double e4[4] = {0., 0., 0., 0.};
+	if(p != q && bd.x != q && bd.y != q) {
+		dv3 fq = {0, 0, 0}, tp = {0, 0, 0};
+		nonbonded<double>(S, np, nq, r, fq, tp, e4);
+	}
+	out[k] = e4[1];
+}
+
 }  // namespace rna
+
+void launch_rna_pair_quantities(hipStream_t st, const Batch &b, int sys, int kind, int n, const int4 *d_pairs, double *d_out) {
+	if(n <= 0) return;
+	rna::k_rna_pair_quantities<<<(n + 255) / 256, 256, 0, st>>>(n, kind, sys, d_pairs, b.pos, b.q64, b.pinfo, b.bonds, b.slot_of, b.d_sys,
+			d_out);
+}
 
 void launch_rna_forces(hipStream_t st, const Batch &b, bool split, int k) {
 	if(b.N == 0) return;

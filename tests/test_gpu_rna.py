@@ -249,3 +249,35 @@ def test_driver_runs_reference_rna_test_input(hip, tmp_path, precision):
     other = [0, 1, 2, 3, 5, 6, 7]
     assert np.all(np.abs(got[other] - want[other]) <= 1.01e-6)
     assert abs(got[4] - want[4]) <= 5e-5 * abs(want[4]) + 1.01e-6
+
+
+def test_rna_pair_quantities(hip):
+    """FFS order-parameter quantities of oxRNA2 pairs: the HB energies of
+    every non-bonded pair sum to the split's HB term, base-base distances
+    follow the sites (0.4 a1) under the nearest image, and the oxDNA-only
+    near-HB criterion is refused."""
+    s, opts, ref = load_rna("rna_contacts_45C")
+    L = s.box
+    d = s.pos[:, None, :] - s.pos[None, :, :]
+    d -= L * np.rint(d / L)
+    close = np.linalg.norm(d, axis=2) < 2.0
+    bonded = np.zeros_like(close)
+    for i in range(s.N):
+        for j in (s.n3[i], s.n5[i]):
+            if j >= 0:
+                bonded[i, j] = True
+    ii, jj = np.nonzero(np.triu(close & ~bonded, 1))
+    pairs = np.stack([ii, jj], axis=1)
+    with hip.HIPBatch(s, backend_precision="double", **opts) as b:
+        e8 = b.energy_split()
+        hb = b.pair_quantities("hb_energy", pairs)
+        other = pairs[s.strand[ii] != s.strand[jj]]
+        dist = b.pair_quantities("base_distance", other)
+        with pytest.raises(Exception):
+            b.pair_quantities("near_hb", pairs[:4])
+    print(f"sum of pair HB {hb.sum():.12f}, split HB {e8[4]:.12f}, {len(pairs)} pairs")
+    assert abs(hb.sum() - e8[4]) <= 1e-10 * abs(e8[4])
+    bi, bj = s.pos[other[:, 0]] + 0.4 * s.a1[other[:, 0]], s.pos[other[:, 1]] + 0.4 * s.a1[other[:, 1]]
+    db = bj - bi
+    db -= L * np.rint((s.pos[other[:, 1]] - s.pos[other[:, 0]]) / L)
+    assert np.allclose(dist, np.linalg.norm(db, axis=1), rtol=0, atol=1e-12)
