@@ -280,4 +280,9 @@ def test_rna_pair_quantities(hip):
     bi, bj = s.pos[other[:, 0]] + 0.4 * s.a1[other[:, 0]], s.pos[other[:, 1]] + 0.4 * s.a1[other[:, 1]]
     db = bj - bi
     db -= L * np.rint((s.pos[other[:, 1]] - s.pos[other[:, 0]]) / L)
-    assert np.allclose(dist, np.linalg.norm(db, axis=1), rtol=0, atol=1e-12)
+    dd = np.abs(dist - np.linalg.norm(db, axis=1)).max()
+    print(f"base distances: {len(other)} pairs, largest difference {dd:.2e}")
+    # the GPU's a1 is the first column of the unit quaternion fitted to the
+    # stored (a1, a3), which the configuration file carries orthonormal to
+    # ~1e-8 only (measured difference 1.2e-8)
+    assert dd <= 1e-7
