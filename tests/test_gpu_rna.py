@@ -254,7 +254,7 @@ def test_driver_runs_reference_rna_test_input(hip, tmp_path, precision):
 def test_rna_pair_quantities(hip):
     """FFS order-parameter quantities of oxRNA2 pairs: the HB energies of
     every non-bonded pair sum to the split's HB term, base-base distances
-    follow the sites (0.4 a1) under the nearest image, and the oxDNA-only
+    follow the sites (0.4f a1) under the nearest image, and the oxDNA-only
     near-HB criterion is refused."""
     s, opts, ref = load_rna("rna_contacts_45C")
     L = s.box
@@ -277,12 +277,10 @@ def test_rna_pair_quantities(hip):
             b.pair_quantities("near_hb", pairs[:4])
     print(f"sum of pair HB {hb.sum():.12f}, split HB {e8[4]:.12f}, {len(pairs)} pairs")
     assert abs(hb.sum() - e8[4]) <= 1e-10 * abs(e8[4])
-    bi, bj = s.pos[other[:, 0]] + 0.4 * s.a1[other[:, 0]], s.pos[other[:, 1]] + 0.4 * s.a1[other[:, 1]]
+    pb = float(np.float32(0.4))   # RNA_POS_BASE = 0.4f (src/Interactions/rna_model.h:315)
+    bi, bj = s.pos[other[:, 0]] + pb * s.a1[other[:, 0]], s.pos[other[:, 1]] + pb * s.a1[other[:, 1]]
     db = bj - bi
     db -= L * np.rint((s.pos[other[:, 1]] - s.pos[other[:, 0]]) / L)
     dd = np.abs(dist - np.linalg.norm(db, axis=1)).max()
     print(f"base distances: {len(other)} pairs, largest difference {dd:.2e}")
-    # the GPU's a1 is the first column of the unit quaternion fitted to the
-    # stored (a1, a3), which the configuration file carries orthonormal to
-    # ~1e-8 only (measured difference 1.2e-8)
-    assert dd <= 1e-7
+    assert dd <= 1e-12
